@@ -1,0 +1,1765 @@
+// ORACLE — test infrastructure only (see kumo_oracle.hpp). CPU FP64
+// restatement of the reference chem-kumo 2.1.0 evaluator; citations are to
+// the reference sources. Built with g++ -std=c++20 -O3 -fopenmp
+// -ffp-contract=off so that no multiply-add is fused, as in the reference's
+// x86-64 builds.
+#include "kumo_oracle.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <numbers>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Basis preparation: src/basis_math.cpp, src/basis_evaluator.cpp:15-202.
+
+struct Power {
+    int x, y, z;
+};
+
+// Gaussian FCHK order of Cartesian components (src/basis_evaluator.cpp:135-152).
+std::vector<Power> cartesian_order(int l) {
+    switch (l) {
+    case 0: return {{0, 0, 0}};
+    case 1: return {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    case 2: return {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    case 3:
+        return {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {1, 2, 0}, {2, 1, 0},
+                {2, 0, 1}, {1, 0, 2}, {0, 1, 2}, {0, 2, 1}, {1, 1, 1}};
+    default: break;
+    }
+    std::vector<Power> out;
+    for (int lx = 0; lx <= l; ++lx)
+        for (int ly = 0; ly <= l - lx; ++ly)
+            out.push_back({lx, ly, l - lx - ly});
+    return out;
+}
+
+double fact(int n) {
+    double v = 1.0;
+    for (long long i = 2; i <= n; ++i)
+        v *= static_cast<double>(i);
+    return v;
+}
+
+// Schlegel & Frisch (1995) closed form of the Cartesian->pure coefficient,
+// as in src/basis_evaluator.cpp:30-96 (same published formula and factor
+// grouping, so the transform rounds the same way).
+double pure_coefficient(int l, int m, int lx, int ly) {
+    const int lz = l - lx - ly;
+    const int ma = std::abs(m);
+    if (ma > l || lz < 0) return 0.0;
+    const double a = std::sqrt(fact(2 * lx) * fact(2 * ly) * fact(2 * lz) * fact(l) /
+                               (fact(2 * l) * fact(lx) * fact(ly) * fact(lz)));
+    const double b = std::sqrt(fact(l - ma) / fact(l + ma)) / (std::pow(2.0, l) * fact(l));
+    const int ilim = (l - ma) / 2;
+    const int j = (l - ma - lz) / 2;
+    if (2 * j != l - ma - lz) return 0.0;
+    double g = 0.0;
+    for (int i = 0; i <= ilim; ++i) {
+        if (j < 0 || j > i) continue;
+        const double c = fact(l) / (fact(i) * fact(l - i)) * fact(2 * l - 2 * i) *
+                         std::pow(-1.0, i) / fact(l - ma - 2 * i) * fact(i) /
+                         (fact(j) * fact(i - j));
+        for (int k = 0; k <= j; ++k) {
+            if (lx - 2 * k < 0 || lx - 2 * k > ma) continue;
+            const double d = fact(j) / (fact(k) * fact(j - k)) * fact(ma) /
+                             (fact(lx - 2 * k) * fact(ma + 2 * k - lx));
+            double e = 0.0;
+            if (m == 0 && lx % 2 == 0) e = std::pow(-1.0, k - lx / 2);
+            const int kt = (2 * k + ma - lx) / 2;
+            if (m > 0 && std::abs(ma - lx) % 2 == 0) e = std::sqrt(2.0) * std::pow(-1.0, kt);
+            if (m < 0 && std::abs(ma - lx) % 2 == 1) e = std::sqrt(2.0) * std::pow(-1.0, kt);
+            g += c * d * e;
+        }
+    }
+    return a * b * g;
+}
+
+// (2l+1) x ncart, rows m = 0, +1, -1, ... (src/basis_evaluator.cpp:155-175).
+std::vector<double> pure_transform(int l) {
+    const auto order = cartesian_order(l);
+    const int nc = static_cast<int>(order.size());
+    std::vector<double> t(static_cast<std::size_t>((2 * l + 1) * nc), 0.0);
+    for (int g = 0; g < nc; ++g) {
+        const auto& p = order[static_cast<std::size_t>(g)];
+        t[static_cast<std::size_t>(g)] = pure_coefficient(l, 0, p.x, p.y);
+        for (int m = 1; m <= l; ++m) {
+            t[static_cast<std::size_t>((2 * m - 1) * nc + g)] = pure_coefficient(l, m, p.x, p.y);
+            t[static_cast<std::size_t>((2 * m) * nc + g)] = pure_coefficient(l, -m, p.x, p.y);
+        }
+    }
+    return t;
+}
+
+double double_factorial_odd(int n) {
+    if (n <= 0) return 1.0;
+    double v = 1.0;
+    for (int i = n; i >= 1; i -= 2)
+        v *= static_cast<double>(i);
+    return v;
+}
+
+// src/basis_math.cpp:17-25.
+double primitive_normalization(double alpha, const Power& p) {
+    const int l = p.x + p.y + p.z;
+    const double radial = std::pow(2.0 * alpha / std::numbers::pi, 0.75);
+    const double angular = std::sqrt(std::pow(4.0 * alpha, static_cast<double>(l)) /
+                                     (double_factorial_odd(2 * p.x - 1) *
+                                      double_factorial_odd(2 * p.y - 1) *
+                                      double_factorial_odd(2 * p.z - 1)));
+    return radial * angular;
+}
+
+struct Block {
+    double origin[3];
+    int l = 0;
+    int atom = 0;
+    std::vector<double> exps;
+    std::vector<Power> powers;
+    std::vector<double> weights; // ncart x nprim, row-major
+    std::vector<double> pure;    // count x ncart, row-major; empty if Cartesian
+    int offset = 0, count = 0;
+    double cutoff_r2 = 0.0;
+    int ncart() const { return static_cast<int>(powers.size()); }
+    int nprim() const { return static_cast<int>(exps.size()); }
+    double w(int k, int p) const { return weights[static_cast<std::size_t>(k * nprim() + p)]; }
+};
+
+// src/basis_evaluator.cpp:98-127: radius beyond which every AO of the block
+// and its first and second derivatives are at most `cutoff`.
+double shell_cutoff_radius(const Block& b, double transform_norm, double cutoff) {
+    const int l = b.l;
+    const int np = b.nprim();
+    std::vector<double> coefficient(static_cast<std::size_t>(np));
+    double start = 1.0;
+    for (int p = 0; p < np; ++p) {
+        const double a = b.exps[static_cast<std::size_t>(p)];
+        double weight = 0.0;
+        for (int k = 0; k < b.ncart(); ++k)
+            weight = std::max(weight, std::abs(b.w(k, p)));
+        coefficient[static_cast<std::size_t>(p)] =
+            transform_norm * weight * (1.0 + l + l * l + 2.0 * a * (2.0 * l + 2.0) + 4.0 * a * a);
+        start = std::max(start, std::sqrt((l + 2.0) / (2.0 * a)));
+    }
+    const auto bound = [&](double r) {
+        double sum = 0.0;
+        for (int p = 0; p < np; ++p)
+            sum += coefficient[static_cast<std::size_t>(p)] * std::pow(r, l + 2) *
+                   std::exp(-b.exps[static_cast<std::size_t>(p)] * r * r);
+        return sum;
+    };
+    if (bound(start) <= cutoff) return start;
+    double low = start, high = 2.0 * start;
+    while (bound(high) > cutoff)
+        high *= 2.0;
+    for (int it = 0; it < 60; ++it) {
+        const double mid = 0.5 * (low + high);
+        (bound(mid) > cutoff ? low : high) = mid;
+    }
+    return high;
+}
+
+constexpr double ao_cutoff = 1e-14; // include/chem/kumo/basis_evaluator.hpp:66
+
+struct Basis {
+    std::vector<Block> blocks;
+    int nao = 0;
+    int max_l = 0, max_cart = 0, max_prim = 0;
+};
+
+// src/basis.cpp:21-120 + src/basis_math.cpp:27-66 + src/basis_evaluator.cpp:177-202.
+std::unique_ptr<Basis> make_basis(int nshell, const int* types, const int* nprim,
+                                  const int* shell_atom, const double* xyz, const double* exps,
+                                  const double* coef, const double* pcoef) {
+    auto basis = std::make_unique<Basis>();
+    std::size_t prim = 0;
+    int offset = 0;
+    const auto add = [&](int s, int l, const double* c, bool pure) {
+        Block b;
+        for (int a = 0; a < 3; ++a)
+            b.origin[a] = xyz[3 * s + a];
+        b.l = l;
+        b.atom = shell_atom[s] - 1;
+        b.exps.assign(exps + prim, exps + prim + nprim[s]);
+        b.powers = cartesian_order(l);
+        const int nc = b.ncart(), np = nprim[s];
+        b.weights.resize(static_cast<std::size_t>(nc * np));
+        for (int k = 0; k < nc; ++k)
+            for (int p = 0; p < np; ++p)
+                b.weights[static_cast<std::size_t>(k * np + p)] =
+                    c[prim + static_cast<std::size_t>(p)] *
+                    primitive_normalization(b.exps[static_cast<std::size_t>(p)],
+                                            b.powers[static_cast<std::size_t>(k)]);
+        if (pure && l >= 2) b.pure = pure_transform(l);
+        b.offset = offset;
+        b.count = b.pure.empty() ? nc : 2 * l + 1;
+        offset += b.count;
+        double tnorm = 1.0;
+        if (!b.pure.empty()) {
+            tnorm = 0.0;
+            for (int r = 0; r < b.count; ++r) {
+                double s2 = 0.0;
+                for (int k = 0; k < nc; ++k)
+                    s2 += std::abs(b.pure[static_cast<std::size_t>(r * nc + k)]);
+                tnorm = std::max(tnorm, s2);
+            }
+        }
+        const double radius = shell_cutoff_radius(b, tnorm, ao_cutoff);
+        b.cutoff_r2 = radius * radius;
+        basis->max_l = std::max(basis->max_l, l);
+        basis->max_cart = std::max(basis->max_cart, nc);
+        basis->max_prim = std::max(basis->max_prim, np);
+        basis->blocks.push_back(std::move(b));
+    };
+    for (int s = 0; s < nshell; ++s) {
+        const int t = types[s];
+        if (nprim[s] <= 0) throw std::runtime_error("invalid primitive count in fchk shell");
+        if (t == -1) {
+            if (!pcoef) throw std::runtime_error("missing P(S=P) coefficients for sp shell");
+            add(s, 0, coef, false);
+            add(s, 1, pcoef, false);
+        } else {
+            const int l = t < 0 ? -t : t;
+            add(s, l, coef, t <= -2);
+        }
+        prim += static_cast<std::size_t>(nprim[s]);
+    }
+    basis->nao = offset;
+    return basis;
+}
+
+// ---------------------------------------------------------------------------
+// AO evaluation: src/basis_evaluator.cpp:413-643.
+
+inline double sqnorm(double x, double y, double z) { return (x * x + y * y) + z * z; }
+
+struct Scratch {
+    std::vector<double> radial, cart, pw, factors;
+};
+
+// AO values at one point (evaluate_into_impl, src/basis_evaluator.cpp:442-525):
+// Cartesian component = ((x^i y^j) z^k) * sum_p w_kp exp(-a_p r^2).
+void ao_values_point(const Basis& B, const double* pt, double* out, Scratch& s) {
+    s.radial.resize(static_cast<std::size_t>(B.max_prim));
+    s.cart.resize(static_cast<std::size_t>(B.max_cart));
+    for (const auto& b : B.blocks) {
+        const double x = pt[0] - b.origin[0], y = pt[1] - b.origin[1], z = pt[2] - b.origin[2];
+        const double r2 = sqnorm(x, y, z);
+        if (r2 > b.cutoff_r2) {
+            for (int i = 0; i < b.count; ++i)
+                out[b.offset + i] = 0.0;
+            continue;
+        }
+        for (int p = 0; p < b.nprim(); ++p)
+            s.radial[static_cast<std::size_t>(p)] = std::exp(-b.exps[static_cast<std::size_t>(p)] * r2);
+        double xp[32], yp[32], zp[32];
+        xp[0] = yp[0] = zp[0] = 1.0;
+        for (int n = 1; n <= b.l; ++n) {
+            xp[n] = xp[n - 1] * x;
+            yp[n] = yp[n - 1] * y;
+            zp[n] = zp[n - 1] * z;
+        }
+        for (int k = 0; k < b.ncart(); ++k) {
+            const auto& pw = b.powers[static_cast<std::size_t>(k)];
+            const double poly = (xp[pw.x] * yp[pw.y]) * zp[pw.z];
+            double rs = 0.0;
+            for (int p = 0; p < b.nprim(); ++p)
+                rs += b.w(k, p) * s.radial[static_cast<std::size_t>(p)];
+            s.cart[static_cast<std::size_t>(k)] = poly * rs;
+        }
+        if (b.pure.empty()) {
+            for (int k = 0; k < b.ncart(); ++k)
+                out[b.offset + k] = s.cart[static_cast<std::size_t>(k)];
+        } else {
+            for (int r = 0; r < b.count; ++r) {
+                double acc = 0.0;
+                for (int k = 0; k < b.ncart(); ++k)
+                    acc += b.pure[static_cast<std::size_t>(r * b.ncart() + k)] *
+                           s.cart[static_cast<std::size_t>(k)];
+                out[b.offset + r] = acc;
+            }
+        }
+    }
+}
+
+// Row-major panel matrix: rows = AOs, columns = panels * n (component c of
+// point j at column c * n + j), the transpose of the reference's
+// column-major layout with the same (row, column) meaning.
+struct Panels {
+    int rows = 0, cols = 0;
+    std::vector<double> v;
+    void resize(int r, int c) {
+        rows = r;
+        cols = c;
+        v.assign(static_cast<std::size_t>(r) * static_cast<std::size_t>(c), 0.0);
+    }
+    double& at(int r, int c) { return v[static_cast<std::size_t>(r) * static_cast<std::size_t>(cols) + static_cast<std::size_t>(c)]; }
+    double at(int r, int c) const { return v[static_cast<std::size_t>(r) * static_cast<std::size_t>(cols) + static_cast<std::size_t>(c)]; }
+    const double* row(int r) const { return v.data() + static_cast<std::size_t>(r) * static_cast<std::size_t>(cols); }
+    double* row(int r) { return v.data() + static_cast<std::size_t>(r) * static_cast<std::size_t>(cols); }
+};
+
+int panel_count(int order) { return order == 0 ? 1 : (order == 2 ? 10 : 4); }
+
+// evaluate_batch (order 0) or evaluate_derivatives_batch (order 1, 2):
+// src/basis_evaluator.cpp:512-643, 665-678.
+void ao_panels(const Basis& B, int n, const double* pts, int order, Panels& out, Scratch& s) {
+    const int panels = panel_count(order);
+    out.resize(B.nao, panels * n);
+    if (order == 0) {
+        std::vector<double> col(static_cast<std::size_t>(B.nao));
+        for (int j = 0; j < n; ++j) {
+            ao_values_point(B, pts + 3 * j, col.data(), s);
+            for (int a = 0; a < B.nao; ++a)
+                out.at(a, j) = col[static_cast<std::size_t>(a)];
+        }
+        return;
+    }
+    const int deg = B.max_l + 1;
+    s.radial.resize(static_cast<std::size_t>(B.max_prim));
+    s.pw.resize(static_cast<std::size_t>(3 * deg));
+    s.factors.resize(static_cast<std::size_t>(B.max_prim * 3 * deg * 3));
+    std::vector<double> cart(static_cast<std::size_t>(B.max_cart * panels));
+    for (const auto& b : B.blocks) {
+        const int nc = b.ncart(), np = b.nprim(), L = b.l;
+        for (int j = 0; j < n; ++j) {
+            const double r[3] = {pts[3 * j] - b.origin[0], pts[3 * j + 1] - b.origin[1],
+                                 pts[3 * j + 2] - b.origin[2]};
+            const double r2 = sqnorm(r[0], r[1], r[2]);
+            if (!(r2 <= b.cutoff_r2)) continue; // zeros stay
+            for (int p = 0; p < np; ++p)
+                s.radial[static_cast<std::size_t>(p)] = std::exp(-b.exps[static_cast<std::size_t>(p)] * r2);
+            for (int axis = 0; axis < 3; ++axis) {
+                double* xp = &s.pw[static_cast<std::size_t>(axis * deg)];
+                xp[0] = 1.0;
+                for (int m = 1; m <= L; ++m)
+                    xp[m] = xp[m - 1] * r[axis];
+            }
+            const auto F = [&](int p, int axis, int m) {
+                return &s.factors[static_cast<std::size_t>(((p * 3 + axis) * deg + m) * 3)];
+            };
+            for (int p = 0; p < np; ++p) {
+                const double a = b.exps[static_cast<std::size_t>(p)];
+                for (int axis = 0; axis < 3; ++axis) {
+                    const double* xp = &s.pw[static_cast<std::size_t>(axis * deg)];
+                    const double x = r[axis];
+                    for (int l = 0; l <= L; ++l) {
+                        double* f = F(p, axis, l);
+                        const double xl = xp[l];
+                        f[0] = xl;
+                        f[1] = (l ? l * xp[l - 1] : 0.0) - 2.0 * a * x * xl;
+                        f[2] = order == 2 ? (l >= 2 ? l * (l - 1) * xp[l - 2] : 0.0) -
+                                                2.0 * a * (2 * l + 1) * xl +
+                                                4.0 * a * a * x * x * xl
+                                          : 0.0;
+                    }
+                }
+            }
+            for (int k = 0; k < nc; ++k) {
+                const auto& pw = b.powers[static_cast<std::size_t>(k)];
+                double sums[10] = {};
+                for (int p = 0; p < np; ++p) {
+                    const double wgt = b.w(k, p) * s.radial[static_cast<std::size_t>(p)];
+                    const double* fx = F(p, 0, pw.x);
+                    const double* fy = F(p, 1, pw.y);
+                    const double* fz = F(p, 2, pw.z);
+                    sums[0] += wgt * fx[0] * fy[0] * fz[0];
+                    sums[1] += wgt * fx[1] * fy[0] * fz[0];
+                    sums[2] += wgt * fx[0] * fy[1] * fz[0];
+                    sums[3] += wgt * fx[0] * fy[0] * fz[1];
+                    if (order == 2) {
+                        sums[4] += wgt * fx[2] * fy[0] * fz[0];
+                        sums[5] += wgt * fx[1] * fy[1] * fz[0];
+                        sums[6] += wgt * fx[1] * fy[0] * fz[1];
+                        sums[7] += wgt * fx[0] * fy[2] * fz[0];
+                        sums[8] += wgt * fx[0] * fy[1] * fz[1];
+                        sums[9] += wgt * fx[0] * fy[0] * fz[2];
+                    }
+                }
+                for (int c = 0; c < panels; ++c)
+                    cart[static_cast<std::size_t>(k * panels + c)] = sums[c];
+            }
+            for (int c = 0; c < panels; ++c) {
+                if (b.pure.empty()) {
+                    for (int k = 0; k < nc; ++k)
+                        out.at(b.offset + k, c * n + j) = cart[static_cast<std::size_t>(k * panels + c)];
+                } else {
+                    for (int row = 0; row < b.count; ++row) {
+                        double acc = 0.0;
+                        for (int k = 0; k < nc; ++k)
+                            acc += b.pure[static_cast<std::size_t>(row * nc + k)] *
+                                   cart[static_cast<std::size_t>(k * panels + c)];
+                        out.at(b.offset + row, c * n + j) = acc;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Analytic overlap: src/basis_evaluator.cpp:210-278.
+
+double ipow(double x, int n) {
+    double r = 1.0;
+    for (int i = 0; i < n; ++i)
+        r *= x;
+    return r;
+}
+
+double overlap_axis(int a, int b, double alpha, double beta, double A, double Bc) {
+    const double p = alpha + beta;
+    const double center = (alpha * A + beta * Bc) / p;
+    const double PA = center - A, PB = center - Bc;
+    const double pref = std::exp(-alpha * beta / p * (A - Bc) * (A - Bc));
+    std::vector<double> mom(static_cast<std::size_t>(a + b + 1), 0.0);
+    mom[0] = std::sqrt(std::numbers::pi / p);
+    for (int k = 2; k <= a + b; k += 2)
+        mom[static_cast<std::size_t>(k)] = (k - 1) * mom[static_cast<std::size_t>(k - 2)] / (2.0 * p);
+    double res = 0.0, ba = 1.0;
+    for (int i = 0; i <= a; ++i) {
+        double bb = 1.0;
+        for (int j = 0; j <= b; ++j) {
+            res += ba * bb * ipow(PA, a - i) * ipow(PB, b - j) * mom[static_cast<std::size_t>(i + j)];
+            if (j < b) bb *= static_cast<double>(b - j) / (j + 1);
+        }
+        if (i < a) ba *= static_cast<double>(a - i) / (i + 1);
+    }
+    return pref * res;
+}
+
+std::vector<double> overlap(const Basis& B) {
+    const int n = B.nao;
+    std::vector<double> S(static_cast<std::size_t>(n) * static_cast<std::size_t>(n), 0.0);
+    for (std::size_t ia = 0; ia < B.blocks.size(); ++ia) {
+        const auto& A = B.blocks[ia];
+        for (std::size_t ib = ia; ib < B.blocks.size(); ++ib) {
+            const auto& Bb = B.blocks[ib];
+            const int na = A.ncart(), nb = Bb.ncart();
+            std::vector<double> cart(static_cast<std::size_t>(na * nb), 0.0);
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j) {
+                    const auto& x = A.powers[static_cast<std::size_t>(i)];
+                    const auto& y = Bb.powers[static_cast<std::size_t>(j)];
+                    double v = 0.0;
+                    for (int p = 0; p < A.nprim(); ++p)
+                        for (int q = 0; q < Bb.nprim(); ++q) {
+                            const double al = A.exps[static_cast<std::size_t>(p)], be = Bb.exps[static_cast<std::size_t>(q)];
+                            const double prim = overlap_axis(x.x, y.x, al, be, A.origin[0], Bb.origin[0]) *
+                                                overlap_axis(x.y, y.y, al, be, A.origin[1], Bb.origin[1]) *
+                                                overlap_axis(x.z, y.z, al, be, A.origin[2], Bb.origin[2]);
+                            v += A.w(i, p) * Bb.w(j, q) * prim;
+                        }
+                    cart[static_cast<std::size_t>(i * nb + j)] = v;
+                }
+            // block = TA * cart * TB^T
+            std::vector<double> left(static_cast<std::size_t>(A.count * nb));
+            for (int r = 0; r < A.count; ++r)
+                for (int j = 0; j < nb; ++j) {
+                    if (A.pure.empty()) {
+                        left[static_cast<std::size_t>(r * nb + j)] = cart[static_cast<std::size_t>(r * nb + j)];
+                    } else {
+                        double acc = 0.0;
+                        for (int k = 0; k < na; ++k)
+                            acc += A.pure[static_cast<std::size_t>(r * na + k)] * cart[static_cast<std::size_t>(k * nb + j)];
+                        left[static_cast<std::size_t>(r * nb + j)] = acc;
+                    }
+                }
+            for (int r = 0; r < A.count; ++r)
+                for (int c = 0; c < Bb.count; ++c) {
+                    double v;
+                    if (Bb.pure.empty()) {
+                        v = left[static_cast<std::size_t>(r * nb + c)];
+                    } else {
+                        v = 0.0;
+                        for (int k = 0; k < nb; ++k)
+                            v += left[static_cast<std::size_t>(r * nb + k)] * Bb.pure[static_cast<std::size_t>(c * nb + k)];
+                    }
+                    const std::size_t i = static_cast<std::size_t>(A.offset + r), j = static_cast<std::size_t>(Bb.offset + c);
+                    S[i * static_cast<std::size_t>(n) + j] = v;
+                    S[j * static_cast<std::size_t>(n) + i] = v;
+                }
+        }
+    }
+    return S;
+}
+
+// ---------------------------------------------------------------------------
+// Molecule: src/molecule.cpp, src/backend_selection.hpp:43-78.
+
+struct Atom {
+    int z;
+    double charge;
+    double pos[3];
+};
+
+struct Molecule {
+    const Basis* basis = nullptr;
+    std::vector<Atom> atoms;
+    int n = 0;
+    std::vector<double> P, S; // total, spin (row-major n x n)
+    std::vector<double> Ca, Cb; // n x nmo, row-major (AO rows, MO columns)
+    int nmo_a = 0, nmo_b = 0, nalpha = 0, nbeta = 0;
+    bool unrestricted = false;
+    bool eligible = false;
+    const std::vector<double>& C(int channel) const { return (channel == 1 && unrestricted) ? Cb : Ca; }
+    int nmo(int channel) const { return (channel == 1 && unrestricted) ? nmo_b : nmo_a; }
+};
+
+double frobenius(const std::vector<double>& m) {
+    double s = 0.0;
+    for (double v : m)
+        s += v * v;
+    return std::sqrt(s);
+}
+
+// validate_orbital_backend (src/backend_selection.hpp:43-78).
+bool check_eligible(const Molecule& m) {
+    const int n = m.n;
+    const auto& cb = m.unrestricted ? m.Cb : m.Ca;
+    const int nmob = m.unrestricted ? m.nmo_b : m.nmo_a;
+    if (m.nalpha > m.nmo_a || m.nbeta > nmob || (!m.unrestricted && m.nalpha != m.nbeta)) return false;
+    if (m.Ca.empty()) return false;
+    std::vector<double> pa(static_cast<std::size_t>(n * n), 0.0), pb(static_cast<std::size_t>(n * n), 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double sa = 0.0, sb = 0.0;
+            for (int k = 0; k < m.nalpha; ++k)
+                sa += m.Ca[static_cast<std::size_t>(i * m.nmo_a + k)] * m.Ca[static_cast<std::size_t>(j * m.nmo_a + k)];
+            for (int k = 0; k < m.nbeta; ++k)
+                sb += cb[static_cast<std::size_t>(i * nmob + k)] * cb[static_cast<std::size_t>(j * nmob + k)];
+            pa[static_cast<std::size_t>(i * n + j)] = sa;
+            pb[static_cast<std::size_t>(i * n + j)] = sb;
+        }
+    std::vector<double> rt(pa.size()), rs(pa.size()), et(pa.size()), es(pa.size());
+    for (std::size_t q = 0; q < pa.size(); ++q) {
+        rt[q] = pa[q] + pb[q];
+        rs[q] = pa[q] - pb[q];
+        et[q] = m.P[q] - rt[q];
+        es[q] = m.S[q] - rs[q];
+    }
+    const double te = frobenius(et), se = frobenius(es);
+    const double ts = std::max(frobenius(m.P), frobenius(rt));
+    const double ss = std::max({frobenius(m.S), frobenius(rs), ts});
+    return te <= 1e-10 + 1e-8 * ts && se <= 1e-10 + 1e-8 * ss;
+}
+
+// ---------------------------------------------------------------------------
+// Request plan: src/evaluation_plan.hpp:37-273, src/automatic_backend.hpp:38-50.
+
+enum Kind {
+    Density = 0, Gradient, Hessian, Laplacian, Tau, Orbital, Elf, ReducedGradient,
+    SignLambda2Density, ElectrostaticPotential, Lol, InteractionRegionIndicator,
+    DensityOverlapRegionsIndicator, HamiltonianKineticEnergy, PotentialEnergyDensity,
+    EnergyDensity, SpinPolarization, ShannonEntropyDensity, FisherInformationDensity,
+    AverageLocalIonizationEnergy, LocalElectronAffinity, LocalElectronAttachmentEnergy,
+    LocalElectronegativity, LocalHardness, OrbitalGradient, OrbitalHessian,
+    OrbitalWeightedFukuiPlus, OrbitalWeightedFukuiMinus, OrbitalWeightedFukuiZero,
+    OrbitalWeightedDualDescriptor, TauComponents, HamiltonianKineticEnergyComponents,
+    PotentialKineticRatio, EnergyDensityPerElectron, KineticEnergyPerElectron,
+    ElfTsirelsonStash, LolTsirelsonStash, KindCount
+};
+enum Comp { Total = 0, Alpha = 1, Beta = 2, Difference = 3 };
+enum Backend { AUTO = 0, AO_P = 1, AO_MO = 2 };
+
+int channels(int k) {
+    switch (k) {
+    case Gradient: case OrbitalGradient: case TauComponents: case HamiltonianKineticEnergyComponents: return 3;
+    case Hessian: case OrbitalHessian: return 6;
+    default: return 1;
+    }
+}
+
+bool is_orbital_property(int k) { return k == Orbital || k == OrbitalGradient || k == OrbitalHessian; }
+bool is_orbital_energy(int k) {
+    return (k >= AverageLocalIonizationEnergy && k <= LocalHardness) ||
+           (k >= OrbitalWeightedFukuiPlus && k <= OrbitalWeightedDualDescriptor);
+}
+
+struct ComponentPlan {
+    bool density = false, gradient = false, hessian = false, laplacian = false, tau = false;
+    bool elf = false, elf_input = false, reduced_gradient = false, sign_lambda2 = false;
+    bool lol = false, lol_input = false, ge_input = false, gradient_input = false;
+    bool hessian_input = false, energy_input = false, tau_input = false, value_input = false;
+    bool esp = false;
+    bool derivatives() const {
+        return gradient || hessian || laplacian || tau || elf_input || reduced_gradient || sign_lambda2 ||
+               lol_input || gradient_input || hessian_input || energy_input || tau_input || ge_input ||
+               value_input;
+    }
+    int ao_order() const {
+        if (hessian || laplacian || sign_lambda2 || hessian_input || energy_input || ge_input) return 2;
+        if (gradient || tau || elf_input || reduced_gradient || lol_input || gradient_input || tau_input || value_input)
+            return 1;
+        return density ? 0 : -1;
+    }
+};
+
+struct Plan {
+    std::array<ComponentPlan, 4> c{};
+    bool orbitals = false, orbital_energies = false;
+    int max_order = -1;
+};
+
+void require_total(const orc_property& p, const char* what) {
+    if (p.component != Total) throw std::invalid_argument(std::string(what) + " is defined only for the total density");
+}
+
+void spin_inputs(Plan& plan, const orc_property& p, bool ComponentPlan::*input, const char* what) {
+    if (p.component == Difference) throw std::invalid_argument(std::string(what) + " is not defined for the spin-difference density");
+    if (p.component == Total) {
+        plan.c[Alpha].*input = true;
+        plan.c[Beta].*input = true;
+    } else {
+        plan.c[static_cast<std::size_t>(p.component)].*input = true;
+    }
+}
+
+Plan make_plan(const std::vector<orc_property>& props) {
+    Plan plan;
+    for (const auto& p : props) {
+        if (p.component < 0 || p.component > 3) throw std::invalid_argument("invalid density spin component");
+        if (is_orbital_property(p.kind)) {
+            if (p.orbital_index == 0) throw std::invalid_argument("orbital indices are one-based");
+            plan.orbitals = true;
+            plan.max_order = std::max(plan.max_order, 0);
+            continue;
+        }
+        auto& c = plan.c[static_cast<std::size_t>(p.component)];
+        switch (p.kind) {
+        case Density: c.density = true; break;
+        case Gradient: c.gradient = true; break;
+        case Hessian: c.hessian = true; break;
+        case Laplacian: c.laplacian = true; break;
+        case Tau: c.tau = true; break;
+        case Elf: c.elf = true; spin_inputs(plan, p, &ComponentPlan::elf_input, "ELF"); break;
+        case ReducedGradient: require_total(p, "the reduced density gradient"); c.reduced_gradient = true; break;
+        case SignLambda2Density: require_total(p, "sign(lambda2) rho"); c.sign_lambda2 = true; break;
+        case ElectrostaticPotential: require_total(p, "the electrostatic potential"); c.esp = true; break;
+        case Lol: c.lol = true; spin_inputs(plan, p, &ComponentPlan::lol_input, "LOL"); break;
+        case ElfTsirelsonStash: spin_inputs(plan, p, &ComponentPlan::ge_input, "the Tsirelson-Stash ELF"); break;
+        case LolTsirelsonStash: spin_inputs(plan, p, &ComponentPlan::ge_input, "the Tsirelson-Stash LOL"); break;
+        case InteractionRegionIndicator: require_total(p, "the interaction region indicator"); c.gradient_input = true; break;
+        case ShannonEntropyDensity: require_total(p, "the Shannon entropy density"); c.gradient_input = true; break;
+        case FisherInformationDensity: require_total(p, "the Fisher information density"); c.gradient_input = true; break;
+        case DensityOverlapRegionsIndicator: require_total(p, "the density overlap regions indicator"); c.hessian_input = true; break;
+        case HamiltonianKineticEnergy: c.energy_input = true; break;
+        case PotentialEnergyDensity: require_total(p, "the potential energy density"); c.energy_input = true; break;
+        case EnergyDensity: require_total(p, "the energy density"); c.energy_input = true; break;
+        case HamiltonianKineticEnergyComponents: c.energy_input = true; break;
+        case TauComponents: c.tau_input = true; break;
+        case PotentialKineticRatio: require_total(p, "|V| / G"); c.energy_input = true; break;
+        case EnergyDensityPerElectron: require_total(p, "the energy density per electron"); c.energy_input = true; break;
+        case KineticEnergyPerElectron: require_total(p, "the kinetic energy per electron"); c.tau_input = true; break;
+        case SpinPolarization:
+            if (p.component != Difference) throw std::invalid_argument("the spin polarization is defined only for the spin component");
+            plan.c[Total].value_input = true;
+            c.value_input = true;
+            break;
+        default:
+            if (is_orbital_energy(p.kind))
+                throw std::invalid_argument("orbital-energy fields are not part of this evaluator");
+            throw std::invalid_argument("unknown property kind");
+        }
+    }
+    for (const auto& c : plan.c)
+        plan.max_order = std::max(plan.max_order, c.ao_order());
+    return plan;
+}
+
+int choose_backend(int requested, bool eligible, const Plan& plan) {
+    if (requested == AO_P) return AO_P;
+    if (requested == AO_MO) {
+        if (!eligible) throw std::invalid_argument("AO_MO requested for ineligible orbital representation");
+        return AO_MO;
+    }
+    if (requested != AUTO) throw std::invalid_argument("unknown evaluation backend");
+    if (!eligible) return AO_P;
+    for (const auto& c : plan.c)
+        if (c.ao_order() >= 0) return AO_MO;
+    return AO_P;
+}
+
+// ---------------------------------------------------------------------------
+// Density kernels: src/density.cpp, src/ao_screening.hpp.
+
+struct DD {
+    double value = 0.0;
+    double gradient[3] = {0, 0, 0};
+    double hessian[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double laplacian = 0.0;
+    double tau = 0.0;
+    double tau_axes[3] = {0, 0, 0};
+};
+
+constexpr int hcol[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}; // src/derivative_layout.hpp:13
+
+double trace(const DD& d) { return d.hessian[0][0] + d.hessian[1][1] + d.hessian[2][2]; }
+
+// src/density.cpp:50-60.
+DD combine(const DD& a, const DD& b, double sign, double scale) {
+    DD d;
+    d.value = scale * (a.value + sign * b.value);
+    for (int i = 0; i < 3; ++i) {
+        d.gradient[i] = scale * (a.gradient[i] + sign * b.gradient[i]);
+        d.tau_axes[i] = scale * (a.tau_axes[i] + sign * b.tau_axes[i]);
+        for (int j = 0; j < 3; ++j)
+            d.hessian[i][j] = scale * (a.hessian[i][j] + sign * b.hessian[i][j]);
+    }
+    d.laplacian = trace(d);
+    d.tau = scale * (a.tau + sign * b.tau);
+    return d;
+}
+
+// significant_aos (src/ao_screening.hpp:35-50).
+std::vector<int> significant(const Panels& ao, double threshold) {
+    std::vector<int> rows;
+    if (threshold == 0.0 || ao.cols == 0) {
+        for (int r = 0; r < ao.rows; ++r)
+            rows.push_back(r);
+        return rows;
+    }
+    for (int r = 0; r < ao.rows; ++r) {
+        double mx = 0.0;
+        const double* row = ao.row(r);
+        for (int c = 0; c < ao.cols; ++c)
+            mx = std::max(mx, std::abs(row[c]));
+        if (mx > threshold) rows.push_back(r);
+    }
+    return rows;
+}
+
+// out(i, col) = sum_k A(kept[k], i) * X(kept[k] row, col), A row-major n x lda.
+// Blocked over 4 output rows; the inner loop runs along contiguous columns.
+void project(const double* A, int lda, int m_out, const std::vector<int>& kept, const Panels& X,
+             int col0, int ncols, std::vector<double>& out) {
+    out.assign(static_cast<std::size_t>(m_out) * static_cast<std::size_t>(ncols), 0.0);
+    int i = 0;
+    for (; i + 4 <= m_out; i += 4) {
+        double* o0 = out.data() + static_cast<std::size_t>(i) * ncols;
+        double* o1 = o0 + ncols;
+        double* o2 = o1 + ncols;
+        double* o3 = o2 + ncols;
+        for (int k : kept) {
+            const double* a = A + static_cast<std::size_t>(k) * lda + i;
+            const double c0 = a[0], c1 = a[1], c2 = a[2], c3 = a[3];
+            const double* x = X.row(k) + col0;
+            for (int c = 0; c < ncols; ++c) {
+                const double v = x[c];
+                o0[c] += c0 * v;
+                o1[c] += c1 * v;
+                o2[c] += c2 * v;
+                o3[c] += c3 * v;
+            }
+        }
+    }
+    for (; i < m_out; ++i) {
+        double* o = out.data() + static_cast<std::size_t>(i) * ncols;
+        for (int k : kept) {
+            const double c0 = A[static_cast<std::size_t>(k) * lda + i];
+            const double* x = X.row(k) + col0;
+            for (int c = 0; c < ncols; ++c)
+                o[c] += c0 * x[c];
+        }
+    }
+}
+
+using BlockDD = std::map<int, std::vector<DD>>;
+
+// density_derivatives_mo (src/density.cpp:234-351), one block of b points.
+BlockDD density_mo(const Molecule& M, const double* pts, int b, const std::vector<int>& comps, int ao_order,
+                   double threshold, Scratch& s) {
+    BlockDD result;
+    for (int c : comps)
+        result[c].assign(static_cast<std::size_t>(b), DD{});
+    if (result.empty() || b == 0) return result;
+    const bool need_a = result.count(Alpha) || result.count(Total) || result.count(Difference);
+    const bool need_b = result.count(Beta) || result.count(Total) || result.count(Difference);
+    const bool beta_is_alpha = need_a && need_b && !M.unrestricted && M.nalpha == M.nbeta;
+    const int panels = panel_count(ao_order);
+    Panels ao;
+    ao_panels(*M.basis, b, pts, ao_order, ao, s);
+    const auto kept = significant(ao, threshold);
+    // Psi[panel] = C_occ,kept^T * Phi_panel,kept, rows = occupied MOs.
+    const auto projectall = [&](int channel, int nocc, std::vector<std::vector<double>>& psi) {
+        psi.assign(static_cast<std::size_t>(panels), {});
+        if (nocc == 0) return;
+        const auto& C = M.C(channel);
+        const int nmo = M.nmo(channel);
+        for (int a = 0; a < panels; ++a)
+            project(C.data(), nmo, nocc, kept, ao, a * b, b, psi[static_cast<std::size_t>(a)]);
+    };
+    std::vector<std::vector<double>> va, vb;
+    if (need_a) projectall(0, M.nalpha, va);
+    if (need_b && !beta_is_alpha) projectall(1, M.nbeta, vb);
+    const auto contract = [&](const std::vector<std::vector<double>>& v, int nocc, int j) {
+        DD d;
+        if (nocc == 0) return d;
+        const auto dot = [&](int p, int q) {
+            double sum = 0.0;
+            for (int i = 0; i < nocc; ++i)
+                sum += v[static_cast<std::size_t>(p)][static_cast<std::size_t>(i * b + j)] *
+                       v[static_cast<std::size_t>(q)][static_cast<std::size_t>(i * b + j)];
+            return sum;
+        };
+        d.value = dot(0, 0);
+        if (ao_order == 0) return d;
+        for (int a = 0; a < 3; ++a) {
+            d.gradient[a] = 2.0 * dot(0, a + 1);
+            d.tau_axes[a] = 0.5 * dot(a + 1, a + 1);
+            d.tau += d.tau_axes[a];
+            if (ao_order == 2)
+                for (int c = a; c < 3; ++c) {
+                    const double v2 = 2.0 * (dot(0, 4 + hcol[a][c]) + dot(a + 1, c + 1));
+                    d.hessian[a][c] = d.hessian[c][a] = v2;
+                }
+        }
+        d.laplacian = trace(d);
+        return d;
+    };
+    for (int j = 0; j < b; ++j) {
+        const DD alpha = need_a ? contract(va, M.nalpha, j) : DD{};
+        DD beta;
+        if (beta_is_alpha) beta = alpha;
+        else if (need_b) beta = contract(vb, M.nbeta, j);
+        const auto put = [&](int c, const DD& d) {
+            auto it = result.find(c);
+            if (it != result.end()) it->second[static_cast<std::size_t>(j)] = d;
+        };
+        put(Alpha, alpha);
+        put(Beta, beta);
+        if (result.count(Total)) put(Total, combine(alpha, beta, 1.0, 1.0));
+        if (result.count(Difference)) put(Difference, combine(alpha, beta, -1.0, 1.0));
+    }
+    return result;
+}
+
+// contract_derivatives (src/density.cpp:26-44).
+void contract_p(const Panels& ao, const std::vector<int>& kept, const std::vector<double>& prod, int b, int j,
+                int ao_order, DD& out) {
+    // prod: m x 4b (row i = kept row i); ao rows indexed through kept.
+    const int m = static_cast<int>(kept.size());
+    const auto dot = [&](int panel, int ppanel) {
+        double s = 0.0;
+        for (int i = 0; i < m; ++i)
+            s += ao.at(kept[static_cast<std::size_t>(i)], panel * b + j) * prod[static_cast<std::size_t>(i) * static_cast<std::size_t>(4 * b) + static_cast<std::size_t>(ppanel * b + j)];
+        return s;
+    };
+    out.value = dot(0, 0);
+    for (int a = 0; a < 3; ++a) {
+        out.gradient[a] = 2.0 * dot(a + 1, 0);
+        out.tau_axes[a] = 0.5 * dot(a + 1, a + 1);
+        out.tau += out.tau_axes[a];
+        if (ao_order == 2)
+            for (int c = a; c < 3; ++c) {
+                const double v = 2.0 * (dot(4 + hcol[a][c], 0) + dot(a + 1, c + 1));
+                out.hessian[a][c] = out.hessian[c][a] = v;
+            }
+    }
+    out.laplacian = trace(out);
+}
+
+// W = P_kept * ao_kept[:, 0:4b] (row i of W = kept AO i).
+void p_product(const std::vector<double>& P, int n, const std::vector<int>& kept, const Panels& ao, int ncols,
+               std::vector<double>& out) {
+    const int m = static_cast<int>(kept.size());
+    out.assign(static_cast<std::size_t>(m) * static_cast<std::size_t>(ncols), 0.0);
+    for (int i = 0; i < m; ++i) {
+        double* o = out.data() + static_cast<std::size_t>(i) * ncols;
+        const double* prow = P.data() + static_cast<std::size_t>(kept[static_cast<std::size_t>(i)]) * n;
+        for (int k : kept) {
+            const double c = prow[k];
+            const double* x = ao.row(k);
+            for (int col = 0; col < ncols; ++col)
+                o[col] += c * x[col];
+        }
+    }
+}
+
+// density_derivatives_shared (src/density.cpp:149-229), one block.
+BlockDD density_p(const Molecule& M, const double* pts, int b, const std::vector<int>& comps, int ao_order,
+                  double threshold, Scratch& s) {
+    BlockDD result;
+    bool tn = false, sn = false;
+    for (int c : comps) {
+        if (c == Total) tn = true;
+        else if (c == Difference) sn = true;
+        else tn = sn = true;
+        result[c].assign(static_cast<std::size_t>(b), DD{});
+    }
+    if (result.empty() || b == 0) return result;
+    Panels ao;
+    ao_panels(*M.basis, b, pts, ao_order, ao, s);
+    const auto kept = significant(ao, threshold);
+    std::vector<double> pt, ps;
+    if (tn) p_product(M.P, M.n, kept, ao, 4 * b, pt);
+    if (sn) p_product(M.S, M.n, kept, ao, 4 * b, ps);
+    for (int j = 0; j < b; ++j) {
+        DD t, sp;
+        if (tn) contract_p(ao, kept, pt, b, j, ao_order, t);
+        if (sn) contract_p(ao, kept, ps, b, j, ao_order, sp);
+        const auto put = [&](int c, const DD& d) {
+            auto it = result.find(c);
+            if (it != result.end()) it->second[static_cast<std::size_t>(j)] = d;
+        };
+        put(Total, t);
+        put(Difference, sp);
+        if (result.count(Alpha)) put(Alpha, combine(t, sp, 1.0, 0.5));
+        if (result.count(Beta)) put(Beta, combine(t, sp, -1.0, 0.5));
+    }
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+// Derived fields: src/evaluator.cpp:127-353.
+
+const double kFermi = 0.3 * std::pow(6.0 * std::numbers::pi * std::numbers::pi, 2.0 / 3.0);
+
+double gnorm2(const DD& d) { return (d.gradient[0] * d.gradient[0] + d.gradient[1] * d.gradient[1]) + d.gradient[2] * d.gradient[2]; }
+
+double elf(std::initializer_list<const DD*> ch) {
+    double pauli = 0.0, uniform = 0.0;
+    for (const DD* d : ch) {
+        if (!(d->value > 0.0)) continue;
+        pauli += d->tau - gnorm2(*d) / (8.0 * d->value);
+        uniform += kFermi * std::pow(d->value, 5.0 / 3.0);
+    }
+    if (!(uniform > 0.0)) return 0.0;
+    const double chi = pauli / uniform;
+    return 1.0 / (1.0 + chi * chi);
+}
+
+double reduced_gradient(const DD& d) {
+    const double scale = 2.0 * std::cbrt(3.0 * std::numbers::pi * std::numbers::pi);
+    if (!(d.value > 0.0)) return 100.0;
+    const double den = scale * std::pow(d.value, 4.0 / 3.0);
+    const double norm = std::sqrt(gnorm2(d));
+    if (!(den > 0.0) || !(norm < 100.0 * den)) return 100.0;
+    return norm / den;
+}
+
+// Middle eigenvalue of the symmetric 3x3 Hessian by cyclic Jacobi rotations.
+double middle_eigenvalue(const double h[3][3]) {
+    double a[3][3];
+    std::memcpy(a, h, sizeof(a));
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        if (off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+            }
+    }
+    double e[3] = {a[0][0], a[1][1], a[2][2]};
+    std::sort(e, e + 3);
+    return e[1];
+}
+
+double sign_lambda2(const DD& d) { return middle_eigenvalue(d.hessian) < 0.0 ? -d.value : d.value; }
+
+double lol(std::initializer_list<const DD*> ch) {
+    double tau = 0.0, uniform = 0.0;
+    for (const DD* d : ch) {
+        tau += d->tau;
+        if (d->value > 0.0) uniform += kFermi * std::pow(d->value, 5.0 / 3.0);
+    }
+    if (!(uniform > 0.0)) return 0.0;
+    if (!(tau > 0.0)) return 1.0;
+    return uniform / (tau + uniform);
+}
+
+double ge_locator(std::initializer_list<const DD*> ch, bool is_lol) {
+    double uniform = 0.0, pauli = 0.0, kinetic = 0.0;
+    for (const DD* d : ch) {
+        if (!(d->value > 0.0)) continue;
+        const double ref = kFermi * std::pow(d->value, 5.0 / 3.0);
+        const double wz = gnorm2(*d) / d->value;
+        uniform += ref;
+        pauli += ref - wz / 9.0 + d->laplacian / 6.0;
+        kinetic += ref + wz / 72.0 + d->laplacian / 6.0;
+    }
+    if (!(uniform > 0.0)) return 0.0;
+    if (is_lol) {
+        const double den = kinetic + uniform;
+        return den != 0.0 ? uniform / den : 0.0;
+    }
+    const double chi = pauli / uniform;
+    return 1.0 / (1.0 + chi * chi);
+}
+
+double iri(const DD& d) {
+    if (!(d.value > 0.0)) return 100.0;
+    const double v = std::sqrt(gnorm2(d)) / std::pow(d.value, 1.1);
+    return v < 100.0 ? v : 100.0;
+}
+
+double dori(const DD& d) {
+    if (!(d.value > 0.0)) return 0.0;
+    const double q[3] = {d.gradient[0] / d.value, d.gradient[1] / d.value, d.gradient[2] / d.value};
+    const double q2 = (q[0] * q[0] + q[1] * q[1]) + q[2] * q[2];
+    if (!(q2 > 0.0)) return 1.0;
+    double w[3];
+    for (int a = 0; a < 3; ++a)
+        w[a] = (d.hessian[a][0] * q[0] + d.hessian[a][1] * q[1] + d.hessian[a][2] * q[2]) / d.value - q2 * q[a];
+    const double theta = 4.0 * ((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]) / (q2 * q2 * q2);
+    if (!std::isfinite(theta)) return 1.0;
+    return theta / (1.0 + theta);
+}
+
+// store_block (src/evaluator.cpp:248-353).
+void store_block(int kind, int component, const BlockDD& block, int count, std::int64_t start, double** out) {
+    if ((kind == Elf || kind == Lol || kind == ElfTsirelsonStash || kind == LolTsirelsonStash) && component == Total) {
+        const auto& a = block.at(Alpha);
+        const auto& b = block.at(Beta);
+        for (int l = 0; l < count; ++l) {
+            const DD* pa = &a[static_cast<std::size_t>(l)];
+            const DD* pb = &b[static_cast<std::size_t>(l)];
+            double v;
+            switch (kind) {
+            case Elf: v = elf({pa, pb}); break;
+            case Lol: v = lol({pa, pb}); break;
+            case ElfTsirelsonStash: v = ge_locator({pa, pb}, false); break;
+            default: v = ge_locator({pa, pb}, true); break;
+            }
+            out[0][start + l] = v;
+        }
+        return;
+    }
+    if (kind == SpinPolarization) {
+        const auto& t = block.at(Total);
+        const auto& s = block.at(Difference);
+        for (int l = 0; l < count; ++l)
+            out[0][start + l] = t[static_cast<std::size_t>(l)].value > 0.0 ? s[static_cast<std::size_t>(l)].value / t[static_cast<std::size_t>(l)].value : 0.0;
+        return;
+    }
+    const auto& dv = block.at(component);
+    for (int l = 0; l < count; ++l) {
+        const DD& d = dv[static_cast<std::size_t>(l)];
+        const std::int64_t p = start + l;
+        switch (kind) {
+        case Density: out[0][p] = d.value; break;
+        case Gradient: for (int a = 0; a < 3; ++a) out[a][p] = d.gradient[a]; break;
+        case Hessian:
+            out[0][p] = d.hessian[0][0]; out[1][p] = d.hessian[0][1]; out[2][p] = d.hessian[0][2];
+            out[3][p] = d.hessian[1][1]; out[4][p] = d.hessian[1][2]; out[5][p] = d.hessian[2][2];
+            break;
+        case Laplacian: out[0][p] = d.laplacian; break;
+        case Tau: out[0][p] = d.tau; break;
+        case Elf: out[0][p] = elf({&d}); break;
+        case ReducedGradient: out[0][p] = reduced_gradient(d); break;
+        case SignLambda2Density: out[0][p] = sign_lambda2(d); break;
+        case Lol: out[0][p] = lol({&d}); break;
+        case ElfTsirelsonStash: out[0][p] = ge_locator({&d}, false); break;
+        case LolTsirelsonStash: out[0][p] = ge_locator({&d}, true); break;
+        case InteractionRegionIndicator: out[0][p] = iri(d); break;
+        case DensityOverlapRegionsIndicator: out[0][p] = dori(d); break;
+        case HamiltonianKineticEnergy: out[0][p] = d.tau - 0.25 * d.laplacian; break;
+        case PotentialEnergyDensity: out[0][p] = 0.25 * d.laplacian - 2.0 * d.tau; break;
+        case EnergyDensity: out[0][p] = 0.25 * d.laplacian - d.tau; break;
+        case TauComponents: for (int a = 0; a < 3; ++a) out[a][p] = d.tau_axes[a]; break;
+        case HamiltonianKineticEnergyComponents:
+            for (int a = 0; a < 3; ++a) out[a][p] = d.tau_axes[a] - 0.25 * d.hessian[a][a];
+            break;
+        case PotentialKineticRatio: out[0][p] = d.tau > 0.0 ? std::abs(0.25 * d.laplacian - 2.0 * d.tau) / d.tau : 0.0; break;
+        case EnergyDensityPerElectron: out[0][p] = d.value > 0.0 ? (0.25 * d.laplacian - d.tau) / d.value : 0.0; break;
+        case KineticEnergyPerElectron: out[0][p] = d.value > 0.0 ? d.tau / d.value : 0.0; break;
+        case ShannonEntropyDensity: out[0][p] = d.value > 0.0 ? -d.value * std::log(d.value) : 0.0; break;
+        case FisherInformationDensity: out[0][p] = d.value > 0.0 ? gnorm2(d) / d.value : 0.0; break;
+        default: throw std::logic_error("invalid derivative property");
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boys function: src/boys_function.cpp (table + 7-term Taylor below T = 40,
+// closed form and upward recursion above; the same published scheme and
+// arithmetic order, so that it matches the reference bit for bit).
+
+constexpr int kBoysMax = 32, kTaylor = 7, kOrders = kBoysMax + kTaylor, kPoints = 801;
+constexpr double kSpacing = 0.05, kEnd = 40.0;
+
+const std::vector<double>& boys_table() {
+    static const std::vector<double> table = [] {
+        std::vector<double> t(static_cast<std::size_t>(kPoints * kOrders));
+        constexpr int top = kOrders - 1;
+        for (int i = 0; i < kPoints; ++i) {
+            const double T = i * kSpacing;
+            const double decay = std::exp(-T);
+            double term = 1.0 / (2.0 * top + 1.0), sum = term;
+            for (int k = 1; k < 10000 && term > 1e-18 * sum; ++k) {
+                term *= 2.0 * T / (2.0 * top + 2.0 * k + 1.0);
+                sum += term;
+            }
+            double* row = t.data() + static_cast<std::size_t>(i * kOrders);
+            row[top] = decay * sum;
+            for (int m = top; m > 0; --m)
+                row[m - 1] = (2.0 * T * row[m] + decay) / (2.0 * m - 1.0);
+        }
+        return t;
+    }();
+    return table;
+}
+
+void boys(double T, int top, double* values) {
+    const double decay = std::exp(-T);
+    if (T >= kEnd) {
+        values[0] = 0.5 * std::sqrt(std::numbers::pi / T);
+        for (int m = 0; m < top; ++m)
+            values[m + 1] = ((2.0 * m + 1.0) * values[m] - decay) / (2.0 * T);
+        return;
+    }
+    const int point = static_cast<int>(std::lround(T / kSpacing));
+    const double delta = T - point * kSpacing;
+    const double* row = boys_table().data() + static_cast<std::size_t>(point * kOrders);
+    double sum = 0.0, factor = 1.0;
+    for (int k = 0; k < kTaylor; ++k) {
+        sum += row[top + k] * factor;
+        factor *= -delta / (k + 1.0);
+    }
+    values[top] = sum;
+    for (int m = top; m > 0; --m)
+        values[m - 1] = (2.0 * T * values[m] + decay) / (2.0 * m - 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// Exact electrostatic potential: src/electrostatic_potential.cpp:17-361.
+
+struct Triple {
+    int t, u, v;
+};
+
+std::vector<Triple> triples(int L) {
+    std::vector<Triple> out;
+    for (int s = 0; s <= L; ++s)
+        for (int t = s; t >= 0; --t)
+            for (int u = s - t; u >= 0; --u)
+                out.push_back({t, u, s - t - u});
+    return out;
+}
+
+// E^{ij}_t Hermite expansion table (src/electrostatic_potential.cpp:57-93).
+struct Hermite {
+    int lb, width;
+    std::vector<double> v;
+    Hermite(int la, int lb_, double p, double pa, double pb) : lb(lb_), width(la + lb_ + 1) {
+        v.assign(static_cast<std::size_t>((la + 1) * (lb + 1) * width), 0.0);
+        const double half = 0.5 / p;
+        at(0, 0, 0) = 1.0;
+        const auto step = [&](int si, int sj, int ti, int tj, double shift) {
+            for (int t = 0; t <= ti + tj; ++t) {
+                double value = shift * get(si, sj, t);
+                if (t > 0) value += half * get(si, sj, t - 1);
+                if (t + 1 <= si + sj) value += (t + 1) * get(si, sj, t + 1);
+                at(ti, tj, t) = value;
+            }
+        };
+        for (int i = 0; i < la; ++i)
+            step(i, 0, i + 1, 0, pa);
+        for (int i = 0; i <= la; ++i)
+            for (int j = 0; j < lb; ++j)
+                step(i, j, i, j + 1, pb);
+    }
+    double get(int i, int j, int t) const {
+        if (t < 0 || t > i + j) return 0.0;
+        return v[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)];
+    }
+    double& at(int i, int j, int t) { return v[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)]; }
+};
+
+struct EspPair {
+    double p;
+    double Q[3];
+    int L;
+    std::size_t first;
+};
+
+struct Esp {
+    std::vector<EspPair> pairs;
+    std::vector<double> coef;
+    int max_L = 0;
+    std::vector<Atom> atoms;
+};
+
+std::unique_ptr<Esp> make_esp(const Molecule& M) {
+    auto esp = std::make_unique<Esp>();
+    esp->atoms = M.atoms;
+    const auto& blocks = M.basis->blocks;
+    for (const auto& b : blocks) {
+        if (b.l > 13) throw std::invalid_argument("the electrostatic potential supports shells up to l = 13");
+        esp->max_L = std::max(esp->max_L, 2 * b.l);
+    }
+    const int n = M.n;
+    for (std::size_t ia = 0; ia < blocks.size(); ++ia) {
+        const auto& A = blocks[ia];
+        for (std::size_t ib = ia; ib < blocks.size(); ++ib) {
+            const auto& B = blocks[ib];
+            const int na = A.ncart(), nb = B.ncart();
+            // Cartesian density block: TA^T * P(A, B) * TB (x2 off the diagonal).
+            std::vector<double> pblk(static_cast<std::size_t>(A.count * B.count));
+            for (int r = 0; r < A.count; ++r)
+                for (int c = 0; c < B.count; ++c)
+                    pblk[static_cast<std::size_t>(r * B.count + c)] = M.P[static_cast<std::size_t>(A.offset + r) * n + static_cast<std::size_t>(B.offset + c)];
+            std::vector<double> left(static_cast<std::size_t>(na * B.count));
+            for (int i = 0; i < na; ++i)
+                for (int c = 0; c < B.count; ++c) {
+                    if (A.pure.empty()) {
+                        left[static_cast<std::size_t>(i * B.count + c)] = pblk[static_cast<std::size_t>(i * B.count + c)];
+                    } else {
+                        double acc = 0.0;
+                        for (int r = 0; r < A.count; ++r)
+                            acc += A.pure[static_cast<std::size_t>(r * na + i)] * pblk[static_cast<std::size_t>(r * B.count + c)];
+                        left[static_cast<std::size_t>(i * B.count + c)] = acc;
+                    }
+                }
+            std::vector<double> cart(static_cast<std::size_t>(na * nb));
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j) {
+                    double v;
+                    if (B.pure.empty()) {
+                        v = left[static_cast<std::size_t>(i * B.count + j)];
+                    } else {
+                        v = 0.0;
+                        for (int c = 0; c < B.count; ++c)
+                            v += left[static_cast<std::size_t>(i * B.count + c)] * B.pure[static_cast<std::size_t>(c * nb + j)];
+                    }
+                    if (ia != ib) v *= 2.0;
+                    cart[static_cast<std::size_t>(i * nb + j)] = v;
+                }
+            if (std::all_of(cart.begin(), cart.end(), [](double v) { return v == 0.0; })) continue;
+            const int la = A.l, lb = B.l, L = la + lb;
+            const auto list = triples(L);
+            const double AB[3] = {A.origin[0] - B.origin[0], A.origin[1] - B.origin[1], A.origin[2] - B.origin[2]};
+            const double ab2 = sqnorm(AB[0], AB[1], AB[2]);
+            std::vector<double> pc(list.size());
+            for (int p = 0; p < A.nprim(); ++p)
+                for (int q = 0; q < B.nprim(); ++q) {
+                    const double al = A.exps[static_cast<std::size_t>(p)], be = B.exps[static_cast<std::size_t>(q)];
+                    const double ex = al + be;
+                    double Q[3];
+                    for (int a = 0; a < 3; ++a)
+                        Q[a] = (al * A.origin[a] + be * B.origin[a]) / ex;
+                    const double scale = std::exp(-al * be / ex * ab2) * 2.0 * std::numbers::pi / ex;
+                    const Hermite hx(la, lb, ex, Q[0] - A.origin[0], Q[0] - B.origin[0]);
+                    const Hermite hy(la, lb, ex, Q[1] - A.origin[1], Q[1] - B.origin[1]);
+                    const Hermite hz(la, lb, ex, Q[2] - A.origin[2], Q[2] - B.origin[2]);
+                    std::fill(pc.begin(), pc.end(), 0.0);
+                    for (int i = 0; i < na; ++i) {
+                        const auto& pi = A.powers[static_cast<std::size_t>(i)];
+                        for (int j = 0; j < nb; ++j) {
+                            const auto& pj = B.powers[static_cast<std::size_t>(j)];
+                            const double wgt = cart[static_cast<std::size_t>(i * nb + j)] * A.w(i, p) * B.w(j, q);
+                            if (wgt == 0.0) continue;
+                            for (std::size_t k = 0; k < list.size(); ++k) {
+                                const auto& h = list[k];
+                                if (h.t > pi.x + pj.x || h.u > pi.y + pj.y || h.v > pi.z + pj.z) continue;
+                                pc[k] += wgt * hx.get(pi.x, pj.x, h.t) * hy.get(pi.y, pj.y, h.u) * hz.get(pi.z, pj.z, h.v);
+                            }
+                        }
+                    }
+                    if (std::all_of(pc.begin(), pc.end(), [](double c) { return c == 0.0; })) continue;
+                    esp->pairs.push_back({ex, {Q[0], Q[1], Q[2]}, L, esp->coef.size()});
+                    for (double c : pc)
+                        esp->coef.push_back(scale * c);
+                }
+        }
+    }
+    return esp;
+}
+
+double esp_nuclear(const Esp& e, const double* r) {
+    double sum = 0.0;
+    for (const auto& a : e.atoms) {
+        const double d = std::sqrt(sqnorm(a.pos[0] - r[0], a.pos[1] - r[1], a.pos[2] - r[2]));
+        if (a.charge == 0.0 || d < 1e-8) continue;
+        sum += a.charge / d;
+    }
+    return sum;
+}
+
+// hermite_integrals + hermite_recursion + pair_potential (src/electrostatic_potential.cpp:269-338).
+double esp_electronic(const Esp& e, const double* r) {
+    const int side = e.max_L + 1;
+    const std::size_t cube = static_cast<std::size_t>(side * side * side);
+    std::vector<double> prev(cube, 0.0), cur(cube, 0.0), seeds(static_cast<std::size_t>(side));
+    const auto all = triples(e.max_L);
+    const auto off = [&](int t, int u, int v) { return static_cast<std::size_t>((t * side + u) * side + v); };
+    double total = 0.0;
+    for (const auto& pr : e.pairs) {
+        const double pc[3] = {pr.Q[0] - r[0], pr.Q[1] - r[1], pr.Q[2] - r[2]};
+        const int L = pr.L;
+        boys(pr.p * sqnorm(pc[0], pc[1], pc[2]), L, seeds.data());
+        double power = 1.0;
+        for (int n = 1; n <= L; ++n) {
+            power *= -2.0 * pr.p;
+            seeds[static_cast<std::size_t>(n)] *= power;
+        }
+        for (int n = L; n >= 0; --n) {
+            std::swap(prev, cur);
+            cur[0] = seeds[static_cast<std::size_t>(n)];
+            const std::size_t entries = static_cast<std::size_t>((L - n + 1) * (L - n + 2) * (L - n + 3) / 6);
+            for (std::size_t k = 1; k < entries; ++k) {
+                const auto& h = all[k];
+                double value;
+                if (h.t > 0) {
+                    value = pc[0] * prev[off(h.t - 1, h.u, h.v)];
+                    if (h.t > 1) value += (h.t - 1) * prev[off(h.t - 2, h.u, h.v)];
+                } else if (h.u > 0) {
+                    value = pc[1] * prev[off(h.t, h.u - 1, h.v)];
+                    if (h.u > 1) value += (h.u - 1) * prev[off(h.t, h.u - 2, h.v)];
+                } else {
+                    value = pc[2] * prev[off(h.t, h.u, h.v - 1)];
+                    if (h.v > 1) value += (h.v - 1) * prev[off(h.t, h.u, h.v - 2)];
+                }
+                cur[off(h.t, h.u, h.v)] = value;
+            }
+        }
+        const double* c = e.coef.data() + pr.first;
+        const std::size_t nk = static_cast<std::size_t>((L + 1) * (L + 2) * (L + 3) / 6);
+        double sum = 0.0;
+        for (std::size_t k = 0; k < nk; ++k)
+            sum += c[k] * cur[off(all[k].t, all[k].u, all[k].v)];
+        total += sum;
+    }
+    return -total;
+}
+
+// ---------------------------------------------------------------------------
+// Points and the evaluation driver: src/grid_geometry.hpp, src/evaluator.cpp.
+
+struct PointSet {
+    const double* origin = nullptr;
+    const double* steps = nullptr; // column-major 3x3
+    const int* counts = nullptr;
+    const double* pts = nullptr;
+    std::int64_t n = 0;
+    bool grid() const { return pts == nullptr; }
+    // grid_point (src/grid_geometry.hpp:59-68): origin + steps * (i, j, k).
+    void point(std::int64_t q, double* out) const {
+        if (!grid()) {
+            out[0] = pts[3 * q];
+            out[1] = pts[3 * q + 1];
+            out[2] = pts[3 * q + 2];
+            return;
+        }
+        const std::int64_t ny = counts[1], nz = counts[2];
+        const double f[3] = {static_cast<double>(q / (ny * nz)), static_cast<double>((q / nz) % ny),
+                             static_cast<double>(q % nz)};
+        for (int a = 0; a < 3; ++a)
+            out[a] = origin[a] + ((steps[a] * f[0] + steps[3 + a] * f[1]) + steps[6 + a] * f[2]);
+    }
+    void block(std::int64_t start, int size, std::vector<double>& out) const {
+        out.resize(static_cast<std::size_t>(3 * size));
+        for (int j = 0; j < size; ++j)
+            point(start + j, out.data() + 3 * j);
+    }
+};
+
+bool uses_point_path(const ComponentPlan& c, int backend, bool grid) {
+    return c.derivatives() || ((backend == AO_MO || !grid) && c.density);
+}
+bool on_grid_path(const orc_property& p, int backend, bool grid) {
+    return grid && (p.kind == Orbital || (p.kind == Density && backend == AO_P));
+}
+bool from_point_path(const orc_property& p, int backend, bool grid) {
+    return !on_grid_path(p, backend, grid) && !is_orbital_property(p.kind) && p.kind != ElectrostaticPotential &&
+           !is_orbital_energy(p.kind);
+}
+
+template <typename Body> void for_blocks(std::int64_t n, int bs, const Body& body) {
+    const std::int64_t nb = n / bs + (n % bs != 0);
+    std::exception_ptr err;
+#pragma omp parallel for schedule(dynamic) if (nb > 1)
+    for (std::int64_t b = 0; b < nb; ++b) {
+        try {
+            const std::int64_t start = b * bs;
+            body(start, static_cast<int>(std::min<std::int64_t>(bs, n - start)));
+        } catch (...) {
+#pragma omp critical(orc_err)
+            if (!err) err = std::current_exception();
+        }
+    }
+    if (err) std::rethrow_exception(err);
+}
+
+struct Field {
+    orc_property key;
+    double** out;
+};
+
+void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_property>& props, const orc_options& opt,
+              double** outputs, int* backend_used) {
+    if (opt.block_size < 1 || opt.block_size > 65536) throw std::invalid_argument("evaluation block_size must be in [1, 65536]");
+    if (!std::isfinite(opt.screening_threshold) || opt.screening_threshold < 0.0)
+        throw std::invalid_argument("screening threshold must be finite and nonnegative");
+    if (opt.backend == AO_MO && !M.eligible)
+        throw std::invalid_argument("AO_MO backend requested but orbital representation is not eligible");
+    const Plan plan = make_plan(props);
+    const int backend = choose_backend(opt.backend, M.eligible, plan);
+    *backend_used = backend;
+    const bool grid = set.grid();
+    const std::int64_t n = set.n;
+    // Requested fields (duplicates evaluated once; they share outputs).
+    std::vector<Field> fields;
+    for (std::size_t i = 0; i < props.size(); ++i)
+        fields.push_back({props[i], outputs + 0});
+    {
+        // channel offsets into the flat outputs array
+        std::size_t off = 0;
+        for (std::size_t i = 0; i < props.size(); ++i) {
+            fields[i].out = outputs + off;
+            off += static_cast<std::size_t>(channels(props[i].kind));
+        }
+    }
+    // Orbital checks (src/evaluator.cpp:618-632).
+    for (const auto& f : fields)
+        if (is_orbital_property(f.key.kind)) {
+            if (M.nmo(f.key.channel) == 0) throw std::runtime_error("FCHK has no MO coefficients");
+            if (f.key.orbital_index > M.nmo(f.key.channel)) throw std::out_of_range("MO index outside available coefficient range");
+        }
+    const int bs = opt.block_size;
+    const double thr = opt.screening_threshold;
+    // Point pass (src/evaluator.cpp:634-657).
+    std::vector<int> needed;
+    for (int c = 0; c < 4; ++c)
+        if (uses_point_path(plan.c[static_cast<std::size_t>(c)], backend, grid)) needed.push_back(c);
+    if (!needed.empty()) {
+        const int ao_order = grid ? plan.max_order : std::max(1, plan.max_order);
+        for_blocks(n, bs, [&](std::int64_t start, int size) {
+            Scratch s;
+            std::vector<double> pts;
+            set.block(start, size, pts);
+            const BlockDD block = backend == AO_MO ? density_mo(M, pts.data(), size, needed, std::max(0, ao_order), thr, s)
+                                                   : density_p(M, pts.data(), size, needed, ao_order, thr, s);
+            for (const auto& f : fields)
+                if (from_point_path(f.key, backend, grid)) store_block(f.key.kind, f.key.component, block, size, start, f.out);
+        });
+    }
+    // Electrostatic potential (exact at every tolerance).
+    for (const auto& f : fields)
+        if (f.key.kind == ElectrostaticPotential) {
+            const auto esp = make_esp(M);
+            for_blocks(n, 64, [&](std::int64_t start, int size) {
+                double r[3];
+                for (int j = 0; j < size; ++j) {
+                    set.point(start + j, r);
+                    f.out[0][start + j] = esp_nuclear(*esp, r) + esp_electronic(*esp, r);
+                }
+            });
+        }
+    // Orbital pass (src/evaluator.cpp:688-730): derivatives on grids, all at points.
+    std::vector<Field> orbital_fields;
+    int dorder = 0;
+    for (const auto& f : fields)
+        if (is_orbital_property(f.key.kind) && !on_grid_path(f.key, backend, grid)) {
+            orbital_fields.push_back(f);
+            if (f.key.kind == OrbitalGradient) dorder = std::max(dorder, 1);
+            if (f.key.kind == OrbitalHessian) dorder = 2;
+        }
+    if (!orbital_fields.empty()) {
+        for_blocks(n, bs, [&](std::int64_t start, int size) {
+            Scratch s;
+            std::vector<double> pts;
+            set.block(start, size, pts);
+            Panels phi, pan;
+            ao_panels(*M.basis, size, pts.data(), 0, phi, s);
+            if (dorder > 0) ao_panels(*M.basis, size, pts.data(), dorder, pan, s);
+            for (const auto& f : orbital_fields) {
+                const auto& C = M.C(f.key.channel);
+                const int nmo = M.nmo(f.key.channel), mo = f.key.orbital_index - 1;
+                const auto store = [&](const Panels& X, int panel, int comp) {
+                    for (int t = 0; t < size; ++t) {
+                        double v = 0.0;
+                        for (int a = 0; a < M.n; ++a)
+                            v += X.at(a, panel * size + t) * C[static_cast<std::size_t>(a * nmo + mo)];
+                        f.out[comp][start + t] = v;
+                    }
+                };
+                if (f.key.kind == Orbital) store(phi, 0, 0);
+                else if (f.key.kind == OrbitalGradient) for (int a = 0; a < 3; ++a) store(pan, 1 + a, a);
+                else for (int h = 0; h < 6; ++h) store(pan, 4 + h, h);
+            }
+        });
+    }
+    // Grid path (src/evaluator.cpp:754-777): AO-P densities, one pass each.
+    for (const auto& f : fields) {
+        if (!(on_grid_path(f.key, backend, grid) && f.key.kind == Density)) continue;
+        const int comp = f.key.component;
+        // densities_blocked with one component (src/density.cpp:365-444).
+        std::vector<double> Pc;
+        const std::vector<double>* Pm = &M.P;
+        if (comp == Difference) Pm = &M.S;
+        else if (comp == Alpha || comp == Beta) {
+            Pc.resize(M.P.size());
+            for (std::size_t q = 0; q < Pc.size(); ++q)
+                Pc[q] = 0.5 * (comp == Alpha ? (M.P[q] + M.S[q]) : (M.P[q] - M.S[q]));
+            Pm = &Pc;
+        }
+        for_blocks(n, bs, [&](std::int64_t start, int size) {
+            Scratch s;
+            std::vector<double> pts;
+            set.block(start, size, pts);
+            Panels phi;
+            ao_panels(*M.basis, size, pts.data(), 0, phi, s);
+            const auto kept = significant(phi, thr);
+            std::vector<double> y;
+            p_product(*Pm, M.n, kept, phi, size, y);
+            for (int t = 0; t < size; ++t) {
+                double r = 0.0;
+                for (std::size_t i = 0; i < kept.size(); ++i)
+                    r += phi.at(kept[i], t) * y[i * static_cast<std::size_t>(size) + static_cast<std::size_t>(t)];
+                f.out[0][start + t] = r;
+            }
+        });
+    }
+    // Grid path: orbital amplitudes (src/orbitals.cpp:66-102), blocks of 128, no screening.
+    for (const auto& f : fields) {
+        if (!(on_grid_path(f.key, backend, grid) && f.key.kind == Orbital)) continue;
+        const auto& C = M.C(f.key.channel);
+        const int nmo = M.nmo(f.key.channel), mo = f.key.orbital_index - 1;
+        for_blocks(n, 128, [&](std::int64_t start, int size) {
+            Scratch s;
+            std::vector<double> pts;
+            set.block(start, size, pts);
+            Panels phi;
+            ao_panels(*M.basis, size, pts.data(), 0, phi, s);
+            for (int t = 0; t < size; ++t) {
+                double v = 0.0;
+                for (int a = 0; a < M.n; ++a)
+                    v += C[static_cast<std::size_t>(a * nmo + mo)] * phi.at(a, t);
+                f.out[0][start + t] = v;
+            }
+        });
+    }
+}
+
+void set_error(char* buf, int len, const char* msg) {
+    if (buf && len > 0) {
+        std::strncpy(buf, msg, static_cast<std::size_t>(len - 1));
+        buf[len - 1] = '\0';
+    }
+}
+
+} // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+void* orc_basis_create(int nshell, const int* shell_types, const int* nprim, const int* shell_atom,
+                       const double* shell_xyz, const double* exponents, const double* coefficients,
+                       const double* p_coefficients, char* error, int error_len) {
+    try {
+        return make_basis(nshell, shell_types, nprim, shell_atom, shell_xyz, exponents, coefficients, p_coefficients)
+            .release();
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return nullptr;
+    }
+}
+
+void orc_basis_destroy(void* b) { delete static_cast<Basis*>(b); }
+int orc_basis_nao(const void* b) { return static_cast<const Basis*>(b)->nao; }
+int orc_basis_nblocks(const void* b) { return static_cast<int>(static_cast<const Basis*>(b)->blocks.size()); }
+
+void orc_basis_blocks(const void* b, int* l, int* offset, int* count, double* cutoff_r2) {
+    const auto* B = static_cast<const Basis*>(b);
+    for (std::size_t i = 0; i < B->blocks.size(); ++i) {
+        l[i] = B->blocks[i].l;
+        offset[i] = B->blocks[i].offset;
+        count[i] = B->blocks[i].count;
+        cutoff_r2[i] = B->blocks[i].cutoff_r2;
+    }
+}
+
+int orc_ao_eval(const void* b, int n, const double* points, int order, double* out) {
+    if (order < 0 || order > 2) return 1;
+    const auto* B = static_cast<const Basis*>(b);
+    Panels p;
+    Scratch s;
+    ao_panels(*B, n, points, order, p, s);
+    // column-major output: (row, col) at row + nao * col
+    for (int r = 0; r < p.rows; ++r)
+        for (int c = 0; c < p.cols; ++c)
+            out[static_cast<std::size_t>(r) + static_cast<std::size_t>(p.rows) * static_cast<std::size_t>(c)] = p.at(r, c);
+    return 0;
+}
+
+void orc_overlap(const void* b, double* out) {
+    const auto S = overlap(*static_cast<const Basis*>(b));
+    std::copy(S.begin(), S.end(), out);
+}
+
+int orc_pure_transform(int l, double* out) {
+    if (l < 2) return 1;
+    const auto t = pure_transform(l);
+    std::copy(t.begin(), t.end(), out);
+    return 0;
+}
+
+void* orc_molecule_create(void* basis, int natoms, const int* atomic_numbers, const double* charges,
+                          const double* xyz, const double* p_total, const double* p_spin, int nmo_alpha,
+                          const double* c_alpha, int nmo_beta, const double* c_beta, int nalpha, int nbeta,
+                          char* error, int error_len) {
+    try {
+        auto m = std::make_unique<Molecule>();
+        m->basis = static_cast<const Basis*>(basis);
+        m->n = m->basis->nao;
+        const std::size_t n = static_cast<std::size_t>(m->n);
+        for (int a = 0; a < natoms; ++a)
+            m->atoms.push_back({atomic_numbers[a], charges[a], {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]}});
+        m->P.assign(p_total, p_total + n * n);
+        if (p_spin) m->S.assign(p_spin, p_spin + n * n);
+        else m->S.assign(n * n, 0.0);
+        // MO-major (AO fastest) -> AO rows x MO columns.
+        const auto load = [&](const double* src, int nmo, std::vector<double>& dst) {
+            dst.assign(n * static_cast<std::size_t>(nmo), 0.0);
+            for (int mo = 0; mo < nmo; ++mo)
+                for (std::size_t ao = 0; ao < n; ++ao)
+                    dst[ao * static_cast<std::size_t>(nmo) + static_cast<std::size_t>(mo)] = src[static_cast<std::size_t>(mo) * n + ao];
+        };
+        if (c_alpha && nmo_alpha > 0) {
+            load(c_alpha, nmo_alpha, m->Ca);
+            m->nmo_a = nmo_alpha;
+        }
+        if (c_beta && nmo_beta > 0) {
+            load(c_beta, nmo_beta, m->Cb);
+            m->nmo_b = nmo_beta;
+        }
+        m->unrestricted = m->nmo_b != 0;
+        m->nalpha = nalpha;
+        m->nbeta = nbeta;
+        if (nalpha != nbeta && !m->unrestricted) throw std::runtime_error("open-shell FCHK lacks Beta MO coefficients");
+        m->eligible = check_eligible(*m);
+        return m.release();
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return nullptr;
+    }
+}
+
+void orc_molecule_destroy(void* m) { delete static_cast<Molecule*>(m); }
+int orc_molecule_eligible(const void* m) { return static_cast<const Molecule*>(m)->eligible ? 1 : 0; }
+
+static PointSet make_set(const double* origin, const double* steps, const int* counts, std::int64_t npoints,
+                         const double* points) {
+    PointSet set;
+    if (points) {
+        set.pts = points;
+        set.n = npoints;
+    } else {
+        set.origin = origin;
+        set.steps = steps;
+        set.counts = counts;
+        set.n = static_cast<std::int64_t>(counts[0]) * counts[1] * counts[2];
+    }
+    return set;
+}
+
+int orc_evaluate(const void* molecule, const double* origin, const double* steps, const int* counts,
+                 std::int64_t npoints, const double* points, const orc_property* props, int nprops,
+                 const orc_options* options, double** outputs, int* backend_used, char* error, int error_len) {
+    try {
+        const auto* M = static_cast<const Molecule*>(molecule);
+        const PointSet set = make_set(origin, steps, counts, npoints, points);
+        std::vector<orc_property> p(props, props + nprops);
+        evaluate(*M, set, p, *options, outputs, backend_used);
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        set_error(error, error_len, e.what());
+        return 1;
+    } catch (const std::out_of_range& e) {
+        set_error(error, error_len, e.what());
+        return 2;
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return 3;
+    }
+}
+
+int orc_masks(const void* molecule, const double* origin, const double* steps, const int* counts,
+              std::int64_t npoints, const double* points, int block_size, int ao_order, double threshold,
+              std::uint8_t* kept) {
+    const auto* M = static_cast<const Molecule*>(molecule);
+    const PointSet set = make_set(origin, steps, counts, npoints, points);
+    const int nao = M->n;
+    for_blocks(set.n, block_size, [&](std::int64_t start, int size) {
+        Scratch s;
+        std::vector<double> pts;
+        set.block(start, size, pts);
+        Panels ao;
+        ao_panels(*M->basis, size, pts.data(), ao_order, ao, s);
+        const auto rows = significant(ao, threshold);
+        std::uint8_t* dst = kept + static_cast<std::size_t>(start / block_size) * static_cast<std::size_t>(nao);
+        std::fill(dst, dst + nao, 0);
+        for (int r : rows)
+            dst[r] = 1;
+    });
+    return 0;
+}
+
+int orc_boys(double T, int m, double* out) {
+    if (m < 0 || m > kBoysMax || !std::isfinite(T) || T < 0.0) return 1;
+    boys(T, m, out);
+    return 0;
+}
+
+int orc_esp(const void* molecule, std::int64_t n, const double* points, double* out) {
+    try {
+        const auto esp = make_esp(*static_cast<const Molecule*>(molecule));
+        for_blocks(n, 64, [&](std::int64_t start, int size) {
+            for (int j = 0; j < size; ++j)
+                out[start + j] = esp_nuclear(*esp, points + 3 * (start + j)) + esp_electronic(*esp, points + 3 * (start + j));
+        });
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+std::int64_t orc_esp_pair_count(const void* molecule) {
+    return static_cast<std::int64_t>(make_esp(*static_cast<const Molecule*>(molecule))->pairs.size());
+}
+
+void orc_grid_points(const double* origin, const double* steps, const int* counts, std::int64_t start, std::int64_t n,
+                     double* out) {
+    PointSet set = make_set(origin, steps, counts, 0, nullptr);
+    for (std::int64_t q = 0; q < n; ++q)
+        set.point(start + q, out + 3 * q);
+}
+
+int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+}

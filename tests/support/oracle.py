@@ -1,0 +1,228 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so). Test
+infrastructure: used only as the checker of the HIP path."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_lib = None
+_ref = None
+
+
+class Prop(C.Structure):
+    _fields_ = [("kind", C.c_int), ("component", C.c_int), ("channel", C.c_int), ("orbital_index", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("block_size", C.c_int), ("backend", C.c_int), ("screening_threshold", C.c_double)]
+
+
+def _ptr(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"])
+        L = C.CDLL(path)
+        L.orc_basis_create.restype = C.c_void_p
+        L.orc_molecule_create.restype = C.c_void_p
+        L.orc_basis_nao.argtypes = [C.c_void_p]
+        L.orc_basis_nblocks.argtypes = [C.c_void_p]
+        L.orc_basis_blocks.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.orc_ao_eval.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_overlap.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_molecule_eligible.argtypes = [C.c_void_p]
+        L.orc_evaluate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_masks.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                C.c_int, C.c_int, C.c_double, C.c_void_p]
+        L.orc_boys.argtypes = [C.c_double, C.c_int, C.c_void_p]
+        L.orc_esp.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.orc_esp_pair_count.argtypes = [C.c_void_p]
+        L.orc_esp_pair_count.restype = C.c_int64
+        L.orc_grid_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
+        L.orc_pure_transform.argtypes = [C.c_int, C.c_void_p]
+        L.orc_molecule_destroy.argtypes = [C.c_void_p]
+        L.orc_basis_destroy.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def ref_boys_lib():
+    """The reference's own Boys function, if oracle/_ref was built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(ORACLE_DIR, "_ref", "libref_boys.so")
+        if not os.path.exists(path):
+            return None
+        _ref = C.CDLL(path)
+        _ref.ref_boys.argtypes = [C.c_double, C.c_int, C.c_void_p]
+    return _ref
+
+
+def boys(T, m):
+    out = np.zeros(m + 1)
+    assert lib().orc_boys(float(T), int(m), _ptr(out)) == 0
+    return out
+
+
+def pure_transform(l):
+    n = 2 * l + 1
+    nc = (l + 1) * (l + 2) // 2
+    out = np.zeros((n, nc))
+    assert lib().orc_pure_transform(l, _ptr(out)) == 0
+    return out
+
+
+class Basis:
+    def __init__(self, d):
+        err = C.create_string_buffer(512)
+        self._keep = [np.ascontiguousarray(x) for x in (d.shell_types, d.nprim, d.shell_atom)]
+        self._keepf = [np.ascontiguousarray(x, dtype=np.float64) for x in (d.shell_xyz, d.exponents, d.coefficients)]
+        pc = None if d.p_coefficients is None else np.ascontiguousarray(d.p_coefficients, dtype=np.float64)
+        self._pc = pc
+        st, npr, sa = [np.ascontiguousarray(x, dtype=np.int32) for x in (d.shell_types, d.nprim, d.shell_atom)]
+        self._ints = (st, npr, sa)
+        self.h = lib().orc_basis_create(len(st), _ptr(st, C.c_int), _ptr(npr, C.c_int), _ptr(sa, C.c_int),
+                                        _ptr(self._keepf[0]), _ptr(self._keepf[1]), _ptr(self._keepf[2]),
+                                        _ptr(pc), err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.nao = lib().orc_basis_nao(self.h)
+
+    def blocks(self):
+        nb = lib().orc_basis_nblocks(self.h)
+        l, off, cnt = (np.zeros(nb, np.int32) for _ in range(3))
+        r2 = np.zeros(nb)
+        lib().orc_basis_blocks(self.h, _ptr(l, C.c_int), _ptr(off, C.c_int), _ptr(cnt, C.c_int), _ptr(r2))
+        return l, off, cnt, r2
+
+    def ao(self, points, order=0):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        panels = {0: 1, 1: 4, 2: 10}[order]
+        out = np.zeros((panels * n, self.nao))  # column-major nao x (panels*n)
+        assert lib().orc_ao_eval(self.h, n, _ptr(pts), order, _ptr(out)) == 0
+        return out.T.copy()
+
+    def overlap(self):
+        out = np.zeros((self.nao, self.nao))
+        lib().orc_overlap(self.h, _ptr(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_basis_destroy(self.h)
+
+
+class Molecule:
+    def __init__(self, d):
+        self.data = d
+        self.basis = Basis(d)
+        n = self.basis.nao
+        err = C.create_string_buffer(512)
+        self._arrs = []
+
+        def keep(a, t=np.float64):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=t)
+            self._arrs.append(a)
+            return a
+        z = keep(d.atomic_numbers, np.int32)
+        ch = keep(d.charges)
+        xyz = keep(d.xyz)
+        pt = keep(d.p_total)
+        ps = keep(d.p_spin)
+        ca = keep(d.c_alpha)
+        cb = keep(d.c_beta)
+        nmo_a = 0 if ca is None else ca.size // n
+        nmo_b = 0 if cb is None else cb.size // n
+        self.h = lib().orc_molecule_create(C.c_void_p(self.basis.h), len(z), _ptr(z, C.c_int), _ptr(ch), _ptr(xyz),
+                                           _ptr(pt), _ptr(ps), nmo_a, _ptr(ca), nmo_b, _ptr(cb),
+                                           d.nalpha, d.nbeta, err, 512)
+        if not self.h:
+            raise RuntimeError(err.value.decode())
+        self.eligible = bool(lib().orc_molecule_eligible(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_molecule_destroy(self.h)
+
+    def evaluate(self, props, grid=None, points=None, block_size=128, backend=0, threshold=1e-12):
+        """props: list of (kind, component, channel, orbital_index).
+        Returns (list of arrays channels x n, backend_used)."""
+        from kumo_hip.api import channels
+        if grid is not None:
+            origin = np.ascontiguousarray(grid["origin"], dtype=np.float64)
+            steps = np.ascontiguousarray(np.asarray(grid["steps"], dtype=np.float64).T)  # column-major
+            counts = np.ascontiguousarray(grid["counts"], dtype=np.int32)
+            n = int(np.prod(counts.astype(np.int64)))
+            pts = None
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            n = pts.shape[0]
+            origin = steps = counts = None
+        P = (Prop * len(props))(*[Prop(*p) for p in props])
+        outs = [np.zeros((channels(p[0]), n)) for p in props]
+        ptrs = []
+        for o in outs:
+            for c in range(o.shape[0]):
+                ptrs.append(o[c].ctypes.data)
+        arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
+        used = C.c_int(0)
+        err = C.create_string_buffer(512)
+        opt = Options(block_size, backend, threshold)
+        rc = lib().orc_evaluate(self.h, _ptr(origin), _ptr(steps), _ptr(counts, C.c_int), n, _ptr(pts), P,
+                                len(props), C.byref(opt), arr, C.byref(used), err, 512)
+        if rc == 1:
+            raise ValueError(err.value.decode())
+        if rc == 2:
+            raise IndexError(err.value.decode())
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return outs, used.value
+
+    def masks(self, block_size, ao_order, grid=None, points=None, threshold=1e-12):
+        if grid is not None:
+            origin = np.ascontiguousarray(grid["origin"], dtype=np.float64)
+            steps = np.ascontiguousarray(np.asarray(grid["steps"], dtype=np.float64).T)
+            counts = np.ascontiguousarray(grid["counts"], dtype=np.int32)
+            n = int(np.prod(counts.astype(np.int64)))
+            pts = None
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            n = pts.shape[0]
+            origin = steps = counts = None
+        nb = (n + block_size - 1) // block_size
+        out = np.zeros((nb, self.basis.nao), dtype=np.uint8)
+        lib().orc_masks(self.h, _ptr(origin), _ptr(steps), _ptr(counts, C.c_int), n, _ptr(pts), block_size,
+                        ao_order, threshold, _ptr(out, C.c_uint8))
+        return out
+
+    def esp(self, points):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0])
+        assert lib().orc_esp(self.h, pts.shape[0], _ptr(pts), _ptr(out)) == 0
+        return out
+
+    def esp_pair_count(self):
+        return lib().orc_esp_pair_count(self.h)
+
+
+def grid_points(grid, start=0, n=None):
+    origin = np.ascontiguousarray(grid["origin"], dtype=np.float64)
+    steps = np.ascontiguousarray(np.asarray(grid["steps"], dtype=np.float64).T)
+    counts = np.ascontiguousarray(grid["counts"], dtype=np.int32)
+    total = int(np.prod(counts.astype(np.int64)))
+    n = total - start if n is None else n
+    out = np.zeros((n, 3))
+    lib().orc_grid_points(_ptr(origin), _ptr(steps), _ptr(counts, C.c_int), start, n, _ptr(out))
+    return out
