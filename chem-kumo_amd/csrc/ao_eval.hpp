@@ -1,0 +1,337 @@
+// Device evaluation of contracted Gaussian AO blocks at one point.
+//
+// The arithmetic follows the reference term by term
+// (src/basis_evaluator.cpp:442-525 for values, :512-643 for derivatives):
+// the same products in the same association and no fused multiply-add
+// (`#pragma clang fp contract(off)`), so that FP64 AO values equal the CPU
+// ones up to the last-bit differences of exp(), and the screening masks built
+// from them are the reference's. The shell's angular momentum is a template
+// parameter so every per-component array lives in registers.
+#pragma once
+
+#include "device.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace kumo {
+
+struct MolView {
+    const DevShell* shells;
+    int nshells;
+    int nao;
+    const double* exps;
+    const double* weights;
+    const double* tdense; // [kMaxL+1][kMaxPure][kMaxCart] Cartesian->pure, zero padded
+};
+
+// Cartesian powers in Gaussian FCHK order (src/basis_evaluator.cpp:135-152).
+__device__ constexpr signed char kPow[kMaxL + 1][kMaxCart][3] = {
+    {{0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}},
+    {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {1, 2, 0}, {2, 1, 0}, {2, 0, 1}, {1, 0, 2}, {0, 1, 2}, {0, 2, 1}, {1, 1, 1}},
+    {{0, 0, 4}, {0, 1, 3}, {0, 2, 2}, {0, 3, 1}, {0, 4, 0}, {1, 0, 3}, {1, 1, 2}, {1, 2, 1}, {1, 3, 0}, {2, 0, 2},
+     {2, 1, 1}, {2, 2, 0}, {3, 0, 1}, {3, 1, 0}, {4, 0, 0}},
+    {{0, 0, 5}, {0, 1, 4}, {0, 2, 3}, {0, 3, 2}, {0, 4, 1}, {0, 5, 0}, {1, 0, 4}, {1, 1, 3}, {1, 2, 2}, {1, 3, 1},
+     {1, 4, 0}, {2, 0, 3}, {2, 1, 2}, {2, 2, 1}, {2, 3, 0}, {3, 0, 2}, {3, 1, 1}, {3, 2, 0}, {4, 0, 1}, {4, 1, 0},
+     {5, 0, 0}}};
+
+// Panel sets: bit c selects derivative component c of
+// value, x, y, z, xx, xy, xz, yy, yz, zz (src/derivative_layout.hpp:13).
+constexpr unsigned kPanelsValue = 0x001;
+constexpr unsigned kPanelsGrad = 0x00f;
+constexpr unsigned kPanelsDiag = 0x08f | 0x010 | 0x200; // value, grad, xx, yy, zz
+constexpr unsigned kPanelsAll = 0x3ff;
+
+__host__ __device__ constexpr int popcount10(unsigned m) {
+    int n = 0;
+    for (int i = 0; i < 10; ++i)
+        n += (m >> i) & 1u;
+    return n;
+}
+// Index of the derivative component of output panel i.
+__host__ __device__ constexpr int panel_component(unsigned m, int i) {
+    for (int c = 0; c < 10; ++c)
+        if ((m >> c) & 1u) {
+            if (i == 0) return c;
+            --i;
+        }
+    return -1;
+}
+
+// Grid or listed point q (src/grid_geometry.hpp:59-68): origin + steps*(i,j,k),
+// each coordinate summed as ((s0 i + s1 j) + s2 k) without fused operations.
+__device__ inline void point_coords(const PointSource& ps, long long q, double& x, double& y, double& z) {
+#pragma clang fp contract(off)
+    if (ps.points) {
+        x = ps.points[3 * q];
+        y = ps.points[3 * q + 1];
+        z = ps.points[3 * q + 2];
+        return;
+    }
+    const long long plane = ps.ny * ps.nz;
+    const double fi = static_cast<double>(q / plane);
+    const double fj = static_cast<double>((q / ps.nz) % ps.ny);
+    const double fk = static_cast<double>(q % ps.nz);
+    x = ps.origin[0] + ((ps.steps[0] * fi + ps.steps[3] * fj) + ps.steps[6] * fk);
+    y = ps.origin[1] + ((ps.steps[1] * fi + ps.steps[4] * fj) + ps.steps[7] * fk);
+    z = ps.origin[2] + ((ps.steps[2] * fi + ps.steps[5] * fj) + ps.steps[8] * fk);
+}
+
+__device__ inline double sq_norm(double x, double y, double z) {
+#pragma clang fp contract(off)
+    return (x * x + y * y) + z * z;
+}
+
+// Cartesian components of block `s` (angular momentum L) at displacement
+// (x, y, z), r2 = |r|^2: cart[k][i] for output panel i of PSET.
+// ORDER 0 uses the value kernel (evaluate_into_impl), 1 and 2 the derivative
+// kernel (evaluate_derivatives_batch), as the reference does.
+template <int L, int ORDER, unsigned PSET>
+__device__ inline void eval_cartesian(const DevShell& s, const double* __restrict__ exps,
+                                      const double* __restrict__ wts, double x, double y, double z, double r2,
+                                      double (&cart)[(L + 1) * (L + 2) / 2][popcount10(PSET)]) {
+#pragma clang fp contract(off)
+    constexpr int NC = (L + 1) * (L + 2) / 2;
+    constexpr int NP = popcount10(PSET);
+    const int np = s.nprim;
+    const double* e = exps + s.prim;
+    const double* w = wts + s.wofs;
+    double xp[L + 1], yp[L + 1], zp[L + 1];
+    xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+    for (int n = 1; n <= L; ++n) {
+        xp[n] = xp[n - 1] * x;
+        yp[n] = yp[n - 1] * y;
+        zp[n] = zp[n - 1] * z;
+    }
+    if constexpr (ORDER == 0) {
+        double rs[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+            rs[k] = 0.0;
+        for (int p = 0; p < np; ++p) {
+            const double radial = exp(-e[p] * r2);
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                rs[k] += w[k * np + p] * radial;
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const double poly = (xp[kPow[L][k][0]] * yp[kPow[L][k][1]]) * zp[kPow[L][k][2]];
+            cart[k][0] = poly * rs[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                cart[k][i] = 0.0;
+        const double r[3] = {x, y, z};
+        for (int p = 0; p < np; ++p) {
+            const double a = e[p];
+            const double radial = exp(-a * r2);
+            // f[axis][m][q]: x^m exp(-a x^2) and its derivatives over exp(-a x^2).
+            double f[3][L + 1][3];
+#pragma unroll
+            for (int axis = 0; axis < 3; ++axis) {
+                const double* pw = axis == 0 ? xp : (axis == 1 ? yp : zp);
+                const double c = r[axis];
+#pragma unroll
+                for (int m = 0; m <= L; ++m) {
+                    const double xm = pw[m];
+                    f[axis][m][0] = xm;
+                    f[axis][m][1] = (m ? m * pw[m > 0 ? m - 1 : 0] : 0.0) - 2.0 * a * c * xm;
+                    if constexpr (ORDER == 2)
+                        f[axis][m][2] = (m >= 2 ? m * (m - 1) * pw[m >= 2 ? m - 2 : 0] : 0.0) -
+                                        2.0 * a * (2 * m + 1) * xm + 4.0 * a * a * c * c * xm;
+                    else
+                        f[axis][m][2] = 0.0;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const double wgt = w[k * np + p] * radial;
+                const double* fx = f[0][kPow[L][k][0]];
+                const double* fy = f[1][kPow[L][k][1]];
+                const double* fz = f[2][kPow[L][k][2]];
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    constexpr int dummy = 0;
+                    (void)dummy;
+                    const int c = panel_component(PSET, i);
+                    double t;
+                    switch (c) {
+                    case 0: t = wgt * fx[0] * fy[0] * fz[0]; break;
+                    case 1: t = wgt * fx[1] * fy[0] * fz[0]; break;
+                    case 2: t = wgt * fx[0] * fy[1] * fz[0]; break;
+                    case 3: t = wgt * fx[0] * fy[0] * fz[1]; break;
+                    case 4: t = wgt * fx[2] * fy[0] * fz[0]; break;
+                    case 5: t = wgt * fx[1] * fy[1] * fz[0]; break;
+                    case 6: t = wgt * fx[1] * fy[0] * fz[1]; break;
+                    case 7: t = wgt * fx[0] * fy[2] * fz[0]; break;
+                    case 8: t = wgt * fx[0] * fy[1] * fz[1]; break;
+                    default: t = wgt * fx[0] * fy[0] * fz[2]; break;
+                    }
+                    cart[k][i] += t;
+                }
+            }
+        }
+    }
+}
+
+// AO values of block `s` at the point: out(row, i) for each AO row of the
+// block and output panel i; rows are pure (transformed, in increasing
+// Cartesian-component order) or Cartesian. Calls emit(row, vals[NP]).
+template <int L, int ORDER, unsigned PSET, typename Emit>
+__device__ inline void eval_block_l(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
+                                    Emit&& emit) {
+#pragma clang fp contract(off)
+    constexpr int NC = (L + 1) * (L + 2) / 2;
+    constexpr int NP = popcount10(PSET);
+    double cart[NC][NP];
+    eval_cartesian<L, ORDER, PSET>(s, m.exps, m.weights, x, y, z, r2, cart);
+    if (L >= 2 && s.pure) {
+        const double* T = m.tdense + L * kMaxPure * kMaxCart;
+#pragma unroll
+        for (int row = 0; row < 2 * L + 1; ++row) {
+            double v[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                v[i] = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const double t = T[row * kMaxCart + k];
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+                    v[i] += t * cart[k][i];
+            }
+            emit(row, v);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+            emit(k, cart[k]);
+    }
+}
+
+// Runtime dispatch on the block's angular momentum.
+template <int ORDER, unsigned PSET, typename Emit>
+__device__ inline void eval_block(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
+                                  Emit&& emit) {
+    switch (s.l) {
+    case 0: eval_block_l<0, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 1: eval_block_l<1, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 2: eval_block_l<2, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 3: eval_block_l<3, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 4: eval_block_l<4, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    default: eval_block_l<5, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    }
+}
+
+// Cartesian components of block `s` for the contraction kernels: the same
+// mathematics as eval_cartesian, reorganised for registers (component-outer,
+// primitives inner with their radial factors cached) and free to fuse
+// multiply-adds, so values agree with the reference kernels to a few ulps.
+// The Cartesian->pure transform is folded into the MO coefficients by the
+// caller. Calls emit(k, vals[NP]) for each Cartesian component k.
+template <int L, int ORDER, unsigned PSET, typename Emit>
+__device__ inline void eval_cartesian_rows(const DevShell& s, const MolView& m, double x, double y, double z,
+                                           double r2, Emit&& emit) {
+    constexpr int NC = (L + 1) * (L + 2) / 2;
+    constexpr int NP = popcount10(PSET);
+    constexpr int PC = 8;
+    const int np = s.nprim;
+    const double* e = m.exps + s.prim;
+    const double* w = m.weights + s.wofs;
+    double xp[L + 3], yp[L + 3], zp[L + 3];
+    xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+    for (int n = 1; n <= L + 2; ++n) {
+        xp[n] = xp[n - 1] * x;
+        yp[n] = yp[n - 1] * y;
+        zp[n] = zp[n - 1] * z;
+    }
+    // One primitive of component k: adds weight * product of axis factors.
+    const auto add = [&](int k, int p, double radial, double (&sums)[NP]) {
+        const double a = e[p];
+        const double wt = w[k * np + p] * radial;
+        const double two_a = 2.0 * a;
+        double f[3][3];
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+            const int q = kPow[L][k][axis];
+            const double* pw = axis == 0 ? xp : (axis == 1 ? yp : zp);
+            f[axis][0] = pw[q];
+            if constexpr (ORDER >= 1) f[axis][1] = (q ? q * pw[q > 0 ? q - 1 : 0] : 0.0) - two_a * pw[q + 1];
+            if constexpr (ORDER == 2)
+                f[axis][2] = (q >= 2 ? q * (q - 1) * pw[q >= 2 ? q - 2 : 0] : 0.0) - two_a * (2 * q + 1) * pw[q] +
+                             two_a * two_a * pw[q + 2];
+        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int c = panel_component(PSET, i);
+            const int dx = c == 1 || c == 5 || c == 6 ? 1 : (c == 4 ? 2 : 0);
+            const int dy = c == 2 || c == 5 || c == 8 ? 1 : (c == 7 ? 2 : 0);
+            const int dz = c == 3 || c == 6 || c == 8 ? 1 : (c == 9 ? 2 : 0);
+            sums[i] += wt * f[0][dx] * f[1][dy] * f[2][dz];
+        }
+    };
+    if (np <= PC) {
+        double rad[PC];
+#pragma unroll
+        for (int p = 0; p < PC; ++p)
+            rad[p] = p < np ? exp(-e[p] * r2) : 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            double sums[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                sums[i] = 0.0;
+#pragma unroll
+            for (int p = 0; p < PC; ++p)
+                if (p < np) add(k, p, rad[p], sums);
+            emit(k, sums);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            double sums[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                sums[i] = 0.0;
+            for (int p = 0; p < np; ++p)
+                add(k, p, exp(-e[p] * r2), sums);
+            emit(k, sums);
+        }
+    }
+}
+
+template <int ORDER, unsigned PSET, typename Emit>
+__device__ inline void eval_block_cartesian(const DevShell& s, const MolView& m, double x, double y, double z,
+                                            double r2, Emit&& emit) {
+    switch (s.l) {
+    case 0: eval_cartesian_rows<0, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 1: eval_cartesian_rows<1, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 2: eval_cartesian_rows<2, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 3: eval_cartesian_rows<3, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    case 4: eval_cartesian_rows<4, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    default: eval_cartesian_rows<5, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    }
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+} // namespace kumo

@@ -1,0 +1,67 @@
+// Data layouts shared by the host code and the HIP kernels.
+//
+// A molecule lives in HBM as structure-of-arrays tables built once at load:
+//  - shells[]: one entry per normalized AO block (an SP shell gives an s and a
+//    p block), in FCHK AO order;
+//  - exps[], weights[]: primitive exponents and contraction weights
+//    (coefficient x primitive normalization), weights row-major per block
+//    (Cartesian component x primitive);
+//  - pure[]: sparse Cartesian->pure transform entries per angular momentum,
+//    ordered by Cartesian component so that each pure AO accumulates its terms
+//    in increasing component order;
+//  - the occupied MO coefficients, AO-major ([AO][MO slot]) in FP64, FP32 and
+//    bf16 splits, both spins side by side in one padded slot range;
+//  - P total and P spin, row-major FP64.
+#pragma once
+
+#include <cstdint>
+
+namespace kumo {
+
+constexpr int kMaxL = 5;                 // h shells; higher l is rejected at load
+constexpr int kMaxCart = (kMaxL + 1) * (kMaxL + 2) / 2;
+constexpr int kMaxPure = 2 * kMaxL + 1;
+constexpr int kMaxPrim = 64;
+
+struct DevShell {
+    double o[3];
+    double cutoff_r2;
+    int32_t l;
+    int32_t nprim;
+    int32_t prim;   // first primitive in exps[]
+    int32_t wofs;   // first weight in weights[] (ncart x nprim, row-major)
+    int32_t ncart;
+    int32_t count;  // AOs of the block
+    int32_t ao;     // first AO
+    int32_t pure;   // 1 if the block is pure (l >= 2)
+    int32_t atom;
+    int32_t pad;
+};
+
+// Sparse transform entry: pure row `row` += coef * Cartesian component `cart`.
+struct PureTerm {
+    double coef;
+    int32_t cart;
+    int32_t row;
+};
+
+// Offsets of the pure terms of angular momentum l in the PureTerm table.
+struct PureIndex {
+    int32_t first[kMaxL + 2];
+};
+
+// Point source of one evaluation: a grid or an explicit list.
+struct PointSource {
+    double origin[3];
+    double steps[9]; // column-major: column a = step vector of axis a
+    int64_t ny, nz;
+    const double* points; // xyz per point, or null for a grid
+};
+
+// Per-point moments of one density channel, as DensityDerivatives
+// (include/chem/kumo/density.hpp:151-160): value, gradient x y z, tau terms
+// x y z, Hessian xx xy xz yy yz zz.
+constexpr int kMoments = 13;
+enum Moment { M_VALUE = 0, M_GX, M_GY, M_GZ, M_TX, M_TY, M_TZ, M_HXX, M_HXY, M_HXZ, M_HYY, M_HYZ, M_HZZ };
+
+} // namespace kumo

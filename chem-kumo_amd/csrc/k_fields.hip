@@ -1,0 +1,308 @@
+// Per-point combination of channel moments into density components
+// (combine, src/density.cpp:50-60) and the derived fields of store_block
+// (src/evaluator.cpp:127-353), in FP64. One thread per point; every output is
+// a coalesced FP64 stream.
+#include "kernels.hpp"
+
+#include "common.hpp"
+
+#include <math.h>
+
+namespace kumo {
+namespace {
+
+constexpr int kComp = 15; // 13 moments + Laplacian + tau
+constexpr int M_LAP = 13, M_TAU = 14;
+
+struct DDv {
+    double v, g[3], t[3], h[6], lap, tau;
+};
+
+__device__ inline DDv load_channel(const double* m, long long n, long long j) {
+    DDv d;
+    d.v = m[M_VALUE * n + j];
+    for (int a = 0; a < 3; ++a) {
+        d.g[a] = m[(M_GX + a) * n + j];
+        d.t[a] = m[(M_TX + a) * n + j];
+    }
+    for (int k = 0; k < 6; ++k)
+        d.h[k] = m[(M_HXX + k) * n + j];
+    d.tau = 0.0;
+    for (int a = 0; a < 3; ++a)
+        d.tau += d.t[a];
+    d.lap = d.h[0] + d.h[3] + d.h[5];
+    return d;
+}
+
+__device__ inline DDv combine(const DDv& a, const DDv& b, double sign, double scale) {
+    DDv d;
+    d.v = scale * (a.v + sign * b.v);
+    for (int k = 0; k < 3; ++k) {
+        d.g[k] = scale * (a.g[k] + sign * b.g[k]);
+        d.t[k] = scale * (a.t[k] + sign * b.t[k]);
+    }
+    for (int k = 0; k < 6; ++k)
+        d.h[k] = scale * (a.h[k] + sign * b.h[k]);
+    d.lap = d.h[0] + d.h[3] + d.h[5];
+    d.tau = scale * (a.tau + sign * b.tau);
+    return d;
+}
+
+__device__ inline void store_comp(double* m, long long n, long long j, const DDv& d) {
+    m[M_VALUE * n + j] = d.v;
+    for (int a = 0; a < 3; ++a) {
+        m[(M_GX + a) * n + j] = d.g[a];
+        m[(M_TX + a) * n + j] = d.t[a];
+    }
+    for (int k = 0; k < 6; ++k)
+        m[(M_HXX + k) * n + j] = d.h[k];
+    m[M_LAP * n + j] = d.lap;
+    m[M_TAU * n + j] = d.tau;
+}
+
+__global__ void combine_kernel(const double* ch, double* comp, long long n, int from_spin, int closed) {
+    const long long j = blockIdx.x * 256ll + threadIdx.x;
+    if (j >= n) return;
+    const DDv c0 = load_channel(ch, n, j);
+    const DDv c1 = closed ? c0 : load_channel(ch + kMoments * n, n, j);
+    const long long stride = kComp * n;
+    if (from_spin) { // channels: alpha, beta
+        store_comp(comp + 1 * stride, n, j, c0);
+        store_comp(comp + 2 * stride, n, j, c1);
+        store_comp(comp + 0 * stride, n, j, combine(c0, c1, 1.0, 1.0));
+        store_comp(comp + 3 * stride, n, j, combine(c0, c1, -1.0, 1.0));
+    } else { // channels: total, spin difference
+        store_comp(comp + 0 * stride, n, j, c0);
+        store_comp(comp + 3 * stride, n, j, c1);
+        store_comp(comp + 1 * stride, n, j, combine(c0, c1, 1.0, 0.5));
+        store_comp(comp + 2 * stride, n, j, combine(c0, c1, -1.0, 0.5));
+    }
+}
+
+__device__ inline DDv load_comp(const double* m, long long n, long long j) {
+    DDv d;
+    d.v = m[M_VALUE * n + j];
+    for (int a = 0; a < 3; ++a) {
+        d.g[a] = m[(M_GX + a) * n + j];
+        d.t[a] = m[(M_TX + a) * n + j];
+    }
+    for (int k = 0; k < 6; ++k)
+        d.h[k] = m[(M_HXX + k) * n + j];
+    d.lap = m[M_LAP * n + j];
+    d.tau = m[M_TAU * n + j];
+    return d;
+}
+
+__device__ inline double gnorm2(const DDv& d) { return (d.g[0] * d.g[0] + d.g[1] * d.g[1]) + d.g[2] * d.g[2]; }
+
+
+// elf(), lol(), gradient_expansion_locator() of src/evaluator.cpp:127-215
+// over one or two channels.
+__device__ double elf(const DDv* c, int nc, double kFermi) {
+    double pauli = 0.0, uniform = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        if (!(c[i].v > 0.0)) continue;
+        pauli += c[i].tau - gnorm2(c[i]) / (8.0 * c[i].v);
+        uniform += kFermi * pow(c[i].v, 5.0 / 3.0);
+    }
+    if (!(uniform > 0.0)) return 0.0;
+    const double chi = pauli / uniform;
+    return 1.0 / (1.0 + chi * chi);
+}
+
+__device__ double lol(const DDv* c, int nc, double kFermi) {
+    double tau = 0.0, uniform = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        tau += c[i].tau;
+        if (c[i].v > 0.0) uniform += kFermi * pow(c[i].v, 5.0 / 3.0);
+    }
+    if (!(uniform > 0.0)) return 0.0;
+    if (!(tau > 0.0)) return 1.0;
+    return uniform / (tau + uniform);
+}
+
+__device__ double ge_locator(const DDv* c, int nc, bool is_lol, double kFermi) {
+    double uniform = 0.0, pauli = 0.0, kinetic = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        if (!(c[i].v > 0.0)) continue;
+        const double ref = kFermi * pow(c[i].v, 5.0 / 3.0);
+        const double wz = gnorm2(c[i]) / c[i].v;
+        uniform += ref;
+        pauli += ref - wz / 9.0 + c[i].lap / 6.0;
+        kinetic += ref + wz / 72.0 + c[i].lap / 6.0;
+    }
+    if (!(uniform > 0.0)) return 0.0;
+    if (is_lol) {
+        const double den = kinetic + uniform;
+        return den != 0.0 ? uniform / den : 0.0;
+    }
+    const double chi = pauli / uniform;
+    return 1.0 / (1.0 + chi * chi);
+}
+
+// Middle eigenvalue of the symmetric Hessian (xx, xy, xz, yy, yz, zz) by
+// cyclic Jacobi rotations in FP64 (the sign decides sign(lambda2) rho).
+__device__ double middle_eigenvalue(const double* h6) {
+    double a[3][3] = {{h6[0], h6[1], h6[2]}, {h6[1], h6[3], h6[4]}, {h6[2], h6[4], h6[5]}};
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        if (off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (a[p][q] == 0.0) continue;
+                const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+            }
+    }
+    const double e0 = a[0][0], e1 = a[1][1], e2 = a[2][2];
+    if ((e0 <= e1 && e1 <= e2) || (e2 <= e1 && e1 <= e0)) return e1;
+    if ((e1 <= e0 && e0 <= e2) || (e2 <= e0 && e0 <= e1)) return e0;
+    return e2;
+}
+
+__global__ void fields_kernel(FieldsLaunch f) {
+    const long long j = blockIdx.x * 256ll + threadIdx.x;
+    if (j >= f.n) return;
+    const long long n = f.n, stride = kComp * n;
+    const long long o = f.out_offset + j;
+    for (int t = 0; t < f.ntasks; ++t) {
+        const FieldTask& task = f.tasks[t];
+        const int kind = task.kind;
+        // Total ELF / LOL / Tsirelson-Stash forms use both spin channels.
+        if ((kind == 6 || kind == 10 || kind == 35 || kind == 36) && task.component == 0) {
+            DDv c[2] = {load_comp(f.moments + 1 * stride, n, j), load_comp(f.moments + 2 * stride, n, j)};
+            double v;
+            if (kind == 6) v = elf(c, 2, f.fermi);
+            else if (kind == 10) v = lol(c, 2, f.fermi);
+            else v = ge_locator(c, 2, kind == 36, f.fermi);
+            task.out[0][o] = v;
+            continue;
+        }
+        if (kind == 16) { // spin polarization
+            const double tv = f.moments[0 * stride + M_VALUE * n + j];
+            const double sv = f.moments[3 * stride + M_VALUE * n + j];
+            task.out[0][o] = tv > 0.0 ? sv / tv : 0.0;
+            continue;
+        }
+        const DDv d = load_comp(f.moments + task.component * stride, n, j);
+        double* out0 = task.out[0];
+        switch (kind) {
+        case 0: out0[o] = d.v; break;
+        case 1: for (int a = 0; a < 3; ++a) task.out[a][o] = d.g[a]; break;
+        case 2: for (int k = 0; k < 6; ++k) task.out[k][o] = d.h[k]; break;
+        case 3: out0[o] = d.lap; break;
+        case 4: out0[o] = d.tau; break;
+        case 6: out0[o] = elf(&d, 1, f.fermi); break;
+        case 7: { // reduced gradient
+            const double scale = f.rdg_scale;
+            double v = 100.0;
+            if (d.v > 0.0) {
+                const double den = scale * pow(d.v, 4.0 / 3.0);
+                const double norm = sqrt(gnorm2(d));
+                if (den > 0.0 && norm < 100.0 * den) v = norm / den;
+            }
+            out0[o] = v;
+            break;
+        }
+        case 8: out0[o] = middle_eigenvalue(d.h) < 0.0 ? -d.v : d.v; break;
+        case 10: out0[o] = lol(&d, 1, f.fermi); break;
+        case 35: out0[o] = ge_locator(&d, 1, false, f.fermi); break;
+        case 36: out0[o] = ge_locator(&d, 1, true, f.fermi); break;
+        case 11: { // IRI
+            double v = 100.0;
+            if (d.v > 0.0) {
+                v = sqrt(gnorm2(d)) / pow(d.v, 1.1);
+                v = v < 100.0 ? v : 100.0;
+            }
+            out0[o] = v;
+            break;
+        }
+        case 12: { // DORI
+            double v = 0.0;
+            if (d.v > 0.0) {
+                const double q[3] = {d.g[0] / d.v, d.g[1] / d.v, d.g[2] / d.v};
+                const double q2 = (q[0] * q[0] + q[1] * q[1]) + q[2] * q[2];
+                if (!(q2 > 0.0)) {
+                    v = 1.0;
+                } else {
+                    const double H[3][3] = {{d.h[0], d.h[1], d.h[2]}, {d.h[1], d.h[3], d.h[4]}, {d.h[2], d.h[4], d.h[5]}};
+                    double w[3];
+                    for (int a = 0; a < 3; ++a)
+                        w[a] = (H[a][0] * q[0] + H[a][1] * q[1] + H[a][2] * q[2]) / d.v - q2 * q[a];
+                    const double theta = 4.0 * ((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]) / (q2 * q2 * q2);
+                    v = isfinite(theta) ? theta / (1.0 + theta) : 1.0;
+                }
+            }
+            out0[o] = v;
+            break;
+        }
+        case 13: out0[o] = d.tau - 0.25 * d.lap; break;
+        case 14: out0[o] = 0.25 * d.lap - 2.0 * d.tau; break;
+        case 15: out0[o] = 0.25 * d.lap - d.tau; break;
+        case 30: for (int a = 0; a < 3; ++a) task.out[a][o] = d.t[a]; break;
+        case 31: {
+            const double diag[3] = {d.h[0], d.h[3], d.h[5]};
+            for (int a = 0; a < 3; ++a) task.out[a][o] = d.t[a] - 0.25 * diag[a];
+            break;
+        }
+        case 32: out0[o] = d.tau > 0.0 ? fabs(0.25 * d.lap - 2.0 * d.tau) / d.tau : 0.0; break;
+        case 33: out0[o] = d.v > 0.0 ? (0.25 * d.lap - d.tau) / d.v : 0.0; break;
+        case 34: out0[o] = d.v > 0.0 ? d.tau / d.v : 0.0; break;
+        case 17: out0[o] = d.v > 0.0 ? -d.v * log(d.v) : 0.0; break;
+        case 18: out0[o] = d.v > 0.0 ? gnorm2(d) / d.v : 0.0; break;
+        default: break;
+        }
+    }
+}
+
+__global__ void add_kernel(double* dst, const double* src, long long n) {
+    const long long j = blockIdx.x * 256ll + threadIdx.x;
+    if (j < n) dst[j] += src[j];
+}
+
+__global__ void fill_kernel(double* dst, long long n, double v) {
+    const long long j = blockIdx.x * 256ll + threadIdx.x;
+    if (j < n) dst[j] = v;
+}
+
+} // namespace
+
+void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream) {
+    if (n <= 0) return;
+    add_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(dst, src, n);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_fill(double* dst, long long n, double value, hipStream_t stream) {
+    if (n <= 0) return;
+    fill_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(dst, n, value);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_combine(const double* channel_moments, double* component_moments, long long n, int from_spin_channels,
+                    int order, int closed_shell, hipStream_t stream) {
+    (void)order;
+    if (n <= 0) return;
+    combine_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(channel_moments, component_moments, n,
+                                                                               from_spin_channels, closed_shell);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_fields(const FieldsLaunch& f, hipStream_t stream) {
+    if (f.n <= 0 || f.ntasks == 0) return;
+    fields_kernel<<<static_cast<unsigned>((f.n + 255) / 256), 256, 0, stream>>>(f);
+    KUMO_HIP(hipGetLastError());
+}
+
+} // namespace kumo

@@ -1,0 +1,137 @@
+// Host-callable launchers of the HIP kernels.
+#pragma once
+
+#include "ao_eval.hpp"
+#include "device.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kumo {
+
+// Rows of one contraction chunk: whole kept AO blocks are packed greedily
+// into chunks of at most kChunkRows AO rows.
+constexpr int kChunkRows = 32;
+
+// Per screening block outputs of the mask kernel.
+struct KeptTables {
+    uint8_t* kept;          // [nblk][nao] significant_aos flags
+    int32_t* kept_list;     // [nblk][nao] kept AO indices, ascending
+    int32_t* nkept;         // [nblk]
+    int32_t* kept_shell;    // [nblk][nshells] AO blocks with at least one kept AO
+    uint32_t* kept_bits;    // [nblk][nshells] bit r: AO row r of that block is kept
+    uint8_t* kept_row;      // [nblk][nshells] first row of the kept block in its chunk
+    int32_t* chunk_begin;   // [nblk][nshells + 1]
+    int32_t* nchunks;       // [nblk]
+};
+
+// significant_aos (src/ao_screening.hpp:35-50) for screening blocks
+// [first_block, first_block + nblk) of `block_size` points; order selects
+// the AO panels screened (0: values, 1: + gradient, 2: + Hessian).
+void launch_masks(const MolView& m, const PointSource& ps, long long npoints, int block_size, long long first_block,
+                  int nblk, int order, double threshold, const KeptTables& t, hipStream_t stream);
+
+// AO panels at listed points, column-major nao x (panels * n).
+void launch_ao_eval(const MolView& m, const PointSource& ps, long long n, int order, double* out, hipStream_t stream);
+
+// Occupied-MO coefficient operands of the AO-MO contraction, AO-major with
+// `slots` columns (alpha then beta, each padded to 16).
+struct MoOperand {
+    const double* c64;      // [nao][slots]
+    const float* c32;       // [nao][slots]
+    const uint16_t* cbf;    // [3][nao][slots] bf16 hi, mid, lo
+    int slots;
+    int alpha_slots;        // slots of the alpha channel (multiple of 16)
+};
+
+// Fused AO-MO contraction (density_derivatives_mo, src/density.cpp:234-351):
+// for every point of the batch, the moments of each channel, written to
+// moments[channel][kMoments][npoints_batch] (FP64). slot range [slot0,
+// slot1) selects the channels contracted.
+struct AoMoLaunch {
+    MolView mol;
+    PointSource ps;
+    MoOperand mo;
+    KeptTables tables;
+    long long npoints;       // all points of the evaluation
+    long long batch_first;   // first point of the batch (a block boundary)
+    long long batch_points;  // points in the batch
+    int block_size;
+    long long first_block;   // screening block of batch_first
+    int nblk;
+    int order;               // AO arithmetic: 0 value kernel, 1/2 derivative kernel
+    int hessian;             // 0 none, 1 diagonal, 2 full
+    int precision;           // kumo_hip_precision
+    int slot0, slot1;
+    double* moments;         // [2][kMoments][batch_points]
+};
+void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream);
+
+// AO-P contraction (density_derivatives_shared / densities_blocked,
+// src/density.cpp:149-229, 365-444): W = P_kept [Phi Phi_x Phi_y Phi_z] and
+// the per-point moments of each matrix (0: total / alpha, 1: spin / beta).
+struct AoPLaunch {
+    MolView mol;
+    PointSource ps;
+    KeptTables tables;
+    const double* p[2];      // row-major nao x nao, null if unused
+    long long npoints, batch_first, batch_points;
+    int block_size;
+    long long first_block;
+    int nblk;
+    int order;               // 0 (densities_blocked) or 1, 2
+    double* phi;             // workspace: [nblk][nao][panels][block_size]
+    double* w;               // workspace: [nblk][nao][4][block_size]
+    double* moments;         // [2][kMoments][batch_points]
+};
+void launch_ao_p(const AoPLaunch& a, hipStream_t stream);
+
+// Per-point derived fields (store_block, src/evaluator.cpp:248-353).
+struct FieldTask {
+    int kind, component;
+    double* out[6];
+};
+struct FieldsLaunch {
+    const double* moments;   // [4 components][kMoments][n]: total, alpha, beta, difference
+    long long n;
+    long long out_offset;    // index of the batch's first point in the outputs
+    int ntasks;
+    const FieldTask* tasks;  // device array
+    double fermi;            // 0.3 (6 pi^2)^(2/3), as src/evaluator.cpp:128-129 computes it
+    double rdg_scale;        // 2 (3 pi^2)^(1/3), src/evaluator.cpp:148
+};
+FieldsLaunch fields_launch(const double* moments, long long n, long long out_offset, int ntasks, const FieldTask* tasks);
+void launch_fields(const FieldsLaunch& f, hipStream_t stream);
+// Channel moments -> component moments: from (alpha, beta) (AO-MO) or
+// (total, spin) (AO-P) build total, alpha, beta, difference
+// (combine, src/density.cpp:50-60).
+void launch_combine(const double* channel_moments, double* component_moments, long long n, int from_spin_channels,
+                    int order, int closed_shell, hipStream_t stream);
+
+// dst[i] += src[i]; dst[i] = value.
+void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream);
+void launch_fill(double* dst, long long n, double value, hipStream_t stream);
+
+// MO amplitudes (and derivatives) of selected MOs at points:
+// out[mo][component][n] for components value (order 0) or value, gradient,
+// Hessian (xx, xy, xz, yy, yz, zz).
+void launch_orbitals(const MolView& m, const PointSource& ps, long long first, long long n, const double* coeff,
+                     int ldc, const int* mo_index, int nmo, int order, double* out, long long ld_out,
+                     hipStream_t stream);
+
+// Electrostatic potential at points (src/electrostatic_potential.cpp:231-361).
+struct EspPairDev {
+    double p, qx, qy, qz;
+    int32_t L;
+    int32_t first;
+};
+struct EspAtomDev {
+    double x, y, z, charge;
+};
+void launch_esp(const EspPairDev* pairs, int npairs, const double* coef, const EspAtomDev* atoms, int natoms,
+                const double* boys_table, const PointSource& ps, long long first, long long n, double* out,
+                hipStream_t stream);
+void launch_boys(const double* table, const double* T, long long n, int m, double* out, hipStream_t stream);
+
+} // namespace kumo

@@ -1,0 +1,475 @@
+// Molecule loading and residency (src/molecule.cpp:20-185,
+// src/backend_selection.hpp:43-78, src/mo_coefficients.hpp:21-34) and the
+// host construction of the ESP primitive pairs
+// (src/electrostatic_potential.cpp:117-229).
+#include "fchk.hpp"
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numbers>
+
+namespace kumo {
+namespace {
+
+double frobenius(const std::vector<double>& m) {
+    double s = 0.0;
+    for (double v : m)
+        s += v * v;
+    return std::sqrt(s);
+}
+
+// P_alpha = C_a,occ C_a,occ^T and likewise for beta; checks total and spin
+// against the FCHK matrices (validate_orbital_backend).
+void assess(kumo_hip_molecule& m) {
+    const int n = m.n;
+    const auto& cb = m.unrestricted ? m.Cb : m.Ca;
+    const int nmob = m.unrestricted ? m.nmo_b : m.nmo_a;
+    if (m.Ca.empty() || m.nalpha > m.nmo_a || m.nbeta > nmob || (!m.unrestricted && m.nalpha != m.nbeta)) return;
+    std::vector<double> at(static_cast<std::size_t>(m.nalpha) * n), bt(static_cast<std::size_t>(m.nbeta) * n);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < m.nalpha; ++k)
+            at[static_cast<std::size_t>(k) * n + i] = m.Ca[static_cast<std::size_t>(i) * m.nmo_a + k];
+        for (int k = 0; k < m.nbeta; ++k)
+            bt[static_cast<std::size_t>(k) * n + i] = cb[static_cast<std::size_t>(i) * nmob + k];
+    }
+    const std::size_t nn = static_cast<std::size_t>(n) * n;
+    std::vector<double> pa(nn, 0.0), pb(nn, 0.0);
+    for (int k = 0; k < m.nalpha; ++k) {
+        const double* c = at.data() + static_cast<std::size_t>(k) * n;
+        for (int i = 0; i < n; ++i) {
+            const double ci = c[i];
+            double* row = pa.data() + static_cast<std::size_t>(i) * n;
+            for (int j = 0; j < n; ++j)
+                row[j] += ci * c[j];
+        }
+    }
+    for (int k = 0; k < m.nbeta; ++k) {
+        const double* c = bt.data() + static_cast<std::size_t>(k) * n;
+        for (int i = 0; i < n; ++i) {
+            const double ci = c[i];
+            double* row = pb.data() + static_cast<std::size_t>(i) * n;
+            for (int j = 0; j < n; ++j)
+                row[j] += ci * c[j];
+        }
+    }
+    std::vector<double> rt(nn), rs(nn), et(nn), es(nn);
+    for (std::size_t q = 0; q < nn; ++q) {
+        rt[q] = pa[q] + pb[q];
+        rs[q] = pa[q] - pb[q];
+        et[q] = m.P[q] - rt[q];
+        es[q] = m.S[q] - rs[q];
+    }
+    const double te = frobenius(et), se = frobenius(es);
+    const double ts = std::max(frobenius(m.P), frobenius(rt));
+    const double ss = std::max({frobenius(m.S), frobenius(rs), ts});
+    m.total_rel = ts > 0 ? te / ts : te;
+    m.spin_rel = ss > 0 ? se / ss : se;
+    m.eligible = te <= 1e-10 + 1e-8 * ts && se <= 1e-10 + 1e-8 * ss;
+}
+
+// bf16 round to nearest even from float.
+uint16_t bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+double bf16_value(uint16_t h) {
+    const uint32_t u = static_cast<uint32_t>(h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+void upload_device(kumo_hip_molecule& m, hipStream_t s) {
+    const auto& b = m.basis;
+    m.d_shells = upload(b.shells.data(), b.shells.size(), s);
+    m.d_exps = upload(b.exps.data(), b.exps.size(), s);
+    m.d_weights = upload(b.weights.data(), b.weights.size(), s);
+    std::vector<double> tdense(static_cast<std::size_t>((kMaxL + 1) * kMaxPure * kMaxCart), 0.0);
+    for (int l = 2; l <= kMaxL; ++l) {
+        const auto t = gaussian_cartesian_to_pure(l);
+        const int nc = (l + 1) * (l + 2) / 2;
+        for (int r = 0; r < 2 * l + 1; ++r)
+            for (int k = 0; k < nc; ++k)
+                tdense[static_cast<std::size_t>((l * kMaxPure + r) * kMaxCart + k)] = t[static_cast<std::size_t>(r * nc + k)];
+    }
+    m.d_tdense = upload(tdense.data(), tdense.size(), s);
+    m.d_p[0] = upload(m.P.data(), m.P.size(), s);
+    m.d_p[1] = upload(m.S.data(), m.S.size(), s);
+    if (!m.Ca.empty()) m.d_ca = upload(m.Ca.data(), m.Ca.size(), s);
+    if (!m.Cb.empty()) m.d_cb = upload(m.Cb.data(), m.Cb.size(), s);
+    // Occupied-MO operand: alpha slots [0, A16), beta slots [A16, A16 + B16)
+    // (closed shells contract the alpha channel only).
+    const bool closed = !m.unrestricted;
+    const int a16 = (m.nalpha + 15) / 16 * 16;
+    const int b16 = closed ? 0 : (m.nbeta + 15) / 16 * 16;
+    m.alpha_slots = a16;
+    m.slots = std::max(16, a16 + b16);
+    if (!m.Ca.empty() && m.nalpha <= m.nmo_a && (closed || m.nbeta <= m.nmo_b)) {
+        const std::size_t ns = static_cast<std::size_t>(m.slots), n = static_cast<std::size_t>(m.n);
+        std::vector<double> c64(n * ns, 0.0);
+        for (std::size_t i = 0; i < n; ++i) {
+            for (int k = 0; k < m.nalpha; ++k)
+                c64[i * ns + static_cast<std::size_t>(k)] = m.Ca[i * static_cast<std::size_t>(m.nmo_a) + static_cast<std::size_t>(k)];
+            if (!closed)
+                for (int k = 0; k < m.nbeta; ++k)
+                    c64[i * ns + static_cast<std::size_t>(a16 + k)] =
+                        m.Cb[i * static_cast<std::size_t>(m.nmo_b) + static_cast<std::size_t>(k)];
+        }
+        std::vector<float> c32(c64.size());
+        std::vector<uint16_t> cbf(3 * c64.size());
+        for (std::size_t q = 0; q < c64.size(); ++q) {
+            const double v = c64[q];
+            c32[q] = static_cast<float>(v);
+            const uint16_t hi = bf16(static_cast<float>(v));
+            const double r1 = v - bf16_value(hi);
+            const uint16_t mid = bf16(static_cast<float>(r1));
+            const double r2 = r1 - bf16_value(mid);
+            cbf[q] = hi;
+            cbf[c64.size() + q] = mid;
+            cbf[2 * c64.size() + q] = bf16(static_cast<float>(r2));
+        }
+        m.d_mo64 = upload(c64.data(), c64.size(), s);
+        m.d_mo32 = upload(c32.data(), c32.size(), s);
+        m.d_mobf = upload(cbf.data(), cbf.size(), s);
+    }
+    KUMO_HIP(hipStreamSynchronize(s));
+    m.device_bytes = 0;
+    for (const DeviceBuffer* d : {&m.d_shells, &m.d_exps, &m.d_weights, &m.d_tdense, &m.d_p[0], &m.d_p[1], &m.d_ca,
+                                  &m.d_cb, &m.d_mo64, &m.d_mo32, &m.d_mobf})
+        m.device_bytes += static_cast<long long>(d->bytes());
+}
+
+std::vector<double> mo_matrix(const double* flat, int nmo, int n) {
+    // FCHK: MO-major, AO fastest -> AO rows x MO columns.
+    std::vector<double> c(static_cast<std::size_t>(n) * nmo);
+    for (int mo = 0; mo < nmo; ++mo)
+        for (int ao = 0; ao < n; ++ao)
+            c[static_cast<std::size_t>(ao) * nmo + mo] = flat[static_cast<std::size_t>(mo) * n + ao];
+    return c;
+}
+
+} // namespace
+
+std::unique_ptr<kumo_hip_molecule> load_molecule(kumo_hip_context* ctx, const kumo_hip_molecule_desc& d) {
+    auto m = std::make_unique<kumo_hip_molecule>();
+    m->ctx = ctx;
+    if (d.natoms <= 0 || !d.atomic_numbers || !d.coordinates) runtime("invalid FCHK geometry");
+    for (int a = 0; a < d.natoms; ++a) {
+        const int z = d.atomic_numbers[a];
+        const double q = d.nuclear_charges ? d.nuclear_charges[a] : static_cast<double>(z);
+        if (z <= 0 || z > 118 || !std::isfinite(d.coordinates[3 * a]) || !std::isfinite(d.coordinates[3 * a + 1]) ||
+            !std::isfinite(d.coordinates[3 * a + 2]))
+            runtime("invalid FCHK atom or coordinates");
+        if (!std::isfinite(q) || q < 0.0) runtime("invalid FCHK nuclear charge");
+        m->atoms.push_back({z, q, {d.coordinates[3 * a], d.coordinates[3 * a + 1], d.coordinates[3 * a + 2]}});
+    }
+    ShellRecords r;
+    r.nshells = d.nshells;
+    r.types = d.shell_types;
+    r.nprim = d.primitives_per_shell;
+    r.atom = d.shell_to_atom;
+    r.xyz = d.shell_coordinates;
+    r.exps = d.exponents;
+    r.coef = d.coefficients;
+    r.pcoef = d.p_coefficients;
+    r.natoms = d.natoms;
+    r.nbasis = d.nbasis;
+    m->basis = prepare_basis(r);
+    m->n = m->basis.nao;
+    const std::size_t nn = static_cast<std::size_t>(m->n) * m->n;
+    if (!d.p_total) runtime("missing fchk record: Total SCF Density");
+    m->P.assign(d.p_total, d.p_total + nn);
+    if (d.p_spin) m->S.assign(d.p_spin, d.p_spin + nn);
+    else if (d.nalpha != d.nbeta) runtime("open-shell FCHK lacks Spin SCF Density");
+    else m->S.assign(nn, 0.0);
+    for (double v : m->P)
+        if (!std::isfinite(v)) runtime("nonfinite AO density matrix");
+    if (d.nalpha < 0 || d.nbeta < 0 || d.nalpha > m->n || d.nbeta > m->n) runtime("invalid electron counts in FCHK");
+    m->nalpha = d.nalpha;
+    m->nbeta = d.nbeta;
+    if (d.c_alpha && d.nmo_alpha > 0) {
+        if (d.nmo_alpha > m->n) runtime("invalid MO coefficient dimensions: Alpha MO coefficients");
+        m->Ca = mo_matrix(d.c_alpha, d.nmo_alpha, m->n);
+        m->nmo_a = d.nmo_alpha;
+    }
+    if (d.c_beta && d.nmo_beta > 0) {
+        if (d.nmo_beta > m->n) runtime("invalid MO coefficient dimensions: Beta MO coefficients");
+        m->Cb = mo_matrix(d.c_beta, d.nmo_beta, m->n);
+        m->nmo_b = d.nmo_beta;
+    }
+    m->unrestricted = m->nmo_b != 0;
+    if (m->nalpha != m->nbeta && !m->unrestricted) runtime("open-shell FCHK lacks Beta MO coefficients");
+    if (d.e_alpha && m->nmo_a) m->Ea.assign(d.e_alpha, d.e_alpha + m->nmo_a);
+    if (d.e_beta && m->nmo_b) m->Eb.assign(d.e_beta, d.e_beta + m->nmo_b);
+    assess(*m);
+    upload_device(*m, ctx->stream);
+    return m;
+}
+
+std::unique_ptr<kumo_hip_molecule> load_fchk(kumo_hip_context* ctx, const std::string& path) {
+    const Fchk f(path);
+    kumo_hip_molecule_desc d{};
+    const auto& z = f.integers("Atomic numbers");
+    std::vector<int32_t> zi(z.begin(), z.end());
+    const auto& xyz = f.reals("Current cartesian coordinates");
+    if (xyz.size() != 3 * zi.size() || zi.empty()) runtime("invalid FCHK geometry");
+    std::vector<double> charges;
+    if (f.contains("Nuclear charges")) {
+        charges = f.reals("Nuclear charges");
+        if (charges.size() != zi.size()) runtime("invalid FCHK nuclear charge count");
+    } else {
+        for (const char* label : {"ECP-LMax", "ECP-LPSkip", "ECP-RNFroz", "ECP-NLP", "ECP-CLP1", "ECP-CLP2",
+                                  "ECP-ZLP", "ECP-KFirst", "ECP-KLast"})
+            if (f.contains(label))
+                runtime("FCHK has an effective core potential (ECP) but no \"Nuclear charges\" record, so the "
+                        "effective nuclear charges are unknown");
+    }
+    const auto& types = f.integers("Shell types");
+    const auto& nprim = f.integers("Number of primitives per shell");
+    const auto& centers = f.integers("Shell to atom map");
+    if (types.size() != nprim.size() || types.size() != centers.size()) runtime("inconsistent shell arrays in fchk");
+    std::vector<int32_t> ti(types.begin(), types.end()), ni(nprim.begin(), nprim.end()), ci(centers.begin(), centers.end());
+    const auto& exps = f.reals("Primitive exponents");
+    const auto& coef = f.reals("Contraction coefficients");
+    const std::vector<double>* pc = f.contains("P(S=P) Contraction coefficients") ? &f.reals("P(S=P) Contraction coefficients") : nullptr;
+    const auto& sxyz = f.reals("Coordinates of each shell");
+    if (sxyz.size() != 3 * ti.size()) runtime("invalid shell-coordinate array in fchk");
+    if (exps.size() != coef.size() || (pc && pc->size() != exps.size())) runtime("inconsistent primitive arrays in fchk");
+    long long total_prims = 0;
+    for (auto v : nprim) {
+        if (v <= 0) runtime("invalid primitive count in fchk shell");
+        total_prims += v;
+    }
+    if (static_cast<std::size_t>(total_prims) != exps.size()) runtime("unused primitive data after parsing fchk basis");
+    const int n = static_cast<int>(f.integer("Number of basis functions"));
+    if (n <= 0) runtime("invalid Number of basis functions");
+    const auto unpack = [&](const std::vector<double>& packed) {
+        const std::size_t nn = static_cast<std::size_t>(n);
+        if (packed.size() != nn * (nn + 1) / 2) runtime("invalid packed AO density matrix size");
+        std::vector<double> p(nn * nn);
+        std::size_t q = 0;
+        for (std::size_t i = 0; i < nn; ++i)
+            for (std::size_t j = 0; j <= i; ++j, ++q)
+                p[i * nn + j] = p[j * nn + i] = packed[q];
+        return p;
+    };
+    const auto P = unpack(f.reals("Total SCF Density"));
+    std::vector<double> S;
+    if (f.contains("Spin SCF Density")) S = unpack(f.reals("Spin SCF Density"));
+    if (!f.contains("Number of alpha electrons") || !f.contains("Number of beta electrons"))
+        runtime("missing electron counts in FCHK");
+    d.natoms = static_cast<int32_t>(zi.size());
+    d.atomic_numbers = zi.data();
+    d.nuclear_charges = charges.empty() ? nullptr : charges.data();
+    d.coordinates = xyz.data();
+    d.nshells = static_cast<int32_t>(ti.size());
+    d.shell_types = ti.data();
+    d.primitives_per_shell = ni.data();
+    d.shell_to_atom = ci.data();
+    d.shell_coordinates = sxyz.data();
+    d.exponents = exps.data();
+    d.coefficients = coef.data();
+    d.p_coefficients = pc ? pc->data() : nullptr;
+    d.nbasis = n;
+    d.p_total = P.data();
+    d.p_spin = S.empty() ? nullptr : S.data();
+    d.nalpha = static_cast<int32_t>(f.integer("Number of alpha electrons"));
+    d.nbeta = static_cast<int32_t>(f.integer("Number of beta electrons"));
+    const auto mo = [&](const char* label, int32_t& nmo) -> const double* {
+        if (!f.contains(label)) return nullptr;
+        const auto& v = f.reals(label);
+        if (v.size() % static_cast<std::size_t>(n) != 0 || v.size() / static_cast<std::size_t>(n) > static_cast<std::size_t>(n))
+            runtime(std::string("invalid MO coefficient dimensions: ") + label);
+        nmo = static_cast<int32_t>(v.size() / static_cast<std::size_t>(n));
+        return v.data();
+    };
+    d.c_alpha = mo("Alpha MO coefficients", d.nmo_alpha);
+    d.c_beta = mo("Beta MO coefficients", d.nmo_beta);
+    if (f.contains("Alpha Orbital Energies")) d.e_alpha = f.reals("Alpha Orbital Energies").data();
+    if (f.contains("Beta Orbital Energies")) d.e_beta = f.reals("Beta Orbital Energies").data();
+    return load_molecule(ctx, d);
+}
+
+// Boys table F_m(i * 0.05), m < 39, i < 801 (src/boys_function.cpp:20-44).
+std::vector<double> boys_table_host() {
+    constexpr int orders = 39, points = 801, top = orders - 1;
+    std::vector<double> t(static_cast<std::size_t>(orders * points));
+    for (int i = 0; i < points; ++i) {
+        const double T = i * 0.05;
+        const double decay = std::exp(-T);
+        double term = 1.0 / (2.0 * top + 1.0), sum = term;
+        for (int k = 1; k < 10000 && term > 1e-18 * sum; ++k) {
+            term *= 2.0 * T / (2.0 * top + 2.0 * k + 1.0);
+            sum += term;
+        }
+        double* row = t.data() + static_cast<std::size_t>(i * orders);
+        row[top] = decay * sum;
+        for (int m = top; m > 0; --m)
+            row[m - 1] = (2.0 * T * row[m] + decay) / (2.0 * m - 1.0);
+    }
+    return t;
+}
+
+} // namespace kumo
+
+const double* kumo_hip_molecule::p_matrix(int component) {
+    using namespace kumo;
+    if (component <= 1) return d_p[component].as<double>();
+    if (!d_p[component].get()) {
+        std::vector<double> m(P.size());
+        for (std::size_t q = 0; q < P.size(); ++q)
+            m[q] = 0.5 * (component == 2 ? (P[q] + S[q]) : (P[q] - S[q]));
+        d_p[component] = upload(m.data(), m.size(), ctx->stream);
+        KUMO_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return d_p[component].as<double>();
+}
+
+// Primitive pairs of the exact ESP: P contracted with the Hermite expansion
+// coefficients of every pair of Cartesian primitives of blocks a <= b
+// (src/electrostatic_potential.cpp:117-229), grouped by L = la + lb.
+void kumo_hip_molecule::ensure_esp() {
+    using namespace kumo;
+    if (esp_ready) return;
+    struct Triple {
+        int t, u, v;
+    };
+    const auto triples = [](int L) {
+        std::vector<Triple> out;
+        for (int s = 0; s < L + 1; ++s)
+            for (int t = s; t >= 0; --t)
+                for (int u = s - t; u >= 0; --u)
+                    out.push_back({t, u, s - t - u});
+        return out;
+    };
+    struct Herm {
+        int lb, width;
+        std::vector<double> v;
+        Herm(int la, int lb_, double p, double pa, double pb) : lb(lb_), width(la + lb_ + 1) {
+            v.assign(static_cast<std::size_t>((la + 1) * (lb + 1) * width), 0.0);
+            const double half = 0.5 / p;
+            at(0, 0, 0) = 1.0;
+            const auto step = [&](int si, int sj, int ti, int tj, double shift) {
+                for (int t = 0; t <= ti + tj; ++t) {
+                    double value = shift * get(si, sj, t);
+                    if (t > 0) value += half * get(si, sj, t - 1);
+                    if (t + 1 <= si + sj) value += (t + 1) * get(si, sj, t + 1);
+                    at(ti, tj, t) = value;
+                }
+            };
+            for (int i = 0; i < la; ++i)
+                step(i, 0, i + 1, 0, pa);
+            for (int i = 0; i <= la; ++i)
+                for (int j = 0; j < lb; ++j)
+                    step(i, j, i, j + 1, pb);
+        }
+        double get(int i, int j, int t) const {
+            return (t < 0 || t > i + j) ? 0.0 : v[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)];
+        }
+        double& at(int i, int j, int t) { return v[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)]; }
+    };
+    std::vector<std::vector<EspPairDev>> groups(2 * kMaxL + 1);
+    std::vector<std::vector<double>> gcoef(2 * kMaxL + 1);
+    const auto& blocks = basis.blocks;
+    const int nn = n;
+    for (std::size_t ia = 0; ia < blocks.size(); ++ia) {
+        const auto& A = blocks[ia];
+        for (std::size_t ib = ia; ib < blocks.size(); ++ib) {
+            const auto& B = blocks[ib];
+            const int na = A.ncart(), nb = B.ncart();
+            std::vector<double> left(static_cast<std::size_t>(na * B.count));
+            for (int i = 0; i < na; ++i)
+                for (int c = 0; c < B.count; ++c) {
+                    double acc;
+                    if (A.pure.empty()) {
+                        acc = P[static_cast<std::size_t>(A.offset + i) * nn + static_cast<std::size_t>(B.offset + c)];
+                    } else {
+                        acc = 0.0;
+                        for (int r = 0; r < A.count; ++r)
+                            acc += A.pure[static_cast<std::size_t>(r * na + i)] *
+                                   P[static_cast<std::size_t>(A.offset + r) * nn + static_cast<std::size_t>(B.offset + c)];
+                    }
+                    left[static_cast<std::size_t>(i * B.count + c)] = acc;
+                }
+            std::vector<double> cart(static_cast<std::size_t>(na * nb));
+            bool any = false;
+            for (int i = 0; i < na; ++i)
+                for (int j = 0; j < nb; ++j) {
+                    double v;
+                    if (B.pure.empty()) {
+                        v = left[static_cast<std::size_t>(i * B.count + j)];
+                    } else {
+                        v = 0.0;
+                        for (int c = 0; c < B.count; ++c)
+                            v += left[static_cast<std::size_t>(i * B.count + c)] * B.pure[static_cast<std::size_t>(c * nb + j)];
+                    }
+                    if (ia != ib) v *= 2.0;
+                    cart[static_cast<std::size_t>(i * nb + j)] = v;
+                    any = any || v != 0.0;
+                }
+            if (!any) continue;
+            const int la = A.l, lb = B.l, L = la + lb;
+            const auto list = triples(L);
+            const double ab[3] = {A.origin[0] - B.origin[0], A.origin[1] - B.origin[1], A.origin[2] - B.origin[2]};
+            const double ab2 = (ab[0] * ab[0] + ab[1] * ab[1]) + ab[2] * ab[2];
+            std::vector<double> pc(list.size());
+            for (int p = 0; p < A.nprim(); ++p)
+                for (int q = 0; q < B.nprim(); ++q) {
+                    const double al = A.exps[static_cast<std::size_t>(p)], be = B.exps[static_cast<std::size_t>(q)];
+                    const double ex = al + be;
+                    double Q[3];
+                    for (int a = 0; a < 3; ++a)
+                        Q[a] = (al * A.origin[a] + be * B.origin[a]) / ex;
+                    const double scale = std::exp(-al * be / ex * ab2) * 2.0 * std::numbers::pi / ex;
+                    const Herm hx(la, lb, ex, Q[0] - A.origin[0], Q[0] - B.origin[0]);
+                    const Herm hy(la, lb, ex, Q[1] - A.origin[1], Q[1] - B.origin[1]);
+                    const Herm hz(la, lb, ex, Q[2] - A.origin[2], Q[2] - B.origin[2]);
+                    std::fill(pc.begin(), pc.end(), 0.0);
+                    for (int i = 0; i < na; ++i) {
+                        const auto& pi = A.powers[static_cast<std::size_t>(i)];
+                        for (int j = 0; j < nb; ++j) {
+                            const auto& pj = B.powers[static_cast<std::size_t>(j)];
+                            const double w = cart[static_cast<std::size_t>(i * nb + j)] *
+                                             A.weights[static_cast<std::size_t>(i * A.nprim() + p)] *
+                                             B.weights[static_cast<std::size_t>(j * B.nprim() + q)];
+                            if (w == 0.0) continue;
+                            for (std::size_t k = 0; k < list.size(); ++k) {
+                                const auto& h = list[k];
+                                if (h.t > pi.x + pj.x || h.u > pi.y + pj.y || h.v > pi.z + pj.z) continue;
+                                pc[k] += w * hx.get(pi.x, pj.x, h.t) * hy.get(pi.y, pj.y, h.u) * hz.get(pi.z, pj.z, h.v);
+                            }
+                        }
+                    }
+                    if (std::all_of(pc.begin(), pc.end(), [](double c) { return c == 0.0; })) continue;
+                    auto& g = groups[static_cast<std::size_t>(L)];
+                    auto& gc = gcoef[static_cast<std::size_t>(L)];
+                    g.push_back({ex, Q[0], Q[1], Q[2], L, static_cast<int32_t>(gc.size())});
+                    for (double c : pc)
+                        gc.push_back(scale * c);
+                }
+        }
+    }
+    std::vector<EspPairDev> pairs;
+    std::vector<double> coef;
+    for (std::size_t L = 0; L < groups.size(); ++L) {
+        const int base = static_cast<int>(coef.size());
+        for (auto p : groups[L]) {
+            p.first += base;
+            pairs.push_back(p);
+        }
+        coef.insert(coef.end(), gcoef[L].begin(), gcoef[L].end());
+    }
+    std::vector<EspAtomDev> at;
+    for (const auto& a : atoms)
+        at.push_back({a.xyz[0], a.xyz[1], a.xyz[2], a.charge});
+    d_esp_pairs = upload(pairs.data(), pairs.size(), ctx->stream);
+    d_esp_coef = upload(coef.data(), coef.size(), ctx->stream);
+    d_esp_atoms = upload(at.data(), at.size(), ctx->stream);
+    esp_npairs = static_cast<int>(pairs.size());
+    KUMO_HIP(hipStreamSynchronize(ctx->stream));
+    esp_ready = true;
+}

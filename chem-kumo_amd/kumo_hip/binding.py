@@ -1,0 +1,279 @@
+"""ctypes binding of the C-ABI in include/kumo_hip.h (libkumo_hip.so, built
+in-tree by chem-kumo_amd/Makefile). The library is the product: there is no
+CPU fallback, and every entry point raises if the library is missing."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .api import Backend, Precision, channels
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libkumo_hip.so")
+_lib = None
+
+STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: RuntimeError, 5: RuntimeError}
+
+
+class _Property(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("component", C.c_int32), ("channel", C.c_int32), ("orbital_index", C.c_int32)]
+
+
+class _Grid(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("steps", C.c_double * 9), ("counts", C.c_int32 * 3)]
+
+
+PROGRESS = C.CFUNCTYPE(C.c_int, C.c_size_t, C.c_size_t, C.c_void_p)
+
+
+class _Options(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("block_size", C.c_int32), ("backend", C.c_int32), ("precision", C.c_int32),
+                ("screening_threshold", C.c_double), ("potential_tolerance", C.c_double), ("fukui_delta", C.c_double),
+                ("device_outputs", C.c_int32), ("reserved", C.c_int32), ("progress", PROGRESS),
+                ("progress_user", C.c_void_p)]
+
+
+class _Desc(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("atomic_numbers", C.c_void_p), ("nuclear_charges", C.c_void_p),
+                ("coordinates", C.c_void_p), ("nshells", C.c_int32), ("shell_types", C.c_void_p),
+                ("primitives_per_shell", C.c_void_p), ("shell_to_atom", C.c_void_p),
+                ("shell_coordinates", C.c_void_p), ("exponents", C.c_void_p), ("coefficients", C.c_void_p),
+                ("p_coefficients", C.c_void_p), ("nbasis", C.c_int32), ("p_total", C.c_void_p),
+                ("p_spin", C.c_void_p), ("nalpha", C.c_int32), ("nbeta", C.c_int32), ("nmo_alpha", C.c_int32),
+                ("nmo_beta", C.c_int32), ("c_alpha", C.c_void_p), ("c_beta", C.c_void_p), ("e_alpha", C.c_void_p),
+                ("e_beta", C.c_void_p)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("nbasis", C.c_int32), ("nblocks", C.c_int32), ("max_l", C.c_int32),
+                ("nalpha", C.c_int32), ("nbeta", C.c_int32), ("nmo_alpha", C.c_int32), ("nmo_beta", C.c_int32),
+                ("unrestricted", C.c_int32), ("ao_mo_eligible", C.c_int32), ("total_relative_error", C.c_double),
+                ("spin_relative_error", C.c_double), ("device_bytes", C.c_int64)]
+
+
+EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip_default_options",
+           "kumo_hip_molecule_from_fchk", "kumo_hip_molecule_create", "kumo_hip_molecule_info_get",
+           "kumo_hip_molecule_release", "kumo_hip_molecule_blocks", "kumo_hip_molecule_atoms",
+           "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
+           "kumo_hip_boys", "kumo_hip_last_kernel_stats"]
+
+
+def lib():
+    """Loads libkumo_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError(f"{LIBRARY} is missing: build it with `make -C chem-kumo_amd` "
+                               "(the HIP backend has no CPU fallback)")
+        L = C.CDLL(LIBRARY)
+        vp = C.c_void_p
+        L.kumo_hip_init.argtypes = [C.c_int, C.POINTER(vp)]
+        L.kumo_hip_release.argtypes = [vp]
+        L.kumo_hip_last_error.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.kumo_hip_default_options.argtypes = [C.POINTER(_Options)]
+        L.kumo_hip_default_options.restype = None
+        L.kumo_hip_molecule_from_fchk.argtypes = [vp, C.c_char_p, C.POINTER(vp)]
+        L.kumo_hip_molecule_create.argtypes = [vp, C.POINTER(_Desc), C.POINTER(vp)]
+        L.kumo_hip_molecule_info_get.argtypes = [vp, C.POINTER(_Info)]
+        L.kumo_hip_molecule_release.argtypes = [vp]
+        L.kumo_hip_molecule_blocks.argtypes = [vp, vp, vp, vp, vp]
+        L.kumo_hip_molecule_atoms.argtypes = [vp, vp, vp, vp]
+        L.kumo_hip_evaluate_grid.argtypes = [vp, vp, C.POINTER(_Grid), vp, C.c_size_t, C.POINTER(_Options), vp, vp]
+        L.kumo_hip_evaluate_points.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(_Options), vp, vp]
+        L.kumo_hip_ao_eval.argtypes = [vp, vp, vp, C.c_size_t, C.c_int32, vp]
+        L.kumo_hip_screening_masks.argtypes = [vp, vp, vp, vp, C.c_size_t, C.c_int32, C.c_int32, C.c_double, vp]
+        L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
+        L.kumo_hip_last_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device, one stream (kumo_hip_init)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().kumo_hip_init(device, C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"kumo_hip_init failed with status {rc}")
+
+    def check(self, rc):
+        if rc != 0:
+            buf = C.create_string_buffer(1024)
+            lib().kumo_hip_last_error(self.h, buf, 1024)
+            raise STATUS.get(rc, RuntimeError)(buf.value.decode())
+
+    def close(self):
+        if self.h:
+            lib().kumo_hip_release(self.h)
+            self.h = C.c_void_p()
+
+    def boys(self, T, m):
+        T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+        out = np.zeros((T.size, m + 1))
+        self.check(lib().kumo_hip_boys(self.h, _p(T), T.size, m, _p(out)))
+        return out
+
+    def kernel_stats(self):
+        ms, fl, pfl = C.c_double(), C.c_double(), C.c_double()
+        n = C.c_int64()
+        lib().kumo_hip_last_kernel_stats(self.h, C.byref(ms), C.byref(n), C.byref(fl), C.byref(pfl))
+        return {"kernel_ms": ms.value, "launches": n.value, "flops": fl.value, "padded_flops": pfl.value}
+
+
+def grid_struct(grid):
+    g = _Grid()
+    o = np.asarray(grid["origin"], dtype=np.float64)
+    st = np.asarray(grid["steps"], dtype=np.float64)  # columns are axis steps
+    for i in range(3):
+        g.origin[i] = o[i]
+        g.counts[i] = int(grid["counts"][i])
+    colmajor = st.T.ravel()
+    for i in range(9):
+        g.steps[i] = colmajor[i]
+    return g
+
+
+def options(block_size=128, backend=Backend.AUTO, precision=Precision.FP64, screening_threshold=1e-12,
+            potential_tolerance=1e-8, fukui_delta=0.1, device_outputs=False, progress=None, threads=0):
+    o = _Options()
+    lib().kumo_hip_default_options(C.byref(o))
+    o.threads = threads
+    o.block_size = block_size
+    o.backend = int(backend)
+    o.precision = int(precision)
+    o.screening_threshold = screening_threshold
+    o.potential_tolerance = potential_tolerance
+    o.fukui_delta = fukui_delta
+    o.device_outputs = 1 if device_outputs else 0
+    if progress is not None:
+        cb = PROGRESS(lambda d, t, u: int(bool(progress(d, t))))
+        o.progress = cb
+        o._cb = cb
+    return o
+
+
+class Molecule:
+    """A molecule resident in HBM (kumo_hip_molecule_from_fchk / _create)."""
+
+    def __init__(self, ctx, fchk_path=None, data=None):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        if fchk_path is not None:
+            ctx.check(lib().kumo_hip_molecule_from_fchk(ctx.h, os.fsencode(fchk_path), C.byref(self.h)))
+        else:
+            self._create(data)
+        info = _Info()
+        lib().kumo_hip_molecule_info_get(self.h, C.byref(info))
+        self.info = {f: getattr(info, f) for f, _ in _Info._fields_}
+        self.nao = info.nbasis
+
+    def _create(self, d):
+        keep = []
+
+        def arr(x, t=np.float64):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(x, dtype=t)
+            keep.append(a)
+            return a.ctypes.data
+        desc = _Desc()
+        desc.natoms = len(d.atomic_numbers)
+        desc.atomic_numbers = arr(d.atomic_numbers, np.int32)
+        desc.nuclear_charges = arr(d.charges)
+        desc.coordinates = arr(d.xyz)
+        desc.nshells = len(d.shell_types)
+        desc.shell_types = arr(d.shell_types, np.int32)
+        desc.primitives_per_shell = arr(d.nprim, np.int32)
+        desc.shell_to_atom = arr(d.shell_atom, np.int32)
+        desc.shell_coordinates = arr(d.shell_xyz)
+        desc.exponents = arr(d.exponents)
+        desc.coefficients = arr(d.coefficients)
+        desc.p_coefficients = arr(d.p_coefficients)
+        desc.nbasis = d.nbasis
+        desc.p_total = arr(d.p_total)
+        desc.p_spin = arr(d.p_spin)
+        desc.nalpha, desc.nbeta = d.nalpha, d.nbeta
+        desc.c_alpha = arr(d.c_alpha)
+        desc.c_beta = arr(d.c_beta)
+        desc.nmo_alpha = 0 if d.c_alpha is None else len(d.c_alpha) // d.nbasis
+        desc.nmo_beta = 0 if d.c_beta is None else len(d.c_beta) // d.nbasis
+        desc.e_alpha = arr(d.e_alpha)
+        desc.e_beta = arr(d.e_beta)
+        self.ctx.check(lib().kumo_hip_molecule_create(self.ctx.h, C.byref(desc), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().kumo_hip_molecule_release(self.h)
+            self.h = C.c_void_p()
+
+    def blocks(self):
+        nb = self.info["nblocks"]
+        l, off, cnt = (np.zeros(nb, np.int32) for _ in range(3))
+        r2 = np.zeros(nb)
+        lib().kumo_hip_molecule_blocks(self.h, _p(l), _p(off), _p(cnt), _p(r2))
+        return l, off, cnt, r2
+
+    def ao(self, points, order=0):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        panels = {0: 1, 1: 4, 2: 10}[order]
+        out = np.zeros((panels * pts.shape[0], self.nao))
+        self.ctx.check(lib().kumo_hip_ao_eval(self.ctx.h, self.h, _p(pts), pts.shape[0], order, _p(out)))
+        return out.T.copy()
+
+    def masks(self, block_size, ao_order, grid=None, points=None, threshold=1e-12):
+        if grid is not None:
+            g = grid_struct(grid)
+            n = int(np.prod(np.asarray(grid["counts"], dtype=np.int64)))
+            nb = (n + block_size - 1) // block_size
+            out = np.zeros((nb, self.nao), np.uint8)
+            self.ctx.check(lib().kumo_hip_screening_masks(self.ctx.h, self.h, C.byref(g), None, 0, block_size,
+                                                          ao_order, threshold, _p(out)))
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            nb = (pts.shape[0] + block_size - 1) // block_size
+            out = np.zeros((nb, self.nao), np.uint8)
+            self.ctx.check(lib().kumo_hip_screening_masks(self.ctx.h, self.h, None, _p(pts), pts.shape[0],
+                                                          block_size, ao_order, threshold, _p(out)))
+        return out
+
+    def evaluate(self, props, grid=None, points=None, **opts):
+        """props: sequence of (kind, component, channel, orbital_index). Returns
+        (list of arrays channels x n, backend_used)."""
+        P = (_Property * max(1, len(props)))(*[_Property(*map(int, p)) for p in props])
+        o = options(**opts)
+        if grid is not None:
+            n = int(np.prod(np.asarray(grid["counts"], dtype=np.int64)))
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            n = pts.shape[0]
+        outs = [np.zeros((channels(p[0]), n)) for p in props]
+        ptrs = [o_[c].ctypes.data for o_ in outs for c in range(o_.shape[0])]
+        arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
+        used = C.c_int32(-1)
+        if grid is not None:
+            g = grid_struct(grid)
+            rc = lib().kumo_hip_evaluate_grid(self.ctx.h, self.h, C.byref(g), P, len(props), C.byref(o), arr,
+                                              C.byref(used))
+        else:
+            rc = lib().kumo_hip_evaluate_points(self.ctx.h, self.h, _p(pts), n, P, len(props), C.byref(o), arr,
+                                                C.byref(used))
+        self.ctx.check(rc)
+        return outs, used.value
+
+    def evaluate_device(self, props, grid, outputs, **opts):
+        """Evaluation into device buffers (torch tensors or raw pointers);
+        nothing is copied to the host."""
+        P = (_Property * max(1, len(props)))(*[_Property(*map(int, p)) for p in props])
+        o = options(device_outputs=True, **opts)
+        ptrs = [int(x) for x in outputs]
+        arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
+        used = C.c_int32(-1)
+        g = grid_struct(grid)
+        self.ctx.check(lib().kumo_hip_evaluate_grid(self.ctx.h, self.h, C.byref(g), P, len(props), C.byref(o), arr,
+                                                    C.byref(used)))
+        return used.value

@@ -1,0 +1,194 @@
+/*
+ * kumo_hip — C-ABI of the MI355X (gfx950) FCHK->grid field evaluator.
+ *
+ * The drop-in boundary for chem-kumo's evaluation path. Every entry point is
+ * extern "C", takes plain pointers and sizes, and returns a status code
+ * (KUMO_HIP_OK on success). Status codes map one to one onto the exception
+ * types the reference throws (include/chem/kumo/evaluator.hpp:497-506):
+ * INVALID_ARGUMENT -> std::invalid_argument, OUT_OF_RANGE -> std::out_of_range,
+ * RUNTIME_ERROR / DEVICE_ERROR -> std::runtime_error. The message of the last
+ * failure on a context is returned by kumo_hip_last_error().
+ *
+ * Property keys, density components, spin channels and backends use the
+ * numbering of the reference enums (include/chem/kumo/evaluator.hpp:112-150,
+ * 403; include/chem/kumo/spin.hpp:8-12).
+ */
+#ifndef KUMO_HIP_H
+#define KUMO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum kumo_hip_status {
+    KUMO_HIP_OK = 0,
+    KUMO_HIP_INVALID_ARGUMENT = 1,
+    KUMO_HIP_OUT_OF_RANGE = 2,
+    KUMO_HIP_RUNTIME_ERROR = 3,
+    KUMO_HIP_DEVICE_ERROR = 4,
+    KUMO_HIP_ABORTED = 5
+};
+
+/* Backend {AUTO, AO_P, AO_MO} (include/chem/kumo/evaluator.hpp:403): the
+ * density contraction; every one of them runs on the GPU here. */
+enum kumo_hip_backend { KUMO_HIP_AUTO = 0, KUMO_HIP_AO_P = 1, KUMO_HIP_AO_MO = 2 };
+
+/* Arithmetic of the dense contractions (AO->MO and AO.P). AO values, the
+ * screening masks, the per-point reductions, the derived fields, the Boys
+ * function and the ESP are FP64 in every mode. */
+enum kumo_hip_precision {
+    KUMO_HIP_FP64 = 0,
+    KUMO_HIP_FP32 = 1,
+    KUMO_HIP_BF16X3 = 2, /* hi/lo bf16 split, 3 MFMA products, FP32 accumulation */
+    KUMO_HIP_BF16X6 = 3  /* hi/mid/lo bf16 split, 6 MFMA products, FP32 accumulation */
+};
+
+typedef struct kumo_hip_context kumo_hip_context;
+typedef struct kumo_hip_molecule kumo_hip_molecule;
+
+/* Property key (include/chem/kumo/evaluator.hpp:166-180). orbital_index is
+ * one-based and used by the Orbital kinds only. */
+typedef struct kumo_hip_property {
+    int32_t kind;
+    int32_t component;
+    int32_t channel;
+    int32_t orbital_index;
+} kumo_hip_property;
+
+/* Regular grid r(i,j,k) = origin + steps * (i,j,k), k fastest
+ * (include/chem/kumo/density.hpp:61-72). steps is column-major: column a
+ * (steps[3a..3a+2]) is the step vector of axis a. */
+typedef struct kumo_hip_grid {
+    double origin[3];
+    double steps[9];
+    int32_t counts[3];
+} kumo_hip_grid;
+
+/* Progress report (include/chem/kumo/density.hpp:34-39); a nonzero return
+ * aborts the evaluation with KUMO_HIP_ABORTED. */
+typedef int (*kumo_hip_progress_fn)(size_t done, size_t total, void* user);
+
+/* EvaluationOptions (include/chem/kumo/evaluator.hpp:406-441) plus the
+ * precision of the contractions. Zero-initialise, then call
+ * kumo_hip_default_options(). */
+typedef struct kumo_hip_options {
+    int32_t threads;            /* 0 or 1, as the reference */
+    int32_t block_size;         /* 1..65536, points per screening block */
+    int32_t backend;            /* enum kumo_hip_backend */
+    int32_t precision;          /* enum kumo_hip_precision */
+    double screening_threshold; /* >= 0; 0 keeps every AO */
+    double potential_tolerance; /* >= 0; the ESP is evaluated exactly */
+    double fukui_delta;         /* > 0 */
+    int32_t device_outputs;     /* 1: outputs are device pointers (HBM) */
+    int32_t reserved;
+    kumo_hip_progress_fn progress;
+    void* progress_user;
+} kumo_hip_options;
+
+/* Molecule from raw FCHK records (src/basis.cpp:21-120, src/molecule.cpp:153-180).
+ * Matrices are row-major n x n (full, symmetric); MO coefficients as in the
+ * FCHK record: MO-major with the AO index fastest (src/mo_coefficients.hpp:21-34).
+ * c_beta is NULL for a restricted calculation; p_spin may be NULL (zero);
+ * energies may be NULL. */
+typedef struct kumo_hip_molecule_desc {
+    int32_t natoms;
+    const int32_t* atomic_numbers;
+    const double* nuclear_charges; /* NULL: the atomic numbers */
+    const double* coordinates;     /* 3 * natoms, bohr */
+    int32_t nshells;
+    const int32_t* shell_types;    /* FCHK "Shell types": -1 SP, <= -2 pure */
+    const int32_t* primitives_per_shell;
+    const int32_t* shell_to_atom;  /* one-based */
+    const double* shell_coordinates; /* 3 * nshells */
+    const double* exponents;
+    const double* coefficients;
+    const double* p_coefficients;  /* SP shells; may be NULL without SP shells */
+    int32_t nbasis;
+    const double* p_total;
+    const double* p_spin;
+    int32_t nalpha, nbeta;
+    int32_t nmo_alpha, nmo_beta;
+    const double* c_alpha;
+    const double* c_beta;
+    const double* e_alpha;
+    const double* e_beta;
+} kumo_hip_molecule_desc;
+
+typedef struct kumo_hip_molecule_info {
+    int32_t natoms, nbasis, nblocks, max_l;
+    int32_t nalpha, nbeta, nmo_alpha, nmo_beta;
+    int32_t unrestricted;
+    int32_t ao_mo_eligible;             /* ElectronicStructureInfo::ao_mo_eligible() */
+    double total_relative_error, spin_relative_error;
+    int64_t device_bytes;               /* HBM held by the molecule */
+} kumo_hip_molecule_info;
+
+/* ---- context (replaces nothing in the reference: it owns the device) ---- */
+int kumo_hip_init(int device, kumo_hip_context** ctx);
+int kumo_hip_release(kumo_hip_context* ctx);
+/* Copy of the last error message of the context (NUL-terminated). */
+int kumo_hip_last_error(const kumo_hip_context* ctx, char* buffer, size_t length);
+void kumo_hip_default_options(kumo_hip_options* options);
+
+/* ---- molecule residency ----
+ * Molecule::from_fchk (include/chem/kumo/density.hpp:215-221): reads a plain
+ * or gzip FCHK, prepares the basis (BasisEvaluator ctor,
+ * src/basis_evaluator.cpp:177-202), checks the MOs against P
+ * (src/backend_selection.hpp:43-78) and keeps basis, P and C in HBM. */
+int kumo_hip_molecule_from_fchk(kumo_hip_context* ctx, const char* path, kumo_hip_molecule** mol);
+/* The same from records in memory. */
+int kumo_hip_molecule_create(kumo_hip_context* ctx, const kumo_hip_molecule_desc* desc,
+                             kumo_hip_molecule** mol);
+int kumo_hip_molecule_info_get(const kumo_hip_molecule* mol, kumo_hip_molecule_info* info);
+int kumo_hip_molecule_release(kumo_hip_molecule* mol);
+/* Prepared AO blocks (src/basis_evaluator.cpp:177-202): per block the angular
+ * momentum, first AO, AO count and squared cutoff radius. Arrays of nblocks. */
+int kumo_hip_molecule_blocks(const kumo_hip_molecule* mol, int32_t* l, int32_t* offset, int32_t* count,
+                             double* cutoff_r2);
+/* Atoms: atomic numbers, charges, coordinates (3 per atom). */
+int kumo_hip_molecule_atoms(const kumo_hip_molecule* mol, int32_t* z, double* charges, double* xyz);
+
+/* ---- evaluation (Evaluator::evaluate, include/chem/kumo/evaluator.hpp:487-538) ----
+ * outputs holds one pointer per component of every property, in request
+ * order (sum over properties of channels(kind)); each receives npoints
+ * doubles in Field order (grid) or point order. backend_used may be NULL. */
+int kumo_hip_evaluate_grid(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const kumo_hip_grid* grid,
+                           const kumo_hip_property* properties, size_t nproperties,
+                           const kumo_hip_options* options, double* const* outputs, int32_t* backend_used);
+/* points: xyz of each point (3 * npoints doubles, host memory; device memory
+ * when options->device_outputs is set). */
+int kumo_hip_evaluate_points(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const double* points,
+                             size_t npoints, const kumo_hip_property* properties, size_t nproperties,
+                             const kumo_hip_options* options, double* const* outputs, int32_t* backend_used);
+
+/* ---- kernel-level seams used by the parity tests ----
+ * BasisEvaluator::evaluate_batch / evaluate_derivatives_batch
+ * (include/chem/kumo/basis_evaluator.hpp:99-117): order 0, 1 or 2; out is
+ * column-major nbasis x (panels * npoints), panels = 1, 4 or 10. */
+int kumo_hip_ao_eval(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const double* points, size_t npoints,
+                     int32_t order, double* out);
+/* significant_aos per screening block (src/ao_screening.hpp:35-50) as the
+ * point path applies it (src/density.cpp:454-467): one byte per
+ * (block, AO); grid may be NULL when points is given. */
+int kumo_hip_screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const kumo_hip_grid* grid,
+                             const double* points, size_t npoints, int32_t block_size, int32_t ao_order,
+                             double threshold, uint8_t* kept);
+/* Boys function F_0..F_m(T) on the device (src/boys_function.cpp:47-76),
+ * out is n x (m + 1). */
+int kumo_hip_boys(kumo_hip_context* ctx, const double* T, size_t n, int32_t m, double* out);
+
+/* ---- timing hooks for the benchmark ----
+ * Device time (ms) of the last evaluation's dominant contraction kernel,
+ * summed over its launches and measured with HIP events on the stream the
+ * kernel ran on, and the number of launches and of algorithmic FLOPs
+ * (2 * kept AOs * MOs * panels * points, summed over blocks). */
+int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, int64_t* launches,
+                               double* algorithmic_flops, double* padded_flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
