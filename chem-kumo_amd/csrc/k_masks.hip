@@ -14,7 +14,7 @@
 namespace kumo {
 namespace {
 
-constexpr int kMaskThreads = 256;
+constexpr int kMaskThreads = 128; // one lane per point of a default 128-point block
 
 template <int ORDER>
 __global__ void __launch_bounds__(kMaskThreads)
