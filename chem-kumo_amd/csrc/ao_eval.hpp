@@ -61,7 +61,7 @@ __host__ __device__ constexpr int panel_component(unsigned m, int i) {
 
 // Grid or listed point q (src/grid_geometry.hpp:59-68): origin + steps*(i,j,k),
 // each coordinate summed as ((s0 i + s1 j) + s2 k) without fused operations.
-__device__ inline void point_coords(const PointSource& ps, long long q, double& x, double& y, double& z) {
+__device__ __attribute__((always_inline)) inline void point_coords(const PointSource& ps, long long q, double& x, double& y, double& z) {
 #pragma clang fp contract(off)
     if (ps.points) {
         x = ps.points[3 * q];
@@ -78,7 +78,7 @@ __device__ inline void point_coords(const PointSource& ps, long long q, double& 
     z = ps.origin[2] + ((ps.steps[2] * fi + ps.steps[5] * fj) + ps.steps[8] * fk);
 }
 
-__device__ inline double sq_norm(double x, double y, double z) {
+__device__ __attribute__((always_inline)) inline double sq_norm(double x, double y, double z) {
 #pragma clang fp contract(off)
     return (x * x + y * y) + z * z;
 }
@@ -88,7 +88,7 @@ __device__ inline double sq_norm(double x, double y, double z) {
 // ORDER 0 uses the value kernel (evaluate_into_impl), 1 and 2 the derivative
 // kernel (evaluate_derivatives_batch), as the reference does.
 template <int L, int ORDER, unsigned PSET>
-__device__ inline void eval_cartesian(const DevShell& s, const double* __restrict__ exps,
+__device__ __attribute__((always_inline)) inline void eval_cartesian(const DevShell& s, const double* __restrict__ exps,
                                       const double* __restrict__ wts, double x, double y, double z, double r2,
                                       double (&cart)[(L + 1) * (L + 2) / 2][popcount10(PSET)]) {
 #pragma clang fp contract(off)
@@ -184,7 +184,7 @@ __device__ inline void eval_cartesian(const DevShell& s, const double* __restric
 // block and output panel i; rows are pure (transformed, in increasing
 // Cartesian-component order) or Cartesian. Calls emit(row, vals[NP]).
 template <int L, int ORDER, unsigned PSET, typename Emit>
-__device__ inline void eval_block_l(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
+__device__ __attribute__((always_inline)) inline void eval_block_l(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
                                     Emit&& emit) {
 #pragma clang fp contract(off)
     constexpr int NC = (L + 1) * (L + 2) / 2;
@@ -217,7 +217,7 @@ __device__ inline void eval_block_l(const DevShell& s, const MolView& m, double 
 
 // Runtime dispatch on the block's angular momentum.
 template <int ORDER, unsigned PSET, typename Emit>
-__device__ inline void eval_block(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
+__device__ __attribute__((always_inline)) inline void eval_block(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
                                   Emit&& emit) {
     switch (s.l) {
     case 0: eval_block_l<0, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
@@ -236,7 +236,7 @@ __device__ inline void eval_block(const DevShell& s, const MolView& m, double x,
 // The Cartesian->pure transform is folded into the MO coefficients by the
 // caller. Calls emit(k, vals[NP]) for each Cartesian component k.
 template <int L, int ORDER, unsigned PSET, typename Emit>
-__device__ inline void eval_cartesian_rows(const DevShell& s, const MolView& m, double x, double y, double z,
+__device__ __attribute__((always_inline)) inline void eval_cartesian_rows(const DevShell& s, const MolView& m, double x, double y, double z,
                                            double r2, Emit&& emit) {
     constexpr int NC = (L + 1) * (L + 2) / 2;
     constexpr int NP = popcount10(PSET);
@@ -253,7 +253,7 @@ __device__ inline void eval_cartesian_rows(const DevShell& s, const MolView& m, 
         zp[n] = zp[n - 1] * z;
     }
     // One primitive of component k: adds weight * product of axis factors.
-    const auto add = [&](int k, int p, double radial, double (&sums)[NP]) {
+    const auto add = [&](int k, int p, double radial, double (&sums)[NP]) __attribute__((always_inline)) {
         const double a = e[p];
         const double wt = w[k * np + p] * radial;
         const double two_a = 2.0 * a;
@@ -308,7 +308,7 @@ __device__ inline void eval_cartesian_rows(const DevShell& s, const MolView& m, 
 }
 
 template <int ORDER, unsigned PSET, typename Emit>
-__device__ inline void eval_block_cartesian(const DevShell& s, const MolView& m, double x, double y, double z,
+__device__ __attribute__((always_inline)) inline void eval_block_cartesian(const DevShell& s, const MolView& m, double x, double y, double z,
                                             double r2, Emit&& emit) {
     switch (s.l) {
     case 0: eval_cartesian_rows<0, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
@@ -320,14 +320,14 @@ __device__ inline void eval_block_cartesian(const DevShell& s, const MolView& m,
     }
 }
 
-__device__ inline double wave_max(double v) {
+__device__ __attribute__((always_inline)) inline double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         v = fmax(v, __shfl_xor(v, o, 64));
     return v;
 }
 
-__device__ inline double wave_sum(double v) {
+__device__ __attribute__((always_inline)) inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
         v += __shfl_xor(v, o, 64);

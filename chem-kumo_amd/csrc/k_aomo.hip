@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
                 inside = r2 <= sh.cutoff_r2;
             }
             if (inside) {
-                eval_block_cartesian<ORDER, PSET>(sh, m, x, y, z, r2, [&](int k, const double (&v)[NP]) {
+                eval_block_cartesian<ORDER, PSET>(sh, m, x, y, z, r2, [&](int k, const double (&v)[NP]) __attribute__((always_inline)) {
 #pragma unroll
                     for (int i = 0; i < NP; ++i)
                         store_split<PREC>(dst + static_cast<size_t>(i) * T * LD + k, phi_plane, v[i]);

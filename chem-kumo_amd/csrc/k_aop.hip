@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(128) aop_phi_kernel(AoPLaunch a, int panels) {
             rr += (bits >> r) & 1u;
         }
         if (r2 <= sh.cutoff_r2) {
-            eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int r, const double (&v)[NP]) {
+            eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int r, const double (&v)[NP]) __attribute__((always_inline)) {
                 if (rowidx[r] >= 0)
 #pragma unroll
                     for (int i = 0; i < NP; ++i)

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kMaskThreads)
                 for (int r = 0; r < kMaxCart; ++r)
                     amax[r] = 0.0;
                 if (inside) {
-                    eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) {
+                    eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) __attribute__((always_inline)) {
                         double a = 0.0;
 #pragma unroll
                         for (int i = 0; i < NP; ++i)
@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(128) ao_eval_kernel(MolView m, PointSource ps,
         const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
         const double r2 = sq_norm(x, y, z);
         if (r2 <= sh.cutoff_r2) {
-            eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) {
+            eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) __attribute__((always_inline)) {
 #pragma unroll
                 for (int i = 0; i < NP; ++i)
                     out[(sh.ao + row) + nao * (i * n + j)] = v[i];

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(128) orbital_kernel(MolView m, PointSource ps,
         const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
         const double r2 = sq_norm(x, y, z);
         if (!(r2 <= sh.cutoff_r2)) continue;
-        eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) {
+        eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) __attribute__((always_inline)) {
             const double* c = coeff + static_cast<long long>(sh.ao + row) * ldc;
 #pragma unroll
             for (int k = 0; k < kOrbChunk; ++k) {

@@ -364,3 +364,42 @@ def cached_fchk(mol_fn, name, directory=None):
         mol.write_fchk(path + ".tmp")
         os.replace(path + ".tmp", path)
     return path, mol, grid
+
+
+# Bondi van der Waals radii (angstrom), J. Phys. Chem. 68, 441 (1964).
+BONDI = {1: 1.20, 6: 1.70, 7: 1.55, 8: 1.52}
+BOHR_PER_ANGSTROM = 1.8897261246257702  # src/cube.cpp:32
+
+
+def pople_molecule(seed=3, natoms=20, nheavy=12):
+    """Configuration 3 molecule: `natoms` atoms (12 C/N/O + 8 H) at least
+    1.8 bohr apart, 6-31G*-shaped Cartesian shells (~200 AOs), closed shell."""
+    rng = np.random.default_rng(seed)
+    z = _element_mix(rng, nheavy, natoms - nheavy)
+    xyz = _positions(rng, natoms, (102.4 * natoms) ** (1 / 3), 1.8)
+    ne = int(sum(z))
+    if ne % 2:
+        z[-1] = 1 if z[-1] != 1 else 1
+        ne = int(sum(z))
+    na = ne // 2
+    mol = SyntheticMolecule(rng, z, xyz, lambda zz: POPLE_HEAVY if zz > 1 else POPLE_H, na, ne - na, ne % 2 == 1)
+    return mol
+
+
+def vdw_shell_points(z, xyz, npoints, scale=1.4, seed=11):
+    """Points uniformly on the union of spheres of radius scale x r_vdW around
+    the atoms, rejecting points inside any other sphere (a molecular surface
+    for the electrostatic potential)."""
+    rng = np.random.default_rng(seed)
+    radii = np.array([BONDI.get(int(a), 1.7) * BOHR_PER_ANGSTROM * scale for a in z])
+    area = 4 * np.pi * radii ** 2
+    out = []
+    while sum(len(o) for o in out) < npoints:
+        k = rng.choice(len(z), size=4 * npoints, p=area / area.sum())
+        d = rng.normal(size=(4 * npoints, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        p = xyz[k] + d * radii[k][:, None]
+        dist = np.linalg.norm(p[:, None, :] - xyz[None, :, :], axis=2)
+        inside = (dist < radii[None, :] * (1 - 1e-12)).any(axis=1)
+        out.append(p[~inside])
+    return np.concatenate(out)[:npoints]
