@@ -51,10 +51,11 @@ int kumo_hip_init(int device, kumo_hip_context** out) {
         if (device < 0 || device >= count) kumo::invalid("no such HIP device");
         KUMO_HIP(hipSetDevice(device));
         ctx->device = device;
-        KUMO_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->exec.create(device, nullptr);
         const auto table = kumo::boys_table_host();
-        ctx->boys_table = kumo::upload(table.data(), table.size(), ctx->stream);
-        KUMO_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->boys_table = kumo::upload(table.data(), table.size(), ctx->exec.stream);
+        KUMO_HIP(hipStreamSynchronize(ctx->exec.stream));
+        ctx->exec.boys_table = ctx->boys_table.as<double>();
     });
     if (rc != KUMO_HIP_OK) {
         delete ctx;
@@ -66,26 +67,9 @@ int kumo_hip_init(int device, kumo_hip_context** out) {
 
 int kumo_hip_release(kumo_hip_context* ctx) {
     if (!ctx) return KUMO_HIP_OK;
-    for (auto& e : ctx->events) {
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    {
-        // release device buffers before the stream
-        kumo_hip_context* c = ctx;
-        c->boys_table.release();
-        c->ws_tables.release();
-        c->ws_moments.release();
-        c->ws_comp.release();
-        c->ws_phi.release();
-        c->ws_w.release();
-        c->ws_out.release();
-        c->ws_points.release();
-        c->ws_tasks.release();
-        c->ws_misc.release();
-    }
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    ctx->exec.destroy();
+    ctx->ws_out.release();
+    ctx->boys_table.release();
     delete ctx;
     return KUMO_HIP_OK;
 }
@@ -137,7 +121,7 @@ int kumo_hip_molecule_info_get(const kumo_hip_molecule* m, kumo_hip_molecule_inf
 
 int kumo_hip_molecule_release(kumo_hip_molecule* m) {
     if (!m) return KUMO_HIP_OK;
-    if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+    if (m->ctx) (void)hipStreamSynchronize(m->ctx->exec.stream);
     delete m;
     return KUMO_HIP_OK;
 }
@@ -207,14 +191,14 @@ int kumo_hip_ao_eval(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const 
         const long long n = static_cast<long long>(npoints);
         if (n == 0) return;
         const int panels = order == 0 ? 1 : (order == 1 ? 4 : 10);
-        kumo::DeviceBuffer dp = kumo::upload(points, 3 * static_cast<std::size_t>(n), ctx->stream);
+        kumo::DeviceBuffer dp = kumo::upload(points, 3 * static_cast<std::size_t>(n), ctx->exec.stream);
         kumo::DeviceBuffer dout(static_cast<std::size_t>(mol->n) * panels * n * sizeof(double));
         kumo::PointSource ps{};
         ps.ny = ps.nz = 1;
         ps.points = dp.as<double>();
-        kumo::launch_ao_eval(mol->view(), ps, n, order, dout.as<double>(), ctx->stream);
-        KUMO_HIP(hipMemcpyAsync(out, dout.get(), dout.bytes(), hipMemcpyDeviceToHost, ctx->stream));
-        KUMO_HIP(hipStreamSynchronize(ctx->stream));
+        kumo::launch_ao_eval(mol->view(), ps, n, order, dout.as<double>(), ctx->exec.stream);
+        KUMO_HIP(hipMemcpyAsync(out, dout.get(), dout.bytes(), hipMemcpyDeviceToHost, ctx->exec.stream));
+        KUMO_HIP(hipStreamSynchronize(ctx->exec.stream));
     });
 }
 
@@ -241,22 +225,22 @@ int kumo_hip_boys(kumo_hip_context* ctx, const double* T, size_t n, int32_t m, d
             if (!(T[i] >= 0.0) || !std::isfinite(T[i]))
                 kumo::invalid("Boys function argument must be finite and nonnegative");
         if (n == 0) return;
-        kumo::DeviceBuffer dt = kumo::upload(T, n, ctx->stream);
+        kumo::DeviceBuffer dt = kumo::upload(T, n, ctx->exec.stream);
         kumo::DeviceBuffer dout(n * static_cast<std::size_t>(m + 1) * sizeof(double));
         kumo::launch_boys(ctx->boys_table.as<double>(), dt.as<double>(), static_cast<long long>(n), m, dout.as<double>(),
-                          ctx->stream);
-        KUMO_HIP(hipMemcpyAsync(out, dout.get(), dout.bytes(), hipMemcpyDeviceToHost, ctx->stream));
-        KUMO_HIP(hipStreamSynchronize(ctx->stream));
+                          ctx->exec.stream);
+        KUMO_HIP(hipMemcpyAsync(out, dout.get(), dout.bytes(), hipMemcpyDeviceToHost, ctx->exec.stream));
+        KUMO_HIP(hipStreamSynchronize(ctx->exec.stream));
     });
 }
 
 int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, int64_t* launches, double* flops,
                                double* padded_flops) {
     if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
-    if (kernel_ms) *kernel_ms = ctx->last_kernel_ms;
-    if (launches) *launches = ctx->last_launches;
-    if (flops) *flops = ctx->last_flops;
-    if (padded_flops) *padded_flops = ctx->last_padded_flops;
+    if (kernel_ms) *kernel_ms = ctx->exec.last_kernel_ms;
+    if (launches) *launches = ctx->exec.last_launches;
+    if (flops) *flops = ctx->exec.last_flops;
+    if (padded_flops) *padded_flops = ctx->exec.last_flops;
     return KUMO_HIP_OK;
 }
 

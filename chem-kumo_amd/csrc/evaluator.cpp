@@ -2,8 +2,10 @@
 // (src/evaluator.cpp:556-802) in the reference order — point path, ESP,
 // orbital pass, grid path — each over batches of whole screening blocks.
 // Every screening block is evaluated by the same kernels whatever batch it
-// falls in, so the results do not depend on the batch size or on other work
-// on the device.
+// falls in, so the results do not depend on the batch size, on the range a
+// caller evaluates or on other work on the device; a serving loop can
+// therefore cut a request into block-aligned ranges and run them on several
+// streams.
 #include "plan.hpp"
 #include "runtime.hpp"
 
@@ -16,16 +18,6 @@ namespace kumo {
 namespace {
 
 constexpr int kComp = 15;
-
-struct Progress {
-    const kumo_hip_options& opt;
-    std::size_t done = 0, total = 0;
-    void advance(std::size_t n) {
-        done += n;
-        if (opt.progress && opt.progress(done, total, opt.progress_user) != 0)
-            throw Error(KUMO_HIP_ABORTED, "evaluation aborted by the progress callback");
-    }
-};
 
 void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
     if (o.block_size < 1 || o.block_size > 65536) invalid("evaluation block_size must be in [1, 65536]");
@@ -42,6 +34,57 @@ void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
         invalid("AO_MO backend requested but orbital representation is not eligible");
     if (o.threads > 1) invalid("explicit multi-thread control is not implemented in the API adapter");
 }
+
+struct TableSet {
+    KeptTables t;
+};
+
+TableSet tables(Executor& ex, int nblk, int nao, int nsh) {
+    const std::size_t b = static_cast<std::size_t>(nblk);
+    const std::size_t bytes_kept = b * nao, bytes_list = 4 * b * nao, bytes_n = 4 * b, bytes_ks = 4 * b * nsh,
+                      bytes_kb = 4 * b * nsh, bytes_kr = b * nsh, bytes_cb = 4 * b * (nsh + 1), bytes_nc = 4 * b;
+    const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };
+    const std::size_t total = al(bytes_kept) + al(bytes_list) + al(bytes_n) + al(bytes_ks) + al(bytes_kb) + al(bytes_kr) +
+                              al(bytes_cb) + al(bytes_nc);
+    ex.ws_tables.resize(total);
+    char* p = ex.ws_tables.as<char>();
+    TableSet s;
+    s.t.kept = reinterpret_cast<uint8_t*>(p);
+    p += al(bytes_kept);
+    s.t.kept_list = reinterpret_cast<int32_t*>(p);
+    p += al(bytes_list);
+    s.t.nkept = reinterpret_cast<int32_t*>(p);
+    p += al(bytes_n);
+    s.t.kept_shell = reinterpret_cast<int32_t*>(p);
+    p += al(bytes_ks);
+    s.t.kept_bits = reinterpret_cast<uint32_t*>(p);
+    p += al(bytes_kb);
+    s.t.kept_row = reinterpret_cast<uint8_t*>(p);
+    p += al(bytes_kr);
+    s.t.chunk_begin = reinterpret_cast<int32_t*>(p);
+    p += al(bytes_cb);
+    s.t.nchunks = reinterpret_cast<int32_t*>(p);
+    return s;
+}
+
+template <typename T> T* grow(DeviceBuffer& b, std::size_t count) {
+    b.resize(std::max<std::size_t>(1, count) * sizeof(T));
+    return b.as<T>();
+}
+
+hipEvent_t next_event(Executor& ex, bool start) {
+    if (ex.events_used == ex.events.size()) {
+        hipEvent_t a, b;
+        KUMO_HIP(hipEventCreate(&a));
+        KUMO_HIP(hipEventCreate(&b));
+        ex.events.push_back({a, b});
+    }
+    auto& pr = ex.events[ex.events_used];
+    if (!start) ++ex.events_used;
+    return start ? pr.first : pr.second;
+}
+
+} // namespace
 
 // validated_point_count (src/grid_geometry.hpp:30-52).
 long long validated_point_count(const kumo_hip_grid& g) {
@@ -71,79 +114,6 @@ long long validated_point_count(const kumo_hip_grid& g) {
     return count;
 }
 
-struct Ws {
-    kumo_hip_context* ctx;
-    template <typename T> T* get(DeviceBuffer& b, std::size_t count) {
-        b.resize(count * sizeof(T));
-        return b.as<T>();
-    }
-};
-
-struct TableSet {
-    KeptTables t;
-    long long nblk_capacity = 0;
-};
-
-TableSet tables(kumo_hip_context* ctx, int nblk, int nao, int nsh) {
-    const std::size_t b = static_cast<std::size_t>(nblk);
-    const std::size_t bytes_kept = b * nao, bytes_list = 4 * b * nao, bytes_n = 4 * b, bytes_ks = 4 * b * nsh,
-                      bytes_kb = 4 * b * nsh, bytes_kr = b * nsh, bytes_cb = 4 * b * (nsh + 1), bytes_nc = 4 * b;
-    const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };
-    const std::size_t total = al(bytes_kept) + al(bytes_list) + al(bytes_n) + al(bytes_ks) + al(bytes_kb) + al(bytes_kr) +
-                              al(bytes_cb) + al(bytes_nc);
-    ctx->ws_tables.resize(total);
-    char* p = ctx->ws_tables.as<char>();
-    TableSet s;
-    s.t.kept = reinterpret_cast<uint8_t*>(p);
-    p += al(bytes_kept);
-    s.t.kept_list = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_list);
-    s.t.nkept = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_n);
-    s.t.kept_shell = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_ks);
-    s.t.kept_bits = reinterpret_cast<uint32_t*>(p);
-    p += al(bytes_kb);
-    s.t.kept_row = reinterpret_cast<uint8_t*>(p);
-    p += al(bytes_kr);
-    s.t.chunk_begin = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_cb);
-    s.t.nchunks = reinterpret_cast<int32_t*>(p);
-    s.nblk_capacity = nblk;
-    return s;
-}
-
-PointSource make_source(const PointsArg& pts, const double* device_points) {
-    PointSource ps{};
-    if (pts.grid) {
-        for (int i = 0; i < 3; ++i)
-            ps.origin[i] = pts.grid->origin[i];
-        for (int i = 0; i < 9; ++i)
-            ps.steps[i] = pts.grid->steps[i];
-        ps.ny = pts.grid->counts[1];
-        ps.nz = pts.grid->counts[2];
-        ps.points = nullptr;
-    } else {
-        ps.ny = ps.nz = 1;
-        ps.points = device_points;
-    }
-    return ps;
-}
-
-hipEvent_t next_event(kumo_hip_context* ctx, bool start) {
-    if (ctx->events_used == ctx->events.size()) {
-        hipEvent_t a, b;
-        KUMO_HIP(hipEventCreate(&a));
-        KUMO_HIP(hipEventCreate(&b));
-        ctx->events.push_back({a, b});
-    }
-    auto& pr = ctx->events[ctx->events_used];
-    if (!start) ++ctx->events_used;
-    return start ? pr.first : pr.second;
-}
-
-} // namespace
-
 // Constants of the derived fields computed with the host libm exactly as the
 // reference computes them.
 FieldsLaunch fields_launch(const double* moments, long long n, long long out_offset, int ntasks, const FieldTask* tasks) {
@@ -153,51 +123,39 @@ FieldsLaunch fields_launch(const double* moments, long long n, long long out_off
     return f;
 }
 
-void screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsArg& pts, int block_size,
-                     int order, double threshold, uint8_t* kept) {
-    if (block_size < 1 || block_size > 65536) invalid("block_size must be in [1, 65536]");
-    if (order < 0 || order > 2) invalid("AO order must be 0, 1 or 2");
-    const long long n = pts.grid ? validated_point_count(*pts.grid) : pts.n;
-    const double* dpts = nullptr;
-    if (!pts.grid) {
-        ctx->ws_points.resize(static_cast<std::size_t>(std::max(1ll, 3 * n)) * sizeof(double));
-        KUMO_HIP(hipMemcpyAsync(ctx->ws_points.get(), pts.points, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        dpts = ctx->ws_points.as<double>();
+PointSource make_source(const kumo_hip_grid* grid, const double* device_points) {
+    PointSource ps{};
+    if (grid) {
+        for (int i = 0; i < 3; ++i)
+            ps.origin[i] = grid->origin[i];
+        for (int i = 0; i < 9; ++i)
+            ps.steps[i] = grid->steps[i];
+        ps.ny = grid->counts[1];
+        ps.nz = grid->counts[2];
+        ps.points = nullptr;
+    } else {
+        ps.ny = ps.nz = 1;
+        ps.points = device_points;
     }
-    const PointSource ps = make_source(pts, dpts);
-    const long long nblocks = (n + block_size - 1) / block_size;
-    const int nao = mol->n, nsh = static_cast<int>(mol->basis.shells.size());
-    const int cap = static_cast<int>(std::min<long long>(nblocks, 4096));
-    const auto ts = tables(ctx, std::max(1, cap), nao, nsh);
-    for (long long b0 = 0; b0 < nblocks; b0 += cap) {
-        const int nb = static_cast<int>(std::min<long long>(cap, nblocks - b0));
-        launch_masks(mol->view(), ps, n, block_size, b0, nb, order, threshold, ts.t, ctx->stream);
-        KUMO_HIP(hipMemcpyAsync(kept + b0 * nao, ts.t.kept, static_cast<std::size_t>(nb) * nao, hipMemcpyDeviceToHost,
-                                ctx->stream));
-    }
-    KUMO_HIP(hipStreamSynchronize(ctx->stream));
+    return ps;
 }
 
-void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const PointsArg& pts,
-              const kumo_hip_property* props, std::size_t nprops, const kumo_hip_options& opt, double* const* outputs,
-              int32_t* backend_used) {
-    auto* mol = const_cast<kumo_hip_molecule*>(mol_c);
-    hipStream_t s = ctx->stream;
+Prepared prepare(const kumo_hip_molecule* mol, const PointsArg& pts, const kumo_hip_property* props,
+                 std::size_t nprops, const kumo_hip_options& opt, bool points_on_device) {
     validate_options(opt, mol);
-    const bool grid = pts.grid != nullptr;
-    long long n = 0;
-    if (grid) {
-        n = validated_point_count(*pts.grid);
+    Prepared r;
+    r.grid = pts.grid != nullptr;
+    if (r.grid) {
+        r.n = validated_point_count(*pts.grid);
     } else {
-        n = pts.n;
-        if (n < 0) invalid("invalid point count");
-        if (!opt.device_outputs)
-            for (long long q = 0; q < 3 * n; ++q)
+        r.n = pts.n;
+        if (r.n < 0) invalid("invalid point count");
+        if (!points_on_device)
+            for (long long q = 0; q < 3 * r.n; ++q)
                 if (!std::isfinite(pts.points[q])) invalid("evaluation points must be finite");
     }
-    const Plan plan = make_plan(props, nprops);
-    const int backend = choose_backend(opt.backend, mol->eligible, plan);
-    if (backend_used) *backend_used = backend;
+    r.plan = make_plan(props, nprops);
+    r.backend = choose_backend(opt.backend, mol->eligible, r.plan);
     for (std::size_t i = 0; i < nprops; ++i) {
         const auto& p = props[i];
         if (is_orbital_property(p.kind)) {
@@ -206,67 +164,45 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
             if (p.orbital_index > nmo) out_of_range("MO index outside available coefficient range");
         }
     }
-    ctx->events_used = 0;
-    ctx->last_flops = ctx->last_padded_flops = 0.0;
-    ctx->last_launches = 0;
-    if (nprops == 0 || n == 0) return;
+    r.props.assign(props, props + nprops);
+    std::size_t c = 0;
+    for (std::size_t i = 0; i < nprops; ++i) {
+        r.first_channel.push_back(c);
+        c += static_cast<std::size_t>(channels(props[i].kind));
+    }
+    r.nout = c;
+    for (int comp = 0; comp < 4; ++comp)
+        if (uses_point_path(r.plan.c[static_cast<std::size_t>(comp)], r.backend, r.grid)) r.needed.push_back(comp);
+    bool orb_ch[2] = {false, false};
+    for (const auto& p : r.props) {
+        if (is_orbital_property(p.kind) && !on_grid_path(p, r.backend, r.grid)) r.orbital_pass = true;
+        if (on_grid_path(p, r.backend, r.grid) && p.kind == kDensity) ++r.grid_density_passes;
+        if (on_grid_path(p, r.backend, r.grid) && p.kind == kOrbital) orb_ch[p.channel] = true;
+    }
+    r.grid_orbital_passes = (orb_ch[0] ? 1 : 0) + (orb_ch[1] ? 1 : 0);
+    r.passes = (r.needed.empty() ? 0 : 1) + (r.plan.esp() ? 1 : 0) + (r.orbital_pass ? 1 : 0) +
+               r.grid_density_passes + r.grid_orbital_passes;
+    return r;
+}
 
-    // Outputs: device pointers per channel (staging buffers for host outputs).
-    std::size_t nout = 0;
-    for (std::size_t i = 0; i < nprops; ++i)
-        nout += static_cast<std::size_t>(channels(props[i].kind));
-    std::vector<double*> dout(nout);
-    if (opt.device_outputs) {
-        for (std::size_t c = 0; c < nout; ++c)
-            dout[c] = outputs[c];
-    } else {
-        ctx->ws_out.resize(nout * static_cast<std::size_t>(n) * sizeof(double));
-        for (std::size_t c = 0; c < nout; ++c)
-            dout[c] = ctx->ws_out.as<double>() + c * static_cast<std::size_t>(n);
-    }
-    std::vector<std::size_t> first_channel(nprops);
-    {
-        std::size_t c = 0;
-        for (std::size_t i = 0; i < nprops; ++i) {
-            first_channel[i] = c;
-            c += static_cast<std::size_t>(channels(props[i].kind));
-        }
-    }
-    // Points on the device.
-    const double* dpts = nullptr;
-    if (!grid) {
-        if (opt.device_outputs) {
-            dpts = pts.points;
-        } else {
-            ctx->ws_points.resize(static_cast<std::size_t>(3 * n) * sizeof(double));
-            KUMO_HIP(hipMemcpyAsync(ctx->ws_points.get(), pts.points, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
-            dpts = ctx->ws_points.as<double>();
-        }
-    }
-    const PointSource ps = make_source(pts, dpts);
+void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const PointSource& ps,
+               const kumo_hip_options& opt, double* const* dout, long long q0, long long q1, const Advance& advance) {
+    hipStream_t s = ex.stream;
+    const auto& props = pr.props;
+    const std::size_t nprops = props.size();
+    const long long n = pr.n;
+    const bool grid = pr.grid;
+    const int backend = pr.backend;
+    const Plan& plan = pr.plan;
     const MolView mv = mol->view();
     const int nao = mol->n, nsh = static_cast<int>(mol->basis.shells.size());
     const int bs = opt.block_size;
     const double thr = opt.screening_threshold;
-
-    // Progress over all passes (src/evaluator.cpp:597-603).
-    std::vector<int> needed;
-    for (int c = 0; c < 4; ++c)
-        if (uses_point_path(plan.c[static_cast<std::size_t>(c)], backend, grid)) needed.push_back(c);
-    bool any_orbital_pass = false, esp = plan.esp();
-    int grid_density_passes = 0, grid_orbital_passes = 0;
-    bool orb_ch[2] = {false, false};
-    for (std::size_t i = 0; i < nprops; ++i) {
-        const auto& p = props[i];
-        if (is_orbital_property(p.kind) && !on_grid_path(p, backend, grid)) any_orbital_pass = true;
-        if (on_grid_path(p, backend, grid) && p.kind == kDensity) ++grid_density_passes;
-        if (on_grid_path(p, backend, grid) && p.kind == kOrbital) orb_ch[p.channel] = true;
-    }
-    grid_orbital_passes = (orb_ch[0] ? 1 : 0) + (orb_ch[1] ? 1 : 0);
-    Progress progress{opt};
-    progress.total = static_cast<std::size_t>(n) *
-                     static_cast<std::size_t>((needed.empty() ? 0 : 1) + (esp ? 1 : 0) + (any_orbital_pass ? 1 : 0) +
-                                              grid_density_passes + grid_orbital_passes);
+    if (q0 % bs != 0) invalid("evaluation ranges must start at a screening block boundary");
+    q1 = std::min(q1, n);
+    if (q1 <= q0) return;
+    const long long range = q1 - q0;
+    const auto& needed = pr.needed;
 
     // ---------------------------------------------------------------- point pass
     if (!needed.empty()) {
@@ -274,16 +210,16 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
         bool full_h = false;
         for (int c = 0; c < 4; ++c)
             full_h = full_h || plan.c[static_cast<std::size_t>(c)].full_hessian();
-        const long long nblocks = (n + bs - 1) / bs;
-        long long target = 1ll << 21;
+        const long long b_first = q0 / bs, b_end = (q1 + bs - 1) / bs;
+        const long long nblocks = b_end - b_first;
+        const long long target = 1ll << 21;
         int cap = static_cast<int>(std::min<long long>({nblocks, std::max(1ll, target / bs), 8192ll}));
         if (backend == KUMO_HIP_AO_P) {
             const long long per_block = static_cast<long long>(nao) * (ao_order == 0 ? 1 : (ao_order == 1 ? 4 : 10)) * bs;
             cap = static_cast<int>(std::max(1ll, std::min<long long>(cap, (512ll << 20) / (8 * per_block))));
         }
-        const auto ts = tables(ctx, cap, nao, nsh);
+        const auto ts = tables(ex, cap, nao, nsh);
         const long long batch_max = static_cast<long long>(cap) * bs;
-        // channel moments [groups][2][13][batch] and component moments [4][15][batch]
         int slot0 = 0, slot1 = 0;
         bool need_a = false, need_b = false;
         for (int c : needed) {
@@ -302,9 +238,8 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
             if (slot1 == slot0) slot1 = slot0 + 16;
         }
         const int groups = backend == KUMO_HIP_AO_MO ? (slot1 - slot0 + 127) / 128 : 1;
-        double* chm = Ws{ctx}.get<double>(ctx->ws_moments, static_cast<std::size_t>(groups) * 2 * kMoments * batch_max);
-        double* cmp = Ws{ctx}.get<double>(ctx->ws_comp, static_cast<std::size_t>(4) * kComp * batch_max);
-        // Field tasks of the point path.
+        double* chm = grow<double>(ex.ws_moments, static_cast<std::size_t>(groups) * 2 * kMoments * batch_max);
+        double* cmp = grow<double>(ex.ws_comp, static_cast<std::size_t>(4) * kComp * batch_max);
         std::vector<FieldTask> tasks;
         for (std::size_t i = 0; i < nprops; ++i) {
             const auto& p = props[i];
@@ -313,18 +248,18 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
             t.kind = p.kind;
             t.component = p.component;
             for (int c = 0; c < channels(p.kind); ++c)
-                t.out[c] = dout[first_channel[i] + static_cast<std::size_t>(c)];
+                t.out[c] = dout[pr.first_channel[i] + static_cast<std::size_t>(c)];
             tasks.push_back(t);
         }
-        FieldTask* dtasks = Ws{ctx}.get<FieldTask>(ctx->ws_tasks, std::max<std::size_t>(1, tasks.size()));
+        FieldTask* dtasks = grow<FieldTask>(ex.ws_tasks, tasks.size());
         KUMO_HIP(hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(FieldTask), hipMemcpyHostToDevice, s));
         std::vector<int32_t> nkept(static_cast<std::size_t>(cap));
         double flops = 0.0;
         const int panels = ao_order <= 0 ? 1 : (ao_order == 1 ? 4 : (full_h ? 10 : 7));
-        for (long long b0 = 0; b0 < nblocks; b0 += cap) {
-            const int nb = static_cast<int>(std::min<long long>(cap, nblocks - b0));
+        for (long long b0 = b_first; b0 < b_end; b0 += cap) {
+            const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
             const long long first = b0 * bs;
-            const long long count = std::min<long long>(static_cast<long long>(nb) * bs, n - first);
+            const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
             launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, ts.t, s);
             if (backend == KUMO_HIP_AO_MO) {
                 AoMoLaunch a{};
@@ -345,15 +280,12 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
                 a.slot0 = slot0;
                 a.slot1 = slot1;
                 a.moments = chm;
-                KUMO_HIP(hipEventRecord(next_event(ctx, true), s));
+                KUMO_HIP(hipEventRecord(next_event(ex, true), s));
                 launch_ao_mo(a, s);
-                KUMO_HIP(hipEventRecord(next_event(ctx, false), s));
-                ++ctx->last_launches;
-                if (groups > 1) {
-                    // Sum the slot groups into group 0 (fixed order).
-                    for (int g = 1; g < groups; ++g)
-                        launch_add_into(chm, chm + static_cast<long long>(g) * 2 * kMoments * count, 2 * kMoments * count, s);
-                }
+                KUMO_HIP(hipEventRecord(next_event(ex, false), s));
+                ++ex.last_launches;
+                for (int g = 1; g < groups; ++g)
+                    launch_add_into(chm, chm + static_cast<long long>(g) * 2 * kMoments * count, 2 * kMoments * count, s);
                 launch_combine(chm, cmp, count, 1, ao_order, closed ? 1 : 0, s);
             } else {
                 AoPLaunch a{};
@@ -362,8 +294,8 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
                 a.tables = ts.t;
                 const bool tn = std::any_of(needed.begin(), needed.end(), [](int c) { return c != kDifference; });
                 const bool sn = std::any_of(needed.begin(), needed.end(), [](int c) { return c != kTotal; });
-                a.p[0] = tn ? mol->p_matrix(0) : nullptr;
-                a.p[1] = sn ? mol->p_matrix(1) : nullptr;
+                a.p[0] = tn ? mol->p_matrix(0, s) : nullptr;
+                a.p[1] = sn ? mol->p_matrix(1, s) : nullptr;
                 a.npoints = n;
                 a.batch_first = first;
                 a.batch_points = count;
@@ -372,19 +304,18 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
                 a.nblk = nb;
                 a.order = ao_order;
                 const int pan = ao_order == 0 ? 1 : (ao_order == 1 ? 4 : 10);
-                a.phi = Ws{ctx}.get<double>(ctx->ws_phi, static_cast<std::size_t>(nb) * nao * pan * bs);
-                a.w = Ws{ctx}.get<double>(ctx->ws_w, static_cast<std::size_t>(nb) * nao * 4 * bs);
+                a.phi = grow<double>(ex.ws_phi, static_cast<std::size_t>(nb) * nao * pan * bs);
+                a.w = grow<double>(ex.ws_w, static_cast<std::size_t>(nb) * nao * 4 * bs);
                 a.moments = chm;
                 launch_fill(chm, 2 * kMoments * count, 0.0, s);
-                KUMO_HIP(hipEventRecord(next_event(ctx, true), s));
+                KUMO_HIP(hipEventRecord(next_event(ex, true), s));
                 launch_ao_p(a, s);
-                KUMO_HIP(hipEventRecord(next_event(ctx, false), s));
-                ++ctx->last_launches;
+                KUMO_HIP(hipEventRecord(next_event(ex, false), s));
+                ++ex.last_launches;
                 launch_combine(chm, cmp, count, 0, ao_order, 0, s);
             }
             const FieldsLaunch f = fields_launch(cmp, count, first, static_cast<int>(tasks.size()), dtasks);
             launch_fields(f, s);
-            // Algorithmic FLOPs from the kept counts.
             KUMO_HIP(hipMemcpyAsync(nkept.data(), ts.t.nkept, static_cast<std::size_t>(nb) * 4, hipMemcpyDeviceToHost, s));
             KUMO_HIP(hipStreamSynchronize(s));
             for (int b = 0; b < nb; ++b) {
@@ -397,35 +328,34 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
                     flops += 2.0 * m_b * m_b * (ao_order == 0 ? 1 : 4) * static_cast<double>(pts_b);
                 }
             }
-            progress.advance(static_cast<std::size_t>(count));
+            advance(static_cast<std::size_t>(count));
         }
-        ctx->last_flops = flops;
+        ex.last_flops += flops;
     }
 
     // ---------------------------------------------------------------- ESP
-    if (esp) {
-        mol->ensure_esp();
+    if (plan.esp()) {
+        mol->ensure_esp(s);
         std::size_t idx = 0;
         for (std::size_t i = 0; i < nprops; ++i)
-            if (props[i].kind == kEsp) idx = first_channel[i];
+            if (props[i].kind == kEsp) idx = pr.first_channel[i];
         const long long chunk = 1 << 18;
-        for (long long q0 = 0; q0 < n; q0 += chunk) {
-            const long long cnt = std::min(chunk, n - q0);
+        for (long long p0 = q0; p0 < q1; p0 += chunk) {
+            const long long cnt = std::min(chunk, q1 - p0);
             launch_esp(mol->d_esp_pairs.as<EspPairDev>(), mol->esp_npairs, mol->d_esp_coef.as<double>(),
-                       mol->d_esp_atoms.as<EspAtomDev>(), static_cast<int>(mol->atoms.size()), ctx->boys_table.as<double>(),
-                       ps, q0, cnt, dout[idx] + q0, s);
+                       mol->d_esp_atoms.as<EspAtomDev>(), static_cast<int>(mol->atoms.size()), ex.boys_table, ps, p0,
+                       cnt, dout[idx] + p0, s);
             KUMO_HIP(hipStreamSynchronize(s));
-            progress.advance(static_cast<std::size_t>(cnt));
+            advance(static_cast<std::size_t>(cnt));
         }
-        // Duplicated ESP keys share the value.
         for (std::size_t i = 0; i < nprops; ++i)
-            if (props[i].kind == kEsp && first_channel[i] != idx)
-                KUMO_HIP(hipMemcpyAsync(dout[first_channel[i]], dout[idx], n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            if (props[i].kind == kEsp && pr.first_channel[i] != idx)
+                KUMO_HIP(hipMemcpyAsync(dout[pr.first_channel[i]] + q0, dout[idx] + q0, range * sizeof(double),
+                                        hipMemcpyDeviceToDevice, s));
     }
 
     // ---------------------------------------------------------------- orbitals
     const auto run_orbitals = [&](const std::vector<std::size_t>& keys, int channel, int order) {
-        // distinct MO indices of this channel
         std::vector<int> mos;
         for (auto i : keys)
             if (std::find(mos.begin(), mos.end(), props[i].orbital_index - 1) == mos.end())
@@ -433,38 +363,38 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
         const bool beta = channel == 1 && mol->unrestricted;
         const double* coeff = beta ? mol->d_cb.as<double>() : mol->d_ca.as<double>();
         const int ldc = beta ? mol->nmo_b : mol->nmo_a;
-        int32_t* dmo = Ws{ctx}.get<int32_t>(ctx->ws_misc, mos.size());
+        int32_t* dmo = grow<int32_t>(ex.ws_misc, mos.size());
         KUMO_HIP(hipMemcpyAsync(dmo, mos.data(), mos.size() * 4, hipMemcpyHostToDevice, s));
         const int NP = order == 0 ? 1 : (order == 1 ? 4 : 10);
-        const long long chunk = std::max(1ll, std::min<long long>(n, (256ll << 20) / (8ll * NP * static_cast<long long>(mos.size()))));
-        double* scratch = Ws{ctx}.get<double>(ctx->ws_phi, static_cast<std::size_t>(mos.size()) * NP * chunk);
-        for (long long q0 = 0; q0 < n; q0 += chunk) {
-            const long long cnt = std::min(chunk, n - q0);
-            launch_orbitals(mv, ps, q0, cnt, coeff, ldc, dmo, static_cast<int>(mos.size()), order, scratch, cnt, s);
+        const long long chunk =
+            std::max(1ll, std::min<long long>(range, (256ll << 20) / (8ll * NP * static_cast<long long>(mos.size()))));
+        double* scratch = grow<double>(ex.ws_phi, static_cast<std::size_t>(mos.size()) * NP * chunk);
+        for (long long p0 = q0; p0 < q1; p0 += chunk) {
+            const long long cnt = std::min(chunk, q1 - p0);
+            launch_orbitals(mv, ps, p0, cnt, coeff, ldc, dmo, static_cast<int>(mos.size()), order, scratch, cnt, s);
             for (auto i : keys) {
                 const int k = static_cast<int>(std::find(mos.begin(), mos.end(), props[i].orbital_index - 1) - mos.begin());
                 const int kind = props[i].kind;
                 const int c0 = kind == kOrbital ? 0 : (kind == kOrbitalGradient ? 1 : 4);
-                const int nc = channels(kind);
-                if (order == 0 && kind != kOrbital) continue;
-                if (order > 0 && kind == kOrbital) continue;
-                for (int c = 0; c < nc; ++c)
-                    KUMO_HIP(hipMemcpyAsync(dout[first_channel[i] + static_cast<std::size_t>(c)] + q0,
+                if ((order == 0) != (kind == kOrbital)) continue;
+                for (int c = 0; c < channels(kind); ++c)
+                    KUMO_HIP(hipMemcpyAsync(dout[pr.first_channel[i] + static_cast<std::size_t>(c)] + p0,
                                             scratch + (static_cast<long long>(k) * NP + c0 + c) * cnt, cnt * sizeof(double),
                                             hipMemcpyDeviceToDevice, s));
             }
             KUMO_HIP(hipStreamSynchronize(s));
         }
     };
-    if (any_orbital_pass) {
+    if (pr.orbital_pass) {
         for (int channel = 0; channel < 2; ++channel) {
             std::vector<std::size_t> values, derivs;
             int dorder = 0;
             for (std::size_t i = 0; i < nprops; ++i) {
                 const auto& p = props[i];
                 if (!is_orbital_property(p.kind) || on_grid_path(p, backend, grid) || p.channel != channel) continue;
-                if (p.kind == kOrbital) values.push_back(i);
-                else {
+                if (p.kind == kOrbital) {
+                    values.push_back(i);
+                } else {
                     derivs.push_back(i);
                     dorder = std::max(dorder, p.kind == kOrbitalGradient ? 1 : 2);
                 }
@@ -472,7 +402,7 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
             if (!values.empty()) run_orbitals(values, channel, 0);
             if (!derivs.empty()) run_orbitals(derivs, channel, dorder);
         }
-        progress.advance(static_cast<std::size_t>(n));
+        advance(static_cast<std::size_t>(range));
     }
 
     // ---------------------------------------------------------------- grid path
@@ -481,28 +411,30 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
             const auto& p = props[i];
             if (!(on_grid_path(p, backend, grid) && p.kind == kDensity)) continue;
             // densities_blocked with one component (src/density.cpp:365-444).
-            const long long nblocks = (n + bs - 1) / bs;
+            const long long b_first = q0 / bs, b_end = (q1 + bs - 1) / bs;
+            const long long nblocks = b_end - b_first;
             const long long per_block = static_cast<long long>(nao) * bs;
-            const int cap = static_cast<int>(std::max(1ll, std::min<long long>({nblocks, 8192ll, (256ll << 20) / (8 * per_block)})));
-            const auto ts = tables(ctx, cap, nao, nsh);
-            double* chm = Ws{ctx}.get<double>(ctx->ws_moments, 2 * kMoments * static_cast<std::size_t>(cap) * bs);
-            double* cmp = Ws{ctx}.get<double>(ctx->ws_comp, 4 * kComp * static_cast<std::size_t>(cap) * bs);
+            const int cap =
+                static_cast<int>(std::max(1ll, std::min<long long>({nblocks, 8192ll, (256ll << 20) / (8 * per_block)})));
+            const auto ts = tables(ex, cap, nao, nsh);
+            double* chm = grow<double>(ex.ws_moments, 2 * kMoments * static_cast<std::size_t>(cap) * bs);
+            double* cmp = grow<double>(ex.ws_comp, 4 * kComp * static_cast<std::size_t>(cap) * bs);
             FieldTask t{};
             t.kind = kDensity;
             t.component = kTotal;
-            t.out[0] = dout[first_channel[i]];
-            FieldTask* dtask = Ws{ctx}.get<FieldTask>(ctx->ws_tasks, 1);
+            t.out[0] = dout[pr.first_channel[i]];
+            FieldTask* dtask = grow<FieldTask>(ex.ws_tasks, 1);
             KUMO_HIP(hipMemcpyAsync(dtask, &t, sizeof(t), hipMemcpyHostToDevice, s));
-            for (long long b0 = 0; b0 < nblocks; b0 += cap) {
-                const int nb = static_cast<int>(std::min<long long>(cap, nblocks - b0));
+            for (long long b0 = b_first; b0 < b_end; b0 += cap) {
+                const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
                 const long long first = b0 * bs;
-                const long long count = std::min<long long>(static_cast<long long>(nb) * bs, n - first);
+                const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
                 launch_masks(mv, ps, n, bs, b0, nb, 0, thr, ts.t, s);
                 AoPLaunch a{};
                 a.mol = mv;
                 a.ps = ps;
                 a.tables = ts.t;
-                a.p[0] = mol->p_matrix(p.component == kTotal ? 0 : (p.component == kDifference ? 1 : p.component == kAlpha ? 2 : 3));
+                a.p[0] = mol->p_matrix(p.component == kTotal ? 0 : (p.component == kDifference ? 1 : p.component == kAlpha ? 2 : 3), s);
                 a.p[1] = nullptr;
                 a.npoints = n;
                 a.batch_first = first;
@@ -511,8 +443,8 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
                 a.first_block = b0;
                 a.nblk = nb;
                 a.order = 0;
-                a.phi = Ws{ctx}.get<double>(ctx->ws_phi, static_cast<std::size_t>(nb) * nao * bs);
-                a.w = Ws{ctx}.get<double>(ctx->ws_w, static_cast<std::size_t>(nb) * nao * 4 * bs);
+                a.phi = grow<double>(ex.ws_phi, static_cast<std::size_t>(nb) * nao * bs);
+                a.w = grow<double>(ex.ws_w, static_cast<std::size_t>(nb) * nao * 4 * bs);
                 a.moments = chm;
                 launch_fill(chm, 2 * kMoments * count, 0.0, s);
                 launch_ao_p(a, s);
@@ -520,37 +452,128 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const Point
                 const FieldsLaunch f = fields_launch(cmp, count, first, 1, dtask);
                 launch_fields(f, s);
                 KUMO_HIP(hipStreamSynchronize(s));
-                progress.advance(static_cast<std::size_t>(count));
+                advance(static_cast<std::size_t>(count));
             }
         }
         for (int channel = 0; channel < 2; ++channel) {
-            if (!orb_ch[channel]) continue;
             std::vector<std::size_t> keys;
             for (std::size_t i = 0; i < nprops; ++i)
                 if (on_grid_path(props[i], backend, grid) && props[i].kind == kOrbital && props[i].channel == channel)
                     keys.push_back(i);
+            if (keys.empty()) continue;
             run_orbitals(keys, channel, 0);
-            progress.advance(static_cast<std::size_t>(n));
+            advance(static_cast<std::size_t>(range));
         }
     }
-
-    // Duplicated property keys share their values (the reference evaluates a
-    // repeated property once; ours are written per key by the passes above).
     KUMO_HIP(hipStreamSynchronize(s));
-    // Kernel timing of the contraction launches.
-    double ms = 0.0;
-    for (std::size_t e = 0; e < ctx->events_used; ++e) {
-        float t = 0.f;
-        KUMO_HIP(hipEventElapsedTime(&t, ctx->events[e].first, ctx->events[e].second));
-        ms += t;
+}
+
+void screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsArg& pts, int block_size,
+                     int order, double threshold, uint8_t* kept) {
+    Executor& ex = ctx->exec;
+    if (block_size < 1 || block_size > 65536) invalid("block_size must be in [1, 65536]");
+    if (order < 0 || order > 2) invalid("AO order must be 0, 1 or 2");
+    const long long n = pts.grid ? validated_point_count(*pts.grid) : pts.n;
+    const double* dpts = nullptr;
+    if (!pts.grid) {
+        dpts = grow<double>(ex.ws_points, static_cast<std::size_t>(3 * n));
+        KUMO_HIP(hipMemcpyAsync(const_cast<double*>(dpts), pts.points, 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                                ex.stream));
     }
-    ctx->last_kernel_ms = ms;
-    ctx->last_padded_flops = ctx->last_flops;
+    const PointSource ps = make_source(pts.grid, dpts);
+    const long long nblocks = (n + block_size - 1) / block_size;
+    const int nao = mol->n, nsh = static_cast<int>(mol->basis.shells.size());
+    const int cap = static_cast<int>(std::min<long long>(nblocks, 4096));
+    const auto ts = tables(ex, std::max(1, cap), nao, nsh);
+    for (long long b0 = 0; b0 < nblocks; b0 += cap) {
+        const int nb = static_cast<int>(std::min<long long>(cap, nblocks - b0));
+        launch_masks(mol->view(), ps, n, block_size, b0, nb, order, threshold, ts.t, ex.stream);
+        KUMO_HIP(hipMemcpyAsync(kept + b0 * nao, ts.t.kept, static_cast<std::size_t>(nb) * nao, hipMemcpyDeviceToHost,
+                                ex.stream));
+    }
+    KUMO_HIP(hipStreamSynchronize(ex.stream));
+}
+
+void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol_c, const PointsArg& pts,
+              const kumo_hip_property* props, std::size_t nprops, const kumo_hip_options& opt, double* const* outputs,
+              int32_t* backend_used) {
+    auto* mol = const_cast<kumo_hip_molecule*>(mol_c);
+    Executor& ex = ctx->exec;
+    hipStream_t s = ex.stream;
+    const Prepared pr = prepare(mol, pts, props, nprops, opt, opt.device_outputs != 0);
+    if (backend_used) *backend_used = pr.backend;
+    ex.reset_stats();
+    const long long n = pr.n;
+    if (nprops == 0 || n == 0) return;
+    std::vector<double*> dout(pr.nout);
+    if (opt.device_outputs) {
+        for (std::size_t c = 0; c < pr.nout; ++c)
+            dout[c] = outputs[c];
+    } else {
+        ctx->ws_out.resize(pr.nout * static_cast<std::size_t>(n) * sizeof(double));
+        for (std::size_t c = 0; c < pr.nout; ++c)
+            dout[c] = ctx->ws_out.as<double>() + c * static_cast<std::size_t>(n);
+    }
+    const double* dpts = nullptr;
+    if (!pr.grid) {
+        if (opt.device_outputs) {
+            dpts = pts.points;
+        } else {
+            dpts = grow<double>(ex.ws_points, static_cast<std::size_t>(3 * n));
+            KUMO_HIP(hipMemcpyAsync(const_cast<double*>(dpts), pts.points, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+        }
+    }
+    const PointSource ps = make_source(pts.grid, dpts);
+    std::size_t done = 0;
+    const std::size_t total = static_cast<std::size_t>(n) * static_cast<std::size_t>(pr.passes);
+    const Advance advance = [&](std::size_t count) {
+        done += count;
+        if (opt.progress && opt.progress(done, total, opt.progress_user) != 0)
+            throw Error(KUMO_HIP_ABORTED, "evaluation aborted by the progress callback");
+    };
+    run_range(ex, mol, pr, ps, opt, dout.data(), 0, n, advance);
+    ex.collect_timing();
     if (!opt.device_outputs) {
-        for (std::size_t c = 0; c < nout; ++c)
+        for (std::size_t c = 0; c < pr.nout; ++c)
             KUMO_HIP(hipMemcpyAsync(outputs[c], dout[c], n * sizeof(double), hipMemcpyDeviceToHost, s));
         KUMO_HIP(hipStreamSynchronize(s));
     }
+}
+
+void Executor::create(int device, const double* table) {
+    KUMO_HIP(hipSetDevice(device));
+    KUMO_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    boys_table = table;
+}
+
+void Executor::destroy() {
+    for (auto& e : events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    events.clear();
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_comp, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc})
+        b->release();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+void Executor::reset_stats() {
+    events_used = 0;
+    last_kernel_ms = 0.0;
+    last_flops = 0.0;
+    last_launches = 0;
+}
+
+void Executor::collect_timing() {
+    double ms = 0.0;
+    for (std::size_t e = 0; e < events_used; ++e) {
+        float t = 0.f;
+        KUMO_HIP(hipEventElapsedTime(&t, events[e].first, events[e].second));
+        ms += t;
+    }
+    last_kernel_ms = ms;
 }
 
 } // namespace kumo

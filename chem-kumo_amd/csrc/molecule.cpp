@@ -206,7 +206,7 @@ std::unique_ptr<kumo_hip_molecule> load_molecule(kumo_hip_context* ctx, const ku
     if (d.e_alpha && m->nmo_a) m->Ea.assign(d.e_alpha, d.e_alpha + m->nmo_a);
     if (d.e_beta && m->nmo_b) m->Eb.assign(d.e_beta, d.e_beta + m->nmo_b);
     assess(*m);
-    upload_device(*m, ctx->stream);
+    upload_device(*m, ctx->exec.stream);
     return m;
 }
 
@@ -316,15 +316,16 @@ std::vector<double> boys_table_host() {
 
 } // namespace kumo
 
-const double* kumo_hip_molecule::p_matrix(int component) {
+const double* kumo_hip_molecule::p_matrix(int component, hipStream_t stream) {
     using namespace kumo;
     if (component <= 1) return d_p[component].as<double>();
+    std::lock_guard<std::mutex> guard(lazy);
     if (!d_p[component].get()) {
         std::vector<double> m(P.size());
         for (std::size_t q = 0; q < P.size(); ++q)
             m[q] = 0.5 * (component == 2 ? (P[q] + S[q]) : (P[q] - S[q]));
-        d_p[component] = upload(m.data(), m.size(), ctx->stream);
-        KUMO_HIP(hipStreamSynchronize(ctx->stream));
+        d_p[component] = upload(m.data(), m.size(), stream);
+        KUMO_HIP(hipStreamSynchronize(stream));
     }
     return d_p[component].as<double>();
 }
@@ -332,8 +333,9 @@ const double* kumo_hip_molecule::p_matrix(int component) {
 // Primitive pairs of the exact ESP: P contracted with the Hermite expansion
 // coefficients of every pair of Cartesian primitives of blocks a <= b
 // (src/electrostatic_potential.cpp:117-229), grouped by L = la + lb.
-void kumo_hip_molecule::ensure_esp() {
+void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
     using namespace kumo;
+    std::lock_guard<std::mutex> guard(lazy);
     if (esp_ready) return;
     struct Triple {
         int t, u, v;
@@ -466,10 +468,10 @@ void kumo_hip_molecule::ensure_esp() {
     std::vector<EspAtomDev> at;
     for (const auto& a : atoms)
         at.push_back({a.xyz[0], a.xyz[1], a.xyz[2], a.charge});
-    d_esp_pairs = upload(pairs.data(), pairs.size(), ctx->stream);
-    d_esp_coef = upload(coef.data(), coef.size(), ctx->stream);
-    d_esp_atoms = upload(at.data(), at.size(), ctx->stream);
+    d_esp_pairs = upload(pairs.data(), pairs.size(), stream);
+    d_esp_coef = upload(coef.data(), coef.size(), stream);
+    d_esp_atoms = upload(at.data(), at.size(), stream);
     esp_npairs = static_cast<int>(pairs.size());
-    KUMO_HIP(hipStreamSynchronize(ctx->stream));
+    KUMO_HIP(hipStreamSynchronize(stream));
     esp_ready = true;
 }

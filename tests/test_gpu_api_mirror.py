@@ -18,3 +18,17 @@ def test_cpp_api_mirror():
     r = subprocess.run([BINARY, fchk], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+    # The integral printed by the C++ program equals the oracle's on the same
+    # Grid::around(atoms, 3.0, 0.3) grid.
+    import numpy as np
+    from kumo_hip import synthetic
+    from support import oracle
+    from support.fchk import FchkData
+    d = FchkData(fchk)
+    grid = synthetic.grid_around(d.xyz, 3.0, 0.3)
+    line = [x for x in r.stdout.splitlines() if x.startswith("grid ")][0].split()
+    assert [int(v) for v in line[1:4]] == list(grid["counts"])
+    assert np.array_equal(np.array([float(v) for v in line[5:8]]), grid["origin"])
+    got = float([x for x in r.stdout.splitlines() if x.startswith("density_sum=")][0].split()[0].split("=")[1])
+    ref, _ = oracle.Molecule(d).evaluate([(0, 0, 0, 0)], grid=grid)
+    assert abs(got - ref[0][0].sum()) <= 1e-12 * ref[0][0].sum()
