@@ -22,6 +22,8 @@ struct MolView {
     const double* exps;
     const double* weights;
     const double* tdense; // [kMaxL+1][kMaxPure][kMaxCart] Cartesian->pure, zero padded
+    const double* radw;   // per primitive: weight of the block's first Cartesian component
+    const double* tg;     // [kMaxL+1][kMaxPure][kMaxCart] T[r][k] g[k] (transform of unscaled components)
 };
 
 // Cartesian powers in Gaussian FCHK order (src/basis_evaluator.cpp:135-152).
@@ -317,6 +319,140 @@ __device__ __attribute__((always_inline)) inline void eval_block_cartesian(const
     case 3: eval_cartesian_rows<3, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
     case 4: eval_cartesian_rows<4, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
     default: eval_cartesian_rows<5, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    }
+}
+
+// Factored evaluation for the contraction kernels. The contraction weights of
+// a block factor as weights[k][p] = g[k] radw[p] (g depends on l and the
+// component only), so component k is g[k] P_k(x, y, z) R(r^2) with
+// P_k = x^i y^j z^k and R = sum_p radw[p] exp(-a_p r^2). The kernels fold g[k]
+// into the MO coefficients and generate the unscaled P_k R and its
+// derivatives from three radial sums: R0 = R, R1 = sum_p -2 a_p radw[p] e_p,
+// R2 = sum_p 4 a_p^2 radw[p] e_p, with d/dx R0 = x R1 and d/dx R1 = x R2.
+template <int ORDER>
+__device__ __attribute__((always_inline)) inline void radial_sums(const DevShell& s, const MolView& m, double r2,
+                                                                double (&R)[3]) {
+    const double* e = m.exps + s.prim;
+    const double* u = m.radw + s.prim;
+    double r0 = 0.0, r1 = 0.0, rr = 0.0;
+    for (int p = 0; p < s.nprim; ++p) {
+        const double a = e[p];
+        const double g = u[p] * exp(-a * r2);
+        r0 += g;
+        if constexpr (ORDER >= 1) {
+            const double t = -2.0 * a * g;
+            r1 += t;
+            if constexpr (ORDER == 2) rr += -2.0 * a * t;
+        }
+    }
+    R[0] = r0;
+    R[1] = r1;
+    R[2] = rr;
+}
+
+// Panels of the unscaled component P_k R for each k of angular momentum L:
+// emit(k, v[NP]). With f0 = c^q, f1 = q c^(q-1), f2 = q (q-1) c^(q-2) per axis:
+//   d/da (P R)      = P_a R0 + a P R1
+//   d2/da2 (P R)    = P_aa R0 + (2 a P_a + P) R1 + a^2 P R2
+//   d2/da db (P R)  = P_ab R0 + (b P_a + a P_b) R1 + a b P R2.
+template <int L, int ORDER, unsigned PSET, typename Emit>
+__device__ __attribute__((always_inline)) inline void cart_components(double x, double y, double z, const double (&R)[3],
+                                                                    Emit&& emit) {
+    constexpr int NC = (L + 1) * (L + 2) / 2;
+    constexpr int NP = popcount10(PSET);
+    const double c[3] = {x, y, z};
+    double pw[3][L + 1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        pw[a][0] = 1.0;
+#pragma unroll
+        for (int n = 1; n <= L; ++n)
+            pw[a][n] = pw[a][n - 1] * c[a];
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        double f[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int q = kPow[L][k][a];
+            f[a][0] = pw[a][q];
+            f[a][1] = q >= 1 ? q * pw[a][q >= 1 ? q - 1 : 0] : 0.0;
+            f[a][2] = q >= 2 ? q * (q - 1) * pw[a][q >= 2 ? q - 2 : 0] : 0.0;
+        }
+        const double P = f[0][0] * f[1][0] * f[2][0];
+        double v[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int cc = panel_component(PSET, i);
+            if (cc == 0) {
+                v[i] = P * R[0];
+            } else if (cc <= 3) {
+                const int a = cc - 1;
+                const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+                v[i] = Pa * R[0] + c[a] * P * R[1];
+            } else if (cc == 4 || cc == 7 || cc == 9) {
+                const int a = cc == 4 ? 0 : (cc == 7 ? 1 : 2);
+                const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+                const double Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
+                v[i] = Paa * R[0] + (2.0 * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
+            } else {
+                const int a = cc == 8 ? 1 : 0, b = cc == 5 ? 1 : 2;
+                const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
+                const double Pb = f[0][0] * (b == 1 ? f[1][1] : f[1][0]) * (b == 2 ? f[2][1] : f[2][0]);
+                const double Pab = (a == 0 ? f[0][1] : f[0][0]) * ((a == 1 || b == 1) ? f[1][1] : f[1][0]) *
+                                   (b == 2 ? f[2][1] : f[2][0]);
+                v[i] = Pab * R[0] + (c[b] * Pa + c[a] * Pb) * R[1] + c[a] * c[b] * P * R[2];
+            }
+        }
+        emit(k, v);
+    }
+}
+
+// cart_components with the angular momentum and the component index as
+// runtime values (a loop over components instead of an unrolled body): for
+// the rare high-l blocks and partially kept pure blocks, where unrolling every
+// component would cost registers in every other path of a kernel.
+template <int ORDER, unsigned PSET>
+__device__ __attribute__((always_inline)) inline void cart_component_rt(int L, int k, double x, double y, double z,
+                                                                      const double (&R)[3],
+                                                                      double (&v)[popcount10(PSET)]) {
+    constexpr int NP = popcount10(PSET);
+    const double c[3] = {x, y, z};
+    double f[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int q = kPow[L][k][a];
+        double p2 = 1.0; // c^(q-2)
+        for (int n = 2; n < q; ++n)
+            p2 *= c[a];
+        const double p1 = q >= 2 ? p2 * c[a] : 1.0; // c^(q-1)
+        f[a][0] = q >= 1 ? p1 * c[a] : 1.0;
+        f[a][1] = q >= 1 ? q * p1 : 0.0;
+        f[a][2] = q >= 2 ? q * (q - 1) * p2 : 0.0;
+    }
+    const double P = f[0][0] * f[1][0] * f[2][0];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const int cc = panel_component(PSET, i);
+        if (cc == 0) {
+            v[i] = P * R[0];
+        } else if (cc <= 3) {
+            const int a = cc - 1;
+            const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+            v[i] = Pa * R[0] + c[a] * P * R[1];
+        } else if (cc == 4 || cc == 7 || cc == 9) {
+            const int a = cc == 4 ? 0 : (cc == 7 ? 1 : 2);
+            const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+            const double Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
+            v[i] = Paa * R[0] + (2.0 * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
+        } else {
+            const int a = cc == 8 ? 1 : 0, b = cc == 5 ? 1 : 2;
+            const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
+            const double Pb = f[0][0] * (b == 1 ? f[1][1] : f[1][0]) * (b == 2 ? f[2][1] : f[2][0]);
+            const double Pab = (a == 0 ? f[0][1] : f[0][0]) * ((a == 1 || b == 1) ? f[1][1] : f[1][0]) *
+                               (b == 2 ? f[2][1] : f[2][0]);
+            v[i] = Pab * R[0] + (c[b] * Pa + c[a] * Pb) * R[1] + c[a] * c[b] * P * R[2];
+        }
     }
 }
 

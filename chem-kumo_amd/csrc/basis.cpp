@@ -132,6 +132,14 @@ std::vector<double> gaussian_cartesian_to_pure(int l) {
     return t;
 }
 
+double component_factor(const CartPower& first, const CartPower& p) {
+    const double d0 = odd_double_factorial(2 * first.x - 1) * odd_double_factorial(2 * first.y - 1) *
+                      odd_double_factorial(2 * first.z - 1);
+    const double dk = odd_double_factorial(2 * p.x - 1) * odd_double_factorial(2 * p.y - 1) *
+                      odd_double_factorial(2 * p.z - 1);
+    return std::sqrt(d0 / dk);
+}
+
 PreparedBasis prepare_basis(const ShellRecords& r) {
     PreparedBasis out;
     if (r.nshells <= 0) runtime("fchk basis has no shells");
@@ -215,6 +223,9 @@ PreparedBasis prepare_basis(const ShellRecords& r) {
             }
     }
     out.pure_index.first[kMaxL + 1] = static_cast<int32_t>(out.pure_terms.size());
+    int cart_rows = 0;
+    for (const auto& b : out.blocks)
+        cart_rows += b.ncart();
     for (const auto& b : out.blocks) {
         DevShell d{};
         for (int a = 0; a < 3; ++a)
@@ -231,8 +242,29 @@ PreparedBasis prepare_basis(const ShellRecords& r) {
         d.atom = b.atom;
         out.exps.insert(out.exps.end(), b.exps.begin(), b.exps.end());
         out.weights.insert(out.weights.end(), b.weights.begin(), b.weights.end());
+        // Factored weights: weights[k][p] = g[k] * weights[0][p].
+        const int np = b.nprim(), nc = b.ncart();
+        for (int p = 0; p < np; ++p) {
+            const double w0 = b.weights[static_cast<std::size_t>(p)];
+            out.radw.push_back(w0);
+            for (int k = 0; k < nc; ++k) {
+                const double wk = b.weights[static_cast<std::size_t>(k * np + p)];
+                const double gk = component_factor(b.powers[0], b.powers[static_cast<std::size_t>(k)]);
+                if (std::abs(gk * w0 - wk) > 1e-13 * std::abs(wk))
+                    runtime("contraction weights of a shell do not factor into radial and angular parts");
+            }
+        }
         out.shells.push_back(d);
     }
+    int crow = 0, pure_rows = 0;
+    for (std::size_t i = 0; i < out.shells.size(); ++i) {
+        DevShell& d = out.shells[i];
+        d.crow = crow;
+        crow += d.ncart;
+        d.prow = d.pure ? cart_rows + pure_rows : -1;
+        if (d.pure) pure_rows += d.count;
+    }
+    out.operand_rows = cart_rows + pure_rows;
     return out;
 }
 

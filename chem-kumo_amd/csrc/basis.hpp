@@ -45,13 +45,18 @@ struct ShellRecords {
     int nbasis = 0;
 };
 
+// Angular factor g of Cartesian component p relative to the block's first
+// component: primitive_norm(a, p) = g * primitive_norm(a, first) for every a.
+double component_factor(const CartPower& first, const CartPower& p);
+
 struct PreparedBasis {
     std::vector<HostBlock> blocks;
     int nao = 0;
     int max_l = 0;
+    int operand_rows = 0; // Cartesian rows of all blocks, then pure rows of pure blocks
     // Device tables
     std::vector<DevShell> shells;
-    std::vector<double> exps, weights;
+    std::vector<double> exps, weights, radw;
     std::vector<PureTerm> pure_terms;
     PureIndex pure_index{};
 };

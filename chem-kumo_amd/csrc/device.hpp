@@ -9,8 +9,15 @@
 //  - pure[]: sparse Cartesian->pure transform entries per angular momentum,
 //    ordered by Cartesian component so that each pure AO accumulates its terms
 //    in increasing component order;
-//  - the occupied MO coefficients, AO-major ([AO][MO slot]) in FP64, FP32 and
-//    bf16 splits, both spins side by side in one padded slot range;
+//  - radw[]: per primitive the contraction weight of the block's first
+//    Cartesian component; weights[k][p] = g[k] radw[p] with g[k] depending on
+//    l and the component only, so a block's components are g[k] times
+//    x^i y^j z^k R(r) with R(r) = sum_p radw[p] exp(-a_p r^2);
+//  - the MO operand table of the AO->MO contraction: per block, Cartesian rows
+//    g[k] (T^T C)[k] (the pure transform and the component factor folded into
+//    the occupied-MO coefficients), then for pure blocks the plain pure rows
+//    C[r]; one row per [MO slot] range (alpha then beta, each padded to 16),
+//    in FP64, FP32 and three bf16 split planes;
 //  - P total and P spin, row-major FP64.
 #pragma once
 
@@ -35,6 +42,8 @@ struct DevShell {
     int32_t ao;     // first AO
     int32_t pure;   // 1 if the block is pure (l >= 2)
     int32_t atom;
+    int32_t crow;   // first Cartesian row of the block in the MO operand table
+    int32_t prow;   // first pure row of the block in the MO operand table (-1 if Cartesian)
     int32_t pad;
 };
 

@@ -39,31 +39,48 @@ struct TableSet {
     KeptTables t;
 };
 
-TableSet tables(Executor& ex, int nblk, int nao, int nsh) {
-    const std::size_t b = static_cast<std::size_t>(nblk);
-    const std::size_t bytes_kept = b * nao, bytes_list = 4 * b * nao, bytes_n = 4 * b, bytes_ks = 4 * b * nsh,
-                      bytes_kb = 4 * b * nsh, bytes_kr = b * nsh, bytes_cb = 4 * b * (nsh + 1), bytes_nc = 4 * b;
+TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
+    const std::size_t b = static_cast<std::size_t>(nblk), ns = static_cast<std::size_t>(nsh);
+    const std::size_t mc = static_cast<std::size_t>(max_chunks);
+    const std::size_t sizes[] = {
+        b * nao,                // kept
+        4 * b * nao,            // kept_list
+        4 * b,                  // nkept
+        4 * b * ns,             // kept_shell
+        4 * b * ns,             // kept_bits
+        4 * b,                  // nkshell
+        4 * b * ns,             // ent
+        4 * b * ns,             // ent_bits
+        b * ns,                 // ent_row
+        4 * b * (mc + 1),       // chunk_begin
+        4 * b * mc * kChunkRows, // chunk_rows
+        4 * b,                  // nchunks
+    };
     const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };
-    const std::size_t total = al(bytes_kept) + al(bytes_list) + al(bytes_n) + al(bytes_ks) + al(bytes_kb) + al(bytes_kr) +
-                              al(bytes_cb) + al(bytes_nc);
+    std::size_t total = 0;
+    for (std::size_t x : sizes)
+        total += al(x);
     ex.ws_tables.resize(total);
     char* p = ex.ws_tables.as<char>();
+    void* ptr[12];
+    for (int i = 0; i < 12; ++i) {
+        ptr[i] = p;
+        p += al(sizes[i]);
+    }
     TableSet s;
-    s.t.kept = reinterpret_cast<uint8_t*>(p);
-    p += al(bytes_kept);
-    s.t.kept_list = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_list);
-    s.t.nkept = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_n);
-    s.t.kept_shell = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_ks);
-    s.t.kept_bits = reinterpret_cast<uint32_t*>(p);
-    p += al(bytes_kb);
-    s.t.kept_row = reinterpret_cast<uint8_t*>(p);
-    p += al(bytes_kr);
-    s.t.chunk_begin = reinterpret_cast<int32_t*>(p);
-    p += al(bytes_cb);
-    s.t.nchunks = reinterpret_cast<int32_t*>(p);
+    s.t.kept = static_cast<uint8_t*>(ptr[0]);
+    s.t.kept_list = static_cast<int32_t*>(ptr[1]);
+    s.t.nkept = static_cast<int32_t*>(ptr[2]);
+    s.t.kept_shell = static_cast<int32_t*>(ptr[3]);
+    s.t.kept_bits = static_cast<uint32_t*>(ptr[4]);
+    s.t.nkshell = static_cast<int32_t*>(ptr[5]);
+    s.t.ent = static_cast<int32_t*>(ptr[6]);
+    s.t.ent_bits = static_cast<uint32_t*>(ptr[7]);
+    s.t.ent_row = static_cast<uint8_t*>(ptr[8]);
+    s.t.chunk_begin = static_cast<int32_t*>(ptr[9]);
+    s.t.chunk_rows = static_cast<int32_t*>(ptr[10]);
+    s.t.nchunks = static_cast<int32_t*>(ptr[11]);
+    s.t.max_chunks = max_chunks;
     return s;
 }
 
@@ -218,7 +235,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             const long long per_block = static_cast<long long>(nao) * (ao_order == 0 ? 1 : (ao_order == 1 ? 4 : 10)) * bs;
             cap = static_cast<int>(std::max(1ll, std::min<long long>(cap, (512ll << 20) / (8 * per_block))));
         }
-        const auto ts = tables(ex, cap, nao, nsh);
+        const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks);
         const long long batch_max = static_cast<long long>(cap) * bs;
         int slot0 = 0, slot1 = 0;
         bool need_a = false, need_b = false;
@@ -265,8 +282,8 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 AoMoLaunch a{};
                 a.mol = mv;
                 a.ps = ps;
-                a.mo = {mol->d_mo64.as<double>(), mol->d_mo32.as<float>(), mol->d_mobf.as<uint16_t>(), mol->slots,
-                        mol->alpha_slots};
+                a.mo = {mol->d_op64.as<double>(), mol->d_op32.as<float>(), mol->d_opbf.as<uint16_t>(), mol->operand_rows,
+                        mol->slots, mol->alpha_slots};
                 a.tables = ts.t;
                 a.npoints = n;
                 a.batch_first = first;
@@ -416,7 +433,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             const long long per_block = static_cast<long long>(nao) * bs;
             const int cap =
                 static_cast<int>(std::max(1ll, std::min<long long>({nblocks, 8192ll, (256ll << 20) / (8 * per_block)})));
-            const auto ts = tables(ex, cap, nao, nsh);
+            const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks);
             double* chm = grow<double>(ex.ws_moments, 2 * kMoments * static_cast<std::size_t>(cap) * bs);
             double* cmp = grow<double>(ex.ws_comp, 4 * kComp * static_cast<std::size_t>(cap) * bs);
             FieldTask t{};
@@ -484,7 +501,7 @@ void screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const 
     const long long nblocks = (n + block_size - 1) / block_size;
     const int nao = mol->n, nsh = static_cast<int>(mol->basis.shells.size());
     const int cap = static_cast<int>(std::min<long long>(nblocks, 4096));
-    const auto ts = tables(ex, std::max(1, cap), nao, nsh);
+    const auto ts = tables(ex, std::max(1, cap), nao, nsh, mol->max_chunks);
     for (long long b0 = 0; b0 < nblocks; b0 += cap) {
         const int nb = static_cast<int>(std::min<long long>(cap, nblocks - b0));
         launch_masks(mol->view(), ps, n, block_size, b0, nb, order, threshold, ts.t, ex.stream);

@@ -2,19 +2,23 @@
 //
 // One workgroup (4 waves) owns T points of one screening block and up to 128
 // MO slots (alpha then beta occupied MOs, each padded to 16). It walks the
-// block's kept AO blocks chunk by chunk (whole blocks, at most 32 Cartesian
-// component rows per chunk). The contraction runs over Cartesian components:
-// since the pure transform T is linear, C_pure^T (T Phi_cart) =
-// (T^T C_pure)^T Phi_cart, so
-//   1. every lane evaluates the Cartesian components of one AO block at one
-//      point in FP64 (ao_eval.hpp) and writes the needed derivative panels
-//      into LDS in the contraction precision (FP64, FP32 or bf16 hi/mid/lo
-//      splits);
-//   2. the chunk's coefficient rows C_cart = T^T C_kept are formed in FP64
-//      from the occupied-MO coefficients of the block's kept AOs only (the
-//      screening mask is applied here) and staged in LDS;
-//   3. MFMA tiles accumulate Psi[panel] = C_cart^T Phi_cart[panel]
-//      (16 MOs x 16 points per tile, all panels of a tile in one wave).
+// block's contraction chunks (at most 32 operand rows each, built by the mask
+// kernel from the kept AO blocks, ordered by descending l). Per chunk:
+//   1. the chunk's rows of the MO operand table are copied to LDS with
+//      coalesced 16-byte loads: for a fully kept block these are its
+//      Cartesian rows g[k] (T^T C)[k] (pure transform and component factor
+//      folded into the coefficients), for a partially kept pure block its
+//      plain pure rows C[r];
+//   2. lanes (one point each, T lanes per AO block) generate the matching
+//      Phi rows in FP64 from the block's three radial sums: unscaled
+//      Cartesian components P_k R for the Cartesian rows (zero for dropped AOs
+//      of a partially kept Cartesian block), or the pure AOs sum_k T g P_k R
+//      (zero for dropped ones) for pure rows, and write the needed derivative
+//      panels to LDS in the contraction precision (FP64, FP32 or bf16 splits);
+//   3. MFMA tiles accumulate Psi[panel] = C_rows^T Phi_rows[panel]
+//      (16 MOs x 16 points per tile, all panels of a tile in one wave; a wave
+//      keeps one 16-point column of tiles, so each Phi operand it loads feeds
+//      all of its MO tiles).
 // The epilogue forms the per-point dot products over MOs in FP64 (value,
 // gradient, tau terms, Hessian) and reduces them over the tiles of each
 // spin channel in a fixed order, so the result is independent of batching.
@@ -23,6 +27,10 @@
 // and B column l % 16 at a K slice selected by l / 16; the accumulator lane l
 // holds D rows 4 (l / 16) + r, column l % 16. The K slice order is the same
 // for A and B, so the sum over K is exact whatever order the hardware uses.
+// LDS strides are chosen so that the 16 lanes of each K slice hit distinct
+// banks: coefficient rows have a pitch of 132 (bf16, FP32) or 144 (FP64)
+// elements; Phi rows of a point are padded (FP32: 36, FP64: 34 elements) or,
+// for bf16, have their 16-byte granules XOR-swizzled by bits 2-3 of the point.
 #include "kernels.hpp"
 
 #include "../../include/kumo_hip.h"
@@ -36,85 +44,159 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int KC = kChunkRows; // 32 AO rows per chunk
+constexpr int KC = kChunkRows; // 32 operand rows per chunk
 
 template <int PREC> struct Prec;
 template <> struct Prec<KUMO_HIP_FP64> {
-    using E = double;      // LDS element
+    using E = double;
     using Acc = f64x4;
-    static constexpr int splits = 1;
-    static constexpr int T = 16;   // points per workgroup
-    static constexpr int TPW = 2;  // tiles per wave (8 m-tiles x 1 n-tile / 4 waves)
-    static constexpr int LD = 36;  // LDS row stride (elements)
+    static constexpr int S = 1;       // split planes
+    static constexpr int T = 16;      // points per workgroup
+    static constexpr int LDK = 34;    // Phi elements per point row
+    static constexpr int PITCH = 144; // coefficient row pitch (elements)
 };
 template <> struct Prec<KUMO_HIP_FP32> {
     using E = float;
     using Acc = f32x4;
-    static constexpr int splits = 1;
+    static constexpr int S = 1;
     static constexpr int T = 32;
-    static constexpr int TPW = 4;
-    static constexpr int LD = 36;
+    static constexpr int LDK = 36;
+    static constexpr int PITCH = 132;
 };
 template <> struct Prec<KUMO_HIP_BF16X3> {
     using E = uint16_t;
     using Acc = f32x4;
-    static constexpr int splits = 2;
+    static constexpr int S = 2;
     static constexpr int T = 32;
-    static constexpr int TPW = 4;
-    static constexpr int LD = 40;
+    static constexpr int LDK = 32;
+    static constexpr int PITCH = 132;
 };
 template <> struct Prec<KUMO_HIP_BF16X6> {
     using E = uint16_t;
     using Acc = f32x4;
-    static constexpr int splits = 3;
+    static constexpr int S = 3;
     static constexpr int T = 32;
-    static constexpr int TPW = 4;
-    static constexpr int LD = 40;
+    static constexpr int LDK = 32;
+    static constexpr int PITCH = 132;
 };
 
-__device__ inline uint16_t bf16_rn(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
-}
-__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+template <int PREC> constexpr bool is_bf16 = PREC == KUMO_HIP_BF16X3 || PREC == KUMO_HIP_BF16X6;
 
-// Writes value v (FP64) as the element(s) of split planes.
-template <int PREC>
-__device__ inline void store_split(typename Prec<PREC>::E* base, size_t plane, double v) {
+__device__ __attribute__((always_inline)) inline uint16_t to_bf16(float f) {
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+}
+__device__ __attribute__((always_inline)) inline double from_bf16(uint16_t h) {
+    return static_cast<double>(__uint_as_float(static_cast<uint32_t>(h) << 16));
+}
+
+// Element offset of (point j, row k) within one Phi panel.
+template <int PREC> __device__ __attribute__((always_inline)) inline int phi_at(int j, int k) {
+    using P = Prec<PREC>;
+    if constexpr (is_bf16<PREC>)
+        return j * P::LDK + ((((k >> 3) ^ (j >> 2)) & 3) << 3) + (k & 7);
+    else
+        return j * P::LDK + k;
+}
+
+// Writes FP64 value v at panel i, point j, row k of Phi in the contraction
+// precision (bf16: hi, mid[, lo] splits of v, residuals formed in FP64).
+template <int PREC, int NP>
+__device__ __attribute__((always_inline)) inline void phi_store(typename Prec<PREC>::E* phi, int i, int j, int k,
+                                                               double v) {
+    using P = Prec<PREC>;
+    constexpr int plane = NP * P::T * P::LDK;
+    typename Prec<PREC>::E* d = phi + i * (P::T * P::LDK) + phi_at<PREC>(j, k);
     if constexpr (PREC == KUMO_HIP_FP64) {
-        base[0] = v;
+        d[0] = v;
     } else if constexpr (PREC == KUMO_HIP_FP32) {
-        base[0] = static_cast<float>(v);
+        d[0] = static_cast<float>(v);
     } else {
-        const uint16_t hi = bf16_rn(static_cast<float>(v));
-        const double r1 = v - static_cast<double>(bf16_to_f(hi));
-        const uint16_t mid = bf16_rn(static_cast<float>(r1));
-        base[0] = hi;
-        base[plane] = mid;
-        if constexpr (Prec<PREC>::splits == 3) {
-            const double r2 = r1 - static_cast<double>(bf16_to_f(mid));
-            base[2 * plane] = bf16_rn(static_cast<float>(r2));
+        const uint16_t hi = to_bf16(static_cast<float>(v));
+        const double r1 = v - from_bf16(hi);
+        const uint16_t mid = to_bf16(static_cast<float>(r1));
+        d[0] = hi;
+        d[plane] = mid;
+        if constexpr (P::S == 3) {
+            const double r2 = r1 - from_bf16(mid);
+            d[2 * plane] = to_bf16(static_cast<float>(r2));
         }
     }
 }
 
+// Phi rows of one chunk entry (AO block `sh` in `mode`) at point j, with the
+// components unrolled for angular momentum L.
+template <int PREC, int ORDER, unsigned PSET, int L>
+__device__ __attribute__((always_inline)) inline void gen_cart(typename Prec<PREC>::E* phi, bool masked, uint32_t bits,
+                                                              int row0, int j, double x, double y, double z,
+                                                              const double (&R)[3]) {
+    constexpr int NP = popcount10(PSET);
+    cart_components<L, ORDER, PSET>(x, y, z, R, [&](int k, const double (&c)[NP]) __attribute__((always_inline)) {
+        const bool keep = !masked || ((bits >> k) & 1u);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            phi_store<PREC, NP>(phi, i, j, row0 + k, keep ? c[i] : 0.0);
+    });
+}
+
+// The same with runtime l: Cartesian rows (l >= 4) or, for a partially kept
+// pure block, its pure rows sum_k T[r][k] g[k] P_k R (zero for dropped AOs).
 template <int PREC, int ORDER, unsigned PSET>
-__global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
+__device__ __attribute__((always_inline)) inline void gen_generic(typename Prec<PREC>::E* phi, const MolView& m,
+                                                                 const DevShell& sh, int mode, uint32_t bits, int row0,
+                                                                 int j, double x, double y, double z,
+                                                                 const double (&R)[3]) {
+    constexpr int NP = popcount10(PSET);
+    const int L = sh.l, nc = sh.ncart;
+    if (mode == kEntryPure) {
+        const double* tg = m.tg + L * kMaxPure * kMaxCart;
+        for (int r = 0; r < sh.count; ++r) {
+            double v[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                v[i] = 0.0;
+            if ((bits >> r) & 1u)
+                for (int k = 0; k < nc; ++k) {
+                    const double w = tg[r * kMaxCart + k];
+                    if (w == 0.0) continue;
+                    double cv[NP];
+                    cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
+#pragma unroll
+                    for (int i = 0; i < NP; ++i)
+                        v[i] += w * cv[i];
+                }
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                phi_store<PREC, NP>(phi, i, j, row0 + r, v[i]);
+        }
+        return;
+    }
+    for (int k = 0; k < nc; ++k) {
+        double cv[NP];
+        cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
+        const bool keep = mode != kEntryCartMasked || ((bits >> k) & 1u);
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            phi_store<PREC, NP>(phi, i, j, row0 + k, keep ? cv[i] : 0.0);
+    }
+}
+
+template <int PREC, int ORDER, unsigned PSET>
+__global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel(AoMoLaunch a) {
     using P = Prec<PREC>;
     using E = typename P::E;
-    constexpr int T = P::T, NT = T / 16, LD = P::LD, S = P::splits, TPW = P::TPW;
+    constexpr int T = P::T, NT = T / 16, S = P::S, LDK = P::LDK, PITCH = P::PITCH;
     constexpr int NP = popcount10(PSET);
+    constexpr int TPW = 8 * NT / 4; // tiles per wave at 128 slots
+    constexpr int PHI_PLANE = NP * T * LDK;
+    constexpr int CS_PLANE = KC * PITCH;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    // LDS: phi[S][NP][T][LD], cs[S][128][LD], red[8][kMoments][T] (double)
+    // LDS: phi[S][NP][T][LDK], cs[S][KC][PITCH]; after the chunk loop the
+    // same memory holds red[MT][kMoments][T] (double).
     E* phi = reinterpret_cast<E*>(smem);
-    const size_t phi_plane = static_cast<size_t>(NP) * T * LD;
-    E* cs = phi + S * phi_plane;
-    const size_t cs_plane = static_cast<size_t>(128) * LD;
-    double* red = reinterpret_cast<double*>(cs + S * cs_plane);
+    E* cs = phi + S * PHI_PLANE;
+    double* red = reinterpret_cast<double*>(smem);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // Tile: (screening block, point range) and MO slot group.
     const int tiles_per_block = (a.block_size + T - 1) / T;
     const int blk_local = blockIdx.x / tiles_per_block;
     const int sub = blockIdx.x % tiles_per_block;
@@ -125,17 +207,31 @@ __global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
     if (p0 >= bcount) return; // uniform over the workgroup
     const int nt_valid = min(T, bcount - p0);
     const int slot_lo = a.slot0 + 128 * blockIdx.y;
-    const int slot_hi = min(a.slot1, slot_lo + 128);
-    const int MT = (slot_hi - slot_lo) / 16;
+    const int width = min(a.slot1, slot_lo + 128) - slot_lo;
+    const int MT = width / 16;
     const int ntiles = MT * NT;
 
     const MolView& m = a.mol;
     const int nsh = m.nshells;
-    const int32_t* ks = a.tables.kept_shell + static_cast<long long>(blk_local) * nsh;
-    const uint32_t* kb = a.tables.kept_bits + static_cast<long long>(blk_local) * nsh;
-    const uint8_t* kr = a.tables.kept_row + static_cast<long long>(blk_local) * nsh;
-    const int32_t* cb = a.tables.chunk_begin + static_cast<long long>(blk_local) * (nsh + 1);
+    const int32_t* ent = a.tables.ent + static_cast<long long>(blk_local) * nsh;
+    const uint32_t* ebits = a.tables.ent_bits + static_cast<long long>(blk_local) * nsh;
+    const uint8_t* erow = a.tables.ent_row + static_cast<long long>(blk_local) * nsh;
+    const int32_t* cb = a.tables.chunk_begin + static_cast<long long>(blk_local) * (a.tables.max_chunks + 1);
+    const int32_t* crows = a.tables.chunk_rows + static_cast<long long>(blk_local) * a.tables.max_chunks * KC;
     const int nchunks = a.tables.nchunks[blk_local];
+
+    // Phi rows past a chunk's last row meet zero coefficient rows; they only
+    // need to be finite, so Phi starts zeroed.
+    {
+        uint4* z4 = reinterpret_cast<uint4*>(phi);
+        constexpr int n16 = static_cast<int>(sizeof(E)) * S * PHI_PLANE / 16;
+        for (int i = tid; i < n16; i += 256)
+            z4[i] = uint4{0, 0, 0, 0};
+    }
+    // This thread's point for AO generation (T divides 256).
+    const int j = tid % T;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (j < nt_valid) point_coords(a.ps, bstart + p0 + j, px, py, pz);
 
     typename P::Acc acc[TPW][NP];
 #pragma unroll
@@ -144,148 +240,161 @@ __global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
         for (int i = 0; i < NP; ++i)
             acc[t][i] = typename P::Acc{0, 0, 0, 0};
 
-    // Point coordinates of this lane's column for AO generation.
+    const E* op;
+    if constexpr (PREC == KUMO_HIP_FP64) op = reinterpret_cast<const E*>(a.mo.op64);
+    else if constexpr (PREC == KUMO_HIP_FP32) op = reinterpret_cast<const E*>(a.mo.op32);
+    else op = reinterpret_cast<const E*>(a.mo.opbf);
+    const long long op_plane = static_cast<long long>(a.mo.rows) * a.mo.slots;
+    constexpr int EPG = 16 / static_cast<int>(sizeof(E)); // elements per 16-byte granule
+    const int G = width / EPG;                             // granules per coefficient row
+
     for (int c = 0; c < nchunks; ++c) {
-        const int s_first = cb[c], s_last = cb[c + 1];
-        const int nsc = s_last - s_first;
-        int rows_used = 0;
+        const int e0 = cb[c], e1 = cb[c + 1];
+        int rows_used;
         {
-            const DevShell& lastsh = m.shells[ks[s_last - 1]];
-            rows_used = kr[s_last - 1] + lastsh.ncart;
+            const int last = ent[e1 - 1];
+            const DevShell& ls = m.shells[last & 0xffffff];
+            rows_used = erow[e1 - 1] + ((last >> 24) == kEntryPure ? ls.count : ls.ncart);
         }
-        // 1. Cartesian AO components of the chunk's blocks at the tile's points
-        //    (the block's mask and pure transform live in the staged C).
-        for (int item = tid; item < nsc * T; item += 256) {
-            const int j = item % T;
-            const int q = s_first + item / T;
-            const DevShell sh = m.shells[ks[q]];
-            const int row0 = kr[q];
-            E* dst = phi + static_cast<size_t>(j) * LD + row0;
-            bool inside = false;
-            double x = 0, y = 0, z = 0, r2 = 0;
-            if (j < nt_valid) {
-                double px, py, pz;
-                point_coords(a.ps, bstart + p0 + j, px, py, pz);
-                x = px - sh.o[0];
-                y = py - sh.o[1];
-                z = pz - sh.o[2];
-                r2 = sq_norm(x, y, z);
-                inside = r2 <= sh.cutoff_r2;
-            }
-            if (inside) {
-                eval_block_cartesian<ORDER, PSET>(sh, m, x, y, z, r2, [&](int k, const double (&v)[NP]) __attribute__((always_inline)) {
-#pragma unroll
-                    for (int i = 0; i < NP; ++i)
-                        store_split<PREC>(dst + static_cast<size_t>(i) * T * LD + k, phi_plane, v[i]);
-                });
+        // 1. Coefficient rows of the chunk (zero past its rows).
+        const int32_t* cr = crows + c * KC;
+        for (int item = tid; item < S * KC * G; item += 256) {
+            const int gi = item % G;
+            const int r = (item / G) % KC;
+            const int sp = item / (KC * G);
+            const int grow = cr[r];
+            uint4 v = uint4{0, 0, 0, 0};
+            if (grow >= 0)
+                v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
+                                                    slot_lo + gi * EPG);
+            E* d = cs + sp * CS_PLANE + r * PITCH + gi * EPG;
+            if constexpr (is_bf16<PREC>) {
+                reinterpret_cast<uint2*>(d)[0] = uint2{v.x, v.y};
+                reinterpret_cast<uint2*>(d)[1] = uint2{v.z, v.w};
             } else {
-                for (int k = 0; k < sh.ncart; ++k)
+                *reinterpret_cast<uint4*>(d) = v;
+            }
+        }
+        // 2. Phi rows of the chunk's entries at the tile's points.
+        for (int q = e0 + tid / T; q < e1; q += 256 / T) {
+            const int ev = ent[q];
+            const DevShell& sh = m.shells[ev & 0xffffff];
+            const int mode = ev >> 24;
+            const int row0 = erow[q];
+            const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
+            const double r2 = sq_norm(x, y, z);
+            if (j < nt_valid && r2 <= sh.cutoff_r2) {
+                double R[3];
+                radial_sums<ORDER>(sh, m, r2, R);
+                const uint32_t bits = ebits[q];
+                const bool masked = mode == kEntryCartMasked;
+                if (mode == kEntryPure || sh.l > 3) {
+                    gen_generic<PREC, ORDER, PSET>(phi, m, sh, mode, bits, row0, j, x, y, z, R);
+                } else {
+                    switch (sh.l) {
+                    case 0: gen_cart<PREC, ORDER, PSET, 0>(phi, masked, bits, row0, j, x, y, z, R); break;
+                    case 1: gen_cart<PREC, ORDER, PSET, 1>(phi, masked, bits, row0, j, x, y, z, R); break;
+                    case 2: gen_cart<PREC, ORDER, PSET, 2>(phi, masked, bits, row0, j, x, y, z, R); break;
+                    default: gen_cart<PREC, ORDER, PSET, 3>(phi, masked, bits, row0, j, x, y, z, R); break;
+                    }
+                }
+            } else {
+                const int nrows = mode == kEntryPure ? sh.count : sh.ncart;
+                for (int k = 0; k < nrows; ++k)
 #pragma unroll
                     for (int i = 0; i < NP; ++i)
-                        store_split<PREC>(dst + static_cast<size_t>(i) * T * LD + k, phi_plane, 0.0);
+                        phi_store<PREC, NP>(phi, i, j, row0 + k, 0.0);
             }
-        }
-        // Rows past the chunk's AOs are zero in both operands.
-        for (int item = tid; item < (KC - rows_used) * T * NP; item += 256) {
-            const int row = rows_used + item % (KC - rows_used);
-            const int rest = item / (KC - rows_used);
-            const int j = rest % T, i = rest / T;
-#pragma unroll
-            for (int sp = 0; sp < S; ++sp)
-                phi[sp * phi_plane + (static_cast<size_t>(i) * T + j) * LD + row] = E(0);
-        }
-        // 2. MO coefficients of the chunk's Cartesian rows, cs[slot][row]:
-        //    C_cart[k] = sum over kept pure AOs r of T[r][k] C[r] (pure blocks)
-        //    or C[k] if AO k is kept (Cartesian blocks), in FP64, then split.
-        const int width = slot_hi - slot_lo;
-        for (int item = tid; item < KC * width; item += 256) {
-            const int slot = item % width;
-            const int row = item / width;
-            E* d = cs + static_cast<size_t>(slot) * LD + row;
-            double v = 0.0;
-            if (row < rows_used) {
-                int q = s_first;
-                while (q + 1 < s_last && kr[q + 1] <= row)
-                    ++q;
-                const DevShell& sh = m.shells[ks[q]];
-                const uint32_t bits = kb[q];
-                const int k = row - kr[q];
-                const double* c = a.mo.c64 + static_cast<long long>(sh.ao) * a.mo.slots + slot_lo + slot;
-                if (!sh.pure) {
-                    if ((bits >> k) & 1u) v = c[static_cast<long long>(k) * a.mo.slots];
-                } else {
-                    const double* t = m.tdense + (sh.l * kMaxPure) * kMaxCart + k;
-                    for (int r = 0; r < sh.count; ++r)
-                        if ((bits >> r) & 1u) v += t[r * kMaxCart] * c[static_cast<long long>(r) * a.mo.slots];
-                }
-            }
-            store_split<PREC>(d, cs_plane, v);
         }
         __syncthreads();
-        // 3. MFMA over the chunk's rows.
+        // 3. MFMA over the chunk's rows. The wave's tiles share one 16-point
+        //    column nt and take MO tiles mt = (wave + 4 t) / NT.
         const int g = lane >> 4, r16 = lane & 15;
+        const int nt = wave % NT;
+        const int jb = nt * 16 + r16; // B column (point) of this lane
+        if constexpr (PREC == KUMO_HIP_FP64) {
+            for (int k0 = 0; k0 < rows_used; k0 += 4) {
+                double av[TPW];
 #pragma unroll
-        for (int t = 0; t < TPW; ++t) {
-            const int tile = wave + 4 * t;
-            if (tile >= ntiles) continue;
-            const int mt = tile / NT, nt = tile % NT;
-            const E* arow = cs + static_cast<size_t>(mt * 16 + r16) * LD;
-            const E* brow = phi + static_cast<size_t>(nt * 16 + r16) * LD;
-            if constexpr (PREC == KUMO_HIP_FP64) {
-                for (int k0 = 0; k0 < rows_used; k0 += 8) {
-                    const double2 av = *reinterpret_cast<const double2*>(arow + k0 + 2 * g);
-#pragma unroll
-                    for (int i = 0; i < NP; ++i) {
-                        const double2 bv = *reinterpret_cast<const double2*>(brow + static_cast<size_t>(i) * T * LD + k0 + 2 * g);
-                        acc[t][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.x, bv.x, acc[t][i], 0, 0, 0);
-                        acc[t][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av.y, bv.y, acc[t][i], 0, 0, 0);
-                    }
+                for (int t = 0; t < TPW; ++t) {
+                    const int mt = (wave + 4 * t) / NT;
+                    av[t] = cs[(k0 + g) * PITCH + mt * 16 + r16];
                 }
-            } else if constexpr (PREC == KUMO_HIP_FP32) {
-                for (int k0 = 0; k0 < rows_used; k0 += 16) {
-                    const float4 av = *reinterpret_cast<const float4*>(arow + k0 + 4 * g);
-#pragma unroll
-                    for (int i = 0; i < NP; ++i) {
-                        const float4 bv = *reinterpret_cast<const float4*>(brow + static_cast<size_t>(i) * T * LD + k0 + 4 * g);
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc[t][i], 0, 0, 0);
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc[t][i], 0, 0, 0);
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc[t][i], 0, 0, 0);
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc[t][i], 0, 0, 0);
-                    }
-                }
-            } else {
-                bf16x8 av[S];
-#pragma unroll
-                for (int sp = 0; sp < S; ++sp)
-                    av[sp] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const s16x8*>(arow + sp * cs_plane + 8 * g));
 #pragma unroll
                 for (int i = 0; i < NP; ++i) {
-                    bf16x8 bv[S];
+                    const double bv = phi[i * (T * LDK) + jb * LDK + k0 + g];
 #pragma unroll
-                    for (int sp = 0; sp < S; ++sp)
-                        bv[sp] = __builtin_bit_cast(
-                            bf16x8, *reinterpret_cast<const s16x8*>(brow + sp * phi_plane + static_cast<size_t>(i) * T * LD + 8 * g));
+                    for (int t = 0; t < TPW; ++t)
+                        if (wave + 4 * t < ntiles)
+                            acc[t][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv, acc[t][i], 0, 0, 0);
+                }
+            }
+        } else if constexpr (PREC == KUMO_HIP_FP32) {
+            for (int k0 = 0; k0 < rows_used; k0 += 16) {
+                float av[TPW][4];
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    const int mt = (wave + 4 * t) / NT;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        av[t][e] = cs[(k0 + 4 * g + e) * PITCH + mt * 16 + r16];
+                }
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const float4 bv = *reinterpret_cast<const float4*>(phi + i * (T * LDK) + jb * LDK + k0 + 4 * g);
+#pragma unroll
+                    for (int t = 0; t < TPW; ++t)
+                        if (wave + 4 * t < ntiles) {
+                            acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][0], bv.x, acc[t][i], 0, 0, 0);
+                            acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][1], bv.y, acc[t][i], 0, 0, 0);
+                            acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][2], bv.z, acc[t][i], 0, 0, 0);
+                            acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][3], bv.w, acc[t][i], 0, 0, 0);
+                        }
+                }
+            }
+        } else {
+            bf16x8 av[TPW][S];
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int mt = (wave + 4 * t) / NT;
+#pragma unroll
+                for (int sp = 0; sp < S; ++sp) {
+                    s16x8 w;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        w[e] = static_cast<short>(cs[sp * CS_PLANE + (8 * g + e) * PITCH + mt * 16 + r16]);
+                    av[t][sp] = __builtin_bit_cast(bf16x8, w);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                bf16x8 bv[S];
+#pragma unroll
+                for (int sp = 0; sp < S; ++sp)
+                    bv[sp] = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const s16x8*>(phi + sp * PHI_PLANE + i * (T * LDK) + phi_at<PREC>(jb, 8 * g)));
+#pragma unroll
+                for (int t = 0; t < TPW; ++t) {
+                    if (wave + 4 * t >= ntiles) continue;
                     // small terms first
                     if constexpr (S == 3) {
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[2], bv[0], acc[t][i], 0, 0, 0);
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv[1], acc[t][i], 0, 0, 0);
-                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv[2], acc[t][i], 0, 0, 0);
+                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][2], bv[0], acc[t][i], 0, 0, 0);
+                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][1], bv[1], acc[t][i], 0, 0, 0);
+                        acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][0], bv[2], acc[t][i], 0, 0, 0);
                     }
-                    acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[1], bv[0], acc[t][i], 0, 0, 0);
-                    acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv[1], acc[t][i], 0, 0, 0);
-                    acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[0], bv[0], acc[t][i], 0, 0, 0);
+                    acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][1], bv[0], acc[t][i], 0, 0, 0);
+                    acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][0], bv[1], acc[t][i], 0, 0, 0);
+                    acc[t][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t][0], bv[0], acc[t][i], 0, 0, 0);
                 }
             }
         }
         __syncthreads();
     }
 
+    __syncthreads(); // (a block without kept AOs skips the loop; red aliases Phi)
     // Epilogue: dot products over the MOs of each tile, FP64.
     // Raw sums q: 0 v0.v0, 1-3 v0.va, 4-6 va.va, 7-12 (v0.vac + va.vc).
-    constexpr int kPanelOf[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-    (void)kPanelOf;
-    const int g = lane >> 4, r16 = lane & 15;
-    (void)g;
+    const int r16 = lane & 15;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         const int tile = wave + 4 * t;
@@ -295,13 +404,13 @@ __global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
 #pragma unroll
         for (int k = 0; k < kMoments; ++k)
             q[k] = 0.0;
-        const auto V = [&](int comp, int r) -> double {
+        const auto V = [&](int comp, int r) __attribute__((always_inline)) -> double {
             // output panel index of derivative component `comp`
             int idx = -1, n = 0;
 #pragma unroll
-            for (int c = 0; c < 10; ++c)
-                if ((PSET >> c) & 1u) {
-                    if (c == comp) idx = n;
+            for (int cc = 0; cc < 10; ++cc)
+                if ((PSET >> cc) & 1u) {
+                    if (cc == comp) idx = n;
                     ++n;
                 }
             return idx < 0 ? 0.0 : static_cast<double>(acc[t][idx < 0 ? 0 : idx][r]);
@@ -343,16 +452,16 @@ __global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
     // Sum the m-tiles of each channel in order and store the moments.
     const int alpha_tiles_end = max(0, min(MT, (a.mo.alpha_slots - slot_lo) / 16));
     for (int item = tid; item < 2 * kMoments * T; item += 256) {
-        const int j = item % T;
+        const int jj = item % T;
         const int k = (item / T) % kMoments;
         const int ch = item / (T * kMoments);
-        if (j >= nt_valid) continue;
+        if (jj >= nt_valid) continue;
         const int mt0 = ch == 0 ? 0 : alpha_tiles_end, mt1 = ch == 0 ? alpha_tiles_end : MT;
         double s = 0.0;
         for (int mt = mt0; mt < mt1; ++mt)
-            s += red[(static_cast<size_t>(mt) * kMoments + k) * T + j];
+            s += red[(static_cast<size_t>(mt) * kMoments + k) * T + jj];
         const double f = k == 0 ? 1.0 : (k <= 3 ? 2.0 : (k <= 6 ? 0.5 : 2.0));
-        const long long point = bstart + p0 + j - a.batch_first;
+        const long long point = bstart + p0 + jj - a.batch_first;
         double* dst = a.moments + (static_cast<long long>(blockIdx.y) * 2 + ch) * kMoments * a.batch_points;
         dst[k * a.batch_points + point] = f * s;
     }
@@ -361,8 +470,9 @@ __global__ void __launch_bounds__(256) aomo_kernel(AoMoLaunch a) {
 template <int PREC, int ORDER, unsigned PSET> void launch_one(const AoMoLaunch& a, hipStream_t stream) {
     using P = Prec<PREC>;
     constexpr int NP = popcount10(PSET);
-    const size_t lds = sizeof(typename P::E) * P::splits * (static_cast<size_t>(NP) * P::T * P::LD + 128 * P::LD) +
-                       sizeof(double) * 8 * kMoments * P::T;
+    const size_t work = sizeof(typename P::E) * P::S * (static_cast<size_t>(NP) * P::T * P::LDK + KC * P::PITCH);
+    const size_t red = sizeof(double) * 8 * kMoments * P::T;
+    const size_t lds = work > red ? work : red;
     const int tiles_per_block = (a.block_size + P::T - 1) / P::T;
     const int groups = (a.slot1 - a.slot0 + 127) / 128;
     const dim3 grid(static_cast<unsigned>(a.nblk * tiles_per_block), static_cast<unsigned>(groups));
@@ -387,6 +497,7 @@ template <int PREC> void launch_prec(const AoMoLaunch& a, hipStream_t stream) {
 
 void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream) {
     if (a.nblk <= 0 || a.slot1 <= a.slot0) return;
+    if (a.slot0 % 16 || a.slot1 % 16) invalid("MO slot ranges must be multiples of 16");
     switch (a.precision) {
     case KUMO_HIP_FP64: launch_prec<KUMO_HIP_FP64>(a, stream); break;
     case KUMO_HIP_FP32: launch_prec<KUMO_HIP_FP32>(a, stream); break;

@@ -27,8 +27,7 @@ __global__ void __launch_bounds__(128) aop_phi_kernel(AoPLaunch a, int panels) {
     const int nsh = m.nshells;
     const int32_t* ks = a.tables.kept_shell + static_cast<long long>(blk_local) * nsh;
     const uint32_t* kb = a.tables.kept_bits + static_cast<long long>(blk_local) * nsh;
-    const int32_t* cb = a.tables.chunk_begin + static_cast<long long>(blk_local) * (nsh + 1);
-    const int nkshell = cb[a.tables.nchunks[blk_local]];
+    const int nkshell = a.tables.nkshell[blk_local];
     double* phi = a.phi + static_cast<long long>(blk_local) * nao * panels * B;
     double px, py, pz;
     point_coords(a.ps, bstart + j, px, py, pz);

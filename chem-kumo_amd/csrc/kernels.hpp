@@ -10,21 +10,41 @@
 
 namespace kumo {
 
-// Rows of one contraction chunk: whole kept AO blocks are packed greedily
-// into chunks of at most kChunkRows AO rows.
+// Rows of one contraction chunk: kept AO blocks are packed into chunks of at
+// most kChunkRows operand rows.
 constexpr int kChunkRows = 32;
+
+// How a kept AO block enters the AO->MO contraction.
+enum EntryMode : int32_t {
+    kEntryCart = 0,       // every AO kept: the block's Cartesian rows (transform folded into C)
+    kEntryCartMasked = 1, // Cartesian block, some AOs dropped: their rows are zero
+    kEntryPure = 2,       // pure block, some AOs dropped: pure rows, dropped ones zero
+};
 
 // Per screening block outputs of the mask kernel.
 struct KeptTables {
     uint8_t* kept;          // [nblk][nao] significant_aos flags
     int32_t* kept_list;     // [nblk][nao] kept AO indices, ascending
     int32_t* nkept;         // [nblk]
-    int32_t* kept_shell;    // [nblk][nshells] AO blocks with at least one kept AO
+    int32_t* kept_shell;    // [nblk][nshells] AO blocks with at least one kept AO, ascending
     uint32_t* kept_bits;    // [nblk][nshells] bit r: AO row r of that block is kept
-    uint8_t* kept_row;      // [nblk][nshells] first row of the kept block in its chunk
-    int32_t* chunk_begin;   // [nblk][nshells + 1]
+    int32_t* nkshell;       // [nblk]
+    // AO->MO contraction plan: entries ordered by descending l (so that the
+    // lanes generating one chunk mostly share the block's code path), packed
+    // next-fit into chunks.
+    int32_t* ent;           // [nblk][nshells] block index | mode << 24
+    uint32_t* ent_bits;     // [nblk][nshells] kept bits of the entry's block
+    uint8_t* ent_row;       // [nblk][nshells] first row of the entry in its chunk
+    int32_t* chunk_begin;   // [nblk][max_chunks + 1] first entry of each chunk
+    int32_t* chunk_rows;    // [nblk][max_chunks][kChunkRows] operand table row, -1 past the chunk's rows
     int32_t* nchunks;       // [nblk]
+    int max_chunks;
 };
+
+// Upper bound on the chunks of any screening block of a basis: next-fit
+// packing of blocks sorted by l closes a chunk of l-blocks only when it holds
+// at least floor(kChunkRows / ncart(l)) of them.
+int max_chunks_bound(const DevShell* shells, int nshells);
 
 // significant_aos (src/ao_screening.hpp:35-50) for screening blocks
 // [first_block, first_block + nblk) of `block_size` points; order selects
@@ -35,12 +55,13 @@ void launch_masks(const MolView& m, const PointSource& ps, long long npoints, in
 // AO panels at listed points, column-major nao x (panels * n).
 void launch_ao_eval(const MolView& m, const PointSource& ps, long long n, int order, double* out, hipStream_t stream);
 
-// Occupied-MO coefficient operands of the AO-MO contraction, AO-major with
-// `slots` columns (alpha then beta, each padded to 16).
+// MO operand table of the AO->MO contraction ([plane][row][slot]; rows as
+// DevShell::crow / prow; slots alpha then beta, each padded to 16).
 struct MoOperand {
-    const double* c64;      // [nao][slots]
-    const float* c32;       // [nao][slots]
-    const uint16_t* cbf;    // [3][nao][slots] bf16 hi, mid, lo
+    const double* op64;     // FP64, one plane
+    const float* op32;      // FP32, one plane
+    const uint16_t* opbf;   // bf16 hi, mid, lo planes
+    int rows;               // rows per plane
     int slots;
     int alpha_slots;        // slots of the alpha channel (multiple of 16)
 };

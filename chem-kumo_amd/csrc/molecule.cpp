@@ -101,6 +101,19 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
     m.d_p[1] = upload(m.S.data(), m.S.size(), s);
     if (!m.Ca.empty()) m.d_ca = upload(m.Ca.data(), m.Ca.size(), s);
     if (!m.Cb.empty()) m.d_cb = upload(m.Cb.data(), m.Cb.size(), s);
+    // g[k] T[r][k]: the pure transform applied to unscaled Cartesian components.
+    std::vector<double> tg(tdense.size(), 0.0);
+    for (int l = 2; l <= kMaxL; ++l) {
+        const auto pw = gaussian_cartesian_order(l);
+        const int nc = (l + 1) * (l + 2) / 2;
+        for (int r = 0; r < 2 * l + 1; ++r)
+            for (int k = 0; k < nc; ++k) {
+                const std::size_t q = static_cast<std::size_t>((l * kMaxPure + r) * kMaxCart + k);
+                tg[q] = tdense[q] * component_factor(pw[0], pw[static_cast<std::size_t>(k)]);
+            }
+    }
+    m.d_tg = upload(tg.data(), tg.size(), s);
+    m.d_radw = upload(b.radw.data(), b.radw.size(), s);
     // Occupied-MO operand: alpha slots [0, A16), beta slots [A16, A16 + B16)
     // (closed shells contract the alpha channel only).
     const bool closed = !m.unrestricted;
@@ -108,38 +121,67 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
     const int b16 = closed ? 0 : (m.nbeta + 15) / 16 * 16;
     m.alpha_slots = a16;
     m.slots = std::max(16, a16 + b16);
+    m.operand_rows = b.operand_rows;
+    m.max_chunks = max_chunks_bound(b.shells.data(), static_cast<int>(b.shells.size()));
     if (!m.Ca.empty() && m.nalpha <= m.nmo_a && (closed || m.nbeta <= m.nmo_b)) {
-        const std::size_t ns = static_cast<std::size_t>(m.slots), n = static_cast<std::size_t>(m.n);
-        std::vector<double> c64(n * ns, 0.0);
-        for (std::size_t i = 0; i < n; ++i) {
-            for (int k = 0; k < m.nalpha; ++k)
-                c64[i * ns + static_cast<std::size_t>(k)] = m.Ca[i * static_cast<std::size_t>(m.nmo_a) + static_cast<std::size_t>(k)];
-            if (!closed)
-                for (int k = 0; k < m.nbeta; ++k)
-                    c64[i * ns + static_cast<std::size_t>(a16 + k)] =
-                        m.Cb[i * static_cast<std::size_t>(m.nmo_b) + static_cast<std::size_t>(k)];
+        const std::size_t ns = static_cast<std::size_t>(m.slots);
+        const std::size_t nr = static_cast<std::size_t>(b.operand_rows);
+        // Occupied coefficients of AO i in slot order.
+        const auto coef = [&](int i, std::size_t slot) -> double {
+            if (slot < static_cast<std::size_t>(m.nalpha))
+                return m.Ca[static_cast<std::size_t>(i) * static_cast<std::size_t>(m.nmo_a) + slot];
+            if (!closed && slot >= static_cast<std::size_t>(a16) && slot < static_cast<std::size_t>(a16 + m.nbeta))
+                return m.Cb[static_cast<std::size_t>(i) * static_cast<std::size_t>(m.nmo_b) + (slot - static_cast<std::size_t>(a16))];
+            return 0.0;
+        };
+        std::vector<double> op(nr * ns, 0.0);
+        for (std::size_t blk = 0; blk < b.blocks.size(); ++blk) {
+            const HostBlock& hb = b.blocks[blk];
+            const DevShell& d = b.shells[blk];
+            const int nc = hb.ncart();
+            for (int k = 0; k < nc; ++k) {
+                const double gk = component_factor(hb.powers[0], hb.powers[static_cast<std::size_t>(k)]);
+                double* row = op.data() + static_cast<std::size_t>(d.crow + k) * ns;
+                for (std::size_t slot = 0; slot < ns; ++slot) {
+                    double v;
+                    if (hb.pure.empty()) {
+                        v = coef(hb.offset + k, slot);
+                    } else {
+                        v = 0.0;
+                        for (int r = 0; r < hb.count; ++r)
+                            v += hb.pure[static_cast<std::size_t>(r * nc + k)] * coef(hb.offset + r, slot);
+                    }
+                    row[slot] = gk * v;
+                }
+            }
+            if (!hb.pure.empty())
+                for (int r = 0; r < hb.count; ++r) {
+                    double* row = op.data() + static_cast<std::size_t>(d.prow + r) * ns;
+                    for (std::size_t slot = 0; slot < ns; ++slot)
+                        row[slot] = coef(hb.offset + r, slot);
+                }
         }
-        std::vector<float> c32(c64.size());
-        std::vector<uint16_t> cbf(3 * c64.size());
-        for (std::size_t q = 0; q < c64.size(); ++q) {
-            const double v = c64[q];
-            c32[q] = static_cast<float>(v);
+        std::vector<float> o32(op.size());
+        std::vector<uint16_t> obf(3 * op.size());
+        for (std::size_t q = 0; q < op.size(); ++q) {
+            const double v = op[q];
+            o32[q] = static_cast<float>(v);
             const uint16_t hi = bf16(static_cast<float>(v));
             const double r1 = v - bf16_value(hi);
             const uint16_t mid = bf16(static_cast<float>(r1));
             const double r2 = r1 - bf16_value(mid);
-            cbf[q] = hi;
-            cbf[c64.size() + q] = mid;
-            cbf[2 * c64.size() + q] = bf16(static_cast<float>(r2));
+            obf[q] = hi;
+            obf[op.size() + q] = mid;
+            obf[2 * op.size() + q] = bf16(static_cast<float>(r2));
         }
-        m.d_mo64 = upload(c64.data(), c64.size(), s);
-        m.d_mo32 = upload(c32.data(), c32.size(), s);
-        m.d_mobf = upload(cbf.data(), cbf.size(), s);
+        m.d_op64 = upload(op.data(), op.size(), s);
+        m.d_op32 = upload(o32.data(), o32.size(), s);
+        m.d_opbf = upload(obf.data(), obf.size(), s);
     }
     KUMO_HIP(hipStreamSynchronize(s));
     m.device_bytes = 0;
-    for (const DeviceBuffer* d : {&m.d_shells, &m.d_exps, &m.d_weights, &m.d_tdense, &m.d_p[0], &m.d_p[1], &m.d_ca,
-                                  &m.d_cb, &m.d_mo64, &m.d_mo32, &m.d_mobf})
+    for (const DeviceBuffer* d : {&m.d_shells, &m.d_exps, &m.d_weights, &m.d_tdense, &m.d_radw, &m.d_tg, &m.d_p[0],
+                                  &m.d_p[1], &m.d_ca, &m.d_cb, &m.d_op64, &m.d_op32, &m.d_opbf})
         m.device_bytes += static_cast<long long>(d->bytes());
 }
 

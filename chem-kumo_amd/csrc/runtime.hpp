@@ -64,11 +64,12 @@ struct kumo_hip_molecule {
     bool unrestricted = false, eligible = false;
     double total_rel = 0.0, spin_rel = 0.0;
     // Device residency
-    kumo::DeviceBuffer d_shells, d_exps, d_weights, d_tdense;
+    kumo::DeviceBuffer d_shells, d_exps, d_weights, d_tdense, d_radw, d_tg;
     kumo::DeviceBuffer d_p[4]; // total, spin, alpha, beta (alpha/beta on first use)
     kumo::DeviceBuffer d_ca, d_cb; // full MO matrices
-    kumo::DeviceBuffer d_mo64, d_mo32, d_mobf;
-    int slots = 0, alpha_slots = 0;
+    kumo::DeviceBuffer d_op64, d_op32, d_opbf; // MO operand table (FP64, FP32, 3 bf16 planes)
+    int slots = 0, alpha_slots = 0, operand_rows = 0;
+    int max_chunks = 0; // contraction chunks of any screening block (bound)
     bool esp_ready = false;
     kumo::DeviceBuffer d_esp_pairs, d_esp_coef, d_esp_atoms;
     int esp_npairs = 0;
@@ -77,7 +78,7 @@ struct kumo_hip_molecule {
 
     kumo::MolView view() const {
         return {d_shells.as<kumo::DevShell>(), static_cast<int>(basis.shells.size()), n, d_exps.as<double>(),
-                d_weights.as<double>(), d_tdense.as<double>()};
+                d_weights.as<double>(), d_tdense.as<double>(), d_radw.as<double>(), d_tg.as<double>()};
     }
     const double* p_matrix(int component, hipStream_t stream);
     void ensure_esp(hipStream_t stream);
