@@ -8,13 +8,12 @@
 //      coalesced 16-byte loads: for a fully kept block these are its
 //      Cartesian rows g[k] (T^T C)[k] (pure transform and component factor
 //      folded into the coefficients), for a partially kept pure block its
-//      plain pure rows C[r];
+//      plain pure rows C[r]; rows of dropped AOs are zero (the screening mask);
 //   2. lanes (one point each, T lanes per AO block) generate the matching
 //      Phi rows in FP64 from the block's three radial sums: unscaled
-//      Cartesian components P_k R for the Cartesian rows (zero for dropped AOs
-//      of a partially kept Cartesian block), or the pure AOs sum_k T g P_k R
-//      (zero for dropped ones) for pure rows, and write the needed derivative
-//      panels to LDS in the contraction precision (FP64, FP32 or bf16 splits);
+//      Cartesian components P_k R for Cartesian rows, or the pure AOs
+//      sum_k T g P_k R for pure rows, and write the needed derivative panels
+//      to LDS in the contraction precision (FP64, FP32 or bf16 splits);
 //   3. MFMA tiles accumulate Psi[panel] = C_rows^T Phi_rows[panel]
 //      (16 MOs x 16 points per tile, all panels of a tile in one wave; a wave
 //      keeps one 16-point column of tiles, so each Phi operand it loads feeds
@@ -29,8 +28,7 @@
 // for A and B, so the sum over K is exact whatever order the hardware uses.
 // LDS strides are chosen so that the 16 lanes of each K slice hit distinct
 // banks: coefficient rows have a pitch of 132 (bf16, FP32) or 144 (FP64)
-// elements; Phi rows of a point are padded (FP32: 36, FP64: 34 elements) or,
-// for bf16, have their 16-byte granules XOR-swizzled by bits 2-3 of the point.
+// elements, the Phi row of a point 40 (bf16), 36 (FP32) or 34 (FP64).
 #include "kernels.hpp"
 
 #include "../../include/kumo_hip.h"
@@ -68,7 +66,7 @@ template <> struct Prec<KUMO_HIP_BF16X3> {
     using Acc = f32x4;
     static constexpr int S = 2;
     static constexpr int T = 32;
-    static constexpr int LDK = 32;
+    static constexpr int LDK = 40;
     static constexpr int PITCH = 132;
 };
 template <> struct Prec<KUMO_HIP_BF16X6> {
@@ -76,7 +74,7 @@ template <> struct Prec<KUMO_HIP_BF16X6> {
     using Acc = f32x4;
     static constexpr int S = 3;
     static constexpr int T = 32;
-    static constexpr int LDK = 32;
+    static constexpr int LDK = 40;
     static constexpr int PITCH = 132;
 };
 
@@ -85,85 +83,74 @@ template <int PREC> constexpr bool is_bf16 = PREC == KUMO_HIP_BF16X3 || PREC == 
 __device__ __attribute__((always_inline)) inline uint16_t to_bf16(float f) {
     return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
-__device__ __attribute__((always_inline)) inline double from_bf16(uint16_t h) {
-    return static_cast<double>(__uint_as_float(static_cast<uint32_t>(h) << 16));
-}
-
-// Element offset of (point j, row k) within one Phi panel.
-template <int PREC> __device__ __attribute__((always_inline)) inline int phi_at(int j, int k) {
-    using P = Prec<PREC>;
-    if constexpr (is_bf16<PREC>)
-        return j * P::LDK + ((((k >> 3) ^ (j >> 2)) & 3) << 3) + (k & 7);
-    else
-        return j * P::LDK + k;
+__device__ __attribute__((always_inline)) inline float from_bf16(uint16_t h) {
+    return __uint_as_float(static_cast<uint32_t>(h) << 16);
 }
 
 // Writes FP64 value v at panel i, point j, row k of Phi in the contraction
-// precision (bf16: hi, mid[, lo] splits of v, residuals formed in FP64).
+// precision. bf16: v is rounded once to FP32, f, and split exactly into
+// hi + mid (+ lo) = f (each residual is exact in FP32).
 template <int PREC, int NP>
 __device__ __attribute__((always_inline)) inline void phi_store(typename Prec<PREC>::E* phi, int i, int j, int k,
                                                                double v) {
     using P = Prec<PREC>;
     constexpr int plane = NP * P::T * P::LDK;
-    typename Prec<PREC>::E* d = phi + i * (P::T * P::LDK) + phi_at<PREC>(j, k);
+    typename Prec<PREC>::E* d = phi + (i * P::T + j) * P::LDK + k;
     if constexpr (PREC == KUMO_HIP_FP64) {
         d[0] = v;
     } else if constexpr (PREC == KUMO_HIP_FP32) {
         d[0] = static_cast<float>(v);
     } else {
-        const uint16_t hi = to_bf16(static_cast<float>(v));
-        const double r1 = v - from_bf16(hi);
-        const uint16_t mid = to_bf16(static_cast<float>(r1));
+        float f = static_cast<float>(v);
+        // Keeps the FP64->FP32 rounding apart from the bf16 ones (the compiler
+        // would otherwise fuse them into a slower direct FP64->bf16 rounding).
+        asm("" : "+v"(f));
+        const uint16_t hi = to_bf16(f);
+        const float r1 = f - from_bf16(hi);
+        const uint16_t mid = to_bf16(r1);
         d[0] = hi;
         d[plane] = mid;
-        if constexpr (P::S == 3) {
-            const double r2 = r1 - from_bf16(mid);
-            d[2 * plane] = to_bf16(static_cast<float>(r2));
-        }
+        if constexpr (P::S == 3) d[2 * plane] = to_bf16(r1 - from_bf16(mid));
     }
 }
 
-// Phi rows of one chunk entry (AO block `sh` in `mode`) at point j, with the
-// components unrolled for angular momentum L.
+// Phi rows of one chunk entry at point j: the unscaled Cartesian components
+// of an AO block of angular momentum L (unrolled).
 template <int PREC, int ORDER, unsigned PSET, int L>
-__device__ __attribute__((always_inline)) inline void gen_cart(typename Prec<PREC>::E* phi, bool masked, uint32_t bits,
-                                                              int row0, int j, double x, double y, double z,
-                                                              const double (&R)[3]) {
+__device__ __attribute__((always_inline)) inline void gen_cart(typename Prec<PREC>::E* phi, int row0, int j, double x,
+                                                              double y, double z, const double (&R)[3]) {
     constexpr int NP = popcount10(PSET);
     cart_components<L, ORDER, PSET>(x, y, z, R, [&](int k, const double (&c)[NP]) __attribute__((always_inline)) {
-        const bool keep = !masked || ((bits >> k) & 1u);
 #pragma unroll
         for (int i = 0; i < NP; ++i)
-            phi_store<PREC, NP>(phi, i, j, row0 + k, keep ? c[i] : 0.0);
+            phi_store<PREC, NP>(phi, i, j, row0 + k, c[i]);
     });
 }
 
 // The same with runtime l: Cartesian rows (l >= 4) or, for a partially kept
-// pure block, its pure rows sum_k T[r][k] g[k] P_k R (zero for dropped AOs).
+// pure block, its pure rows sum_k T[r][k] g[k] P_k R.
 template <int PREC, int ORDER, unsigned PSET>
 __device__ __attribute__((always_inline)) inline void gen_generic(typename Prec<PREC>::E* phi, const MolView& m,
-                                                                 const DevShell& sh, int mode, uint32_t bits, int row0,
-                                                                 int j, double x, double y, double z,
-                                                                 const double (&R)[3]) {
+                                                                 const DevShell& sh, bool pure_rows, int row0, int j,
+                                                                 double x, double y, double z, const double (&R)[3]) {
     constexpr int NP = popcount10(PSET);
     const int L = sh.l, nc = sh.ncart;
-    if (mode == kEntryPure) {
+    if (pure_rows) {
         const double* tg = m.tg + L * kMaxPure * kMaxCart;
         for (int r = 0; r < sh.count; ++r) {
             double v[NP];
 #pragma unroll
             for (int i = 0; i < NP; ++i)
                 v[i] = 0.0;
-            if ((bits >> r) & 1u)
-                for (int k = 0; k < nc; ++k) {
-                    const double w = tg[r * kMaxCart + k];
-                    if (w == 0.0) continue;
-                    double cv[NP];
-                    cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
+            for (int k = 0; k < nc; ++k) {
+                const double w = tg[r * kMaxCart + k];
+                if (w == 0.0) continue;
+                double cv[NP];
+                cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
 #pragma unroll
-                    for (int i = 0; i < NP; ++i)
-                        v[i] += w * cv[i];
-                }
+                for (int i = 0; i < NP; ++i)
+                    v[i] += w * cv[i];
+            }
 #pragma unroll
             for (int i = 0; i < NP; ++i)
                 phi_store<PREC, NP>(phi, i, j, row0 + r, v[i]);
@@ -173,10 +160,9 @@ __device__ __attribute__((always_inline)) inline void gen_generic(typename Prec<
     for (int k = 0; k < nc; ++k) {
         double cv[NP];
         cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
-        const bool keep = mode != kEntryCartMasked || ((bits >> k) & 1u);
 #pragma unroll
         for (int i = 0; i < NP; ++i)
-            phi_store<PREC, NP>(phi, i, j, row0 + k, keep ? cv[i] : 0.0);
+            phi_store<PREC, NP>(phi, i, j, row0 + k, cv[i]);
     }
 }
 
@@ -214,7 +200,6 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     const MolView& m = a.mol;
     const int nsh = m.nshells;
     const int32_t* ent = a.tables.ent + static_cast<long long>(blk_local) * nsh;
-    const uint32_t* ebits = a.tables.ent_bits + static_cast<long long>(blk_local) * nsh;
     const uint8_t* erow = a.tables.ent_row + static_cast<long long>(blk_local) * nsh;
     const int32_t* cb = a.tables.chunk_begin + static_cast<long long>(blk_local) * (a.tables.max_chunks + 1);
     const int32_t* crows = a.tables.chunk_rows + static_cast<long long>(blk_local) * a.tables.max_chunks * KC;
@@ -246,6 +231,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     else op = reinterpret_cast<const E*>(a.mo.opbf);
     const long long op_plane = static_cast<long long>(a.mo.rows) * a.mo.slots;
     constexpr int EPG = 16 / static_cast<int>(sizeof(E)); // elements per 16-byte granule
+    constexpr int GMAX = 128 / EPG;                        // granules of a 128-slot row (power of two)
     const int G = width / EPG;                             // granules per coefficient row
 
     for (int c = 0; c < nchunks; ++c) {
@@ -258,10 +244,13 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
         }
         // 1. Coefficient rows of the chunk (zero past its rows).
         const int32_t* cr = crows + c * KC;
-        for (int item = tid; item < S * KC * G; item += 256) {
-            const int gi = item % G;
-            const int r = (item / G) % KC;
-            const int sp = item / (KC * G);
+#pragma unroll
+        for (int u = 0; u < S * KC * GMAX / 256; ++u) {
+            const int item = tid + 256 * u;
+            const int gi = item % GMAX;
+            const int r = (item / GMAX) % KC;
+            const int sp = item / (KC * GMAX);
+            if (gi >= G) continue;
             const int grow = cr[r];
             uint4 v = uint4{0, 0, 0, 0};
             if (grow >= 0)
@@ -286,16 +275,14 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
             if (j < nt_valid && r2 <= sh.cutoff_r2) {
                 double R[3];
                 radial_sums<ORDER>(sh, m, r2, R);
-                const uint32_t bits = ebits[q];
-                const bool masked = mode == kEntryCartMasked;
                 if (mode == kEntryPure || sh.l > 3) {
-                    gen_generic<PREC, ORDER, PSET>(phi, m, sh, mode, bits, row0, j, x, y, z, R);
+                    gen_generic<PREC, ORDER, PSET>(phi, m, sh, mode == kEntryPure, row0, j, x, y, z, R);
                 } else {
                     switch (sh.l) {
-                    case 0: gen_cart<PREC, ORDER, PSET, 0>(phi, masked, bits, row0, j, x, y, z, R); break;
-                    case 1: gen_cart<PREC, ORDER, PSET, 1>(phi, masked, bits, row0, j, x, y, z, R); break;
-                    case 2: gen_cart<PREC, ORDER, PSET, 2>(phi, masked, bits, row0, j, x, y, z, R); break;
-                    default: gen_cart<PREC, ORDER, PSET, 3>(phi, masked, bits, row0, j, x, y, z, R); break;
+                    case 0: gen_cart<PREC, ORDER, PSET, 0>(phi, row0, j, x, y, z, R); break;
+                    case 1: gen_cart<PREC, ORDER, PSET, 1>(phi, row0, j, x, y, z, R); break;
+                    case 2: gen_cart<PREC, ORDER, PSET, 2>(phi, row0, j, x, y, z, R); break;
+                    default: gen_cart<PREC, ORDER, PSET, 3>(phi, row0, j, x, y, z, R); break;
                     }
                 }
             } else {
@@ -372,7 +359,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
 #pragma unroll
                 for (int sp = 0; sp < S; ++sp)
                     bv[sp] = __builtin_bit_cast(
-                        bf16x8, *reinterpret_cast<const s16x8*>(phi + sp * PHI_PLANE + i * (T * LDK) + phi_at<PREC>(jb, 8 * g)));
+                        bf16x8, *reinterpret_cast<const s16x8*>(phi + sp * PHI_PLANE + (i * T + jb) * LDK + 8 * g));
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
                     if (wave + 4 * t >= ntiles) continue;

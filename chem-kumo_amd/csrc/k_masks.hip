@@ -212,9 +212,11 @@ __global__ void __launch_bounds__(kMaskThreads)
         er[q] = static_cast<uint8_t>(row0);
         const int first = mode == kEntryPure ? sh.prow : sh.crow;
         const int n = mode == kEntryPure ? sh.count : sh.ncart;
+        // Rows of dropped AOs (partially kept blocks) get zero coefficients.
+        const uint32_t bits = sbits[s];
         if (c < t.max_chunks)
             for (int i = 0; i < n; ++i)
-                crows[c * kChunkRows + row0 + i] = first + i;
+                crows[c * kChunkRows + row0 + i] = (mode == kEntryCart || ((bits >> i) & 1u)) ? first + i : -1;
     }
     for (int item = tid; item < nc * kChunkRows; item += kMaskThreads) {
         const int c = item / kChunkRows, r = item % kChunkRows;

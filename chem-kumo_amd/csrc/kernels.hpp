@@ -15,10 +15,12 @@ namespace kumo {
 constexpr int kChunkRows = 32;
 
 // How a kept AO block enters the AO->MO contraction.
+// Dropped AOs of a partially kept block get the operand row -1 (zero
+// coefficients) in the chunk's row list.
 enum EntryMode : int32_t {
     kEntryCart = 0,       // every AO kept: the block's Cartesian rows (transform folded into C)
-    kEntryCartMasked = 1, // Cartesian block, some AOs dropped: their rows are zero
-    kEntryPure = 2,       // pure block, some AOs dropped: pure rows, dropped ones zero
+    kEntryCartMasked = 1, // Cartesian block, some AOs dropped
+    kEntryPure = 2,       // pure block, some AOs dropped: its pure rows
 };
 
 // Per screening block outputs of the mask kernel.
