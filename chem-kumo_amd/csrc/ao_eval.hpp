@@ -24,6 +24,8 @@ struct MolView {
     const double* tdense; // [kMaxL+1][kMaxPure][kMaxCart] Cartesian->pure, zero padded
     const double* radw;   // per primitive: weight of the block's first Cartesian component
     const double* tg;     // [kMaxL+1][kMaxPure][kMaxCart] T[r][k] g[k] (transform of unscaled components)
+    const double* gk;     // [kMaxL+1][kMaxCart] g[k]: Cartesian AO = g[k] x unscaled component
+    const int32_t* lorder; // blocks by descending l
 };
 
 // Cartesian powers in Gaussian FCHK order (src/basis_evaluator.cpp:135-152).
@@ -218,6 +220,115 @@ __device__ __attribute__((always_inline)) inline void eval_block_l(const DevShel
 }
 
 // Runtime dispatch on the block's angular momentum.
+// eval_block_l with the angular momentum a runtime value: the same
+// operations in the same order, with the per-component arrays in private
+// memory. Used for l >= 4, where unrolled register arrays would set the
+// register budget of every kernel that can meet such a block.
+template <int ORDER, unsigned PSET, typename Emit>
+__device__ __attribute__((always_inline)) inline void eval_block_rt(const DevShell& s, const MolView& m, double x, double y,
+                                                                  double z, double r2, Emit&& emit) {
+#pragma clang fp contract(off)
+    constexpr int NP = popcount10(PSET);
+    const int L = s.l, NC = s.ncart, np = s.nprim;
+    const double* e = m.exps + s.prim;
+    const double* w = m.weights + s.wofs;
+    double cart[kMaxCart][NP];
+    double xp[kMaxL + 1], yp[kMaxL + 1], zp[kMaxL + 1];
+    xp[0] = yp[0] = zp[0] = 1.0;
+    for (int n = 1; n <= L; ++n) {
+        xp[n] = xp[n - 1] * x;
+        yp[n] = yp[n - 1] * y;
+        zp[n] = zp[n - 1] * z;
+    }
+    if constexpr (ORDER == 0) {
+        double rs[kMaxCart];
+        for (int k = 0; k < NC; ++k)
+            rs[k] = 0.0;
+        for (int p = 0; p < np; ++p) {
+            const double radial = exp(-e[p] * r2);
+            for (int k = 0; k < NC; ++k)
+                rs[k] += w[k * np + p] * radial;
+        }
+        for (int k = 0; k < NC; ++k) {
+            const double poly = (xp[kPow[L][k][0]] * yp[kPow[L][k][1]]) * zp[kPow[L][k][2]];
+            cart[k][0] = poly * rs[k];
+        }
+    } else {
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                cart[k][i] = 0.0;
+        const double r[3] = {x, y, z};
+        for (int p = 0; p < np; ++p) {
+            const double a = e[p];
+            const double radial = exp(-a * r2);
+            double f[3][kMaxL + 1][3];
+            for (int axis = 0; axis < 3; ++axis) {
+                const double* pw = axis == 0 ? xp : (axis == 1 ? yp : zp);
+                const double c = r[axis];
+                for (int mm = 0; mm <= L; ++mm) {
+                    const double xm = pw[mm];
+                    f[axis][mm][0] = xm;
+                    f[axis][mm][1] = (mm ? mm * pw[mm > 0 ? mm - 1 : 0] : 0.0) - 2.0 * a * c * xm;
+                    if constexpr (ORDER == 2)
+                        f[axis][mm][2] = (mm >= 2 ? mm * (mm - 1) * pw[mm >= 2 ? mm - 2 : 0] : 0.0) -
+                                         2.0 * a * (2 * mm + 1) * xm + 4.0 * a * a * c * c * xm;
+                    else
+                        f[axis][mm][2] = 0.0;
+                }
+            }
+            for (int k = 0; k < NC; ++k) {
+                const double wgt = w[k * np + p] * radial;
+                const double* fx = f[0][kPow[L][k][0]];
+                const double* fy = f[1][kPow[L][k][1]];
+                const double* fz = f[2][kPow[L][k][2]];
+#pragma unroll
+                for (int i = 0; i < NP; ++i) {
+                    const int c = panel_component(PSET, i);
+                    double t;
+                    switch (c) {
+                    case 0: t = wgt * fx[0] * fy[0] * fz[0]; break;
+                    case 1: t = wgt * fx[1] * fy[0] * fz[0]; break;
+                    case 2: t = wgt * fx[0] * fy[1] * fz[0]; break;
+                    case 3: t = wgt * fx[0] * fy[0] * fz[1]; break;
+                    case 4: t = wgt * fx[2] * fy[0] * fz[0]; break;
+                    case 5: t = wgt * fx[1] * fy[1] * fz[0]; break;
+                    case 6: t = wgt * fx[1] * fy[0] * fz[1]; break;
+                    case 7: t = wgt * fx[0] * fy[2] * fz[0]; break;
+                    case 8: t = wgt * fx[0] * fy[1] * fz[1]; break;
+                    default: t = wgt * fx[0] * fy[0] * fz[2]; break;
+                    }
+                    cart[k][i] += t;
+                }
+            }
+        }
+    }
+    if (L >= 2 && s.pure) {
+        const double* T = m.tdense + L * kMaxPure * kMaxCart;
+        for (int row = 0; row < 2 * L + 1; ++row) {
+            double v[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                v[i] = 0.0;
+            for (int k = 0; k < NC; ++k) {
+                const double t = T[row * kMaxCart + k];
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+                    v[i] += t * cart[k][i];
+            }
+            emit(row, v);
+        }
+    } else {
+        for (int k = 0; k < NC; ++k) {
+            double v[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                v[i] = cart[k][i];
+            emit(k, v);
+        }
+    }
+}
+
 template <int ORDER, unsigned PSET, typename Emit>
 __device__ __attribute__((always_inline)) inline void eval_block(const DevShell& s, const MolView& m, double x, double y, double z, double r2,
                                   Emit&& emit) {
@@ -226,99 +337,7 @@ __device__ __attribute__((always_inline)) inline void eval_block(const DevShell&
     case 1: eval_block_l<1, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
     case 2: eval_block_l<2, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
     case 3: eval_block_l<3, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    case 4: eval_block_l<4, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    default: eval_block_l<5, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    }
-}
-
-// Cartesian components of block `s` for the contraction kernels: the same
-// mathematics as eval_cartesian, reorganised for registers (component-outer,
-// primitives inner with their radial factors cached) and free to fuse
-// multiply-adds, so values agree with the reference kernels to a few ulps.
-// The Cartesian->pure transform is folded into the MO coefficients by the
-// caller. Calls emit(k, vals[NP]) for each Cartesian component k.
-template <int L, int ORDER, unsigned PSET, typename Emit>
-__device__ __attribute__((always_inline)) inline void eval_cartesian_rows(const DevShell& s, const MolView& m, double x, double y, double z,
-                                           double r2, Emit&& emit) {
-    constexpr int NC = (L + 1) * (L + 2) / 2;
-    constexpr int NP = popcount10(PSET);
-    constexpr int PC = 8;
-    const int np = s.nprim;
-    const double* e = m.exps + s.prim;
-    const double* w = m.weights + s.wofs;
-    double xp[L + 3], yp[L + 3], zp[L + 3];
-    xp[0] = yp[0] = zp[0] = 1.0;
-#pragma unroll
-    for (int n = 1; n <= L + 2; ++n) {
-        xp[n] = xp[n - 1] * x;
-        yp[n] = yp[n - 1] * y;
-        zp[n] = zp[n - 1] * z;
-    }
-    // One primitive of component k: adds weight * product of axis factors.
-    const auto add = [&](int k, int p, double radial, double (&sums)[NP]) __attribute__((always_inline)) {
-        const double a = e[p];
-        const double wt = w[k * np + p] * radial;
-        const double two_a = 2.0 * a;
-        double f[3][3];
-#pragma unroll
-        for (int axis = 0; axis < 3; ++axis) {
-            const int q = kPow[L][k][axis];
-            const double* pw = axis == 0 ? xp : (axis == 1 ? yp : zp);
-            f[axis][0] = pw[q];
-            if constexpr (ORDER >= 1) f[axis][1] = (q ? q * pw[q > 0 ? q - 1 : 0] : 0.0) - two_a * pw[q + 1];
-            if constexpr (ORDER == 2)
-                f[axis][2] = (q >= 2 ? q * (q - 1) * pw[q >= 2 ? q - 2 : 0] : 0.0) - two_a * (2 * q + 1) * pw[q] +
-                             two_a * two_a * pw[q + 2];
-        }
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            const int c = panel_component(PSET, i);
-            const int dx = c == 1 || c == 5 || c == 6 ? 1 : (c == 4 ? 2 : 0);
-            const int dy = c == 2 || c == 5 || c == 8 ? 1 : (c == 7 ? 2 : 0);
-            const int dz = c == 3 || c == 6 || c == 8 ? 1 : (c == 9 ? 2 : 0);
-            sums[i] += wt * f[0][dx] * f[1][dy] * f[2][dz];
-        }
-    };
-    if (np <= PC) {
-        double rad[PC];
-#pragma unroll
-        for (int p = 0; p < PC; ++p)
-            rad[p] = p < np ? exp(-e[p] * r2) : 0.0;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            double sums[NP];
-#pragma unroll
-            for (int i = 0; i < NP; ++i)
-                sums[i] = 0.0;
-#pragma unroll
-            for (int p = 0; p < PC; ++p)
-                if (p < np) add(k, p, rad[p], sums);
-            emit(k, sums);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            double sums[NP];
-#pragma unroll
-            for (int i = 0; i < NP; ++i)
-                sums[i] = 0.0;
-            for (int p = 0; p < np; ++p)
-                add(k, p, exp(-e[p] * r2), sums);
-            emit(k, sums);
-        }
-    }
-}
-
-template <int ORDER, unsigned PSET, typename Emit>
-__device__ __attribute__((always_inline)) inline void eval_block_cartesian(const DevShell& s, const MolView& m, double x, double y, double z,
-                                            double r2, Emit&& emit) {
-    switch (s.l) {
-    case 0: eval_cartesian_rows<0, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    case 1: eval_cartesian_rows<1, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    case 2: eval_cartesian_rows<2, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    case 3: eval_cartesian_rows<3, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    case 4: eval_cartesian_rows<4, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
-    default: eval_cartesian_rows<5, ORDER, PSET>(s, m, x, y, z, r2, emit); break;
+    default: eval_block_rt<ORDER, PSET>(s, m, x, y, z, r2, emit); break;
     }
 }
 
@@ -332,6 +351,7 @@ __device__ __attribute__((always_inline)) inline void eval_block_cartesian(const
 template <int ORDER>
 __device__ __attribute__((always_inline)) inline void radial_sums(const DevShell& s, const MolView& m, double r2,
                                                                 double (&R)[3]) {
+#pragma clang fp contract(fast)
     const double* e = m.exps + s.prim;
     const double* u = m.radw + s.prim;
     double r0 = 0.0, r1 = 0.0, rr = 0.0;
@@ -358,6 +378,7 @@ __device__ __attribute__((always_inline)) inline void radial_sums(const DevShell
 template <int L, int ORDER, unsigned PSET, typename Emit>
 __device__ __attribute__((always_inline)) inline void cart_components(double x, double y, double z, const double (&R)[3],
                                                                     Emit&& emit) {
+#pragma clang fp contract(fast)
     constexpr int NC = (L + 1) * (L + 2) / 2;
     constexpr int NP = popcount10(PSET);
     const double c[3] = {x, y, z};
@@ -416,6 +437,7 @@ template <int ORDER, unsigned PSET>
 __device__ __attribute__((always_inline)) inline void cart_component_rt(int L, int k, double x, double y, double z,
                                                                       const double (&R)[3],
                                                                       double (&v)[popcount10(PSET)]) {
+#pragma clang fp contract(fast)
     constexpr int NP = popcount10(PSET);
     const double c[3] = {x, y, z};
     double f[3][3];

@@ -101,7 +101,36 @@ double cutoff_radius(const HostBlock& b, double tnorm, double cutoff) {
     return hi;
 }
 
+// Largest row sum of |T| (1 for Cartesian blocks), the factor the cutoff
+// bound carries for pure AOs (src/basis_evaluator.cpp:98-127).
+double transform_norm(const HostBlock& b) {
+    if (b.pure.empty()) return 1.0;
+    const int nc = b.ncart();
+    double tnorm = 0.0;
+    for (int row = 0; row < b.count; ++row) {
+        double sum = 0.0;
+        for (int k = 0; k < nc; ++k)
+            sum += std::abs(b.pure[static_cast<std::size_t>(row * nc + k)]);
+        tnorm = std::max(tnorm, sum);
+    }
+    return tnorm;
+}
+
 } // namespace
+
+std::vector<double> screening_radii2(const PreparedBasis& basis, double threshold) {
+    std::vector<double> r2(basis.blocks.size());
+    for (std::size_t i = 0; i < basis.blocks.size(); ++i) {
+        const HostBlock& b = basis.blocks[i];
+        if (!(threshold > 1e-14)) {
+            r2[i] = b.cutoff_r2;
+            continue;
+        }
+        const double r = cutoff_radius(b, transform_norm(b), threshold);
+        r2[i] = std::min(b.cutoff_r2, r * r);
+    }
+    return r2;
+}
 
 std::vector<CartPower> gaussian_cartesian_order(int l) {
     if (l == 0) return {{0, 0, 0}};
@@ -169,17 +198,7 @@ PreparedBasis prepare_basis(const ShellRecords& r) {
         b.offset = offset;
         b.count = b.pure.empty() ? nc : 2 * l + 1;
         offset += b.count;
-        double tnorm = 1.0;
-        if (!b.pure.empty()) {
-            tnorm = 0.0;
-            for (int row = 0; row < b.count; ++row) {
-                double sum = 0.0;
-                for (int k = 0; k < nc; ++k)
-                    sum += std::abs(b.pure[static_cast<std::size_t>(row * nc + k)]);
-                tnorm = std::max(tnorm, sum);
-            }
-        }
-        const double radius = cutoff_radius(b, tnorm, 1e-14);
+        const double radius = cutoff_radius(b, transform_norm(b), 1e-14);
         b.cutoff_r2 = radius * radius;
         out.max_l = std::max(out.max_l, l);
         out.blocks.push_back(std::move(b));
@@ -265,6 +284,9 @@ PreparedBasis prepare_basis(const ShellRecords& r) {
         if (d.pure) pure_rows += d.count;
     }
     out.operand_rows = cart_rows + pure_rows;
+    for (int l = kMaxL; l >= 0; --l)
+        for (std::size_t i = 0; i < out.shells.size(); ++i)
+            if (out.shells[i].l == l) out.lorder.push_back(static_cast<int32_t>(i));
     return out;
 }
 

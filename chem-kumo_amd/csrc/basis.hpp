@@ -59,7 +59,14 @@ struct PreparedBasis {
     std::vector<double> exps, weights, radw;
     std::vector<PureTerm> pure_terms;
     PureIndex pure_index{};
+    std::vector<int32_t> lorder; // blocks by descending l, ascending index within an l
 };
+
+// Squared radius per block beyond which the rigorous bound of
+// src/basis_evaluator.cpp:98-127 keeps every AO of the block and its first and
+// second derivatives at or below `threshold` (the cutoff radius when the
+// threshold is below the cutoff's 1e-14).
+std::vector<double> screening_radii2(const PreparedBasis& basis, double threshold);
 
 // Validates the shell records as src/basis.cpp:21-120 does and prepares the
 // blocks; throws kumo::Error (invalid argument / runtime) on bad records.

@@ -277,7 +277,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
             const long long first = b0 * bs;
             const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
-            launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, ts.t, s);
+            launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, mol->screen_r2(thr, s), ts.t, s);
             if (backend == KUMO_HIP_AO_MO) {
                 AoMoLaunch a{};
                 a.mol = mv;
@@ -446,7 +446,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
                 const long long first = b0 * bs;
                 const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
-                launch_masks(mv, ps, n, bs, b0, nb, 0, thr, ts.t, s);
+                launch_masks(mv, ps, n, bs, b0, nb, 0, thr, mol->screen_r2(thr, s), ts.t, s);
                 AoPLaunch a{};
                 a.mol = mv;
                 a.ps = ps;
@@ -504,7 +504,7 @@ void screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const 
     const auto ts = tables(ex, std::max(1, cap), nao, nsh, mol->max_chunks);
     for (long long b0 = 0; b0 < nblocks; b0 += cap) {
         const int nb = static_cast<int>(std::min<long long>(cap, nblocks - b0));
-        launch_masks(mol->view(), ps, n, block_size, b0, nb, order, threshold, ts.t, ex.stream);
+        launch_masks(mol->view(), ps, n, block_size, b0, nb, order, threshold, const_cast<kumo_hip_molecule*>(mol)->screen_r2(threshold, ex.stream), ts.t, ex.stream);
         KUMO_HIP(hipMemcpyAsync(kept + b0 * nao, ts.t.kept, static_cast<std::size_t>(nb) * nao, hipMemcpyDeviceToHost,
                                 ex.stream));
     }

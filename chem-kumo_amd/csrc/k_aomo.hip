@@ -88,8 +88,10 @@ __device__ __attribute__((always_inline)) inline float from_bf16(uint16_t h) {
 }
 
 // Writes FP64 value v at panel i, point j, row k of Phi in the contraction
-// precision. bf16: v is rounded once to FP32, f, and split exactly into
-// hi + mid (+ lo) = f (each residual is exact in FP32).
+// precision. bf16: hi = bf16(v); the first residual v - hi is formed exactly
+// in FP64 and rounded to FP32, f1, which is split exactly into mid (+ lo)
+// (the second residual is exact in FP32), so hi + mid + lo represents v to
+// about 2^-27 relative (a plain FP32 rounding of v would cost 2^-24).
 template <int PREC, int NP>
 __device__ __attribute__((always_inline)) inline void phi_store(typename Prec<PREC>::E* phi, int i, int j, int k,
                                                                double v) {
@@ -102,15 +104,18 @@ __device__ __attribute__((always_inline)) inline void phi_store(typename Prec<PR
         d[0] = static_cast<float>(v);
     } else {
         float f = static_cast<float>(v);
-        // Keeps the FP64->FP32 rounding apart from the bf16 ones (the compiler
-        // would otherwise fuse them into a slower direct FP64->bf16 rounding).
+        // Keeps each FP64->FP32 rounding apart from the bf16 one that follows
+        // (the compiler would otherwise fuse them into a slower direct
+        // FP64->bf16 rounding; rounding twice is exact here since FP32 has
+        // more than twice bf16's precision).
         asm("" : "+v"(f));
         const uint16_t hi = to_bf16(f);
-        const float r1 = f - from_bf16(hi);
-        const uint16_t mid = to_bf16(r1);
+        float f1 = static_cast<float>(v - static_cast<double>(from_bf16(hi)));
+        asm("" : "+v"(f1));
+        const uint16_t mid = to_bf16(f1);
         d[0] = hi;
         d[plane] = mid;
-        if constexpr (P::S == 3) d[2 * plane] = to_bf16(r1 - from_bf16(mid));
+        if constexpr (P::S == 3) d[2 * plane] = to_bf16(f1 - from_bf16(mid));
     }
 }
 

@@ -1,12 +1,31 @@
 // Screening masks (src/ao_screening.hpp:35-50 as applied per point block in
 // src/density.cpp:454-467) and AO panel evaluation.
 //
-// One workgroup per screening block: each lane evaluates the AO blocks at one
-// point, the largest |value| over the screened panels is reduced across the
-// wave with shuffles and across the workgroup with a 64-bit LDS atomic max on
-// the IEEE bit pattern (monotone for non-negative doubles). The kept AO list,
-// the kept AO blocks and their packing into contraction chunks are written
-// for the contraction kernels. Built with -ffp-contract=off (see ao_eval.hpp).
+// significant_aos keeps AO r of a block of points when the largest |value| of
+// its screened panels (value; + gradient; + Hessian) over the block's points
+// exceeds the threshold. One workgroup per screening block decides every AO
+// block in four phases, and the decisions equal those of the reference
+// arithmetic (src/basis_evaluator.cpp:512-643, no fused multiply-adds; this
+// file is built with -ffp-contract=off, see ao_eval.hpp):
+//   A. lanes = AO blocks (in descending l): the block's point nearest to the
+//      AO block's centre gives the smallest distance. Beyond the cutoff, or
+//      beyond the radius where the rigorous bound of
+//      src/basis_evaluator.cpp:98-127 falls to the threshold, every AO is
+//      dropped. Otherwise the AOs are evaluated at that point with the fast
+//      factored arithmetic; when each AO's largest panel there exceeds the
+//      threshold by more than a relative 1e-6 (far beyond the few-ulp
+//      difference of the two arithmetics), every AO is kept (the maximum
+//      over the block is at least that value);
+//   B. lanes = points: the remaining AO blocks are evaluated at every point
+//      with the fast arithmetic and each AO's maximum is reduced (wave
+//      shuffles, then a 64-bit LDS atomic max on the IEEE bit pattern,
+//      monotone for non-negative doubles);
+//   C. AO blocks with an AO whose fast maximum lies within a relative 1e-6 of
+//      the threshold are evaluated again with the reference arithmetic, which
+//      decides them;
+//   D. the kept AO list, the kept AO blocks (in AO order for the AO-P path)
+//      and the AO->MO chunk plan (entries by descending l, packed next-fit)
+//      are written for the contraction kernels.
 #include "kernels.hpp"
 
 #include "common.hpp"
@@ -15,167 +34,367 @@ namespace kumo {
 namespace {
 
 constexpr int kMaskThreads = 128; // one lane per point of a default 128-point block
+constexpr double kWindow = 1e-6;  // relative band around the threshold decided exactly
+
+enum : int32_t { kDrop = 0, kKeepAll = 1, kFull = 2, kExact = 3 };
+
+// Calls f.template operator()<C>() for each derivative component C of PSET.
+template <unsigned PSET, int I = 0, typename F> __device__ __attribute__((always_inline)) inline void for_panels(F&& f) {
+    if constexpr (I < popcount10(PSET)) {
+        f.template operator()<panel_component(PSET, I)>();
+        for_panels<PSET, I + 1>(f);
+    }
+}
+
+// Fast largest |panel| of each AO of block sh (angular momentum L) at one point.
+template <int L, int ORDER, unsigned PSET>
+__device__ __attribute__((always_inline)) inline void fast_block_max(const DevShell& sh, const MolView& m, double x,
+                                                                   double y, double z, const double (&R)[3],
+                                                                   double (&amax)[kMaxCart]) {
+#pragma clang fp contract(fast)
+    constexpr int NC = (L + 1) * (L + 2) / 2;
+    const bool pure = L >= 2 && sh.pure;
+    const double* tg = m.tg + L * kMaxPure * kMaxCart;
+    const double* gk = m.gk + L * kMaxCart;
+    for_panels<PSET>([&]<int C>() __attribute__((always_inline)) {
+        double cv[NC];
+        cart_components<L, ORDER, (1u << C)>(x, y, z, R, [&](int k, const double (&v)[1]) __attribute__((always_inline)) {
+            cv[k] = v[0];
+        });
+        if (pure) {
+#pragma unroll
+            for (int r = 0; r < 2 * L + 1; ++r) {
+                double a = 0.0;
+#pragma unroll
+                for (int k = 0; k < NC; ++k)
+                    a += tg[r * kMaxCart + k] * cv[k];
+                amax[r] = fmax(amax[r], fabs(a));
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+                amax[k] = fmax(amax[k], fabs(gk[k] * cv[k]));
+        }
+    });
+}
+
+// Fast smallest (over the block's AOs) largest |panel| at one point, with l
+// a runtime value (lanes evaluate different blocks).
+template <int ORDER, unsigned PSET>
+__device__ __attribute__((always_inline)) inline double probe_min_max(const DevShell& sh, const MolView& m, double x,
+                                                                     double y, double z, double r2) {
+#pragma clang fp contract(fast)
+    constexpr int NP = popcount10(PSET);
+    double R[3];
+    radial_sums<ORDER>(sh, m, r2, R);
+    const int L = sh.l, nc = sh.ncart;
+    double mn = 1.0e300;
+    if (sh.pure) {
+        const double* tg = m.tg + L * kMaxPure * kMaxCart;
+        for (int r = 0; r < sh.count; ++r) {
+            double v[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                v[i] = 0.0;
+            for (int k = 0; k < nc; ++k) {
+                const double w = tg[r * kMaxCart + k];
+                if (w == 0.0) continue;
+                double cv[NP];
+                cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+                    v[i] += w * cv[i];
+            }
+            double mx = 0.0;
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                mx = fmax(mx, fabs(v[i]));
+            mn = fmin(mn, mx);
+        }
+    } else {
+        const double* gk = m.gk + L * kMaxCart;
+        for (int k = 0; k < nc; ++k) {
+            double cv[NP];
+            cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
+            double mx = 0.0;
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                mx = fmax(mx, fabs(gk[k] * cv[i]));
+            mn = fmin(mn, mx);
+        }
+    }
+    return mn;
+}
+
+// Exclusive prefix sum of v over the workgroup; total receives the sum.
+__device__ __attribute__((always_inline)) inline int block_scan(int v, int* tmp, int& total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    const int before = w ? tmp[0] : 0;
+    total = tmp[0] + tmp[1];
+    __syncthreads();
+    return before + x - v;
+}
 
 template <int ORDER>
 __global__ void __launch_bounds__(kMaskThreads)
     mask_kernel(MolView m, PointSource ps, long long npoints, int block_size, long long first_block, double threshold,
-                KeptTables t) {
-    // LDS: smax[nao], then per AO block kept bits, entries, entry slots, and
-    // the rows used by each chunk.
+                const double* screen_r2, KeptTables t) {
+    constexpr unsigned PSET = ORDER == 0 ? kPanelsValue : (ORDER == 1 ? kPanelsGrad : kPanelsAll);
+    constexpr int NP = popcount10(PSET);
+    // LDS: smax[nao] (u64), pts[kMaskThreads][4] (double), then per AO block
+    // bits, state, entry, entry slot (int32), then rows used per chunk.
     extern __shared__ unsigned long long smax[];
-    const int tid = threadIdx.x;
+    const int nao = m.nao, nsh = m.nshells;
+    double* spts = reinterpret_cast<double*>(smax + nao);
+    uint32_t* sbits = reinterpret_cast<uint32_t*>(spts + 4 * kMaskThreads);
+    int32_t* sstate = reinterpret_cast<int32_t*>(sbits + nsh);
+    int32_t* sent = sstate + nsh;
+    int32_t* sslot = sent + nsh;
+    int32_t* sused = sslot + nsh;
+    __shared__ int scan_tmp[2];
+    __shared__ int packed[2];
+
+    const int tid = threadIdx.x, lane = tid & 63;
     const long long blk = first_block + blockIdx.x;
     const long long start = blk * block_size;
     const int count = static_cast<int>(min(static_cast<long long>(block_size), npoints - start));
-    const int nao = m.nao;
+    const bool screen = threshold != 0.0 && count > 0;
     for (int a = tid; a < nao; a += kMaskThreads)
         smax[a] = 0ull;
+    const bool cached = count <= kMaskThreads; // points held in LDS
+    if (cached && tid < count) {
+        double px, py, pz;
+        point_coords(ps, start + tid, px, py, pz);
+        spts[4 * tid] = px;
+        spts[4 * tid + 1] = py;
+        spts[4 * tid + 2] = pz;
+    }
     __syncthreads();
-    constexpr unsigned PSET = ORDER == 0 ? kPanelsValue : (ORDER == 1 ? kPanelsGrad : kPanelsAll);
-    constexpr int NP = popcount10(PSET);
-    const bool screen = threshold != 0.0;
-    if (screen) {
-        for (int p0 = 0; p0 < count; p0 += kMaskThreads) {
-            const int j = p0 + tid;
-            const bool active = j < count;
-            double px = 0, py = 0, pz = 0;
-            if (active) point_coords(ps, start + j, px, py, pz);
-            for (int s = 0; s < m.nshells; ++s) {
-                const DevShell sh = m.shells[s];
-                const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
-                const double r2 = sq_norm(x, y, z);
-                const bool inside = active && r2 <= sh.cutoff_r2;
-                if (!__any(inside)) continue;
-                double amax[kMaxCart];
-#pragma unroll
-                for (int r = 0; r < kMaxCart; ++r)
-                    amax[r] = 0.0;
-                if (inside) {
-                    eval_block<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) __attribute__((always_inline)) {
-                        double a = 0.0;
-#pragma unroll
-                        for (int i = 0; i < NP; ++i)
-                            a = fmax(a, fabs(v[i]));
-                        amax[row] = a;
-                    });
+
+    if (screen && !cached) {
+        // Blocks of more than kMaskThreads points skip the classification.
+        for (int s = tid; s < nsh; s += kMaskThreads)
+            sstate[s] = m.shells[s].l > 3 ? kExact : kFull;
+        __syncthreads();
+    }
+    if (screen && cached) {
+        // A. Classification, one lane per AO block.
+        for (int q = tid; q < nsh; q += kMaskThreads) {
+            const int s = m.lorder[q];
+            const DevShell& sh = m.shells[s];
+            double best = 1.0e300;
+            int bj = 0;
+            for (int jj = 0; jj < count; ++jj) {
+                const double r2 = sq_norm(spts[4 * jj] - sh.o[0], spts[4 * jj + 1] - sh.o[1], spts[4 * jj + 2] - sh.o[2]);
+                if (r2 < best) {
+                    best = r2;
+                    bj = jj;
                 }
+            }
+            int st;
+            if (!(best <= sh.cutoff_r2) || best > screen_r2[s]) {
+                st = kDrop;
+            } else {
+                const double x = spts[4 * bj] - sh.o[0], y = spts[4 * bj + 1] - sh.o[1], z = spts[4 * bj + 2] - sh.o[2];
+                const double mn = probe_min_max<ORDER, PSET>(sh, m, x, y, z, best);
+                st = mn > threshold * (1.0 + kWindow) ? kKeepAll : (sh.l > 3 ? kExact : kFull);
+            }
+            sstate[s] = st;
+        }
+        __syncthreads();
+    }
+    if (screen) {
+        // B. Fast maxima over the points of the undecided blocks.
+        for (int p0 = 0; p0 < count; p0 += kMaskThreads) {
+        const bool active = p0 + tid < count;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        if (active) {
+            if (cached) {
+                px = spts[4 * tid];
+                py = spts[4 * tid + 1];
+                pz = spts[4 * tid + 2];
+            } else {
+                point_coords(ps, start + p0 + tid, px, py, pz);
+            }
+        }
+        for (int q = 0; q < nsh; ++q) {
+            const int s = m.lorder[q];
+            if (sstate[s] != kFull) continue;
+            const DevShell& sh = m.shells[s];
+            const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
+            const double r2 = sq_norm(x, y, z);
+            const bool inside = active && r2 <= sh.cutoff_r2;
+            if (!__any(inside)) continue;
+            double amax[kMaxCart];
 #pragma unroll
-                for (int r = 0; r < kMaxCart; ++r) {
-                    if (r < sh.count) {
-                        const double v = wave_max(amax[r]);
-                        if ((threadIdx.x & 63) == 0 && v > 0.0)
-                            atomicMax(&smax[sh.ao + r], static_cast<unsigned long long>(__double_as_longlong(v)));
-                    }
+            for (int r = 0; r < kMaxCart; ++r)
+                amax[r] = 0.0;
+            if (inside) {
+                double R[3];
+                radial_sums<ORDER>(sh, m, r2, R);
+                switch (sh.l) {
+                case 0: fast_block_max<0, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+                case 1: fast_block_max<1, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+                case 2: fast_block_max<2, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+                default: fast_block_max<3, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 2 * 3 + 1 + 3; ++r) { // at most 10 rows for l <= 3
+                if (r < sh.count) {
+                    const double v = wave_max(amax[r]);
+                    if (lane == 0 && v > 0.0)
+                        atomicMax(&smax[sh.ao + r], static_cast<unsigned long long>(__double_as_longlong(v)));
                 }
             }
         }
-    }
-    __syncthreads();
-    // Flags and the ascending kept list (block-wide scan over the AOs).
-    uint8_t* kept = t.kept + static_cast<long long>(blockIdx.x) * nao;
-    int32_t* list = t.kept_list + static_cast<long long>(blockIdx.x) * nao;
-    __shared__ int wave_counts[kMaskThreads / 64];
-    __shared__ int base;
-    if (tid == 0) base = 0;
-    __syncthreads();
-    for (int a0 = 0; a0 < nao; a0 += kMaskThreads) {
-        const int a = a0 + tid;
-        bool k = false;
-        if (a < nao) {
-            k = !screen || count == 0 || __longlong_as_double(static_cast<long long>(smax[a])) > threshold;
-            kept[a] = k ? 1 : 0;
-        }
-        const unsigned long long ballot = __ballot(k);
-        const int lane = tid & 63, wave = tid >> 6;
-        const int before = __popcll(ballot & ((1ull << lane) - 1ull));
-        if (lane == 0) wave_counts[wave] = __popcll(ballot);
-        __syncthreads();
-        int offset = base;
-        for (int w = 0; w < wave; ++w)
-            offset += wave_counts[w];
-        if (k) list[offset + before] = a;
-        __syncthreads();
-        if (tid == 0) {
-            int total = 0;
-            for (int w = 0; w < kMaskThreads / 64; ++w)
-                total += wave_counts[w];
-            base += total;
         }
         __syncthreads();
+        // C. Blocks with an AO near the threshold are decided by the reference
+        //    arithmetic.
+        for (int s = tid; s < nsh; s += kMaskThreads) {
+            if (sstate[s] != kFull) continue;
+            const DevShell& sh = m.shells[s];
+            bool near = false;
+            for (int r = 0; r < sh.count; ++r) {
+                const double v = __longlong_as_double(static_cast<long long>(smax[sh.ao + r]));
+                near = near || fabs(v - threshold) <= kWindow * threshold;
+            }
+            if (near) sstate[s] = kExact;
+        }
+        __syncthreads();
+        for (int s = tid; s < nsh; s += kMaskThreads)
+            if (sstate[s] == kExact) {
+                const DevShell& sh = m.shells[s];
+                for (int r = 0; r < sh.count; ++r)
+                    smax[sh.ao + r] = 0ull;
+            }
+        __syncthreads();
+        for (int p0 = 0; p0 < count; p0 += kMaskThreads) {
+        const bool active = p0 + tid < count;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        if (active) {
+            if (cached) {
+                px = spts[4 * tid];
+                py = spts[4 * tid + 1];
+                pz = spts[4 * tid + 2];
+            } else {
+                point_coords(ps, start + p0 + tid, px, py, pz);
+            }
+        }
+        for (int q = 0; q < nsh; ++q) {
+            const int s = m.lorder[q];
+            if (sstate[s] != kExact) continue;
+            const DevShell& sh = m.shells[s];
+            const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
+            const double r2 = sq_norm(x, y, z);
+            const bool inside = active && r2 <= sh.cutoff_r2;
+            if (!__any(inside)) continue;
+            double amax[kMaxCart];
+#pragma unroll
+            for (int r = 0; r < kMaxCart; ++r)
+                amax[r] = 0.0;
+            if (inside) {
+                // (rarely reached: runtime-l code keeps this path's registers out of the budget)
+                eval_block_rt<ORDER, PSET>(sh, m, x, y, z, r2, [&](int row, const double (&v)[NP]) __attribute__((always_inline)) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NP; ++i)
+                        a = fmax(a, fabs(v[i]));
+                    amax[row] = a;
+                });
+            }
+#pragma unroll
+            for (int r = 0; r < kMaxCart; ++r) {
+                if (r < sh.count) {
+                    const double v = wave_max(amax[r]);
+                    if (lane == 0 && v > 0.0)
+                        atomicMax(&smax[sh.ao + r], static_cast<unsigned long long>(__double_as_longlong(v)));
+                }
+            }
+        }
+        }
+        __syncthreads();
     }
-    if (tid == 0) t.nkept[blockIdx.x] = base;
-    // Kept bits of every AO block.
-    const int nsh = m.nshells;
-    uint32_t* sbits = reinterpret_cast<uint32_t*>(smax + nao);
-    int32_t* sent = reinterpret_cast<int32_t*>(sbits + nsh);
-    int32_t* sslot = sent + nsh;
-    int32_t* sused = sslot + nsh;
+
+    // D. Kept bits of every AO block.
     for (int s = tid; s < nsh; s += kMaskThreads) {
         const DevShell& sh = m.shells[s];
-        uint32_t bits = 0;
-        for (int r = 0; r < sh.count; ++r)
-            if (!screen || count == 0 || __longlong_as_double(static_cast<long long>(smax[sh.ao + r])) > threshold)
-                bits |= 1u << r;
+        const uint32_t all = sh.count >= 32 ? 0xffffffffu : ((1u << sh.count) - 1u);
+        uint32_t bits = all;
+        if (screen) {
+            const int st = sstate[s];
+            if (st == kDrop) {
+                bits = 0;
+            } else if (st != kKeepAll) {
+                bits = 0;
+                for (int r = 0; r < sh.count; ++r)
+                    if (__longlong_as_double(static_cast<long long>(smax[sh.ao + r])) > threshold) bits |= 1u << r;
+            }
+        }
         sbits[s] = bits;
     }
     __syncthreads();
-    // Ordered compaction of the kept blocks: pass 0 in AO order (AO-P
-    // tables), passes l = max..0 by descending l (AO-MO entries).
+    // Flags, the ascending kept AO list and the kept blocks in AO order.
+    uint8_t* kept = t.kept + static_cast<long long>(blockIdx.x) * nao;
+    int32_t* list = t.kept_list + static_cast<long long>(blockIdx.x) * nao;
     int32_t* ks = t.kept_shell + static_cast<long long>(blockIdx.x) * nsh;
     uint32_t* kb = t.kept_bits + static_cast<long long>(blockIdx.x) * nsh;
-    int nent = 0;
-    for (int pass = -1; pass <= kMaxL; ++pass) {
-        const int want_l = pass < 0 ? -1 : kMaxL - pass;
-        if (tid == 0) base = 0;
-        __syncthreads();
-        for (int s0 = 0; s0 < nsh; s0 += kMaskThreads) {
-            const int s = s0 + tid;
-            bool take = false;
-            uint32_t bits = 0;
-            if (s < nsh) {
-                bits = sbits[s];
-                take = bits != 0 && (want_l < 0 || m.shells[s].l == want_l);
+    int aos = 0, blocks = 0;
+    for (int s0 = 0; s0 < nsh; s0 += kMaskThreads) {
+        const int s = s0 + tid;
+        const uint32_t bits = s < nsh ? sbits[s] : 0u;
+        int tot_a, tot_b;
+        const int pa = aos + block_scan(__popc(bits), scan_tmp, tot_a);
+        const int pb = blocks + block_scan(bits ? 1 : 0, scan_tmp, tot_b);
+        if (s < nsh) {
+            const DevShell& sh = m.shells[s];
+            int w = pa;
+            for (int r = 0; r < sh.count; ++r) {
+                const bool k = (bits >> r) & 1u;
+                kept[sh.ao + r] = k ? 1 : 0;
+                if (k) list[w++] = sh.ao + r;
             }
-            const unsigned long long ballot = __ballot(take);
-            const int lane = tid & 63, wave = tid >> 6;
-            const int before = __popcll(ballot & ((1ull << lane) - 1ull));
-            if (lane == 0) wave_counts[wave] = __popcll(ballot);
-            __syncthreads();
-            int offset = base;
-            for (int w = 0; w < wave; ++w)
-                offset += wave_counts[w];
-            if (take) {
-                const int pos = offset + before;
-                if (want_l < 0) {
-                    ks[pos] = s;
-                    kb[pos] = bits;
-                } else {
-                    const DevShell& sh = m.shells[s];
-                    const bool full = bits == (sh.count >= 32 ? 0xffffffffu : ((1u << sh.count) - 1u));
-                    const int mode = full ? kEntryCart : (sh.pure ? kEntryPure : kEntryCartMasked);
-                    sent[nent + pos] = s | (mode << 24);
-                    sslot[nent + pos] = mode == kEntryPure ? sh.count : sh.ncart; // rows, packed below
-                }
+            if (bits) {
+                ks[pb] = s;
+                kb[pb] = bits;
             }
-            __syncthreads();
-            if (tid == 0) {
-                int total = 0;
-                for (int w = 0; w < kMaskThreads / 64; ++w)
-                    total += wave_counts[w];
-                base += total;
-            }
-            __syncthreads();
         }
-        if (want_l < 0) {
-            if (tid == 0) t.nkshell[blockIdx.x] = base;
-        } else {
-            nent += base;
-        }
-        __syncthreads();
+        aos += tot_a;
+        blocks += tot_b;
     }
+    if (tid == 0) {
+        t.nkept[blockIdx.x] = aos;
+        t.nkshell[blockIdx.x] = blocks;
+    }
+    // AO->MO entries: kept blocks in descending l.
+    int nent = 0;
+    for (int q0 = 0; q0 < nsh; q0 += kMaskThreads) {
+        const int q = q0 + tid;
+        const int s = q < nsh ? m.lorder[q] : 0;
+        const uint32_t bits = q < nsh ? sbits[s] : 0u;
+        int tot;
+        const int pos = nent + block_scan(bits ? 1 : 0, scan_tmp, tot);
+        if (bits) {
+            const DevShell& sh = m.shells[s];
+            const uint32_t all = sh.count >= 32 ? 0xffffffffu : ((1u << sh.count) - 1u);
+            const int mode = bits == all ? kEntryCart : (sh.pure ? kEntryPure : kEntryCartMasked);
+            sent[pos] = s | (mode << 24);
+            sslot[pos] = mode == kEntryPure ? sh.count : sh.ncart; // rows, packed below
+        }
+        nent += tot;
+    }
+    __syncthreads();
     // Next-fit packing of the entries into chunks of at most kChunkRows rows.
     int32_t* cb = t.chunk_begin + static_cast<long long>(blockIdx.x) * (t.max_chunks + 1);
-    __shared__ int packed_entries;
     if (tid == 0) {
         int nc = 0, rows = kChunkRows, q = 0; // forces a new chunk at the first entry
         for (; q < nent; ++q) {
@@ -192,12 +411,12 @@ __global__ void __launch_bounds__(kMaskThreads)
         if (nc > 0) sused[nc - 1] = rows;
         cb[nc] = q;
         t.nchunks[blockIdx.x] = nc;
-        base = nc;
-        packed_entries = q;
+        packed[0] = nc;
+        packed[1] = q;
     }
     __syncthreads();
-    const int nc = base;
-    nent = packed_entries;
+    const int nc = packed[0];
+    nent = packed[1];
     int32_t* en = t.ent + static_cast<long long>(blockIdx.x) * nsh;
     uint32_t* eb = t.ent_bits + static_cast<long long>(blockIdx.x) * nsh;
     uint8_t* er = t.ent_row + static_cast<long long>(blockIdx.x) * nsh;
@@ -207,13 +426,13 @@ __global__ void __launch_bounds__(kMaskThreads)
         const int s = e & 0xffffff, mode = e >> 24;
         const int c = sslot[q] >> 8, row0 = sslot[q] & 0xff;
         const DevShell& sh = m.shells[s];
+        const uint32_t bits = sbits[s];
         en[q] = e;
-        eb[q] = sbits[s];
+        eb[q] = bits;
         er[q] = static_cast<uint8_t>(row0);
         const int first = mode == kEntryPure ? sh.prow : sh.crow;
         const int n = mode == kEntryPure ? sh.count : sh.ncart;
         // Rows of dropped AOs (partially kept blocks) get zero coefficients.
-        const uint32_t bits = sbits[s];
         if (c < t.max_chunks)
             for (int i = 0; i < n; ++i)
                 crows[c * kChunkRows + row0 + i] = (mode == kEntryCart || ((bits >> i) & 1u)) ? first + i : -1;
@@ -268,17 +487,18 @@ int max_chunks_bound(const DevShell* shells, int nshells) {
 }
 
 void launch_masks(const MolView& m, const PointSource& ps, long long npoints, int block_size, long long first_block,
-                  int nblk, int order, double threshold, const KeptTables& t, hipStream_t stream) {
+                  int nblk, int order, double threshold, const double* screen_r2, const KeptTables& t,
+                  hipStream_t stream) {
     if (nblk <= 0) return;
-    const size_t shmem = static_cast<size_t>(m.nao) * sizeof(unsigned long long) +
-                         static_cast<size_t>(m.nshells) * 3 * sizeof(int32_t) +
+    const size_t shmem = static_cast<size_t>(m.nao) * sizeof(unsigned long long) + 4 * kMaskThreads * sizeof(double) +
+                         static_cast<size_t>(m.nshells) * 4 * sizeof(int32_t) +
                          static_cast<size_t>(t.max_chunks + 1) * sizeof(int32_t);
     if (shmem > 64 * 1024) invalid("basis too large for the screening kernel (more than about 6000 AOs)");
     const dim3 grid(static_cast<unsigned>(nblk)), block(kMaskThreads);
     switch (order) {
-    case 0: mask_kernel<0><<<grid, block, shmem, stream>>>(m, ps, npoints, block_size, first_block, threshold, t); break;
-    case 1: mask_kernel<1><<<grid, block, shmem, stream>>>(m, ps, npoints, block_size, first_block, threshold, t); break;
-    default: mask_kernel<2><<<grid, block, shmem, stream>>>(m, ps, npoints, block_size, first_block, threshold, t); break;
+    case 0: mask_kernel<0><<<grid, block, shmem, stream>>>(m, ps, npoints, block_size, first_block, threshold, screen_r2, t); break;
+    case 1: mask_kernel<1><<<grid, block, shmem, stream>>>(m, ps, npoints, block_size, first_block, threshold, screen_r2, t); break;
+    default: mask_kernel<2><<<grid, block, shmem, stream>>>(m, ps, npoints, block_size, first_block, threshold, screen_r2, t); break;
     }
     KUMO_HIP(hipGetLastError());
 }

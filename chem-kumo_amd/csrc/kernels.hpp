@@ -52,7 +52,8 @@ int max_chunks_bound(const DevShell* shells, int nshells);
 // [first_block, first_block + nblk) of `block_size` points; order selects
 // the AO panels screened (0: values, 1: + gradient, 2: + Hessian).
 void launch_masks(const MolView& m, const PointSource& ps, long long npoints, int block_size, long long first_block,
-                  int nblk, int order, double threshold, const KeptTables& t, hipStream_t stream);
+                  int nblk, int order, double threshold, const double* screen_r2, const KeptTables& t,
+                  hipStream_t stream);
 
 // AO panels at listed points, column-major nao x (panels * n).
 void launch_ao_eval(const MolView& m, const PointSource& ps, long long n, int order, double* out, hipStream_t stream);

@@ -113,6 +113,14 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
             }
     }
     m.d_tg = upload(tg.data(), tg.size(), s);
+    std::vector<double> gk(static_cast<std::size_t>((kMaxL + 1) * kMaxCart), 0.0);
+    for (int l = 0; l <= kMaxL; ++l) {
+        const auto pw = gaussian_cartesian_order(l);
+        for (std::size_t k = 0; k < pw.size(); ++k)
+            gk[static_cast<std::size_t>(l * kMaxCart) + k] = component_factor(pw[0], pw[k]);
+    }
+    m.d_gk = upload(gk.data(), gk.size(), s);
+    m.d_lorder = upload(b.lorder.data(), b.lorder.size(), s);
     m.d_radw = upload(b.radw.data(), b.radw.size(), s);
     // Occupied-MO operand: alpha slots [0, A16), beta slots [A16, A16 + B16)
     // (closed shells contract the alpha channel only).
@@ -370,6 +378,18 @@ const double* kumo_hip_molecule::p_matrix(int component, hipStream_t stream) {
         KUMO_HIP(hipStreamSynchronize(stream));
     }
     return d_p[component].as<double>();
+}
+
+const double* kumo_hip_molecule::screen_r2(double threshold, hipStream_t stream) {
+    using namespace kumo;
+    std::lock_guard<std::mutex> guard(lazy);
+    auto it = d_screen_r2.find(threshold);
+    if (it == d_screen_r2.end()) {
+        const auto r2 = screening_radii2(basis, threshold);
+        it = d_screen_r2.emplace(threshold, upload(r2.data(), r2.size(), stream)).first;
+        KUMO_HIP(hipStreamSynchronize(stream));
+    }
+    return it->second.as<double>();
 }
 
 // Primitive pairs of the exact ESP: P contracted with the Hermite expansion

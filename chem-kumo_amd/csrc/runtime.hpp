@@ -12,6 +12,7 @@
 #include "../../include/kumo_hip.h"
 
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -64,7 +65,10 @@ struct kumo_hip_molecule {
     bool unrestricted = false, eligible = false;
     double total_rel = 0.0, spin_rel = 0.0;
     // Device residency
-    kumo::DeviceBuffer d_shells, d_exps, d_weights, d_tdense, d_radw, d_tg;
+    kumo::DeviceBuffer d_shells, d_exps, d_weights, d_tdense, d_radw, d_tg, d_gk, d_lorder;
+    // Per screening threshold: squared radius per block beyond which no AO of
+    // the block passes it (kept until the molecule is released).
+    std::map<double, kumo::DeviceBuffer> d_screen_r2;
     kumo::DeviceBuffer d_p[4]; // total, spin, alpha, beta (alpha/beta on first use)
     kumo::DeviceBuffer d_ca, d_cb; // full MO matrices
     kumo::DeviceBuffer d_op64, d_op32, d_opbf; // MO operand table (FP64, FP32, 3 bf16 planes)
@@ -78,9 +82,11 @@ struct kumo_hip_molecule {
 
     kumo::MolView view() const {
         return {d_shells.as<kumo::DevShell>(), static_cast<int>(basis.shells.size()), n, d_exps.as<double>(),
-                d_weights.as<double>(), d_tdense.as<double>(), d_radw.as<double>(), d_tg.as<double>()};
+                d_weights.as<double>(), d_tdense.as<double>(), d_radw.as<double>(), d_tg.as<double>(),
+                d_gk.as<double>(), d_lorder.as<int32_t>()};
     }
     const double* p_matrix(int component, hipStream_t stream);
+    const double* screen_r2(double threshold, hipStream_t stream);
     void ensure_esp(hipStream_t stream);
 };
 

@@ -81,6 +81,18 @@ def test_masks_bit_exact(ctx, name, order, block):
     assert np.array_equal(gm, om)
 
 
+@pytest.mark.parametrize("block", [64, 300])
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("threshold", [1e-12, 1e-6])
+def test_masks_bit_exact_points(ctx, block, order, threshold):
+    # listed points, blocks below and above 128 points, and a coarse threshold
+    rng = np.random.default_rng(5 + block)
+    pts = rng.normal(size=(1500, 3)) * 3.0
+    gm = gmol(ctx, "acrolein").masks(block, order, points=pts, threshold=threshold)
+    om = omol("acrolein").masks(block, order, points=pts, threshold=threshold)
+    assert np.array_equal(gm, om)
+
+
 def test_boys_matches_oracle(ctx):
     rng = np.random.default_rng(1)
     T = np.concatenate([[0.0, 1e-10, 0.025, 39.99, 40.0, 55.5], rng.uniform(0, 60, 2000)])
