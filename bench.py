@@ -55,10 +55,30 @@ def run_gpu(mol, grid, precision, steps, warmup, torch):
         kernel_ms += st["kernel_ms"]
         flops += st["flops"]
         launches = st["launches"]
+        rescue_ms, rescue_points = st["rescue_ms"], st["rescue_points"]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"seconds": dt, "points": n * steps, "kernel_ms": kernel_ms / steps, "flops": flops / steps,
-            "launches": launches, "out": out}
+            "launches": launches, "out": out, "rescue_ms": rescue_ms, "rescue_points": rescue_points}
+
+
+CHANNELS = ["rho", "grad_x", "grad_y", "grad_z", "lap", "tau", "elf", "lol"]
+
+
+def full_grid_errors(got, ref, torch):
+    """Per output channel: max abs error and max abs error relative to the
+    channel's largest |value|, rho relative errors above density floors, and
+    ELF errors where rho >= 1e-4, against the FP64 GPU path on every point."""
+    out = {}
+    rho = ref[0]
+    for c, name in enumerate(CHANNELS):
+        d = (got[c] - ref[c]).abs()
+        out[name] = {"max_abs": float(d.max()), "max_abs_over_max": float(d.max() / ref[c].abs().max())}
+    rel = (got[0] - rho).abs() / rho.clamp_min(1e-300)
+    for floor in (1e-6, 1e-4, 1e-2):
+        out["rho"][f"max_rel_rho_ge_{floor:g}"] = float(rel[rho >= floor].max())
+    out["elf"]["max_abs_rho_ge_1e-4"] = float((got[6] - ref[6]).abs()[rho >= 1e-4].max())
+    return out
 
 
 def main():
@@ -102,8 +122,11 @@ def main():
         log(f"FP64 GPU path: {fp64['value']:.4g} points/s")
         ref_out = r["out"]
         del r
+        full_errors = full_grid_errors(main_run["out"], ref_out, torch)
+        log(f"full-grid errors vs FP64 GPU: rho rel (>=1e-6) {full_errors['rho']['max_rel_rho_ge_1e-06']:.3g}, "
+            f"ELF abs {full_errors['elf']['max_abs']:.3g}")
     else:
-        ref_out = None
+        ref_out, full_errors = None, None
 
     kernel_s = main_run["kernel_ms"] * 1e-3
     achieved = main_run["flops"] / kernel_s if kernel_s > 0 else 0.0
@@ -155,6 +178,9 @@ def main():
         "fp64_gpu": fp64,
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
         "errors": errors,
+        "errors_full_grid": dict(full_errors or {}, vs="FP64 GPU path, all points") if full_errors else None,
+        "elf_rescue": {"points": main_run["rescue_points"], "fraction": main_run["rescue_points"] / n,
+                       "fp64_kernel_ms": main_run["rescue_ms"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

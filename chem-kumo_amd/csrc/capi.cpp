@@ -244,4 +244,11 @@ int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, i
     return KUMO_HIP_OK;
 }
 
+int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points) {
+    if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
+    if (rescue_ms) *rescue_ms = ctx->exec.last_rescue_ms;
+    if (rescue_points) *rescue_points = ctx->exec.last_rescue_points;
+    return KUMO_HIP_OK;
+}
+
 } // extern "C"

@@ -89,16 +89,51 @@ template <typename T> T* grow(DeviceBuffer& b, std::size_t count) {
     return b.as<T>();
 }
 
-hipEvent_t next_event(Executor& ex, bool start) {
-    if (ex.events_used == ex.events.size()) {
+hipEvent_t next_event(Executor& ex, bool start, int kind = 0) {
+    auto& ev = ex.events[kind];
+    std::size_t& used = ex.events_used[kind];
+    if (used == ev.size()) {
         hipEvent_t a, b;
         KUMO_HIP(hipEventCreate(&a));
         KUMO_HIP(hipEventCreate(&b));
-        ex.events.push_back({a, b});
+        ev.push_back({a, b});
     }
-    auto& pr = ex.events[ex.events_used];
-    if (!start) ++ex.events_used;
+    auto& pr = ev[used];
+    if (!start) ++used;
     return start ? pr.first : pr.second;
+}
+
+// Relative error of the channel moments assumed by the ELF rescue, per
+// contraction precision. On a 640-AO UHF molecule over 4.1e6 points the
+// BF16X6 estimate with 5e-7 flags 14 % of the points and leaves at most 8e-7
+// ELF error on the others (BF16X3 moments are about 100x less accurate).
+double rescue_eps(int precision) {
+    switch (precision) {
+    case KUMO_HIP_FP32: return 1e-6;
+    case KUMO_HIP_BF16X3: return 2e-4;
+    default: return 5e-7;
+    }
+}
+constexpr double kElfRescueTol = 1e-6; // largest ELF error left to the contraction precision
+
+RescuePlan rescue_plan(Executor& ex, long long count, int nblk, int block_size, int tiles_per_block) {
+    const std::size_t pts = static_cast<std::size_t>(nblk) * static_cast<std::size_t>(block_size);
+    const std::size_t tiles = static_cast<std::size_t>(nblk) * static_cast<std::size_t>(tiles_per_block);
+    const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };
+    const std::size_t sizes[] = {static_cast<std::size_t>(count), 2 * pts, 4 * static_cast<std::size_t>(nblk),
+                                 4 * tiles, 4 * tiles, 4};
+    std::size_t total = 0;
+    for (std::size_t x : sizes)
+        total += al(x);
+    ex.ws_rescue.resize(total);
+    char* p = ex.ws_rescue.as<char>();
+    void* ptr[6];
+    for (int i = 0; i < 6; ++i) {
+        ptr[i] = p;
+        p += al(sizes[i]);
+    }
+    return {static_cast<uint8_t*>(ptr[0]), static_cast<uint16_t*>(ptr[1]), static_cast<int32_t*>(ptr[2]),
+            static_cast<int32_t*>(ptr[3]), static_cast<int32_t*>(ptr[4]), static_cast<int32_t*>(ptr[5])};
 }
 
 } // namespace
@@ -268,6 +303,10 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 t.out[c] = dout[pr.first_channel[i] + static_cast<std::size_t>(c)];
             tasks.push_back(t);
         }
+        int elf_which = 0; // ELF requests on this pass: bit 0 total, 1 alpha, 2 beta
+        for (const auto& t : tasks)
+            if (t.kind == kElf) elf_which |= t.component == kTotal ? 1 : (t.component == kAlpha ? 2 : (t.component == kBeta ? 4 : 1));
+        int32_t rescue_tiles = 0;
         FieldTask* dtasks = grow<FieldTask>(ex.ws_tasks, tasks.size());
         KUMO_HIP(hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(FieldTask), hipMemcpyHostToDevice, s));
         std::vector<int32_t> nkept(static_cast<std::size_t>(cap));
@@ -303,6 +342,29 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 ++ex.last_launches;
                 for (int g = 1; g < groups; ++g)
                     launch_add_into(chm, chm + static_cast<long long>(g) * 2 * kMoments * count, 2 * kMoments * count, s);
+                if (elf_which && opt.precision != KUMO_HIP_FP64) {
+                    // FP64 rescue of the points whose ELF the contraction
+                    // precision leaves uncertain (same masks, same chunks).
+                    const int tpb = (bs + kRescueTile - 1) / kRescueTile;
+                    const long long rslots = static_cast<long long>(nb) * tpb * kRescueTile;
+                    const RescuePlan rp = rescue_plan(ex, count, nb, bs, tpb);
+                    launch_rescue_plan(chm, count, closed ? 1 : 0, elf_which, rescue_eps(opt.precision),
+                                       kElfRescueTol, bs, nb, kRescueTile, rp, s);
+                    AoMoLaunch r = a;
+                    r.precision = KUMO_HIP_FP64;
+                    r.moments = grow<double>(ex.ws_rmom, static_cast<std::size_t>(groups) * 2 * kMoments * rslots);
+                    r.batch_points = rslots;
+                    r.r_tile_blk = rp.tile_blk;
+                    r.r_tile_sub = rp.tile_sub;
+                    r.r_total = rp.total;
+                    r.r_count = rp.count;
+                    r.r_offs = rp.offs;
+                    KUMO_HIP(hipEventRecord(next_event(ex, true, 1), s));
+                    launch_ao_mo(r, s);
+                    KUMO_HIP(hipEventRecord(next_event(ex, false, 1), s));
+                    launch_rescue_scatter(r.moments, rslots, groups, kRescueTile, bs, rp, chm, count, s);
+                    KUMO_HIP(hipMemcpyAsync(&rescue_tiles, rp.total, 4, hipMemcpyDeviceToHost, s));
+                }
                 launch_combine(chm, cmp, count, 1, ao_order, closed ? 1 : 0, s);
             } else {
                 AoPLaunch a{};
@@ -335,6 +397,8 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             launch_fields(f, s);
             KUMO_HIP(hipMemcpyAsync(nkept.data(), ts.t.nkept, static_cast<std::size_t>(nb) * 4, hipMemcpyDeviceToHost, s));
             KUMO_HIP(hipStreamSynchronize(s));
+            ex.last_rescue_points += static_cast<long long>(rescue_tiles) * kRescueTile;
+            rescue_tiles = 0;
             for (int b = 0; b < nb; ++b) {
                 const long long pts_b = std::min<long long>(bs, n - (b0 + b) * static_cast<long long>(bs));
                 const double m_b = nkept[static_cast<std::size_t>(b)];
@@ -564,33 +628,38 @@ void Executor::create(int device, const double* table) {
 }
 
 void Executor::destroy() {
-    for (auto& e : events) {
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    events.clear();
     if (stream) (void)hipStreamSynchronize(stream);
-    for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_comp, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc})
+    for (auto& list : events) {
+        for (auto& e : list) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        list.clear();
+    }
+    for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_comp, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc,
+                            &ws_rescue, &ws_rmom})
         b->release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
 }
 
 void Executor::reset_stats() {
-    events_used = 0;
-    last_kernel_ms = 0.0;
+    events_used[0] = events_used[1] = 0;
+    last_kernel_ms = last_rescue_ms = 0.0;
     last_flops = 0.0;
-    last_launches = 0;
+    last_launches = last_rescue_points = 0;
 }
 
 void Executor::collect_timing() {
-    double ms = 0.0;
-    for (std::size_t e = 0; e < events_used; ++e) {
-        float t = 0.f;
-        KUMO_HIP(hipEventElapsedTime(&t, events[e].first, events[e].second));
-        ms += t;
+    for (int k = 0; k < 2; ++k) {
+        double ms = 0.0;
+        for (std::size_t e = 0; e < events_used[k]; ++e) {
+            float t = 0.f;
+            KUMO_HIP(hipEventElapsedTime(&t, events[k][e].first, events[k][e].second));
+            ms += t;
+        }
+        (k ? last_rescue_ms : last_kernel_ms) = ms;
     }
-    last_kernel_ms = ms;
 }
 
 } // namespace kumo

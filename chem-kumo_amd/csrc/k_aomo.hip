@@ -188,15 +188,28 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     double* red = reinterpret_cast<double*>(smem);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tiles_per_block = (a.block_size + T - 1) / T;
-    const int blk_local = blockIdx.x / tiles_per_block;
-    const int sub = blockIdx.x % tiles_per_block;
+    // Tile: T points of one screening block, consecutive or (rescue mode)
+    // listed by their offsets in the block.
+    int blk_local, p0 = 0, nt_valid;
+    const uint16_t* roff = nullptr;
+    if (a.r_tile_blk) {
+        const int t = blockIdx.x;
+        if (t >= *a.r_total) return; // uniform over the workgroup
+        blk_local = a.r_tile_blk[t];
+        const int sub = a.r_tile_sub[t];
+        nt_valid = min(T, a.r_count[blk_local] - sub * T);
+        roff = a.r_offs + static_cast<long long>(blk_local) * a.block_size + sub * T;
+    } else {
+        const int tiles_per_block = (a.block_size + T - 1) / T;
+        blk_local = blockIdx.x / tiles_per_block;
+        p0 = (blockIdx.x % tiles_per_block) * T;
+        const long long bs0 = (a.first_block + blk_local) * a.block_size;
+        const int bcount = static_cast<int>(min(static_cast<long long>(a.block_size), a.npoints - bs0));
+        if (p0 >= bcount) return; // uniform over the workgroup
+        nt_valid = min(T, bcount - p0);
+    }
     const long long blk = a.first_block + blk_local;
     const long long bstart = blk * a.block_size;
-    const int bcount = static_cast<int>(min(static_cast<long long>(a.block_size), a.npoints - bstart));
-    const int p0 = sub * T;
-    if (p0 >= bcount) return; // uniform over the workgroup
-    const int nt_valid = min(T, bcount - p0);
     const int slot_lo = a.slot0 + 128 * blockIdx.y;
     const int width = min(a.slot1, slot_lo + 128) - slot_lo;
     const int MT = width / 16;
@@ -221,7 +234,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     // This thread's point for AO generation (T divides 256).
     const int j = tid % T;
     double px = 0.0, py = 0.0, pz = 0.0;
-    if (j < nt_valid) point_coords(a.ps, bstart + p0 + j, px, py, pz);
+    if (j < nt_valid) point_coords(a.ps, bstart + (roff ? roff[j] : p0 + j), px, py, pz);
 
     typename P::Acc acc[TPW][NP];
 #pragma unroll
@@ -453,7 +466,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
         for (int mt = mt0; mt < mt1; ++mt)
             s += red[(static_cast<size_t>(mt) * kMoments + k) * T + jj];
         const double f = k == 0 ? 1.0 : (k <= 3 ? 2.0 : (k <= 6 ? 0.5 : 2.0));
-        const long long point = bstart + p0 + jj - a.batch_first;
+        const long long point = roff ? static_cast<long long>(blockIdx.x) * T + jj : bstart + p0 + jj - a.batch_first;
         double* dst = a.moments + (static_cast<long long>(blockIdx.y) * 2 + ch) * kMoments * a.batch_points;
         dst[k * a.batch_points + point] = f * s;
     }
@@ -467,6 +480,7 @@ template <int PREC, int ORDER, unsigned PSET> void launch_one(const AoMoLaunch& 
     const size_t lds = work > red ? work : red;
     const int tiles_per_block = (a.block_size + P::T - 1) / P::T;
     const int groups = (a.slot1 - a.slot0 + 127) / 128;
+    // (rescue mode: the same bound, every block's points in tiles of T)
     const dim3 grid(static_cast<unsigned>(a.nblk * tiles_per_block), static_cast<unsigned>(groups));
     static bool configured = false;
     if (!configured) {
@@ -490,6 +504,8 @@ template <int PREC> void launch_prec(const AoMoLaunch& a, hipStream_t stream) {
 void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream) {
     if (a.nblk <= 0 || a.slot1 <= a.slot0) return;
     if (a.slot0 % 16 || a.slot1 % 16) invalid("MO slot ranges must be multiples of 16");
+    static_assert(Prec<KUMO_HIP_FP64>::T == kRescueTile, "rescue tiles are FP64 workgroup tiles");
+    if (a.r_tile_blk && a.precision != KUMO_HIP_FP64) invalid("rescue tiles are evaluated in FP64");
     switch (a.precision) {
     case KUMO_HIP_FP64: launch_prec<KUMO_HIP_FP64>(a, stream); break;
     case KUMO_HIP_FP32: launch_prec<KUMO_HIP_FP32>(a, stream); break;

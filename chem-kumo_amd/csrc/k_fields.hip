@@ -276,7 +276,154 @@ __global__ void fill_kernel(double* dst, long long n, double v) {
     if (j < n) dst[j] = v;
 }
 
+// FP64 rescue of ill-conditioned ELF values. The Pauli term tau - |grad
+// rho|^2 / (8 rho) of each spin channel is a difference of two positive
+// terms; with a relative error eps on the contraction's moments it carries
+// an error up to eps (tau + |grad rho|^2 / (8 rho)). A point is recomputed in
+// FP64 when that error, propagated through ELF = 1 / (1 + chi^2) (the largest
+// |dELF/dchi| = 2 chi / (1 + chi^2)^2 over chi +- dchi), could exceed tol.
+__device__ inline bool elf_uncertain(const DDv* c, int nc, double fermi, double eps, double tol) {
+    double P = 0.0, Sum = 0.0, U = 0.0;
+    bool any = false;
+    for (int i = 0; i < nc; ++i) {
+        if (!(c[i].v > 0.0)) continue;
+        any = true;
+        const double q = gnorm2(c[i]) / (8.0 * c[i].v);
+        P += c[i].tau - q;
+        Sum += c[i].tau + q;
+        U += fermi * pow(c[i].v, 5.0 / 3.0);
+    }
+    if (!any) return false; // ELF is 0 in every precision
+    if (!(U > 0.0)) return true;
+    const double chi = fabs(P / U), dchi = eps * Sum / U;
+    const double lo = fmax(chi - dchi, 0.0), hi = chi + dchi;
+    const auto slope = [](double x) { return 2.0 * x / ((1.0 + x * x) * (1.0 + x * x)); };
+    const double peak = 0.5773502691896258; // 1/sqrt(3), where the slope is largest
+    const double smax = (lo <= peak && peak <= hi) ? slope(peak) : fmax(slope(lo), slope(hi));
+    return !(smax * dchi <= tol);
+}
+
+// which: bit 0 total ELF (both channels), bit 1 alpha, bit 2 beta.
+__global__ void rescue_flag_kernel(const double* ch, long long n, int closed, int which, double fermi, double eps,
+                                   double tol, uint8_t* flag) {
+    const long long j = blockIdx.x * 256ll + threadIdx.x;
+    if (j >= n) return;
+    DDv c[2];
+    c[0] = load_channel(ch, n, j);
+    c[1] = closed ? c[0] : load_channel(ch + kMoments * n, n, j);
+    bool f = false;
+    if (which & 1) f = f || elf_uncertain(c, 2, fermi, eps, tol);
+    if (which & 2) f = f || elf_uncertain(c, 1, fermi, eps, tol);
+    if (which & 4) f = f || elf_uncertain(c + 1, 1, fermi, eps, tol);
+    flag[j] = f ? 1 : 0;
+}
+
+// Per screening block (one workgroup each): the flagged points, compacted in
+// ascending order into offs[b * block_size ...], and their count.
+__global__ void __launch_bounds__(256) rescue_compact_kernel(const uint8_t* flag, long long n, int block_size,
+                                                             uint16_t* offs, int32_t* count) {
+    __shared__ int wsum[4];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long long b0 = static_cast<long long>(blockIdx.x) * block_size;
+    const int bn = static_cast<int>(min(static_cast<long long>(block_size), n - b0));
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int i0 = 0; i0 < bn; i0 += 256) {
+        const int i = i0 + tid;
+        const bool f = i < bn && flag[b0 + i];
+        const unsigned long long ballot = __ballot(f);
+        const int before = __popcll(ballot & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(ballot);
+        __syncthreads();
+        int off = base;
+        for (int k = 0; k < w; ++k)
+            off += wsum[k];
+        if (f) offs[b0 + off + before] = static_cast<uint16_t>(i);
+        __syncthreads();
+        if (tid == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (tid == 0) count[blockIdx.x] = base;
+}
+
+// Tiles of `tile` flagged points: tile t covers points [16 sub, ...) of the
+// flagged list of block tile_blk[t]; total receives the number of tiles.
+__global__ void __launch_bounds__(256) rescue_tiles_kernel(const int32_t* count, int nblk, int tile, int32_t* tile_blk,
+                                                           int32_t* tile_sub, int32_t* total) {
+    __shared__ int wsum[4];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblk; b0 += 256) {
+        const int b = b0 + tid;
+        const int nt = b < nblk ? (count[b] + tile - 1) / tile : 0;
+        int x = nt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        int off = base + x - nt;
+        for (int k = 0; k < w; ++k)
+            off += wsum[k];
+        for (int s2 = 0; s2 < nt; ++s2) {
+            tile_blk[off + s2] = b;
+            tile_sub[off + s2] = s2;
+        }
+        __syncthreads();
+        if (tid == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (tid == 0) *total = base;
+}
+
+// Writes the rescued channel moments (summed over MO slot groups) over the
+// contraction's values at the flagged points.
+__global__ void rescue_scatter_kernel(const double* rm, long long rslots, int groups, int tile, int block_size,
+                                      const int32_t* tile_blk, const int32_t* tile_sub, const int32_t* total,
+                                      const int32_t* count, const uint16_t* offs, double* ch, long long n) {
+    const long long q = blockIdx.x * 256ll + threadIdx.x;
+    const long long t = q / tile;
+    const int jj = static_cast<int>(q % tile);
+    if (t >= *total) return;
+    const int b = tile_blk[t], sub = tile_sub[t];
+    if (sub * tile + jj >= count[b]) return;
+    const long long point = static_cast<long long>(b) * block_size + offs[static_cast<long long>(b) * block_size + sub * tile + jj];
+    for (int c = 0; c < 2; ++c)
+        for (int k = 0; k < kMoments; ++k) {
+            double v = 0.0;
+            for (int g = 0; g < groups; ++g)
+                v += rm[((static_cast<long long>(g) * 2 + c) * kMoments + k) * rslots + q];
+            ch[(static_cast<long long>(c) * kMoments + k) * n + point] = v;
+        }
+}
+
 } // namespace
+
+void launch_rescue_plan(const double* channel_moments, long long n, int closed, int which, double eps, double tol,
+                        int block_size, int nblk, int tile, const RescuePlan& r, hipStream_t stream) {
+    if (n <= 0) return;
+    const FieldsLaunch f = fields_launch(nullptr, 0, 0, 0, nullptr);
+    rescue_flag_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(channel_moments, n, closed, which,
+                                                                                  f.fermi, eps, tol, r.flag);
+    KUMO_HIP(hipGetLastError());
+    rescue_compact_kernel<<<static_cast<unsigned>(nblk), 256, 0, stream>>>(r.flag, n, block_size, r.offs, r.count);
+    KUMO_HIP(hipGetLastError());
+    rescue_tiles_kernel<<<1, 256, 0, stream>>>(r.count, nblk, tile, r.tile_blk, r.tile_sub, r.total);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_rescue_scatter(const double* rescue_moments, long long rslots, int groups, int tile, int block_size,
+                           const RescuePlan& r, double* channel_moments, long long n, hipStream_t stream) {
+    if (rslots <= 0) return;
+    rescue_scatter_kernel<<<static_cast<unsigned>((rslots + 255) / 256), 256, 0, stream>>>(
+        rescue_moments, rslots, groups, tile, block_size, r.tile_blk, r.tile_sub, r.total, r.count, r.offs,
+        channel_moments, n);
+    KUMO_HIP(hipGetLastError());
+}
 
 void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream) {
     if (n <= 0) return;

@@ -73,6 +73,24 @@ struct MoOperand {
 // for every point of the batch, the moments of each channel, written to
 // moments[channel][kMoments][npoints_batch] (FP64). slot range [slot0,
 // slot1) selects the channels contracted.
+// Points of a batch recomputed in FP64 (ELF rescue), in tiles of points of
+// one screening block.
+struct RescuePlan {
+    uint8_t* flag;       // [batch points]
+    uint16_t* offs;      // [batch points]: flagged offsets per block, compacted
+    int32_t* count;      // [blocks]
+    int32_t* tile_blk;   // [max tiles]
+    int32_t* tile_sub;   // [max tiles]
+    int32_t* total;      // tiles in use
+};
+// Flags the points whose ELF (which: bit 0 total, 1 alpha, 2 beta) could be
+// off by more than tol when the channel moments carry a relative error eps,
+// and groups them into tiles of `tile` points of one screening block.
+void launch_rescue_plan(const double* channel_moments, long long n, int closed, int which, double eps, double tol,
+                        int block_size, int nblk, int tile, const RescuePlan& r, hipStream_t stream);
+void launch_rescue_scatter(const double* rescue_moments, long long rslots, int groups, int tile, int block_size,
+                           const RescuePlan& r, double* channel_moments, long long n, hipStream_t stream);
+
 struct AoMoLaunch {
     MolView mol;
     PointSource ps;
@@ -89,8 +107,16 @@ struct AoMoLaunch {
     int precision;           // kumo_hip_precision
     int slot0, slot1;
     double* moments;         // [2][kMoments][batch_points]
+    // Rescue mode (FP64 only): workgroup t evaluates the flagged points of
+    // tile t of the plan and writes moments at slot t * 16 + j.
+    const int32_t* r_tile_blk;
+    const int32_t* r_tile_sub;
+    const int32_t* r_total;
+    const int32_t* r_count;
+    const uint16_t* r_offs;
 };
 void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream);
+constexpr int kRescueTile = 16; // points per rescue tile (the FP64 kernel's tile)
 
 // AO-P contraction (density_derivatives_shared / densities_blocked,
 // src/density.cpp:149-229, 365-444): W = P_kept [Phi Phi_x Phi_y Phi_z] and

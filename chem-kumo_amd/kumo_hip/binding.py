@@ -55,7 +55,7 @@ EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip
            "kumo_hip_molecule_from_fchk", "kumo_hip_molecule_create", "kumo_hip_molecule_info_get",
            "kumo_hip_molecule_release", "kumo_hip_molecule_blocks", "kumo_hip_molecule_atoms",
            "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
-           "kumo_hip_boys", "kumo_hip_last_kernel_stats"]
+           "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats"]
 
 
 def lib():
@@ -84,6 +84,7 @@ def lib():
         L.kumo_hip_screening_masks.argtypes = [vp, vp, vp, vp, C.c_size_t, C.c_int32, C.c_int32, C.c_double, vp]
         L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_last_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
+        L.kumo_hip_last_rescue_stats.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -122,7 +123,10 @@ class Context:
         ms, fl, pfl = C.c_double(), C.c_double(), C.c_double()
         n = C.c_int64()
         lib().kumo_hip_last_kernel_stats(self.h, C.byref(ms), C.byref(n), C.byref(fl), C.byref(pfl))
-        return {"kernel_ms": ms.value, "launches": n.value, "flops": fl.value, "padded_flops": pfl.value}
+        rms, rpts = C.c_double(), C.c_int64()
+        lib().kumo_hip_last_rescue_stats(self.h, C.byref(rms), C.byref(rpts))
+        return {"kernel_ms": ms.value, "launches": n.value, "flops": fl.value, "padded_flops": pfl.value,
+                "rescue_ms": rms.value, "rescue_points": rpts.value}
 
 
 def grid_struct(grid):

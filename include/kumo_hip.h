@@ -188,6 +188,13 @@ int kumo_hip_boys(kumo_hip_context* ctx, const double* T, size_t n, int32_t m, d
 int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, int64_t* launches,
                                double* algorithmic_flops, double* padded_flops);
 
+/* Device time (ms) of the last evaluation's FP64 ELF rescue launches and the
+ * number of point slots they covered (points in 16-point tiles). With a
+ * reduced contraction precision and ELF requested, the points whose ELF the
+ * precision leaves uncertain beyond 1e-6 are recomputed in FP64; no
+ * counterpart in the reference (its CPU path is FP64 throughout). */
+int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points);
+
 #ifdef __cplusplus
 }
 #endif

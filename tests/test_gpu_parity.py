@@ -192,7 +192,9 @@ def test_block_size_and_batching_invariance(ctx):
 def test_reduced_precision_errors(ctx, precision, rho_tol):
     """Mixed-precision contractions against the FP64 oracle; the bound is on
     the relative density error where rho >= 1e-6 and the ELF absolute error
-    where rho >= 1e-4 (tests/derived_properties_test.cpp:188-200)."""
+    where rho >= 1e-4 (tests/derived_properties_test.cpp:188-200); with the
+    FP64 rescue of uncertain ELF points, the ELF error stays below 1e-5 at
+    every point, tails included."""
     d = load_sample("acrolein_density")
     grid = {"origin": d["origin"] + 2.0, "steps": np.eye(3) * 0.2, "counts": np.array([40, 38, 30])}
     props = [(K.Density, 0, 0, 0), (K.Elf, 0, 0, 0), (K.Laplacian, 0, 0, 0)]
@@ -202,9 +204,12 @@ def test_reduced_precision_errors(ctx, precision, rho_tol):
     sig = rho >= 1e-6
     rel = np.max(np.abs(g[0][0] - rho)[sig] / rho[sig])
     elf_err = np.max(np.abs(g[1][0] - o[1][0])[rho >= 1e-4])
-    print(f"[gpu] precision {int(precision)}: max rel rho {rel:.3g}, ELF abs {elf_err:.3g}", flush=True)
+    elf_all = np.max(np.abs(g[1][0] - o[1][0]))
+    print(f"[gpu] precision {int(precision)}: max rel rho {rel:.3g}, ELF abs {elf_err:.3g} (all points {elf_all:.3g})",
+          flush=True)
     assert rel < rho_tol
     assert elf_err < 50 * rho_tol
+    assert elf_all < 1e-5
 
 
 def test_esp_matches_oracle(ctx):
