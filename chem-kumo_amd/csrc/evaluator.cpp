@@ -89,6 +89,21 @@ template <typename T> T* grow(DeviceBuffer& b, std::size_t count) {
     return b.as<T>();
 }
 
+void sync(Executor& ex) {
+    if (!ex.capture) KUMO_HIP(hipStreamSynchronize(ex.stream));
+}
+
+template <typename T> void upload_async(Executor& ex, T* dst, const T* src, std::size_t count) {
+    const void* from = src;
+    if (ex.keep) {
+        ex.keep->emplace_back(reinterpret_cast<const char*>(src), reinterpret_cast<const char*>(src + count));
+        from = ex.keep->back().data();
+    }
+    KUMO_HIP(hipMemcpyAsync(dst, from, count * sizeof(T), hipMemcpyHostToDevice, ex.stream));
+}
+
+void record(Executor& ex, bool start, int kind = 0);
+
 hipEvent_t next_event(Executor& ex, bool start, int kind = 0) {
     auto& ev = ex.events[kind];
     std::size_t& used = ex.events_used[kind];
@@ -101,6 +116,10 @@ hipEvent_t next_event(Executor& ex, bool start, int kind = 0) {
     auto& pr = ev[used];
     if (!start) ++used;
     return start ? pr.first : pr.second;
+}
+
+void record(Executor& ex, bool start, int kind) {
+    if (!ex.capture) KUMO_HIP(hipEventRecord(next_event(ex, start, kind), ex.stream));
 }
 
 // Relative error of the channel moments assumed by the ELF rescue, per
@@ -308,7 +327,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             if (t.kind == kElf) elf_which |= t.component == kTotal ? 1 : (t.component == kAlpha ? 2 : (t.component == kBeta ? 4 : 1));
         int32_t rescue_tiles = 0;
         FieldTask* dtasks = grow<FieldTask>(ex.ws_tasks, tasks.size());
-        KUMO_HIP(hipMemcpyAsync(dtasks, tasks.data(), tasks.size() * sizeof(FieldTask), hipMemcpyHostToDevice, s));
+        upload_async(ex, dtasks, tasks.data(), tasks.size());
         std::vector<int32_t> nkept(static_cast<std::size_t>(cap));
         double flops = 0.0;
         const int panels = ao_order <= 0 ? 1 : (ao_order == 1 ? 4 : (full_h ? 10 : 7));
@@ -336,9 +355,9 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.slot0 = slot0;
                 a.slot1 = slot1;
                 a.moments = chm;
-                KUMO_HIP(hipEventRecord(next_event(ex, true), s));
+                record(ex, true);
                 launch_ao_mo(a, s);
-                KUMO_HIP(hipEventRecord(next_event(ex, false), s));
+                record(ex, false);
                 ++ex.last_launches;
                 for (int g = 1; g < groups; ++g)
                     launch_add_into(chm, chm + static_cast<long long>(g) * 2 * kMoments * count, 2 * kMoments * count, s);
@@ -359,11 +378,11 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                     r.r_total = rp.total;
                     r.r_count = rp.count;
                     r.r_offs = rp.offs;
-                    KUMO_HIP(hipEventRecord(next_event(ex, true, 1), s));
+                    record(ex, true, 1);
                     launch_ao_mo(r, s);
-                    KUMO_HIP(hipEventRecord(next_event(ex, false, 1), s));
+                    record(ex, false, 1);
                     launch_rescue_scatter(r.moments, rslots, groups, kRescueTile, bs, rp, chm, count, s);
-                    KUMO_HIP(hipMemcpyAsync(&rescue_tiles, rp.total, 4, hipMemcpyDeviceToHost, s));
+                    if (!ex.capture) KUMO_HIP(hipMemcpyAsync(&rescue_tiles, rp.total, 4, hipMemcpyDeviceToHost, s));
                 }
                 launch_combine(chm, cmp, count, 1, ao_order, closed ? 1 : 0, s);
             } else {
@@ -387,14 +406,18 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.w = grow<double>(ex.ws_w, static_cast<std::size_t>(nb) * nao * 4 * bs);
                 a.moments = chm;
                 launch_fill(chm, 2 * kMoments * count, 0.0, s);
-                KUMO_HIP(hipEventRecord(next_event(ex, true), s));
+                record(ex, true);
                 launch_ao_p(a, s);
-                KUMO_HIP(hipEventRecord(next_event(ex, false), s));
+                record(ex, false);
                 ++ex.last_launches;
                 launch_combine(chm, cmp, count, 0, ao_order, 0, s);
             }
             const FieldsLaunch f = fields_launch(cmp, count, first, static_cast<int>(tasks.size()), dtasks);
             launch_fields(f, s);
+            if (ex.capture) {
+                advance(static_cast<std::size_t>(count));
+                continue;
+            }
             KUMO_HIP(hipMemcpyAsync(nkept.data(), ts.t.nkept, static_cast<std::size_t>(nb) * 4, hipMemcpyDeviceToHost, s));
             KUMO_HIP(hipStreamSynchronize(s));
             ex.last_rescue_points += static_cast<long long>(rescue_tiles) * kRescueTile;
@@ -426,7 +449,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             launch_esp(mol->d_esp_pairs.as<EspPairDev>(), mol->esp_npairs, mol->d_esp_coef.as<double>(),
                        mol->d_esp_atoms.as<EspAtomDev>(), static_cast<int>(mol->atoms.size()), ex.boys_table, ps, p0,
                        cnt, dout[idx] + p0, s);
-            KUMO_HIP(hipStreamSynchronize(s));
+            sync(ex);
             advance(static_cast<std::size_t>(cnt));
         }
         for (std::size_t i = 0; i < nprops; ++i)
@@ -445,7 +468,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         const double* coeff = beta ? mol->d_cb.as<double>() : mol->d_ca.as<double>();
         const int ldc = beta ? mol->nmo_b : mol->nmo_a;
         int32_t* dmo = grow<int32_t>(ex.ws_misc, mos.size());
-        KUMO_HIP(hipMemcpyAsync(dmo, mos.data(), mos.size() * 4, hipMemcpyHostToDevice, s));
+        upload_async(ex, dmo, mos.data(), mos.size());
         const int NP = order == 0 ? 1 : (order == 1 ? 4 : 10);
         const long long chunk =
             std::max(1ll, std::min<long long>(range, (256ll << 20) / (8ll * NP * static_cast<long long>(mos.size()))));
@@ -463,7 +486,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                                             scratch + (static_cast<long long>(k) * NP + c0 + c) * cnt, cnt * sizeof(double),
                                             hipMemcpyDeviceToDevice, s));
             }
-            KUMO_HIP(hipStreamSynchronize(s));
+            sync(ex);
         }
     };
     if (pr.orbital_pass) {
@@ -505,7 +528,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             t.component = kTotal;
             t.out[0] = dout[pr.first_channel[i]];
             FieldTask* dtask = grow<FieldTask>(ex.ws_tasks, 1);
-            KUMO_HIP(hipMemcpyAsync(dtask, &t, sizeof(t), hipMemcpyHostToDevice, s));
+            upload_async(ex, dtask, &t, 1);
             for (long long b0 = b_first; b0 < b_end; b0 += cap) {
                 const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
                 const long long first = b0 * bs;
@@ -532,7 +555,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 launch_combine(chm, cmp, count, 0, 0, 1, s);
                 const FieldsLaunch f = fields_launch(cmp, count, first, 1, dtask);
                 launch_fields(f, s);
-                KUMO_HIP(hipStreamSynchronize(s));
+                sync(ex);
                 advance(static_cast<std::size_t>(count));
             }
         }
@@ -546,7 +569,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             advance(static_cast<std::size_t>(range));
         }
     }
-    KUMO_HIP(hipStreamSynchronize(s));
+    sync(ex);
 }
 
 void screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsArg& pts, int block_size,

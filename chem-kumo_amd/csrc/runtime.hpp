@@ -11,6 +11,7 @@
 
 #include "../../include/kumo_hip.h"
 
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -32,6 +33,11 @@ struct Executor {
     std::size_t events_used[2] = {0, 0};
     double last_kernel_ms = 0.0, last_flops = 0.0, last_rescue_ms = 0.0;
     long long last_launches = 0, last_rescue_points = 0;
+    // Capture mode (a batch recorded into a HIP graph): no host
+    // synchronisation, no timing events, no readback; small host arrays
+    // copied to the device are kept in `keep` so that graph replays read them.
+    bool capture = false;
+    std::deque<std::vector<char>>* keep = nullptr;
     void create(int device, const double* table);
     void destroy();
     void reset_stats();

@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from .api import Backend, Precision, channels
+from .api import Backend, Precision, PropertyKind, channels
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "libkumo_hip.so")
@@ -51,11 +51,23 @@ class _Info(C.Structure):
                 ("spin_relative_error", C.c_double), ("device_bytes", C.c_int64)]
 
 
+class _ServerOptions(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("reserved", C.c_int32), ("hbm_budget", C.c_int64),
+                ("chunk_points", C.c_int64)]
+
+
+class _RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("backend_used", C.c_int32), ("queue_ms", C.c_double),
+                ("run_ms", C.c_double), ("latency_ms", C.c_double), ("error", C.c_char * 256)]
+
+
 EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip_default_options",
            "kumo_hip_molecule_from_fchk", "kumo_hip_molecule_create", "kumo_hip_molecule_info_get",
            "kumo_hip_molecule_release", "kumo_hip_molecule_blocks", "kumo_hip_molecule_atoms",
            "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
-           "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats"]
+           "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_batch_create",
+           "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_release", "kumo_hip_server_create",
+           "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release"]
 
 
 def lib():
@@ -85,6 +97,16 @@ def lib():
         L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_last_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
         L.kumo_hip_last_rescue_stats.argtypes = [vp, vp, vp]
+        L.kumo_hip_batch_create.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_size_t, C.POINTER(_Options), C.c_int32,
+                                            vp, vp]
+        L.kumo_hip_batch_run.argtypes = [vp, C.c_int32]
+        L.kumo_hip_batch_last_ms.argtypes = [vp, vp]
+        L.kumo_hip_batch_release.argtypes = [vp]
+        L.kumo_hip_server_create.argtypes = [vp, C.POINTER(_ServerOptions), vp]
+        L.kumo_hip_server_submit.argtypes = [vp, vp, vp, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(_Options), vp, vp]
+        L.kumo_hip_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(_RequestInfo)]
+        L.kumo_hip_server_query.argtypes = [vp, C.c_uint64, vp]
+        L.kumo_hip_server_release.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -281,3 +303,106 @@ class Molecule:
         self.ctx.check(lib().kumo_hip_evaluate_grid(self.ctx.h, self.h, C.byref(g), P, len(props), C.byref(o), arr,
                                                     C.byref(used)))
         return used.value
+
+
+class Batch:
+    """Many (molecule, grid) items with one property list, evaluated together
+    into device buffers (kumo_hip_batch_*): outputs[i] lists item i's channel
+    pointers (torch tensors or raw device pointers)."""
+
+    def __init__(self, ctx, molecules, grids, props, outputs, streams=4, **opts):
+        self.ctx = ctx
+        self._mols = list(molecules)
+        n = len(self._mols)
+        M = (C.c_void_p * max(1, n))(*[m.h.value for m in self._mols])
+        G = (_Grid * max(1, n))(*[grid_struct(g) for g in grids])
+        P = (_Property * max(1, len(props)))(*[_Property(*map(int, p)) for p in props])
+        o = options(device_outputs=True, **opts)
+        flat = [int(x) for item in outputs for x in item]
+        arr = (C.c_void_p * max(1, len(flat)))(*flat)
+        h = C.c_void_p()
+        ctx.check(lib().kumo_hip_batch_create(ctx.h, n, M, G, P, len(props), C.byref(o), int(streams), arr,
+                                              C.byref(h)))
+        self.h = h
+
+    def run(self, graph=True):
+        self.ctx.check(lib().kumo_hip_batch_run(self.h, 1 if graph else 0))
+        ms = C.c_double()
+        lib().kumo_hip_batch_last_ms(self.h, C.byref(ms))
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().kumo_hip_batch_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Request:
+    """A submitted server request; `outputs` fills in when it completes."""
+
+    def __init__(self, server, rid, outputs, props, keep):
+        self.server, self.id, self.outputs, self.props, self._keep = server, rid, outputs, props, keep
+        self.info = None
+
+    def wait(self):
+        """Blocks until done; returns (results as evaluate() does, info dict)."""
+        if self.info is None:
+            inf = _RequestInfo()
+            rc = lib().kumo_hip_server_wait(self.server.h, self.id, C.byref(inf))
+            self.info = {"status": inf.status, "backend": inf.backend_used, "queue_ms": inf.queue_ms,
+                         "run_ms": inf.run_ms, "latency_ms": inf.latency_ms, "error": inf.error.decode()}
+            if rc != 0:
+                raise RuntimeError(f"request failed ({rc}): {self.info['error']}")
+        res, c = [], 0
+        for p in self.props:
+            k = channels(PropertyKind(int(p[0])))
+            res.append(self.outputs[c:c + k])
+            c += k
+        return res, self.info
+
+
+class Server:
+    """Serving loop over resident molecules (kumo_hip_server_*)."""
+
+    def __init__(self, ctx, workers=4, hbm_budget=0, chunk_points=0):
+        self.ctx = ctx
+        o = _ServerOptions(int(workers), 0, int(hbm_budget), int(chunk_points))
+        h = C.c_void_p()
+        ctx.check(lib().kumo_hip_server_create(ctx.h, C.byref(o), C.byref(h)))
+        self.h = h
+
+    def submit(self, mol, props, grid=None, points=None, **opts):
+        P = (_Property * max(1, len(props)))(*[_Property(*map(int, p)) for p in props])
+        o = options(**opts)
+        if grid is not None:
+            g = grid_struct(grid)
+            n = int(np.prod(np.asarray(grid["counts"], dtype=np.int64)))
+            gp, pp, npts = C.byref(g), None, 0
+        else:
+            pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+            n = pts.shape[0]
+            g, gp, pp, npts = None, None, _p(pts), n
+        nch = sum(channels(PropertyKind(int(p[0]))) for p in props)
+        out = np.zeros((nch, n))
+        arr = (C.c_void_p * max(1, nch))(*[out[c].ctypes.data for c in range(nch)])
+        rid = C.c_uint64()
+        self.ctx.check(lib().kumo_hip_server_submit(self.h, mol.h, gp, pp, npts, P, len(props), C.byref(o), arr,
+                                                    C.byref(rid)))
+        return Request(self, rid.value, out, props, (g, P, arr, o))
+
+    def close(self):
+        if self.h:
+            lib().kumo_hip_server_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

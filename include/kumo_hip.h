@@ -180,6 +180,71 @@ int kumo_hip_screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol
  * out is n x (m + 1). */
 int kumo_hip_boys(kumo_hip_context* ctx, const double* T, size_t n, int32_t m, double* out);
 
+/* ---- batched evaluation ----
+ * A fixed list of (molecule, grid) items sharing one property list and one
+ * set of options, evaluated across `streams` HIP streams (0: 4). outputs
+ * holds, for item i, its channel pointers at outputs[i * nchannels + c]
+ * (nchannels = sum over properties of their component counts); they are
+ * device pointers and must stay valid while the batch lives. With use_graph,
+ * the first kumo_hip_batch_run records every launch of every item into one
+ * HIP graph and each later run replays it; without it, each run issues the
+ * launches again. Each item's results equal kumo_hip_evaluate_grid's on that
+ * item (results do not depend on how items share the device). The options'
+ * progress callback is not used. No counterpart in the reference (its
+ * Evaluator evaluates one grid per call); it serves callers that evaluate
+ * many molecules per step. */
+typedef struct kumo_hip_batch kumo_hip_batch;
+int kumo_hip_batch_create(kumo_hip_context* ctx, size_t nitems, const kumo_hip_molecule* const* molecules,
+                          const kumo_hip_grid* grids, const kumo_hip_property* properties, size_t nproperties,
+                          const kumo_hip_options* options, int32_t streams, double* const* outputs,
+                          kumo_hip_batch** batch);
+int kumo_hip_batch_run(kumo_hip_batch* batch, int32_t use_graph);
+/* Device time (ms) of the last run, measured with HIP events around it. */
+int kumo_hip_batch_last_ms(const kumo_hip_batch* batch, double* ms);
+int kumo_hip_batch_release(kumo_hip_batch* batch);
+
+/* ---- serving loop ----
+ * A request server over resident molecules: `workers` threads, each with its
+ * own HIP stream and workspaces (0: 4), evaluate admitted requests in
+ * block-aligned ranges of about chunk_points points (0: 131072), always
+ * taking the next range of the admitted request with the fewest points left,
+ * so that point and line requests are not queued behind cubes and ranges of
+ * different requests run on the device at the same time. A request is
+ * admitted when its device memory (outputs and listed points) fits in
+ * hbm_budget bytes (0: 8 GiB) beside the requests in flight, in arrival
+ * order; one request alone is always admitted. Results are copied to the
+ * host `outputs` (one pointer per component, as kumo_hip_evaluate_*), which
+ * must stay valid until kumo_hip_server_wait returns, and equal those of a
+ * lone kumo_hip_evaluate_* call. No counterpart in the reference (a library
+ * call per grid there); it is for serving many clients from one GPU. */
+typedef struct kumo_hip_server kumo_hip_server;
+typedef struct kumo_hip_server_options {
+    int32_t workers;
+    int32_t reserved;
+    int64_t hbm_budget;
+    int64_t chunk_points;
+} kumo_hip_server_options;
+typedef struct kumo_hip_request_info {
+    int32_t status;       /* kumo_hip_status of the request */
+    int32_t backend_used;
+    double queue_ms;      /* submission -> first range started */
+    double run_ms;        /* first range started -> results on the host */
+    double latency_ms;    /* submission -> results on the host */
+    char error[256];
+} kumo_hip_request_info;
+int kumo_hip_server_create(kumo_hip_context* ctx, const kumo_hip_server_options* options, kumo_hip_server** server);
+/* grid or points (host memory, 3 * npoints doubles; copied at submission). */
+int kumo_hip_server_submit(kumo_hip_server* server, const kumo_hip_molecule* mol, const kumo_hip_grid* grid,
+                           const double* points, size_t npoints, const kumo_hip_property* properties,
+                           size_t nproperties, const kumo_hip_options* options, double* const* outputs,
+                           uint64_t* request);
+/* Blocks until the request is done, fills info and forgets the request. */
+int kumo_hip_server_wait(kumo_hip_server* server, uint64_t request, kumo_hip_request_info* info);
+/* done = 1 once the request's results are on the host. */
+int kumo_hip_server_query(kumo_hip_server* server, uint64_t request, int32_t* done);
+/* Finishes the requests in flight, then stops the workers. */
+int kumo_hip_server_release(kumo_hip_server* server);
+
 /* ---- timing hooks for the benchmark ----
  * Device time (ms) of the last evaluation's dominant contraction kernel,
  * summed over its launches and measured with HIP events on the stream the
