@@ -81,11 +81,245 @@ def full_grid_errors(got, ref, torch):
     return out
 
 
+PROPS_BATCH = [(K.Density, 0, 0, 0), (K.Gradient, 0, 0, 0), (K.Laplacian, 0, 0, 0), (K.Tau, 0, 0, 0),
+               (K.Elf, 0, 0, 0)]
+
+
+def bench_batch(args):
+    """256 synthetic molecules (3-60 atoms, 6-31G*- or def2-TZVP-shaped, each
+    on its own cube of edge 40-120 at 0.2 bohr), rho, grad, lap, tau, ELF,
+    evaluated as one batch over 4 streams and replayed from a HIP graph."""
+    import torch
+    from kumo_hip import synthetic
+    from kumo_hip.binding import Batch, Context, Molecule
+    rng = np.random.default_rng(2)
+    specs = [(int(rng.integers(3, 61)), "pople" if rng.random() < 0.5 else "tzvp", int(rng.integers(40, 121)))
+             for _ in range(args.molecules)]
+    ctx = Context(0)
+    t = time.perf_counter()
+    smols, grids = [], []
+    for i, (na, fam, edge) in enumerate(specs):
+        seed = 1000 + i
+        while True:  # redraw the rare exponent sets that are numerically linearly dependent
+            try:
+                sm, grid = synthetic.batch_molecule(seed, na, fam, edge)
+                break
+            except ValueError:
+                seed += 100000
+        smols.append(sm)
+        grids.append(grid)
+    mols = [Molecule(ctx, data=sm) for sm in smols]
+    counts = [int(np.prod(g["counts"].astype(np.int64))) for g in grids]
+    total = sum(counts)
+    log(f"{len(mols)} molecules ({sum(m.info['nbasis'] for m in mols)} AOs in all), {total} points, "
+        f"ready in {time.perf_counter() - t:.1f}s")
+    nch = sum(channels(p[0]) for p in PROPS_BATCH)
+    res = {}
+    outs = {}
+    for prec in (args.precision, "FP64"):
+        o = [torch.empty((nch, n), dtype=torch.float64, device="cuda") for n in counts]
+        ptrs = [[t_[c].data_ptr() for c in range(nch)] for t_ in o]
+        b = Batch(ctx, mols, grids, PROPS_BATCH, ptrs, streams=4, precision=int(Precision[prec]))
+        for _ in range(max(1, args.warmup)):
+            b.run(graph=True)
+        ms = [b.run(graph=True) for _ in range(args.steps)]
+        eager = b.run(graph=False)
+        res[prec] = {"value": total / (min(ms) * 1e-3), "ms_per_step": min(ms), "ms_median": float(np.median(ms)),
+                     "ms_without_graph": eager}
+        log(f"{prec}: {res[prec]['value']:.4g} points/s, {min(ms):.1f} ms/step (graph), {eager:.1f} ms without")
+        outs[prec] = o
+        b.close()
+    rel, elf = 0.0, 0.0
+    for a, r in zip(outs[args.precision], outs["FP64"]):
+        rho = r[0]
+        sel = rho >= 1e-6
+        if bool(sel.any()):
+            rel = max(rel, float(((a[0] - rho).abs()[sel] / rho[sel]).max()))
+        elf = max(elf, float((a[6] - r[6]).abs().max()))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from support import oracle
+        pts_done, secs = 0, 0.0
+        for i in range(0, len(mols), max(1, len(mols) // 16)):
+            om = oracle.Molecule(smols[i])
+            pts = oracle.grid_points(grids[i])
+            sel = np.arange(0, len(pts) // 128, max(1, len(pts) // 128 // 32))[:32]
+            idx = (sel[:, None] * 128 + np.arange(128)[None, :]).ravel()
+            t0 = time.perf_counter()
+            om.evaluate(PROPS_BATCH, points=pts[idx], block_size=128)
+            secs += time.perf_counter() - t0
+            pts_done += len(idx)
+        cpu = {"value": pts_done / secs, "unit": "grid points/s", "cores": int(oracle.lib().orc_threads()),
+               "kind": "port", "sample": f"32 screening blocks from each of 16 of the molecules ({pts_done} points)"}
+        log(f"CPU oracle: {cpu['value']:.4g} points/s")
+    out = {"metric": "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error (batched)",
+           "value": res[args.precision]["value"], "unit": "grid points/s", "n_gpus": 1,
+           "ms_per_step": res[args.precision]["ms_per_step"], "higher_is_better": True,
+           "dtype": args.precision.lower(), "data": "synthetic (seeded molecules built in memory)",
+           "config": {"workload": f"{len(mols)} molecules of 3-60 atoms (6-31G*/def2-TZVP shapes 50/50), cubes of "
+                                  "edge 40-120 at 0.2 bohr, rho/grad/lap/tau/ELF, one graph replay per step",
+                      "points": total, "streams": 4},
+           "fp64_gpu": res["FP64"], "speedup_vs_fp64_gpu": res[args.precision]["value"] / res["FP64"]["value"],
+           "errors_vs_fp64_gpu": {"rho_max_rel_rho_ge_1e-6": rel, "elf_max_abs": elf},
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+    for m in mols:
+        m.close()
+    ctx.close()
+
+
+def bench_esp(args):
+    """Exact electrostatic potential of a 20-atom 6-31G*-shaped molecule on
+    100 000 points of its 1.4 x van der Waals surface (FP64, Boys function)."""
+    import torch
+    from kumo_hip import synthetic
+    from kumo_hip.binding import Context, Molecule
+    sm = synthetic.pople_molecule()
+    pts = synthetic.vdw_shell_points(sm.atomic_numbers, sm.xyz, 100000)
+    ctx = Context(0)
+    mol = Molecule(ctx, data=sm)
+    prop = [(K.ElectrostaticPotential, 0, 0, 0)]
+    ms = []
+    for r in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        v, _ = mol.evaluate(prop, points=pts)
+        if r >= args.warmup:
+            ms.append(1e3 * (time.perf_counter() - t0))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from support import oracle
+    om = oracle.Molecule(sm)
+    sample = pts[:: len(pts) // 500][:500]
+    t0 = time.perf_counter()
+    ref = om.esp(sample)
+    cpu_s = time.perf_counter() - t0
+    got = v[0][0][:: len(pts) // 500][:500]
+    pairs = om.esp_pair_count()
+    out = {"metric": "ESP points/s (exact, FP64)", "value": len(pts) / (min(ms) * 1e-3), "unit": "points/s",
+           "ms_per_step": min(ms), "pairs": pairs, "pair_points_per_s": pairs * len(pts) / (min(ms) * 1e-3),
+           "max_abs_err_vs_oracle": float(np.max(np.abs(got - ref))),
+           "max_rel_err_vs_oracle": float(np.max(np.abs(got - ref) / np.abs(ref))),
+           "config": {"workload": f"20-atom 6-31G*-shaped molecule ({sm.nbasis} AOs), {len(pts)} points on the "
+                                  "1.4 x vdW surface", "includes": "host->device points and device->host values"},
+           "cpu_baseline": {"value": len(sample) / cpu_s, "unit": "points/s", "cores": int(oracle.lib().orc_threads()),
+                            "kind": "port", "sample": f"{len(sample)} of the points"}}
+    print(json.dumps(out))
+    mol.close()
+    ctx.close()
+
+
+def bench_serving(args):
+    """Serving load: resident synthetic molecules of 10-150 atoms
+    (def2-TZVP-shaped), Poisson arrivals of point batches (40 %), lines of 512
+    points (30 %), 256 x 256 planes (20 %) and cubes of edge 64-128 (10 %),
+    properties drawn from rho, lap rho, ELF, RDG + sign(lambda2) rho and an MO
+    amplitude; p50 / p99 latency per request kind and aggregate points/s."""
+    from kumo_hip import synthetic
+    from kumo_hip.binding import Context, Molecule, Server
+    rng = np.random.default_rng(4)
+    ctx = Context(0)
+    t = time.perf_counter()
+    smols = []
+    for i in range(args.molecules):
+        na, seed = int(rng.integers(10, 151)), 2000 + i
+        while True:  # redraw the rare exponent sets that are numerically linearly dependent
+            try:
+                smols.append(synthetic.batch_molecule(seed, na, "tzvp", 40)[0])
+                break
+            except ValueError:
+                seed += 100000
+    mols = [Molecule(ctx, data=sm) for sm in smols]
+    log(f"{len(mols)} resident molecules ({sum(m.info['nbasis'] for m in mols)} AOs in all) "
+        f"ready in {time.perf_counter() - t:.1f}s")
+    props_sets = [[(K.Density, 0, 0, 0)], [(K.Laplacian, 0, 0, 0)], [(K.Elf, 0, 0, 0)],
+                  [(K.ReducedGradient, 0, 0, 0), (K.SignLambda2Density, 0, 0, 0)], None]
+    srv = Server(ctx, workers=args.workers, chunk_points=args.chunk)
+    # warm every molecule's lazily built tables and the workers' workspaces
+    for m, sm in zip(mols, smols):
+        srv.submit(m, [(K.Elf, 0, 0, 0), (K.SignLambda2Density, 0, 0, 0)], points=sm.xyz,
+                   precision=int(Precision.BF16X6)).wait()
+    results = []
+    for rate in args.rates:
+        n_req = int(rate * args.seconds)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        reqs = []
+        for _ in range(n_req):
+            i = int(rng.integers(len(mols)))
+            sm = smols[i]
+            c = sm.xyz.mean(0)
+            u = rng.random()
+            if u < 0.4:
+                kind = "points"
+                where = {"points": c + rng.normal(size=(int(rng.integers(1, 1001)), 3)) * 3.0}
+            elif u < 0.7:
+                kind = "line"
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                where = {"points": c + np.linspace(-8, 8, 512)[:, None] * d}
+            elif u < 0.9:
+                kind = "plane"
+                ax = np.linspace(-8, 8, 256)
+                gx, gy = np.meshgrid(ax, ax, indexing="ij")
+                where = {"points": np.stack([gx.ravel(), gy.ravel(), np.zeros(gx.size)], 1) + c}
+            else:
+                kind = "cube"
+                e = int(rng.integers(64, 129))
+                h = 16.0 / e
+                where = {"grid": {"origin": c - 8.0, "steps": np.eye(3) * h, "counts": np.array([e, e, e])}}
+            ps = props_sets[int(rng.integers(len(props_sets)))]
+            if ps is None:
+                ps = [(K.Orbital, 0, 0, int(sm.nalpha))]
+            reqs.append((i, kind, ps, where))
+        t0 = time.perf_counter()
+        handles = []
+        for (i, kind, ps, where), ta in zip(reqs, arrivals):
+            delay = t0 + ta - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            handles.append((kind, srv.submit(mols[i], ps, precision=int(Precision.BF16X6), **where),
+                            len(where["points"]) if "points" in where else int(np.prod(where["grid"]["counts"]))))
+        lat = {}
+        npts = 0
+        for kind, h, n in handles:
+            _, info = h.wait()
+            lat.setdefault(kind, []).append(info["latency_ms"])
+            npts += n
+        wall = time.perf_counter() - t0
+        allv = [x for v in lat.values() for x in v]
+        entry = {"rate_per_s": rate, "requests": n_req, "seconds": wall, "points_per_s": npts / wall,
+                 "p50_ms": float(np.percentile(allv, 50)), "p99_ms": float(np.percentile(allv, 99))}
+        for kind, v in lat.items():
+            entry[kind] = {"n": len(v), "p50_ms": float(np.percentile(v, 50)), "p99_ms": float(np.percentile(v, 99))}
+        small = lat.get("points", []) + lat.get("line", [])
+        entry["point_line_p99_ms"] = float(np.percentile(small, 99)) if small else None
+        log(f"rate {rate}/s: p50 {entry['p50_ms']:.1f} ms, p99 {entry['p99_ms']:.1f} ms, point/line p99 "
+            f"{entry['point_line_p99_ms']:.1f} ms, {entry['points_per_s']:.4g} points/s")
+        results.append(entry)
+    out = {"metric": "serving latency p50/p99 and aggregate points/s", "unit": "ms",
+           "config": {"workload": f"{len(mols)} resident def2-TZVP-shaped molecules of 10-150 atoms; Poisson "
+                                  "arrivals 40 % points (1-1000), 30 % lines (512), 20 % planes (256^2), 10 % cubes "
+                                  "(64^3-128^3); rho, lap, ELF, RDG+sign(l2)rho or MO amplitude; BF16X6",
+                      "workers": args.workers, "chunk_points": args.chunk},
+           "loads": results}
+    print(json.dumps(out))
+    srv.close()
+    for m in mols:
+        m.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1, choices=[1, 2, 3, 4],
+                    help="1: 160^3 cube (headline line); 2: batched molecules; 3: ESP; 4: serving load")
+    ap.add_argument("--molecules", type=int, default=None, help="molecules of configurations 2 and 4")
+    ap.add_argument("--rates", type=float, nargs="*", default=[20.0, 50.0], help="arrival rates (config 4)")
+    ap.add_argument("--seconds", type=float, default=10.0, help="arrival window per rate (config 4)")
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--chunk", type=int, default=131072)
     ap.add_argument("--precision", default="BF16X6", choices=list(PEAK))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -95,6 +329,14 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
               "screening blocks across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.config == 2:
+        args.molecules = args.molecules or 256
+        return bench_batch(args)
+    if args.config == 3:
+        return bench_esp(args)
+    if args.config == 4:
+        args.molecules = args.molecules or 32
+        return bench_serving(args)
 
     import torch
     from kumo_hip import synthetic

@@ -26,6 +26,7 @@ struct MolView {
     const double* tg;     // [kMaxL+1][kMaxPure][kMaxCart] T[r][k] g[k] (transform of unscaled components)
     const double* gk;     // [kMaxL+1][kMaxCart] g[k]: Cartesian AO = g[k] x unscaled component
     const int32_t* lorder; // blocks by descending l
+    const ShellRec* rec;   // per block, for the contraction kernels
 };
 
 // Cartesian powers in Gaussian FCHK order (src/basis_evaluator.cpp:135-152).
@@ -370,6 +371,34 @@ __device__ __attribute__((always_inline)) inline void radial_sums(const DevShell
     R[2] = rr;
 }
 
+// radial_sums from a block's record (primitives inline up to kRecPrim).
+template <int ORDER>
+__device__ __attribute__((always_inline)) inline void radial_sums_rec(const ShellRec& s, const MolView& m, double r2,
+                                                                    double (&R)[3]) {
+#pragma clang fp contract(fast)
+    double r0 = 0.0, r1 = 0.0, rr = 0.0;
+    const auto add = [&](double a, double u) __attribute__((always_inline)) {
+        const double g = u * exp(-a * r2);
+        r0 += g;
+        if constexpr (ORDER >= 1) {
+            const double t = -2.0 * a * g;
+            r1 += t;
+            if constexpr (ORDER == 2) rr += -2.0 * a * t;
+        }
+    };
+    const int np = s.nprim;
+    if (np <= kRecPrim) {
+        for (int p = 0; p < np; ++p)
+            add(s.e[p], s.w[p]);
+    } else {
+        for (int p = 0; p < np; ++p)
+            add(m.exps[s.prim + p], m.radw[s.prim + p]);
+    }
+    R[0] = r0;
+    R[1] = r1;
+    R[2] = rr;
+}
+
 // Panels of the unscaled component P_k R for each k of angular momentum L:
 // emit(k, v[NP]). With f0 = c^q, f1 = q c^(q-1), f2 = q (q-1) c^(q-2) per axis:
 //   d/da (P R)      = P_a R0 + a P R1
@@ -434,8 +463,22 @@ __device__ __attribute__((always_inline)) inline void cart_components(double x, 
 // the rare high-l blocks and partially kept pure blocks, where unrolling every
 // component would cost registers in every other path of a kernel.
 template <int ORDER, unsigned PSET>
+__device__ __attribute__((always_inline)) inline void cart_component_pw(const int (&qq)[3], double x, double y,
+                                                                      double z, const double (&R)[3],
+                                                                      double (&v)[popcount10(PSET)]);
+
+template <int ORDER, unsigned PSET>
 __device__ __attribute__((always_inline)) inline void cart_component_rt(int L, int k, double x, double y, double z,
                                                                       const double (&R)[3],
+                                                                      double (&v)[popcount10(PSET)]) {
+    const int qq[3] = {kPow[L][k][0], kPow[L][k][1], kPow[L][k][2]};
+    cart_component_pw<ORDER, PSET>(qq, x, y, z, R, v);
+}
+
+// The component with powers (qq[0], qq[1], qq[2]) as runtime values.
+template <int ORDER, unsigned PSET>
+__device__ __attribute__((always_inline)) inline void cart_component_pw(const int (&qq)[3], double x, double y,
+                                                                      double z, const double (&R)[3],
                                                                       double (&v)[popcount10(PSET)]) {
 #pragma clang fp contract(fast)
     constexpr int NP = popcount10(PSET);
@@ -443,7 +486,7 @@ __device__ __attribute__((always_inline)) inline void cart_component_rt(int L, i
     double f[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int q = kPow[L][k][a];
+        const int q = qq[a];
         double p2 = 1.0; // c^(q-2)
         for (int n = 2; n < q; ++n)
             p2 *= c[a];

@@ -47,6 +47,19 @@ struct DevShell {
     int32_t pad;
 };
 
+// A block's data for the contraction kernels in one contiguous record, so
+// that a lane reads everything it needs for a row with independent loads:
+// centre, cutoff, l and up to kRecPrim primitives (exponent, factored
+// weight radw); blocks with more primitives read exps[] / radw[] instead.
+constexpr int kRecPrim = 8;
+struct ShellRec {
+    double o[3];
+    double cutoff_r2;
+    int32_t l, nprim, prim, pad;
+    double e[kRecPrim];
+    double w[kRecPrim];
+};
+
 // Sparse transform entry: pure row `row` += coef * Cartesian component `cart`.
 struct PureTerm {
     double coef;

@@ -53,7 +53,7 @@ TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
         4 * b * ns,             // ent_bits
         b * ns,                 // ent_row
         4 * b * (mc + 1),       // chunk_begin
-        4 * b * mc * kChunkRows, // chunk_rows
+        8 * b * mc * kChunkRows, // chunk_rows
         4 * b,                  // nchunks
     };
     const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };

@@ -9,11 +9,12 @@
 //      Cartesian rows g[k] (T^T C)[k] (pure transform and component factor
 //      folded into the coefficients), for a partially kept pure block its
 //      plain pure rows C[r]; rows of dropped AOs are zero (the screening mask);
-//   2. lanes (one point each, T lanes per AO block) generate the matching
-//      Phi rows in FP64 from the block's three radial sums: unscaled
+//   2. each thread evaluates 4 (FP64: 2) consecutive Phi rows of the chunk
+//      at one point in FP64 from the rows' AO block radial sums: unscaled
 //      Cartesian components P_k R for Cartesian rows, or the pure AOs
-//      sum_k T g P_k R for pure rows, and write the needed derivative panels
-//      to LDS in the contraction precision (FP64, FP32 or bf16 splits);
+//      sum_k T g P_k R for pure rows, converts the needed derivative panels to
+//      the contraction precision (FP64, FP32 or bf16 splits) and writes them
+//      with one vector store per plane and panel;
 //   3. MFMA tiles accumulate Psi[panel] = C_rows^T Phi_rows[panel]
 //      (16 MOs x 16 points per tile, all panels of a tile in one wave; a wave
 //      keeps one 16-point column of tiles, so each Phi operand it loads feeds
@@ -80,95 +81,27 @@ template <> struct Prec<KUMO_HIP_BF16X6> {
 
 template <int PREC> constexpr bool is_bf16 = PREC == KUMO_HIP_BF16X3 || PREC == KUMO_HIP_BF16X6;
 
-__device__ __attribute__((always_inline)) inline uint16_t to_bf16(float f) {
-    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
-}
-__device__ __attribute__((always_inline)) inline float from_bf16(uint16_t h) {
-    return __uint_as_float(static_cast<uint32_t>(h) << 16);
-}
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-// Writes FP64 value v at panel i, point j, row k of Phi in the contraction
-// precision. bf16: hi = bf16(v); the first residual v - hi is formed exactly
-// in FP64 and rounded to FP32, f1, which is split exactly into mid (+ lo)
-// (the second residual is exact in FP32), so hi + mid + lo represents v to
-// about 2^-27 relative (a plain FP32 rounding of v would cost 2^-24).
-template <int PREC, int NP>
-__device__ __attribute__((always_inline)) inline void phi_store(typename Prec<PREC>::E* phi, int i, int j, int k,
-                                                               double v) {
-    using P = Prec<PREC>;
-    constexpr int plane = NP * P::T * P::LDK;
-    typename Prec<PREC>::E* d = phi + (i * P::T + j) * P::LDK + k;
-    if constexpr (PREC == KUMO_HIP_FP64) {
-        d[0] = v;
-    } else if constexpr (PREC == KUMO_HIP_FP32) {
-        d[0] = static_cast<float>(v);
-    } else {
-        float f = static_cast<float>(v);
-        // Keeps each FP64->FP32 rounding apart from the bf16 one that follows
-        // (the compiler would otherwise fuse them into a slower direct
-        // FP64->bf16 rounding; rounding twice is exact here since FP32 has
-        // more than twice bf16's precision).
-        asm("" : "+v"(f));
-        const uint16_t hi = to_bf16(f);
-        float f1 = static_cast<float>(v - static_cast<double>(from_bf16(hi)));
-        asm("" : "+v"(f1));
-        const uint16_t mid = to_bf16(f1);
-        d[0] = hi;
-        d[plane] = mid;
-        if constexpr (P::S == 3) d[2 * plane] = to_bf16(f1 - from_bf16(mid));
-    }
-}
-
-// Phi rows of one chunk entry at point j: the unscaled Cartesian components
-// of an AO block of angular momentum L (unrolled).
-template <int PREC, int ORDER, unsigned PSET, int L>
-__device__ __attribute__((always_inline)) inline void gen_cart(typename Prec<PREC>::E* phi, int row0, int j, double x,
-                                                              double y, double z, const double (&R)[3]) {
-    constexpr int NP = popcount10(PSET);
-    cart_components<L, ORDER, PSET>(x, y, z, R, [&](int k, const double (&c)[NP]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int i = 0; i < NP; ++i)
-            phi_store<PREC, NP>(phi, i, j, row0 + k, c[i]);
-    });
-}
-
-// The same with runtime l: Cartesian rows (l >= 4) or, for a partially kept
-// pure block, its pure rows sum_k T[r][k] g[k] P_k R.
-template <int PREC, int ORDER, unsigned PSET>
-__device__ __attribute__((always_inline)) inline void gen_generic(typename Prec<PREC>::E* phi, const MolView& m,
-                                                                 const DevShell& sh, bool pure_rows, int row0, int j,
-                                                                 double x, double y, double z, const double (&R)[3]) {
-    constexpr int NP = popcount10(PSET);
-    const int L = sh.l, nc = sh.ncart;
-    if (pure_rows) {
-        const double* tg = m.tg + L * kMaxPure * kMaxCart;
-        for (int r = 0; r < sh.count; ++r) {
-            double v[NP];
-#pragma unroll
-            for (int i = 0; i < NP; ++i)
-                v[i] = 0.0;
-            for (int k = 0; k < nc; ++k) {
-                const double w = tg[r * kMaxCart + k];
-                if (w == 0.0) continue;
-                double cv[NP];
-                cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
-#pragma unroll
-                for (int i = 0; i < NP; ++i)
-                    v[i] += w * cv[i];
-            }
-#pragma unroll
-            for (int i = 0; i < NP; ++i)
-                phi_store<PREC, NP>(phi, i, j, row0 + r, v[i]);
-        }
-        return;
-    }
-    for (int k = 0; k < nc; ++k) {
-        double cv[NP];
-        cart_component_rt<ORDER, PSET>(L, k, x, y, z, R, cv);
-#pragma unroll
-        for (int i = 0; i < NP; ++i)
-            phi_store<PREC, NP>(phi, i, j, row0 + k, cv[i]);
-    }
+// Splits two FP64 values into bf16 hi, mid, lo packed pairs (element 0 in
+// the low half): hi = bf16(v); the residual v - hi, exact in FP64, is
+// rounded once to FP32 and split exactly into mid + lo, so hi + mid + lo
+// represents v to about 2^-27 relative. The empty asm keeps each FP64->FP32
+// rounding apart from the bf16 rounding after it (the compiler would fuse
+// them into a slower direct FP64->bf16 rounding; rounding twice is exact
+// here since FP32 has more than twice bf16's precision).
+__device__ __attribute__((always_inline)) inline void split_pair(double v0, double v1, uint32_t& hi, uint32_t& mid,
+                                                                uint32_t& lo) {
+    f32x2 f = {static_cast<float>(v0), static_cast<float>(v1)};
+    asm("" : "+v"(f));
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
+    f32x2 f1 = {static_cast<float>(v0 - static_cast<double>(__uint_as_float(hi << 16))),
+                static_cast<float>(v1 - static_cast<double>(__uint_as_float(hi & 0xffff0000u)))};
+    asm("" : "+v"(f1));
+    mid = __builtin_bit_cast(uint32_t, __builtin_convertvector(f1, bf16x2));
+    const f32x2 f2 = {f1.x - __uint_as_float(mid << 16), f1.y - __uint_as_float(mid & 0xffff0000u)};
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2, bf16x2));
 }
 
 template <int PREC, int ORDER, unsigned PSET>
@@ -220,17 +153,9 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     const int32_t* ent = a.tables.ent + static_cast<long long>(blk_local) * nsh;
     const uint8_t* erow = a.tables.ent_row + static_cast<long long>(blk_local) * nsh;
     const int32_t* cb = a.tables.chunk_begin + static_cast<long long>(blk_local) * (a.tables.max_chunks + 1);
-    const int32_t* crows = a.tables.chunk_rows + static_cast<long long>(blk_local) * a.tables.max_chunks * KC;
+    const int32_t* crows = a.tables.chunk_rows + static_cast<long long>(blk_local) * a.tables.max_chunks * KC * 2;
     const int nchunks = a.tables.nchunks[blk_local];
 
-    // Phi rows past a chunk's last row meet zero coefficient rows; they only
-    // need to be finite, so Phi starts zeroed.
-    {
-        uint4* z4 = reinterpret_cast<uint4*>(phi);
-        constexpr int n16 = static_cast<int>(sizeof(E)) * S * PHI_PLANE / 16;
-        for (int i = tid; i < n16; i += 256)
-            z4[i] = uint4{0, 0, 0, 0};
-    }
     // This thread's point for AO generation (T divides 256).
     const int j = tid % T;
     double px = 0.0, py = 0.0, pz = 0.0;
@@ -261,7 +186,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
             rows_used = erow[e1 - 1] + ((last >> 24) == kEntryPure ? ls.count : ls.ncart);
         }
         // 1. Coefficient rows of the chunk (zero past its rows).
-        const int32_t* cr = crows + c * KC;
+        const int32_t* cr = crows + 2 * c * KC;
 #pragma unroll
         for (int u = 0; u < S * KC * GMAX / 256; ++u) {
             const int item = tid + 256 * u;
@@ -269,7 +194,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
             const int r = (item / GMAX) % KC;
             const int sp = item / (KC * GMAX);
             if (gi >= G) continue;
-            const int grow = cr[r];
+            const int grow = cr[2 * r];
             uint4 v = uint4{0, 0, 0, 0};
             if (grow >= 0)
                 v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
@@ -282,33 +207,93 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
                 *reinterpret_cast<uint4*>(d) = v;
             }
         }
-        // 2. Phi rows of the chunk's entries at the tile's points.
-        for (int q = e0 + tid / T; q < e1; q += 256 / T) {
-            const int ev = ent[q];
-            const DevShell& sh = m.shells[ev & 0xffffff];
-            const int mode = ev >> 24;
-            const int row0 = erow[q];
-            const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
-            const double r2 = sq_norm(x, y, z);
-            if (j < nt_valid && r2 <= sh.cutoff_r2) {
-                double R[3];
-                radial_sums<ORDER>(sh, m, r2, R);
-                if (mode == kEntryPure || sh.l > 3) {
-                    gen_generic<PREC, ORDER, PSET>(phi, m, sh, mode == kEntryPure, row0, j, x, y, z, R);
-                } else {
-                    switch (sh.l) {
-                    case 0: gen_cart<PREC, ORDER, PSET, 0>(phi, row0, j, x, y, z, R); break;
-                    case 1: gen_cart<PREC, ORDER, PSET, 1>(phi, row0, j, x, y, z, R); break;
-                    case 2: gen_cart<PREC, ORDER, PSET, 2>(phi, row0, j, x, y, z, R); break;
-                    default: gen_cart<PREC, ORDER, PSET, 3>(phi, row0, j, x, y, z, R); break;
-                    }
-                }
-            } else {
-                const int nrows = mode == kEntryPure ? sh.count : sh.ncart;
-                for (int k = 0; k < nrows; ++k)
+        // 2. Phi: thread (point j, row group) evaluates RPL consecutive rows of
+        //    the chunk at its point (rows past the chunk's end are zero).
+        {
+            constexpr int RPL = KC * T / 256;
+            const int r0 = (tid / T) * RPL;
+            int cached = -1;
+            bool inside = false;
+            double R[3] = {0.0, 0.0, 0.0}, x = 0.0, y = 0.0, z = 0.0;
+            const auto eval_row = [&](int i, double (&v)[NP]) __attribute__((always_inline)) {
 #pragma unroll
-                    for (int i = 0; i < NP; ++i)
-                        phi_store<PREC, NP>(phi, i, j, row0 + k, 0.0);
+                for (int p = 0; p < NP; ++p)
+                    v[p] = 0.0;
+                const int info = cr[2 * (r0 + i) + 1];
+                if (info < 0 || j >= nt_valid) return;
+                const int sidx = info >> 15;
+                const ShellRec& rec = m.rec[sidx];
+                if (sidx != cached) {
+                    cached = sidx;
+                    x = px - rec.o[0];
+                    y = py - rec.o[1];
+                    z = pz - rec.o[2];
+                    const double r2 = sq_norm(x, y, z);
+                    inside = r2 <= rec.cutoff_r2;
+                    if (inside) radial_sums_rec<ORDER>(rec, m, r2, R);
+                }
+                if (!inside) return;
+                if (((info >> 13) & 3) == kEntryPure) {
+                    // pure AO (row index in bits 9-12) from the unscaled components
+                    const int L = rec.l, kp = (info >> 9) & 15;
+                    const double* tg = m.tg + (L * kMaxPure + kp) * kMaxCart;
+                    for (int c = 0; c < (L + 1) * (L + 2) / 2; ++c) {
+                        const double w = tg[c];
+                        if (w == 0.0) continue;
+                        double cv[NP];
+                        cart_component_rt<ORDER, PSET>(L, c, x, y, z, R, cv);
+#pragma unroll
+                        for (int p = 0; p < NP; ++p)
+                            v[p] += w * cv[p];
+                    }
+                    return;
+                }
+                const int qq[3] = {(info >> 6) & 7, (info >> 3) & 7, info & 7};
+                cart_component_pw<ORDER, PSET>(qq, x, y, z, R, v);
+            };
+            constexpr int PLANE = NP * T * LDK;
+            E* d0 = phi + j * LDK + r0;
+            if constexpr (is_bf16<PREC>) {
+                static_assert(RPL == 4, "four bf16 rows per lane");
+                uint32_t hh[2][NP], mm[2][NP], ll[2][NP];
+#pragma unroll
+                for (int pi = 0; pi < 2; ++pi) {
+                    double va[NP], vb[NP];
+                    eval_row(2 * pi, va);
+                    eval_row(2 * pi + 1, vb);
+#pragma unroll
+                    for (int p = 0; p < NP; ++p)
+                        split_pair(va[p], vb[p], hh[pi][p], mm[pi][p], ll[pi][p]);
+                }
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    E* d = d0 + p * T * LDK;
+                    *reinterpret_cast<uint2*>(d) = uint2{hh[0][p], hh[1][p]};
+                    *reinterpret_cast<uint2*>(d + PLANE) = uint2{mm[0][p], mm[1][p]};
+                    if constexpr (S == 3) *reinterpret_cast<uint2*>(d + 2 * PLANE) = uint2{ll[0][p], ll[1][p]};
+                }
+            } else if constexpr (PREC == KUMO_HIP_FP32) {
+                static_assert(RPL == 4, "four FP32 rows per lane");
+                float st[RPL][NP];
+#pragma unroll
+                for (int i = 0; i < RPL; ++i) {
+                    double v[NP];
+                    eval_row(i, v);
+#pragma unroll
+                    for (int p = 0; p < NP; ++p)
+                        st[i][p] = static_cast<float>(v[p]);
+                }
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    *reinterpret_cast<float4*>(d0 + p * T * LDK) = float4{st[0][p], st[1][p], st[2][p], st[3][p]};
+            } else {
+                static_assert(RPL == 2, "two FP64 rows per lane");
+                double va[NP], vb[NP];
+                eval_row(0, va);
+                eval_row(1, vb);
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    *reinterpret_cast<double2*>(d0 + p * T * LDK) = double2{va[p], vb[p]};
             }
         }
         __syncthreads();

@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(kMaskThreads)
     int32_t* en = t.ent + static_cast<long long>(blockIdx.x) * nsh;
     uint32_t* eb = t.ent_bits + static_cast<long long>(blockIdx.x) * nsh;
     uint8_t* er = t.ent_row + static_cast<long long>(blockIdx.x) * nsh;
-    int32_t* crows = t.chunk_rows + static_cast<long long>(blockIdx.x) * t.max_chunks * kChunkRows;
+    int32_t* crows = t.chunk_rows + static_cast<long long>(blockIdx.x) * t.max_chunks * kChunkRows * 2;
     for (int q = tid; q < nent; q += kMaskThreads) {
         const int e = sent[q];
         const int s = e & 0xffffff, mode = e >> 24;
@@ -434,12 +434,21 @@ __global__ void __launch_bounds__(kMaskThreads)
         const int n = mode == kEntryPure ? sh.count : sh.ncart;
         // Rows of dropped AOs (partially kept blocks) get zero coefficients.
         if (c < t.max_chunks)
-            for (int i = 0; i < n; ++i)
-                crows[c * kChunkRows + row0 + i] = (mode == kEntryCart || ((bits >> i) & 1u)) ? first + i : -1;
+            for (int i = 0; i < n; ++i) {
+                int32_t* cr = crows + 2 * (c * kChunkRows + row0 + i);
+                cr[0] = (mode == kEntryCart || ((bits >> i) & 1u)) ? first + i : -1;
+                // Phi row: block << 15 | mode << 13 | row << 9 | Cartesian powers (3 bits each)
+                const int cart = mode == kEntryPure ? 0 : i;
+                cr[1] = (s << 15) | (mode << 13) | (i << 9) | (kPow[sh.l][cart][0] << 6) | (kPow[sh.l][cart][1] << 3) |
+                        kPow[sh.l][cart][2];
+            }
     }
     for (int item = tid; item < nc * kChunkRows; item += kMaskThreads) {
         const int c = item / kChunkRows, r = item % kChunkRows;
-        if (r >= sused[c]) crows[item] = -1;
+        if (r >= sused[c]) {
+            crows[2 * item] = -1;
+            crows[2 * item + 1] = -1;
+        }
     }
 }
 

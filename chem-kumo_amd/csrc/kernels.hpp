@@ -38,7 +38,11 @@ struct KeptTables {
     uint32_t* ent_bits;     // [nblk][nshells] kept bits of the entry's block
     uint8_t* ent_row;       // [nblk][nshells] first row of the entry in its chunk
     int32_t* chunk_begin;   // [nblk][max_chunks + 1] first entry of each chunk
-    int32_t* chunk_rows;    // [nblk][max_chunks][kChunkRows] operand table row, -1 past the chunk's rows
+    // [nblk][max_chunks][kChunkRows][2]: operand table row (-1: zero
+    // coefficients) and the Phi row it multiplies, block << 15 | mode << 13 |
+    // row within the block << 9 | x, y, z powers of a Cartesian row (3 bits
+    // each), or -1 past the chunk's rows.
+    int32_t* chunk_rows;
     int32_t* nchunks;       // [nblk]
     int max_chunks;
 };

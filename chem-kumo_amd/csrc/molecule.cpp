@@ -120,7 +120,25 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
             gk[static_cast<std::size_t>(l * kMaxCart) + k] = component_factor(pw[0], pw[k]);
     }
     m.d_gk = upload(gk.data(), gk.size(), s);
+    if (b.shells.size() >= 65536) runtime("more than 65535 AO blocks are not supported by the HIP backend");
     m.d_lorder = upload(b.lorder.data(), b.lorder.size(), s);
+    std::vector<ShellRec> rec(b.shells.size());
+    for (std::size_t i = 0; i < b.shells.size(); ++i) {
+        const DevShell& d = b.shells[i];
+        ShellRec& r = rec[i];
+        std::memset(&r, 0, sizeof(r));
+        for (int a = 0; a < 3; ++a)
+            r.o[a] = d.o[a];
+        r.cutoff_r2 = d.cutoff_r2;
+        r.l = d.l;
+        r.nprim = d.nprim;
+        r.prim = d.prim;
+        for (int p = 0; p < d.nprim && p < kRecPrim; ++p) {
+            r.e[p] = b.exps[static_cast<std::size_t>(d.prim + p)];
+            r.w[p] = b.radw[static_cast<std::size_t>(d.prim + p)];
+        }
+    }
+    m.d_rec = upload(rec.data(), rec.size(), s);
     m.d_radw = upload(b.radw.data(), b.radw.size(), s);
     // Occupied-MO operand: alpha slots [0, A16), beta slots [A16, A16 + B16)
     // (closed shells contract the alpha channel only).
@@ -188,7 +206,7 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
     }
     KUMO_HIP(hipStreamSynchronize(s));
     m.device_bytes = 0;
-    for (const DeviceBuffer* d : {&m.d_shells, &m.d_exps, &m.d_weights, &m.d_tdense, &m.d_radw, &m.d_tg, &m.d_p[0],
+    for (const DeviceBuffer* d : {&m.d_shells, &m.d_exps, &m.d_weights, &m.d_tdense, &m.d_radw, &m.d_tg, &m.d_rec, &m.d_p[0],
                                   &m.d_p[1], &m.d_ca, &m.d_cb, &m.d_op64, &m.d_op32, &m.d_opbf})
         m.device_bytes += static_cast<long long>(d->bytes());
 }

@@ -73,7 +73,7 @@ struct kumo_hip_molecule {
     bool unrestricted = false, eligible = false;
     double total_rel = 0.0, spin_rel = 0.0;
     // Device residency
-    kumo::DeviceBuffer d_shells, d_exps, d_weights, d_tdense, d_radw, d_tg, d_gk, d_lorder;
+    kumo::DeviceBuffer d_shells, d_exps, d_weights, d_tdense, d_radw, d_tg, d_gk, d_lorder, d_rec;
     // Per screening threshold: squared radius per block beyond which no AO of
     // the block passes it (kept until the molecule is released).
     std::map<double, kumo::DeviceBuffer> d_screen_r2;
@@ -91,7 +91,7 @@ struct kumo_hip_molecule {
     kumo::MolView view() const {
         return {d_shells.as<kumo::DevShell>(), static_cast<int>(basis.shells.size()), n, d_exps.as<double>(),
                 d_weights.as<double>(), d_tdense.as<double>(), d_radw.as<double>(), d_tg.as<double>(),
-                d_gk.as<double>(), d_lorder.as<int32_t>()};
+                d_gk.as<double>(), d_lorder.as<int32_t>(), d_rec.as<kumo::ShellRec>()};
     }
     const double* p_matrix(int component, hipStream_t stream);
     const double* screen_r2(double threshold, hipStream_t stream);
