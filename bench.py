@@ -372,8 +372,13 @@ def main():
 
     kernel_s = main_run["kernel_ms"] * 1e-3
     achieved = main_run["flops"] / kernel_s if kernel_s > 0 else 0.0
+    traffic = None  # HBM bytes per launch, from the FETCH_SIZE / WRITE_SIZE counter runs of this workload
+    tpath = os.path.join(ROOT, "profiles", "traffic_aomo_bf16x6_tzvp640.json")
+    if args.precision == "BF16X6" and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh)["traffic_bytes_per_launch"]
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision], "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision], "traffic": traffic,
                 "kernel": "aomo_kernel", "kernel_ms": main_run["kernel_ms"],
                 "launches_per_step": main_run["launches"],
                 "algorithmic_flops_per_step": main_run["flops"],

@@ -255,22 +255,20 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
             E* d0 = phi + j * LDK + r0;
             if constexpr (is_bf16<PREC>) {
                 static_assert(RPL == 4, "four bf16 rows per lane");
-                uint32_t hh[2][NP], mm[2][NP], ll[2][NP];
 #pragma unroll
                 for (int pi = 0; pi < 2; ++pi) {
                     double va[NP], vb[NP];
                     eval_row(2 * pi, va);
                     eval_row(2 * pi + 1, vb);
 #pragma unroll
-                    for (int p = 0; p < NP; ++p)
-                        split_pair(va[p], vb[p], hh[pi][p], mm[pi][p], ll[pi][p]);
-                }
-#pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    E* d = d0 + p * T * LDK;
-                    *reinterpret_cast<uint2*>(d) = uint2{hh[0][p], hh[1][p]};
-                    *reinterpret_cast<uint2*>(d + PLANE) = uint2{mm[0][p], mm[1][p]};
-                    if constexpr (S == 3) *reinterpret_cast<uint2*>(d + 2 * PLANE) = uint2{ll[0][p], ll[1][p]};
+                    for (int p = 0; p < NP; ++p) {
+                        uint32_t hh, mm, ll;
+                        split_pair(va[p], vb[p], hh, mm, ll);
+                        E* d = d0 + p * T * LDK + 2 * pi;
+                        *reinterpret_cast<uint32_t*>(d) = hh;
+                        *reinterpret_cast<uint32_t*>(d + PLANE) = mm;
+                        if constexpr (S == 3) *reinterpret_cast<uint32_t*>(d + 2 * PLANE) = ll;
+                    }
                 }
             } else if constexpr (PREC == KUMO_HIP_FP32) {
                 static_assert(RPL == 4, "four FP32 rows per lane");
