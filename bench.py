@@ -411,6 +411,14 @@ def main():
         if ref_out is not None:
             g64 = ref_out[:, torch.as_tensor(sample_idx, device="cuda")].cpu().numpy()
             errors["fp64_gpu_rho_max_rel"] = float(np.max(np.abs(g64[0] - rho)[sig] / rho[sig]))
+            d64 = np.abs(g64[6] - o[4][0])
+            errors["fp64_gpu_elf_max_abs"] = float(d64.max())
+            # ELF where rho is so small that tau - |grad rho|^2 / 8 rho is
+            # rounding noise in FP64 itself: largest rho at which either GPU
+            # path differs from the oracle by more than 1e-5
+            off = (np.abs(got[6] - o[4][0]) > 1e-5) | (d64 > 1e-5)
+            errors["elf_points_off_by_1e-5"] = int(off.sum())
+            errors["elf_off_max_rho"] = float(rho[off].max()) if off.any() else None
 
     result = {
         "metric": "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error; p50/p99 latency",
