@@ -22,14 +22,30 @@
 // The epilogue forms the per-point dot products over MOs in FP64 (value,
 // gradient, tau terms, Hessian) and reduces them over the tiles of each
 // spin channel in a fixed order, so the result is independent of batching.
+// Reduced precisions also sum, per point and channel, |Phi_k| times the norm
+// of coefficient row k over the channel's MOs: the scale of the
+// contraction's rounding error that the density rescue tests.
 //
 // MFMA operand layouts (wave64): for 16x16xK the lane l holds A row l % 16
 // and B column l % 16 at a K slice selected by l / 16; the accumulator lane l
 // holds D rows 4 (l / 16) + r, column l % 16. The K slice order is the same
 // for A and B, so the sum over K is exact whatever order the hardware uses.
 // LDS strides are chosen so that the 16 lanes of each K slice hit distinct
-// banks: coefficient rows have a pitch of 132 (bf16, FP32) or 144 (FP64)
-// elements, the Phi row of a point 40 (bf16), 36 (FP32) or 34 (FP64).
+// banks. FP32 and FP64: coefficient rows have a pitch of 132 and 144
+// elements, the Phi row of a point 36 and 34. bf16: coefficient rows are
+// staged in pairs, one 32-bit word (row 2q low, row 2q + 1 high) per slot,
+// 132 words per pair, so a lane reads its 8 K-consecutive values as four
+// words; the 32 Phi values of a point are four 8-row granules whose order
+// is XOR-swizzled by (point / 4) % 4, so 16 consecutive points read
+// distinct banks without padding.
+//
+// Split-bf16 precisions: both operands are sums of bf16 terms (Phi split in
+// the kernel, the coefficients when the molecule is loaded) and the MFMA
+// products of the terms whose combined order is within the precision are
+// accumulated in FP32: BF16X3 two terms, 3 products; BF16X6 three terms,
+// 6 products. More terms gain nothing: the bf16 MFMA aligns the products of
+// each 8-element K group to the group's largest and keeps about 24 bits, so
+// the contraction error is about 2^-24 sum_k |C_k Phi_k| whatever the split.
 #include "kernels.hpp"
 
 #include "../../include/kumo_hip.h"
@@ -62,46 +78,48 @@ template <> struct Prec<KUMO_HIP_FP32> {
     static constexpr int LDK = 36;
     static constexpr int PITCH = 132;
 };
-template <> struct Prec<KUMO_HIP_BF16X3> {
+template <int NSPLIT> struct PrecBf16 {
     using E = uint16_t;
     using Acc = f32x4;
-    static constexpr int S = 2;
+    static constexpr int S = NSPLIT;
     static constexpr int T = 32;
-    static constexpr int LDK = 40;
-    static constexpr int PITCH = 132;
+    static constexpr int LDK = 32;    // swizzled 8-row granules
+    static constexpr int PITCH = 132; // 32-bit words per coefficient row pair
 };
-template <> struct Prec<KUMO_HIP_BF16X6> {
-    using E = uint16_t;
-    using Acc = f32x4;
-    static constexpr int S = 3;
-    static constexpr int T = 32;
-    static constexpr int LDK = 40;
-    static constexpr int PITCH = 132;
-};
+template <> struct Prec<KUMO_HIP_BF16X3> : PrecBf16<2> {};
+template <> struct Prec<KUMO_HIP_BF16X6> : PrecBf16<3> {};
 
 template <int PREC> constexpr bool is_bf16 = PREC == KUMO_HIP_BF16X3 || PREC == KUMO_HIP_BF16X6;
+
+// Element offset of row r (< 32) of point j in a bf16 Phi plane.
+__device__ __attribute__((always_inline)) inline int phi_bf16_offset(int j, int r) {
+    return j * 32 + ((((r >> 3) ^ (j >> 2)) & 3) << 3) + (r & 7);
+}
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
-// Splits two FP64 values into bf16 hi, mid, lo packed pairs (element 0 in
-// the low half): hi = bf16(v); the residual v - hi, exact in FP64, is
-// rounded once to FP32 and split exactly into mid + lo, so hi + mid + lo
-// represents v to about 2^-27 relative. The empty asm keeps each FP64->FP32
-// rounding apart from the bf16 rounding after it (the compiler would fuse
-// them into a slower direct FP64->bf16 rounding; rounding twice is exact
-// here since FP32 has more than twice bf16's precision).
-__device__ __attribute__((always_inline)) inline void split_pair(double v0, double v1, uint32_t& hi, uint32_t& mid,
-                                                                uint32_t& lo) {
+// Splits two FP64 values into S bf16 terms, packed in pairs (element 0 in the
+// low half): t0 = bf16(v); the residual v - t0, exact in FP64, is rounded
+// once to FP32 (2^-33 relative to v) and split into the remaining terms
+// (each FP32 residual exact), so three terms represent v to about 2^-26. The empty asm keeps
+// each FP64->FP32 rounding apart from the bf16 rounding after it (the
+// compiler would fuse them into a slower direct FP64->bf16 rounding;
+// rounding twice is exact here since FP32 has more than twice bf16's
+// precision).
+template <int S>
+__device__ __attribute__((always_inline)) inline void split_pair(double v0, double v1, uint32_t (&t)[S]) {
     f32x2 f = {static_cast<float>(v0), static_cast<float>(v1)};
     asm("" : "+v"(f));
-    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
-    f32x2 f1 = {static_cast<float>(v0 - static_cast<double>(__uint_as_float(hi << 16))),
-                static_cast<float>(v1 - static_cast<double>(__uint_as_float(hi & 0xffff0000u)))};
-    asm("" : "+v"(f1));
-    mid = __builtin_bit_cast(uint32_t, __builtin_convertvector(f1, bf16x2));
-    const f32x2 f2 = {f1.x - __uint_as_float(mid << 16), f1.y - __uint_as_float(mid & 0xffff0000u)};
-    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(f2, bf16x2));
+    t[0] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
+    f32x2 r = {static_cast<float>(v0 - static_cast<double>(__uint_as_float(t[0] << 16))),
+               static_cast<float>(v1 - static_cast<double>(__uint_as_float(t[0] & 0xffff0000u)))};
+    asm("" : "+v"(r));
+#pragma unroll
+    for (int k = 1; k < S; ++k) {
+        t[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+        if (k + 1 < S) r = f32x2{r.x - __uint_as_float(t[k] << 16), r.y - __uint_as_float(t[k] & 0xffff0000u)};
+    }
 }
 
 template <int PREC, int ORDER, unsigned PSET>
@@ -115,10 +133,14 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     constexpr int CS_PLANE = KC * PITCH;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     // LDS: phi[S][NP][T][LDK], cs[S][KC][PITCH]; after the chunk loop the
-    // same memory holds red[MT][kMoments][T] (double).
+    // same memory holds red[MT][kMoments][T] (double), then esl[256][2]
+    // (float, the threads' error-scale sums).
     E* phi = reinterpret_cast<E*>(smem);
     E* cs = phi + S * PHI_PLANE;
     double* red = reinterpret_cast<double*>(smem);
+    float* esl = reinterpret_cast<float*>(smem + sizeof(double) * 8 * kMoments * T);
+    const bool scale = PREC != KUMO_HIP_FP64 && a.rho_scale != nullptr; // uniform
+    float es0 = 0.f, es1 = 0.f;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Tile: T points of one screening block, consecutive or (rescue mode)
@@ -187,24 +209,41 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
         }
         // 1. Coefficient rows of the chunk (zero past its rows).
         const int32_t* cr = crows + 2 * c * KC;
+        if constexpr (is_bf16<PREC>) {
+            // rows 2q and 2q + 1, 8 slots each, interleaved into 32-bit words
 #pragma unroll
-        for (int u = 0; u < S * KC * GMAX / 256; ++u) {
-            const int item = tid + 256 * u;
-            const int gi = item % GMAX;
-            const int r = (item / GMAX) % KC;
-            const int sp = item / (KC * GMAX);
-            if (gi >= G) continue;
-            const int grow = cr[2 * r];
-            uint4 v = uint4{0, 0, 0, 0};
-            if (grow >= 0)
-                v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
-                                                    slot_lo + gi * EPG);
-            E* d = cs + sp * CS_PLANE + r * PITCH + gi * EPG;
-            if constexpr (is_bf16<PREC>) {
-                reinterpret_cast<uint2*>(d)[0] = uint2{v.x, v.y};
-                reinterpret_cast<uint2*>(d)[1] = uint2{v.z, v.w};
-            } else {
-                *reinterpret_cast<uint4*>(d) = v;
+            for (int u = 0; u < S * (KC / 2) * GMAX / 256; ++u) {
+                const int item = tid + 256 * u;
+                const int gi = item % GMAX;
+                const int q = (item / GMAX) % (KC / 2);
+                const int sp = item / ((KC / 2) * GMAX);
+                if (gi >= G) continue;
+                const int g0 = cr[4 * q], g1 = cr[4 * q + 2];
+                const E* src = op + sp * op_plane + slot_lo + gi * EPG;
+                uint4 v0 = uint4{0, 0, 0, 0}, v1 = uint4{0, 0, 0, 0};
+                if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
+                if (g1 >= 0) v1 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
+                const auto lo = [](uint32_t x, uint32_t y) { return (x & 0xffffu) | (y << 16); };
+                const auto hi = [](uint32_t x, uint32_t y) { return (x >> 16) | (y & 0xffff0000u); };
+                uint4* d = reinterpret_cast<uint4*>(reinterpret_cast<uint32_t*>(cs + sp * CS_PLANE) + q * PITCH +
+                                                    gi * EPG);
+                d[0] = uint4{lo(v0.x, v1.x), hi(v0.x, v1.x), lo(v0.y, v1.y), hi(v0.y, v1.y)};
+                d[1] = uint4{lo(v0.z, v1.z), hi(v0.z, v1.z), lo(v0.w, v1.w), hi(v0.w, v1.w)};
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < S * KC * GMAX / 256; ++u) {
+                const int item = tid + 256 * u;
+                const int gi = item % GMAX;
+                const int r = (item / GMAX) % KC;
+                const int sp = item / (KC * GMAX);
+                if (gi >= G) continue;
+                const int grow = cr[2 * r];
+                uint4 v = uint4{0, 0, 0, 0};
+                if (grow >= 0)
+                    v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
+                                                        slot_lo + gi * EPG);
+                *reinterpret_cast<uint4*>(cs + sp * CS_PLANE + r * PITCH + gi * EPG) = v;
             }
         }
         // 2. Phi: thread (point j, row group) evaluates RPL consecutive rows of
@@ -251,23 +290,35 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
                 const int qq[3] = {(info >> 6) & 7, (info >> 3) & 7, info & 7};
                 cart_component_pw<ORDER, PSET>(qq, x, y, z, R, v);
             };
+            const auto add_scale = [&](int i, double v0) __attribute__((always_inline)) {
+                if (!scale) return;
+                const int g = cr[2 * (r0 + i)];
+                if (g < 0) return;
+                const float2 w = a.mo.norm[g];
+                const float av = fabsf(static_cast<float>(v0));
+                es0 += av * w.x;
+                es1 += av * w.y;
+            };
             constexpr int PLANE = NP * T * LDK;
             E* d0 = phi + j * LDK + r0;
             if constexpr (is_bf16<PREC>) {
-                static_assert(RPL == 4, "four bf16 rows per lane");
+                static_assert(RPL == 4 && KC == 32 && LDK == 32, "four bf16 rows per lane, swizzled 32-row points");
+                E* db = phi + phi_bf16_offset(j, r0);
 #pragma unroll
                 for (int pi = 0; pi < 2; ++pi) {
                     double va[NP], vb[NP];
                     eval_row(2 * pi, va);
                     eval_row(2 * pi + 1, vb);
+                    add_scale(2 * pi, va[0]);
+                    add_scale(2 * pi + 1, vb[0]);
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
-                        uint32_t hh, mm, ll;
-                        split_pair(va[p], vb[p], hh, mm, ll);
-                        E* d = d0 + p * T * LDK + 2 * pi;
-                        *reinterpret_cast<uint32_t*>(d) = hh;
-                        *reinterpret_cast<uint32_t*>(d + PLANE) = mm;
-                        if constexpr (S == 3) *reinterpret_cast<uint32_t*>(d + 2 * PLANE) = ll;
+                        uint32_t tt[S];
+                        split_pair<S>(va[p], vb[p], tt);
+                        E* d = db + p * T * LDK + 2 * pi;
+#pragma unroll
+                        for (int k = 0; k < S; ++k)
+                            *reinterpret_cast<uint32_t*>(d + k * PLANE) = tt[k];
                     }
                 }
             } else if constexpr (PREC == KUMO_HIP_FP32) {
@@ -277,6 +328,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
                 for (int i = 0; i < RPL; ++i) {
                     double v[NP];
                     eval_row(i, v);
+                    add_scale(i, v[0]);
 #pragma unroll
                     for (int p = 0; p < NP; ++p)
                         st[i][p] = static_cast<float>(v[p]);
@@ -347,20 +399,19 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
                 const int mt = (wave + 4 * t) / NT;
 #pragma unroll
                 for (int sp = 0; sp < S; ++sp) {
-                    s16x8 w;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e)
-                        w[e] = static_cast<short>(cs[sp * CS_PLANE + (8 * g + e) * PITCH + mt * 16 + r16]);
-                    av[t][sp] = __builtin_bit_cast(bf16x8, w);
+                    const uint32_t* w = reinterpret_cast<const uint32_t*>(cs + sp * CS_PLANE) + 4 * g * PITCH +
+                                        mt * 16 + r16;
+                    av[t][sp] = __builtin_bit_cast(bf16x8, uint4{w[0], w[PITCH], w[2 * PITCH], w[3 * PITCH]});
                 }
             }
+            const int bofs = phi_bf16_offset(jb, 8 * g);
 #pragma unroll
             for (int i = 0; i < NP; ++i) {
                 bf16x8 bv[S];
 #pragma unroll
                 for (int sp = 0; sp < S; ++sp)
                     bv[sp] = __builtin_bit_cast(
-                        bf16x8, *reinterpret_cast<const s16x8*>(phi + sp * PHI_PLANE + (i * T + jb) * LDK + 8 * g));
+                        bf16x8, *reinterpret_cast<const s16x8*>(phi + sp * PHI_PLANE + i * T * LDK + bofs));
 #pragma unroll
                 for (int t = 0; t < TPW; ++t) {
                     if (wave + 4 * t >= ntiles) continue;
@@ -380,6 +431,10 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     }
 
     __syncthreads(); // (a block without kept AOs skips the loop; red aliases Phi)
+    if (scale) {
+        esl[2 * tid] = es0;
+        esl[2 * tid + 1] = es1;
+    }
     // Epilogue: dot products over the MOs of each tile, FP64.
     // Raw sums q: 0 v0.v0, 1-3 v0.va, 4-6 va.va, 7-12 (v0.vac + va.vc).
     const int r16 = lane & 15;
@@ -452,6 +507,12 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
         const long long point = roff ? static_cast<long long>(blockIdx.x) * T + jj : bstart + p0 + jj - a.batch_first;
         double* dst = a.moments + (static_cast<long long>(blockIdx.y) * 2 + ch) * kMoments * a.batch_points;
         dst[k * a.batch_points + point] = f * s;
+        if (k == 0 && scale && blockIdx.y == 0) {
+            float e = 0.f;
+            for (int q = jj; q < 256; q += T)
+                e += esl[2 * q + ch];
+            a.rho_scale[ch * a.batch_points + point] = e;
+        }
     }
 }
 
@@ -459,7 +520,7 @@ template <int PREC, int ORDER, unsigned PSET> void launch_one(const AoMoLaunch& 
     using P = Prec<PREC>;
     constexpr int NP = popcount10(PSET);
     const size_t work = sizeof(typename P::E) * P::S * (static_cast<size_t>(NP) * P::T * P::LDK + KC * P::PITCH);
-    const size_t red = sizeof(double) * 8 * kMoments * P::T;
+    const size_t red = sizeof(double) * 8 * kMoments * P::T + sizeof(float) * 2 * 256;
     const size_t lds = work > red ? work : red;
     const int tiles_per_block = (a.block_size + P::T - 1) / P::T;
     const int groups = (a.slot1 - a.slot0 + 127) / 128;

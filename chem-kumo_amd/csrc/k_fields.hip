@@ -303,18 +303,33 @@ __device__ inline bool elf_uncertain(const DDv* c, int nc, double fermi, double 
     return !(smax * dchi <= tol);
 }
 
-// which: bit 0 total ELF (both channels), bit 1 alpha, bit 2 beta.
-__global__ void rescue_flag_kernel(const double* ch, long long n, int closed, int which, double fermi, double eps,
-                                   double tol, uint8_t* flag) {
+// Relative density error bound of one channel, 2 eps s / sqrt(rho)
+// (0 where the channel holds no density).
+__device__ inline double rho_rel_bound(double rho, double s, double eps) {
+    return rho > 0.0 ? 2.0 * eps * s / sqrt(rho) : 0.0;
+}
+
+// Output bits: 0 total (both channels), 1 alpha, 2 beta.
+__global__ void rescue_flag_kernel(const double* ch, long long n, int closed, RescueCriteria rc, double fermi,
+                                   uint8_t* flag) {
     const long long j = blockIdx.x * 256ll + threadIdx.x;
     if (j >= n) return;
     DDv c[2];
     c[0] = load_channel(ch, n, j);
     c[1] = closed ? c[0] : load_channel(ch + kMoments * n, n, j);
     bool f = false;
-    if (which & 1) f = f || elf_uncertain(c, 2, fermi, eps, tol);
-    if (which & 2) f = f || elf_uncertain(c, 1, fermi, eps, tol);
-    if (which & 4) f = f || elf_uncertain(c + 1, 1, fermi, eps, tol);
+    const int ew = rc.elf_which;
+    if (ew & 1) f = f || elf_uncertain(c, 2, fermi, rc.elf_eps, rc.elf_tol);
+    if (ew & 2) f = f || elf_uncertain(c, 1, fermi, rc.elf_eps, rc.elf_tol);
+    if (ew & 4) f = f || elf_uncertain(c + 1, 1, fermi, rc.elf_eps, rc.elf_tol);
+    if (rc.rho_which) {
+        const double sa = rc.rho_scale[j], sb = closed ? sa : rc.rho_scale[n + j];
+        const double ba = rho_rel_bound(c[0].v, sa, rc.rho_eps), bb = rho_rel_bound(c[1].v, sb, rc.rho_eps);
+        const double rt = c[0].v + c[1].v;
+        const double bt = rt > 0.0 ? (c[0].v * ba + c[1].v * bb) / rt : 0.0;
+        f = f || ((rc.rho_which & 1) && bt > rc.rho_tol) || ((rc.rho_which & 2) && ba > rc.rho_tol) ||
+            ((rc.rho_which & 4) && bb > rc.rho_tol);
+    }
     flag[j] = f ? 1 : 0;
 }
 
@@ -403,12 +418,13 @@ __global__ void rescue_scatter_kernel(const double* rm, long long rslots, int gr
 
 } // namespace
 
-void launch_rescue_plan(const double* channel_moments, long long n, int closed, int which, double eps, double tol,
+void launch_rescue_plan(const double* channel_moments, long long n, int closed, const RescueCriteria& c,
                         int block_size, int nblk, int tile, const RescuePlan& r, hipStream_t stream) {
     if (n <= 0) return;
+    if (c.rho_which && !c.rho_scale) invalid("density rescue without error scales");
     const FieldsLaunch f = fields_launch(nullptr, 0, 0, 0, nullptr);
-    rescue_flag_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(channel_moments, n, closed, which,
-                                                                                  f.fermi, eps, tol, r.flag);
+    rescue_flag_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(channel_moments, n, closed, c,
+                                                                                  f.fermi, r.flag);
     KUMO_HIP(hipGetLastError());
     rescue_compact_kernel<<<static_cast<unsigned>(nblk), 256, 0, stream>>>(r.flag, n, block_size, r.offs, r.count);
     KUMO_HIP(hipGetLastError());

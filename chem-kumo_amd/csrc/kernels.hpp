@@ -13,6 +13,7 @@ namespace kumo {
 // Rows of one contraction chunk: kept AO blocks are packed into chunks of at
 // most kChunkRows operand rows.
 constexpr int kChunkRows = 32;
+constexpr int kBf16Planes = 3; // bf16 terms of the MO operand table
 
 // How a kept AO block enters the AO->MO contraction.
 // Dropped AOs of a partially kept block get the operand row -1 (zero
@@ -67,7 +68,8 @@ void launch_ao_eval(const MolView& m, const PointSource& ps, long long n, int or
 struct MoOperand {
     const double* op64;     // FP64, one plane
     const float* op32;      // FP32, one plane
-    const uint16_t* opbf;   // bf16 hi, mid, lo planes
+    const uint16_t* opbf;   // kBf16Planes bf16 planes, largest term first
+    const float2* norm;     // per row: Euclidean norm over the alpha slots, over the beta slots
     int rows;               // rows per plane
     int slots;
     int alpha_slots;        // slots of the alpha channel (multiple of 16)
@@ -87,10 +89,24 @@ struct RescuePlan {
     int32_t* tile_sub;   // [max tiles]
     int32_t* total;      // tiles in use
 };
-// Flags the points whose ELF (which: bit 0 total, 1 alpha, 2 beta) could be
-// off by more than tol when the channel moments carry a relative error eps,
-// and groups them into tiles of `tile` points of one screening block.
-void launch_rescue_plan(const double* channel_moments, long long n, int closed, int which, double eps, double tol,
+// What a point is rescued for (output bits: 0 total, 1 alpha, 2 beta).
+//  ELF: the channel moments carry a relative error elf_eps; rescue where the
+//   ELF could then be off by more than elf_tol.
+//  Density: the reduced-precision contraction of channel c is off by about
+//   rho_eps * sum_i |Psi_i| sum_k |C_ik Phi_k| <= rho_eps * sqrt(rho_c) s_c
+//   (Cauchy-Schwarz over the MOs), with s_c = sum_k |Phi_k| ||C_k,c|| from
+//   the contraction kernel; rescue where 2 rho_eps s_c / sqrt(rho_c) (or its
+//   rho-weighted mean over both channels for the total) exceeds rho_tol.
+struct RescueCriteria {
+    int elf_which;
+    double elf_eps, elf_tol;
+    int rho_which;
+    const float* rho_scale; // [2][n] s_alpha, s_beta
+    double rho_eps, rho_tol;
+};
+// Flags the points to recompute and groups them into tiles of `tile` points
+// of one screening block.
+void launch_rescue_plan(const double* channel_moments, long long n, int closed, const RescueCriteria& c,
                         int block_size, int nblk, int tile, const RescuePlan& r, hipStream_t stream);
 void launch_rescue_scatter(const double* rescue_moments, long long rslots, int groups, int tile, int block_size,
                            const RescuePlan& r, double* channel_moments, long long n, hipStream_t stream);
@@ -111,6 +127,9 @@ struct AoMoLaunch {
     int precision;           // kumo_hip_precision
     int slot0, slot1;
     double* moments;         // [2][kMoments][batch_points]
+    // Reduced precisions, when not null: [2][batch_points] sum_k |Phi_k| ||C_k,c||
+    // over the kept operand rows for channel c (slot group 0 only).
+    float* rho_scale;
     // Rescue mode (FP64 only): workgroup t evaluates the flagged points of
     // tile t of the plan and writes moments at slot t * 16 + j.
     const int32_t* r_tile_blk;

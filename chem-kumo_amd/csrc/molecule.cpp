@@ -188,18 +188,32 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
                 }
         }
         std::vector<float> o32(op.size());
-        std::vector<uint16_t> obf(3 * op.size());
+        // bf16 planes: each term is the bf16 rounding of what the previous
+        // terms leave (residuals exact in FP64); three terms carry about 24
+        // bits.
+        std::vector<uint16_t> obf(kBf16Planes * op.size());
         for (std::size_t q = 0; q < op.size(); ++q) {
             const double v = op[q];
             o32[q] = static_cast<float>(v);
-            const uint16_t hi = bf16(static_cast<float>(v));
-            const double r1 = v - bf16_value(hi);
-            const uint16_t mid = bf16(static_cast<float>(r1));
-            const double r2 = r1 - bf16_value(mid);
-            obf[q] = hi;
-            obf[op.size() + q] = mid;
-            obf[2 * op.size() + q] = bf16(static_cast<float>(r2));
+            double r = v;
+            for (int t = 0; t < kBf16Planes; ++t) {
+                const uint16_t h = bf16(static_cast<float>(r));
+                obf[static_cast<std::size_t>(t) * op.size() + q] = h;
+                r -= bf16_value(h);
+            }
         }
+        // Row norms over each channel's slots (the density error scale).
+        std::vector<float> nrm(2 * nr);
+        for (std::size_t r = 0; r < nr; ++r) {
+            double sa = 0.0, sb = 0.0;
+            for (std::size_t slot = 0; slot < ns; ++slot) {
+                const double v = op[r * ns + slot];
+                (slot < static_cast<std::size_t>(m.alpha_slots) ? sa : sb) += v * v;
+            }
+            nrm[2 * r] = static_cast<float>(std::sqrt(sa));
+            nrm[2 * r + 1] = static_cast<float>(std::sqrt(sb));
+        }
+        m.d_opnorm = upload(nrm.data(), nrm.size(), s);
         m.d_op64 = upload(op.data(), op.size(), s);
         m.d_op32 = upload(o32.data(), o32.size(), s);
         m.d_opbf = upload(obf.data(), obf.size(), s);
@@ -207,7 +221,7 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
     KUMO_HIP(hipStreamSynchronize(s));
     m.device_bytes = 0;
     for (const DeviceBuffer* d : {&m.d_shells, &m.d_exps, &m.d_weights, &m.d_tdense, &m.d_radw, &m.d_tg, &m.d_rec, &m.d_p[0],
-                                  &m.d_p[1], &m.d_ca, &m.d_cb, &m.d_op64, &m.d_op32, &m.d_opbf})
+                                  &m.d_p[1], &m.d_ca, &m.d_cb, &m.d_op64, &m.d_op32, &m.d_opbf, &m.d_opnorm})
         m.device_bytes += static_cast<long long>(d->bytes());
 }
 

@@ -26,7 +26,7 @@ namespace kumo {
 struct Executor {
     hipStream_t stream = nullptr;
     const double* boys_table = nullptr;
-    DeviceBuffer ws_tables, ws_moments, ws_comp, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom;
+    DeviceBuffer ws_tables, ws_moments, ws_comp, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale;
     // Events bracketing the contraction launches (index 0) and the FP64
     // rescue launches (index 1).
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[2];
@@ -79,7 +79,7 @@ struct kumo_hip_molecule {
     std::map<double, kumo::DeviceBuffer> d_screen_r2;
     kumo::DeviceBuffer d_p[4]; // total, spin, alpha, beta (alpha/beta on first use)
     kumo::DeviceBuffer d_ca, d_cb; // full MO matrices
-    kumo::DeviceBuffer d_op64, d_op32, d_opbf; // MO operand table (FP64, FP32, 3 bf16 planes)
+    kumo::DeviceBuffer d_op64, d_op32, d_opbf, d_opnorm; // MO operand table (FP64, FP32, 3 bf16 planes)
     int slots = 0, alpha_slots = 0, operand_rows = 0;
     int max_chunks = 0; // contraction chunks of any screening block (bound)
     bool esp_ready = false;
