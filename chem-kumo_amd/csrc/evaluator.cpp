@@ -33,6 +33,7 @@ void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
     if (o.backend == KUMO_HIP_AO_MO && !mol->eligible)
         invalid("AO_MO backend requested but orbital representation is not eligible");
     if (o.threads > 1) invalid("explicit multi-thread control is not implemented in the API adapter");
+    if (o.tile_points != 0 && o.tile_points != 16 && o.tile_points != 32) invalid("tile_points must be 0, 16 or 32");
 }
 
 struct TableSet {
@@ -151,6 +152,53 @@ double rho_rescue_eps(int precision) {
     }
 }
 constexpr double kRhoRescueTol = 5e-7; // largest relative density error left to the contraction
+
+// Autotuner of the AO->MO tile. The points per workgroup (16 or 32) change
+// only how points are shared among workgroups (every point's arithmetic and
+// summation order are the same, so results are identical): the first launch
+// of a (precision, derivative order, Hessian) on a molecule with at least
+// kTuneBlocks screening blocks times both tiles on that batch (each launched
+// twice, the second launch timed, so module loading and cold caches do not
+// count) and the faster one is kept for the molecule's lifetime. Smaller
+// batches, and batches being recorded into a graph, use 32 until a choice
+// has been made.
+constexpr int kTuneBlocks = 256;
+
+int tuned_tile(Executor& ex, kumo_hip_molecule* mol, AoMoLaunch a, hipStream_t s) {
+    if (a.precision == KUMO_HIP_FP64) return 16;
+    const int key = (a.precision * 4 + a.order) * 4 + a.hessian;
+    {
+        std::lock_guard<std::mutex> lock(mol->tune_mutex);
+        const auto it = mol->tuned_tile.find(key);
+        if (it != mol->tuned_tile.end()) return it->second;
+    }
+    if (ex.capture || a.nblk < kTuneBlocks) return 32;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    KUMO_HIP(hipEventCreate(&e0));
+    KUMO_HIP(hipEventCreate(&e1));
+    const int cand[2] = {32, 16};
+    float ms[2] = {0.f, 0.f};
+    try {
+        for (int i = 0; i < 2; ++i) {
+            a.tile_points = cand[i];
+            launch_ao_mo(a, s);
+            KUMO_HIP(hipEventRecord(e0, s));
+            launch_ao_mo(a, s);
+            KUMO_HIP(hipEventRecord(e1, s));
+            KUMO_HIP(hipEventSynchronize(e1));
+            KUMO_HIP(hipEventElapsedTime(&ms[i], e0, e1));
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int best = ms[1] < ms[0] ? cand[1] : cand[0];
+    std::lock_guard<std::mutex> lock(mol->tune_mutex);
+    return mol->tuned_tile.emplace(key, best).first->second;
+}
 
 RescuePlan rescue_plan(Executor& ex, long long count, int nblk, int block_size, int tiles_per_block) {
     const std::size_t pts = static_cast<std::size_t>(nblk) * static_cast<std::size_t>(block_size);
@@ -380,6 +428,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.moments = chm;
                 const bool rho_rescue = rho_which && rho_rescue_eps(opt.precision) > 0.0;
                 a.rho_scale = rho_rescue ? grow<float>(ex.ws_rscale, 2 * static_cast<std::size_t>(batch_max)) : nullptr;
+                a.tile_points = opt.tile_points ? opt.tile_points : tuned_tile(ex, mol, a, s);
                 record(ex, true);
                 launch_ao_mo(a, s);
                 record(ex, false);

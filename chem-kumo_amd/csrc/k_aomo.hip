@@ -122,11 +122,19 @@ __device__ __attribute__((always_inline)) inline void split_pair(double v0, doub
     }
 }
 
-template <int PREC, int ORDER, unsigned PSET>
-__global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel(AoMoLaunch a) {
+// Workgroups per CU the register budget is sized for: 32-point tiles keep
+// 4 x NP accumulator tiles per wave, reduced-precision 16-point tiles half as
+// many and run three workgroups per CU (FP64 tiles are always 16 points).
+template <int PREC, unsigned PSET, int TP> constexpr int occupancy() {
+    return popcount10(PSET) > 7 ? 1 : (TP == 16 && PREC != KUMO_HIP_FP64 ? 3 : 2);
+}
+
+template <int PREC, int ORDER, unsigned PSET, int TP>
+__global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kernel(AoMoLaunch a) {
     using P = Prec<PREC>;
     using E = typename P::E;
-    constexpr int T = P::T, NT = T / 16, S = P::S, LDK = P::LDK, PITCH = P::PITCH;
+    static_assert(TP == 16 || TP == 32, "16 or 32 points per workgroup");
+    constexpr int T = TP, NT = T / 16, S = P::S, LDK = P::LDK, PITCH = P::PITCH;
     constexpr int NP = popcount10(PSET);
     constexpr int TPW = 8 * NT / 4; // tiles per wave at 128 slots
     constexpr int PHI_PLANE = NP * T * LDK;
@@ -302,10 +310,10 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
             constexpr int PLANE = NP * T * LDK;
             E* d0 = phi + j * LDK + r0;
             if constexpr (is_bf16<PREC>) {
-                static_assert(RPL == 4 && KC == 32 && LDK == 32, "four bf16 rows per lane, swizzled 32-row points");
+                static_assert(RPL % 2 == 0 && KC == 32 && LDK == 32, "bf16 row pairs, swizzled 32-row points");
                 E* db = phi + phi_bf16_offset(j, r0);
 #pragma unroll
-                for (int pi = 0; pi < 2; ++pi) {
+                for (int pi = 0; pi < RPL / 2; ++pi) {
                     double va[NP], vb[NP];
                     eval_row(2 * pi, va);
                     eval_row(2 * pi + 1, vb);
@@ -322,7 +330,7 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
                     }
                 }
             } else if constexpr (PREC == KUMO_HIP_FP32) {
-                static_assert(RPL == 4, "four FP32 rows per lane");
+                static_assert(RPL == 4 || RPL == 2, "two or four FP32 rows per lane");
                 float st[RPL][NP];
 #pragma unroll
                 for (int i = 0; i < RPL; ++i) {
@@ -334,8 +342,12 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
                         st[i][p] = static_cast<float>(v[p]);
                 }
 #pragma unroll
-                for (int p = 0; p < NP; ++p)
-                    *reinterpret_cast<float4*>(d0 + p * T * LDK) = float4{st[0][p], st[1][p], st[2][p], st[3][p]};
+                for (int p = 0; p < NP; ++p) {
+                    if constexpr (RPL == 4)
+                        *reinterpret_cast<float4*>(d0 + p * T * LDK) = float4{st[0][p], st[1][p], st[2][p], st[3][p]};
+                    else
+                        *reinterpret_cast<float2*>(d0 + p * T * LDK) = float2{st[0][p], st[1][p]};
+                }
             } else {
                 static_assert(RPL == 2, "two FP64 rows per lane");
                 double va[NP], vb[NP];
@@ -516,31 +528,40 @@ __global__ void __launch_bounds__(256, popcount10(PSET) > 7 ? 1 : 2) aomo_kernel
     }
 }
 
-template <int PREC, int ORDER, unsigned PSET> void launch_one(const AoMoLaunch& a, hipStream_t stream) {
+template <int PREC, int ORDER, unsigned PSET, int TP> void launch_one(const AoMoLaunch& a, hipStream_t stream) {
     using P = Prec<PREC>;
     constexpr int NP = popcount10(PSET);
-    const size_t work = sizeof(typename P::E) * P::S * (static_cast<size_t>(NP) * P::T * P::LDK + KC * P::PITCH);
-    const size_t red = sizeof(double) * 8 * kMoments * P::T + sizeof(float) * 2 * 256;
+    const size_t work = sizeof(typename P::E) * P::S * (static_cast<size_t>(NP) * TP * P::LDK + KC * P::PITCH);
+    const size_t red = sizeof(double) * 8 * kMoments * TP + sizeof(float) * 2 * 256;
     const size_t lds = work > red ? work : red;
-    const int tiles_per_block = (a.block_size + P::T - 1) / P::T;
+    const int tiles_per_block = (a.block_size + TP - 1) / TP;
     const int groups = (a.slot1 - a.slot0 + 127) / 128;
     // (rescue mode: the same bound, every block's points in tiles of T)
     const dim3 grid(static_cast<unsigned>(a.nblk * tiles_per_block), static_cast<unsigned>(groups));
     static bool configured = false;
     if (!configured) {
-        KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&aomo_kernel<PREC, ORDER, PSET>),
+        KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&aomo_kernel<PREC, ORDER, PSET, TP>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         configured = true;
     }
-    aomo_kernel<PREC, ORDER, PSET><<<grid, dim3(256), lds, stream>>>(a);
+    aomo_kernel<PREC, ORDER, PSET, TP><<<grid, dim3(256), lds, stream>>>(a);
     KUMO_HIP(hipGetLastError());
 }
 
+template <int PREC, int TP> void launch_tile(const AoMoLaunch& a, hipStream_t stream) {
+    if (a.order == 0) launch_one<PREC, 0, kPanelsValue, TP>(a, stream);
+    else if (a.order == 1) launch_one<PREC, 1, kPanelsGrad, TP>(a, stream);
+    else if (a.hessian == 2) launch_one<PREC, 2, kPanelsAll, TP>(a, stream);
+    else launch_one<PREC, 2, kPanelsDiag, TP>(a, stream);
+}
+
 template <int PREC> void launch_prec(const AoMoLaunch& a, hipStream_t stream) {
-    if (a.order == 0) launch_one<PREC, 0, kPanelsValue>(a, stream);
-    else if (a.order == 1) launch_one<PREC, 1, kPanelsGrad>(a, stream);
-    else if (a.hessian == 2) launch_one<PREC, 2, kPanelsAll>(a, stream);
-    else launch_one<PREC, 2, kPanelsDiag>(a, stream);
+    if constexpr (PREC == KUMO_HIP_FP64) {
+        launch_tile<PREC, 16>(a, stream);
+    } else {
+        if (a.tile_points == 16) launch_tile<PREC, 16>(a, stream);
+        else launch_tile<PREC, 32>(a, stream);
+    }
 }
 
 } // namespace
@@ -550,6 +571,7 @@ void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream) {
     if (a.slot0 % 16 || a.slot1 % 16) invalid("MO slot ranges must be multiples of 16");
     static_assert(Prec<KUMO_HIP_FP64>::T == kRescueTile, "rescue tiles are FP64 workgroup tiles");
     if (a.r_tile_blk && a.precision != KUMO_HIP_FP64) invalid("rescue tiles are evaluated in FP64");
+    if (a.tile_points != 0 && a.tile_points != 16 && a.tile_points != 32) invalid("tile points must be 16 or 32");
     switch (a.precision) {
     case KUMO_HIP_FP64: launch_prec<KUMO_HIP_FP64>(a, stream); break;
     case KUMO_HIP_FP32: launch_prec<KUMO_HIP_FP32>(a, stream); break;

@@ -126,6 +126,7 @@ struct AoMoLaunch {
     int hessian;             // 0 none, 1 diagonal, 2 full
     int precision;           // kumo_hip_precision
     int slot0, slot1;
+    int tile_points;         // points per workgroup: 16 or 32 (0: 32; FP64 always 16)
     double* moments;         // [2][kMoments][batch_points]
     // Reduced precisions, when not null: [2][batch_points] sum_k |Phi_k| ||C_k,c||
     // over the kept operand rows for channel c (slot group 0 only).

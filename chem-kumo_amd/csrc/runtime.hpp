@@ -77,6 +77,9 @@ struct kumo_hip_molecule {
     // Per screening threshold: squared radius per block beyond which no AO of
     // the block passes it (kept until the molecule is released).
     std::map<double, kumo::DeviceBuffer> d_screen_r2;
+    // Autotuned AO->MO tile (points per workgroup) per (precision, order, Hessian).
+    std::mutex tune_mutex;
+    std::map<int, int> tuned_tile;
     kumo::DeviceBuffer d_p[4]; // total, spin, alpha, beta (alpha/beta on first use)
     kumo::DeviceBuffer d_ca, d_cb; // full MO matrices
     kumo::DeviceBuffer d_op64, d_op32, d_opbf, d_opnorm; // MO operand table (FP64, FP32, 3 bf16 planes)

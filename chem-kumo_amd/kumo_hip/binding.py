@@ -29,7 +29,7 @@ PROGRESS = C.CFUNCTYPE(C.c_int, C.c_size_t, C.c_size_t, C.c_void_p)
 class _Options(C.Structure):
     _fields_ = [("threads", C.c_int32), ("block_size", C.c_int32), ("backend", C.c_int32), ("precision", C.c_int32),
                 ("screening_threshold", C.c_double), ("potential_tolerance", C.c_double), ("fukui_delta", C.c_double),
-                ("device_outputs", C.c_int32), ("reserved", C.c_int32), ("progress", PROGRESS),
+                ("device_outputs", C.c_int32), ("tile_points", C.c_int32), ("progress", PROGRESS),
                 ("progress_user", C.c_void_p)]
 
 
@@ -165,7 +165,8 @@ def grid_struct(grid):
 
 
 def options(block_size=128, backend=Backend.AUTO, precision=Precision.FP64, screening_threshold=1e-12,
-            potential_tolerance=1e-8, fukui_delta=0.1, device_outputs=False, progress=None, threads=0):
+            potential_tolerance=1e-8, fukui_delta=0.1, device_outputs=False, progress=None, threads=0,
+            tile_points=0):
     o = _Options()
     lib().kumo_hip_default_options(C.byref(o))
     o.threads = threads
@@ -176,6 +177,7 @@ def options(block_size=128, backend=Backend.AUTO, precision=Precision.FP64, scre
     o.potential_tolerance = potential_tolerance
     o.fukui_delta = fukui_delta
     o.device_outputs = 1 if device_outputs else 0
+    o.tile_points = tile_points
     if progress is not None:
         cb = PROGRESS(lambda d, t, u: int(bool(progress(d, t))))
         o.progress = cb

@@ -238,3 +238,19 @@ def test_errors_map_to_exceptions(ctx):
         m.evaluate([(K.AverageLocalIonizationEnergy, 0, 0, 0)], points=pts)
     with pytest.raises(RuntimeError):
         m.evaluate([(K.Density, 0, 0, 0)], points=pts, progress=lambda d, t: True)
+
+
+@pytest.mark.parametrize("precision", [Precision.FP32, Precision.BF16X3, Precision.BF16X6])
+def test_tile_shapes_give_identical_results(ctx, precision):
+    """The AO->MO tile (16 or 32 points per workgroup, or the autotuner's
+    choice) changes only the work partition: results are bit-identical."""
+    rng = np.random.default_rng(12)
+    pts = rng.normal(size=(300 * 128 + 45, 3)) * 2.0  # above the tuner's batch threshold
+    m = gmol(ctx, "acrolein")
+    for props in ([(K.Density, 0, 0, 0)], [(K.Gradient, 1, 0, 0)], [(K.Laplacian, 0, 0, 0), (K.Elf, 0, 0, 0)],
+                  [(K.Hessian, 0, 0, 0), (K.Density, 2, 0, 0)]):
+        got = [m.evaluate(props, points=pts, precision=precision, backend=Backend.AO_MO, tile_points=t)[0]
+               for t in (16, 32, 0)]
+        for other in got[1:]:
+            for a, b in zip(got[0], other):
+                assert np.array_equal(a, b), (precision, props)
