@@ -42,21 +42,24 @@ __device__ constexpr signed char kPow[kMaxL + 1][kMaxCart][3] = {
      {5, 0, 0}}};
 
 // Panel sets: bit c selects derivative component c of
-// value, x, y, z, xx, xy, xz, yy, yz, zz (src/derivative_layout.hpp:13).
+// value, x, y, z, xx, xy, xz, yy, yz, zz (src/derivative_layout.hpp:13), and
+// bit 10 the Laplacian xx + yy + zz (contractions that need only the trace).
 constexpr unsigned kPanelsValue = 0x001;
 constexpr unsigned kPanelsGrad = 0x00f;
 constexpr unsigned kPanelsDiag = 0x08f | 0x010 | 0x200; // value, grad, xx, yy, zz
 constexpr unsigned kPanelsAll = 0x3ff;
+constexpr unsigned kPanelsLap = 0x40f;                   // value, grad, Laplacian
+constexpr int kPanelLap = 10;
 
 __host__ __device__ constexpr int popcount10(unsigned m) {
     int n = 0;
-    for (int i = 0; i < 10; ++i)
+    for (int i = 0; i <= kPanelLap; ++i)
         n += (m >> i) & 1u;
     return n;
 }
 // Index of the derivative component of output panel i.
 __host__ __device__ constexpr int panel_component(unsigned m, int i) {
-    for (int c = 0; c < 10; ++c)
+    for (int c = 0; c <= kPanelLap; ++c)
         if ((m >> c) & 1u) {
             if (i == 0) return c;
             --i;
@@ -97,6 +100,7 @@ __device__ __attribute__((always_inline)) inline void eval_cartesian(const DevSh
                                       const double* __restrict__ wts, double x, double y, double z, double r2,
                                       double (&cart)[(L + 1) * (L + 2) / 2][popcount10(PSET)]) {
 #pragma clang fp contract(off)
+    static_assert(!(PSET >> kPanelLap), "reference-order evaluation has no Laplacian panel");
     constexpr int NC = (L + 1) * (L + 2) / 2;
     constexpr int NP = popcount10(PSET);
     const int np = s.nprim;
@@ -229,6 +233,7 @@ template <int ORDER, unsigned PSET, typename Emit>
 __device__ __attribute__((always_inline)) inline void eval_block_rt(const DevShell& s, const MolView& m, double x, double y,
                                                                   double z, double r2, Emit&& emit) {
 #pragma clang fp contract(off)
+    static_assert(!(PSET >> kPanelLap), "reference-order evaluation has no Laplacian panel");
     constexpr int NP = popcount10(PSET);
     const int L = s.l, NC = s.ncart, np = s.nprim;
     const double* e = m.exps + s.prim;
@@ -445,6 +450,10 @@ __device__ __attribute__((always_inline)) inline void cart_components(double x, 
                 const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
                 const double Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
                 v[i] = Paa * R[0] + (2.0 * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
+            } else if (cc == kPanelLap) {
+                // sum of the three above; sum_a c_a P_a = L P for a monomial of degree L
+                const double Plap = f[0][2] * f[1][0] * f[2][0] + f[0][0] * f[1][2] * f[2][0] + f[0][0] * f[1][0] * f[2][2];
+                v[i] = Plap * R[0] + (2 * L + 3) * P * R[1] + (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) * P * R[2];
             } else {
                 const int a = cc == 8 ? 1 : 0, b = cc == 5 ? 1 : 2;
                 const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
@@ -510,6 +519,11 @@ __device__ __attribute__((always_inline)) inline void cart_component_pw(const in
             const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
             const double Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
             v[i] = Paa * R[0] + (2.0 * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
+        } else if (cc == kPanelLap) {
+            // sum of the three above; sum_a c_a P_a = (qx + qy + qz) P
+            const double Plap = f[0][2] * f[1][0] * f[2][0] + f[0][0] * f[1][2] * f[2][0] + f[0][0] * f[1][0] * f[2][2];
+            v[i] = Plap * R[0] + (2 * (qq[0] + qq[1] + qq[2]) + 3) * P * R[1] +
+                   (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) * P * R[2];
         } else {
             const int a = cc == 8 ? 1 : 0, b = cc == 5 ? 1 : 2;
             const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];

@@ -343,9 +343,11 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
     // ---------------------------------------------------------------- point pass
     if (!needed.empty()) {
         const int ao_order = grid ? plan.max_order : std::max(1, plan.max_order);
-        bool full_h = false;
-        for (int c = 0; c < 4; ++c)
+        bool full_h = false, diag_h = false;
+        for (int c = 0; c < 4; ++c) {
             full_h = full_h || plan.c[static_cast<std::size_t>(c)].full_hessian();
+            diag_h = diag_h || plan.c[static_cast<std::size_t>(c)].hessian_diagonal();
+        }
         const long long b_first = q0 / bs, b_end = (q1 + bs - 1) / bs;
         const long long nblocks = b_end - b_first;
         const long long target = 1ll << 21;
@@ -401,7 +403,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         upload_async(ex, dtasks, tasks.data(), tasks.size());
         std::vector<int32_t> nkept(static_cast<std::size_t>(cap));
         double flops = 0.0;
-        const int panels = ao_order <= 0 ? 1 : (ao_order == 1 ? 4 : (full_h ? 10 : 7));
+        const int panels = ao_order <= 0 ? 1 : (ao_order == 1 ? 4 : (full_h ? 10 : (diag_h ? 7 : 5)));
         for (long long b0 = b_first; b0 < b_end; b0 += cap) {
             const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
             const long long first = b0 * bs;
@@ -421,7 +423,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.first_block = b0;
                 a.nblk = nb;
                 a.order = std::max(0, ao_order);
-                a.hessian = ao_order == 2 ? (full_h ? 2 : 1) : 0;
+                a.hessian = ao_order == 2 ? (full_h ? 2 : (diag_h ? 1 : 3)) : 0;
                 a.precision = opt.precision;
                 a.slot0 = slot0;
                 a.slot1 = slot1;

@@ -463,7 +463,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
             // output panel index of derivative component `comp`
             int idx = -1, n = 0;
 #pragma unroll
-            for (int cc = 0; cc < 10; ++cc)
+            for (int cc = 0; cc <= kPanelLap; ++cc)
                 if ((PSET >> cc) & 1u) {
                     if (cc == comp) idx = n;
                     ++n;
@@ -481,7 +481,10 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                     q[1 + ax] += v0 * va[ax];
                     q[4 + ax] += va[ax] * va[ax];
                 }
-                if constexpr (ORDER == 2) {
+                if constexpr (ORDER == 2 && ((PSET >> kPanelLap) & 1u)) {
+                    // Laplacian panel: the whole trace in the xx moment
+                    q[7] += v0 * V(kPanelLap, r) + va[0] * va[0] + va[1] * va[1] + va[2] * va[2];
+                } else if constexpr (ORDER == 2) {
                     constexpr int hc[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
                     int h = 0;
 #pragma unroll
@@ -552,6 +555,7 @@ template <int PREC, int TP> void launch_tile(const AoMoLaunch& a, hipStream_t st
     if (a.order == 0) launch_one<PREC, 0, kPanelsValue, TP>(a, stream);
     else if (a.order == 1) launch_one<PREC, 1, kPanelsGrad, TP>(a, stream);
     else if (a.hessian == 2) launch_one<PREC, 2, kPanelsAll, TP>(a, stream);
+    else if (a.hessian == 3) launch_one<PREC, 2, kPanelsLap, TP>(a, stream);
     else launch_one<PREC, 2, kPanelsDiag, TP>(a, stream);
 }
 

@@ -123,7 +123,7 @@ struct AoMoLaunch {
     long long first_block;   // screening block of batch_first
     int nblk;
     int order;               // AO arithmetic: 0 value kernel, 1/2 derivative kernel
-    int hessian;             // 0 none, 1 diagonal, 2 full
+    int hessian;             // 0 none, 1 diagonal, 2 full, 3 trace only (in the xx moment)
     int precision;           // kumo_hip_precision
     int slot0, slot1;
     int tile_points;         // points per workgroup: 16 or 32 (0: 32; FP64 always 16)

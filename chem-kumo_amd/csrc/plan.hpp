@@ -39,6 +39,7 @@ struct ComponentPlan {
     bool elf = false, elf_input = false, reduced_gradient = false, sign_lambda2 = false;
     bool lol = false, lol_input = false, ge_input = false, gradient_input = false;
     bool hessian_input = false, energy_input = false, tau_input = false, value_input = false, esp = false;
+    bool diagonal_input = false; // the Hessian diagonal element by element
     bool derivatives() const {
         return gradient || hessian || laplacian || tau || elf_input || reduced_gradient || sign_lambda2 || lol_input ||
                gradient_input || hessian_input || energy_input || tau_input || ge_input || value_input;
@@ -52,6 +53,9 @@ struct ComponentPlan {
     // Full Hessian moments (all six) are needed by Hessian, sign(lambda2) rho
     // and DORI; the others use its diagonal only.
     bool full_hessian() const { return hessian || sign_lambda2 || hessian_input; }
+    // The other second-derivative users need only the Laplacian, except the
+    // kinetic-energy components (tau_a - lap_a / 4).
+    bool hessian_diagonal() const { return diagonal_input; }
 };
 
 struct Plan {
@@ -110,7 +114,7 @@ inline Plan make_plan(const kumo_hip_property* props, std::size_t n) {
         case kKinetic: c.energy_input = true; break;
         case kPotentialEnergy: require_total(p, "the potential energy density"); c.energy_input = true; break;
         case kEnergyDensity: require_total(p, "the energy density"); c.energy_input = true; break;
-        case kKineticComponents: c.energy_input = true; break;
+        case kKineticComponents: c.energy_input = true; c.diagonal_input = true; break;
         case kTauComponents: c.tau_input = true; break;
         case kVirialRatio: require_total(p, "|V| / G"); c.energy_input = true; break;
         case kEnergyPerElectron: require_total(p, "the energy density per electron"); c.energy_input = true; break;
