@@ -91,6 +91,19 @@ template <> struct Prec<KUMO_HIP_BF16X6> : PrecBf16<3> {};
 
 template <int PREC> constexpr bool is_bf16 = PREC == KUMO_HIP_BF16X3 || PREC == KUMO_HIP_BF16X6;
 
+// Moments a contraction can make nonzero: the value; + gradient and tau terms
+// (order 1); + the trace (Laplacian panel), the diagonal or all six Hessian
+// moments (order 2). The others are written as zeros without being summed.
+template <int ORDER, unsigned PSET> __host__ __device__ constexpr bool moment_used(int k) {
+    if (k == 0) return true;
+    if (ORDER == 0) return false;
+    if (k <= 6) return true;
+    if (ORDER == 1) return false;
+    if ((PSET >> kPanelLap) & 1u) return k == 7;
+    if (PSET == kPanelsDiag) return k == 7 || k == 10 || k == 12;
+    return true;
+}
+
 // Element offset of row r (< 32) of point j in a bf16 Phi plane.
 __device__ __attribute__((always_inline)) inline int phi_bf16_offset(int j, int r) {
     return j * 32 + ((((r >> 3) ^ (j >> 2)) & 3) << 3) + (r & 7);
@@ -498,13 +511,15 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
         // Sum over the four row groups of the tile.
 #pragma unroll
         for (int k = 0; k < kMoments; ++k) {
+            if (!moment_used<ORDER, PSET>(k)) continue;
             q[k] += __shfl_xor(q[k], 16, 64);
             q[k] += __shfl_xor(q[k], 32, 64);
         }
         if (lane < 16)
 #pragma unroll
             for (int k = 0; k < kMoments; ++k)
-                red[(static_cast<size_t>(mt) * kMoments + k) * T + nt * 16 + r16] = q[k];
+                if (moment_used<ORDER, PSET>(k))
+                    red[(static_cast<size_t>(mt) * kMoments + k) * T + nt * 16 + r16] = q[k];
     }
     __syncthreads();
     // Sum the m-tiles of each channel in order and store the moments.
@@ -516,8 +531,9 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
         if (jj >= nt_valid) continue;
         const int mt0 = ch == 0 ? 0 : alpha_tiles_end, mt1 = ch == 0 ? alpha_tiles_end : MT;
         double s = 0.0;
-        for (int mt = mt0; mt < mt1; ++mt)
-            s += red[(static_cast<size_t>(mt) * kMoments + k) * T + jj];
+        if (moment_used<ORDER, PSET>(k))
+            for (int mt = mt0; mt < mt1; ++mt)
+                s += red[(static_cast<size_t>(mt) * kMoments + k) * T + jj];
         const double f = k == 0 ? 1.0 : (k <= 3 ? 2.0 : (k <= 6 ? 0.5 : 2.0));
         const long long point = roff ? static_cast<long long>(blockIdx.x) * T + jj : bstart + p0 + jj - a.batch_first;
         double* dst = a.moments + (static_cast<long long>(blockIdx.y) * 2 + ch) * kMoments * a.batch_points;
