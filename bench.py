@@ -47,19 +47,22 @@ def run_gpu(mol, grid, precision, steps, warmup, torch):
     for _ in range(warmup):
         mol.evaluate_device(PROPS, grid, ptrs, precision=precision)
     torch.cuda.synchronize()
-    kernel_ms, flops = 0.0, 0.0
+    kernel_ms, flops, fields_ms, fields_bytes = 0.0, 0.0, 0.0, 0
     t0 = time.perf_counter()
     for _ in range(steps):
         mol.evaluate_device(PROPS, grid, ptrs, precision=precision)
         st = mol.ctx.kernel_stats()
         kernel_ms += st["kernel_ms"]
         flops += st["flops"]
+        fields_ms += st["fields_ms"]
+        fields_bytes += st["fields_bytes"]
         launches = st["launches"]
         rescue_ms, rescue_points = st["rescue_ms"], st["rescue_points"]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"seconds": dt, "points": n * steps, "kernel_ms": kernel_ms / steps, "flops": flops / steps,
-            "launches": launches, "out": out, "rescue_ms": rescue_ms, "rescue_points": rescue_points}
+            "launches": launches, "out": out, "rescue_ms": rescue_ms, "rescue_points": rescue_points,
+            "fields_ms": fields_ms / steps, "fields_bytes": fields_bytes / steps}
 
 
 CHANNELS = ["rho", "grad_x", "grad_y", "grad_z", "lap", "tau", "elf", "lol"]
@@ -435,8 +438,14 @@ def main():
         "errors": errors,
         "errors_full_grid": dict(full_errors or {}, vs="FP64 GPU path, all points") if full_errors else None,
         "elf_rescue": {"points": main_run["rescue_points"], "fraction": main_run["rescue_points"] / n,
-                       "fp64_kernel_ms": main_run["rescue_ms"]},
+                       "fp64_kernel_ms": main_run["rescue_ms"], "criteria": "density or ELF uncertain"},
         "roofline": roofline,
+        "field_reductions": {
+            "bound": "hbm", "unit": "GB/s", "peak": 8000.0, "kernel": "fields_kernel",
+            "kernel_ms": main_run["fields_ms"], "bytes_per_step": main_run["fields_bytes"],
+            "achieved": main_run["fields_bytes"] / (main_run["fields_ms"] * 1e-3) / 1e9 if main_run["fields_ms"] else None,
+            "frac": main_run["fields_bytes"] / (main_run["fields_ms"] * 1e-3) / 8e12 if main_run["fields_ms"] else None,
+            "bytes": "channel moments read (2 x 13 x 8 B per point) + output channels written (8 x 8 B per point)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(result))

@@ -244,6 +244,14 @@ int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, i
     return KUMO_HIP_OK;
 }
 
+int kumo_hip_last_field_stats(const kumo_hip_context* ctx, double* fields_ms, int64_t* points, int64_t* bytes) {
+    if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
+    if (fields_ms) *fields_ms = ctx->exec.last_fields_ms;
+    if (points) *points = ctx->exec.last_fields_points;
+    if (bytes) *bytes = ctx->exec.last_fields_bytes;
+    return KUMO_HIP_OK;
+}
+
 int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points) {
     if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
     if (rescue_ms) *rescue_ms = ctx->exec.last_rescue_ms;

@@ -17,7 +17,6 @@
 namespace kumo {
 namespace {
 
-constexpr int kComp = 15;
 
 void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
     if (o.block_size < 1 || o.block_size > 65536) invalid("evaluation block_size must be in [1, 65536]");
@@ -255,9 +254,10 @@ long long validated_point_count(const kumo_hip_grid& g) {
 
 // Constants of the derived fields computed with the host libm exactly as the
 // reference computes them.
-FieldsLaunch fields_launch(const double* moments, long long n, long long out_offset, int ntasks, const FieldTask* tasks) {
+FieldsLaunch fields_launch(const double* channels, int from_spin, int closed, long long n, long long out_offset,
+                          int ntasks, const FieldTask* tasks) {
     constexpr double pi = 3.141592653589793;
-    FieldsLaunch f{moments, n, out_offset, ntasks, tasks, 0.3 * std::pow(6.0 * pi * pi, 2.0 / 3.0),
+    FieldsLaunch f{channels, from_spin, closed, n, out_offset, ntasks, tasks, 0.3 * std::pow(6.0 * pi * pi, 2.0 / 3.0),
                    2.0 * std::cbrt(3.0 * pi * pi)};
     return f;
 }
@@ -380,7 +380,6 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         }
         const int groups = backend == KUMO_HIP_AO_MO ? (slot1 - slot0 + 127) / 128 : 1;
         double* chm = grow<double>(ex.ws_moments, static_cast<std::size_t>(groups) * 2 * kMoments * batch_max);
-        double* cmp = grow<double>(ex.ws_comp, static_cast<std::size_t>(4) * kComp * batch_max);
         std::vector<FieldTask> tasks;
         for (std::size_t i = 0; i < nprops; ++i) {
             const auto& p = props[i];
@@ -472,7 +471,6 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                     launch_rescue_scatter(r.moments, rslots, groups, kRescueTile, bs, rp, chm, count, s);
                     if (!ex.capture) KUMO_HIP(hipMemcpyAsync(&rescue_tiles, rp.total, 4, hipMemcpyDeviceToHost, s));
                 }
-                launch_combine(chm, cmp, count, 1, ao_order, closed ? 1 : 0, s);
             } else {
                 AoPLaunch a{};
                 a.mol = mv;
@@ -498,10 +496,22 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 launch_ao_p(a, s);
                 record(ex, false);
                 ++ex.last_launches;
-                launch_combine(chm, cmp, count, 0, ao_order, 0, s);
             }
-            const FieldsLaunch f = fields_launch(cmp, count, first, static_cast<int>(tasks.size()), dtasks);
+            const FieldsLaunch f = fields_launch(chm, backend == KUMO_HIP_AO_MO ? 1 : 0,
+                                                 backend == KUMO_HIP_AO_MO && closed ? 1 : 0, count, first,
+                                                 static_cast<int>(tasks.size()), dtasks);
+            record(ex, true, 2);
             launch_fields(f, s);
+            record(ex, false, 2);
+            if (!tasks.empty()) {
+                // bytes the reduction must move: the channel moments read, the outputs written
+                std::size_t outs = 0;
+                for (const auto& t : tasks)
+                    outs += static_cast<std::size_t>(channels(t.kind));
+                const int read_ch = backend == KUMO_HIP_AO_MO && closed ? 1 : 2;
+                ex.last_fields_points += count;
+                ex.last_fields_bytes += count * static_cast<long long>(8 * (read_ch * kMoments + outs));
+            }
             if (ex.capture) {
                 advance(static_cast<std::size_t>(count));
                 continue;
@@ -610,7 +620,6 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 static_cast<int>(std::max(1ll, std::min<long long>({nblocks, 8192ll, (256ll << 20) / (8 * per_block)})));
             const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks);
             double* chm = grow<double>(ex.ws_moments, 2 * kMoments * static_cast<std::size_t>(cap) * bs);
-            double* cmp = grow<double>(ex.ws_comp, 4 * kComp * static_cast<std::size_t>(cap) * bs);
             FieldTask t{};
             t.kind = kDensity;
             t.component = kTotal;
@@ -640,8 +649,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.moments = chm;
                 launch_fill(chm, 2 * kMoments * count, 0.0, s);
                 launch_ao_p(a, s);
-                launch_combine(chm, cmp, count, 0, 0, 1, s);
-                const FieldsLaunch f = fields_launch(cmp, count, first, 1, dtask);
+                const FieldsLaunch f = fields_launch(chm, 0, 0, count, first, 1, dtask);
                 launch_fields(f, s);
                 sync(ex);
                 advance(static_cast<std::size_t>(count));
@@ -747,7 +755,7 @@ void Executor::destroy() {
         }
         list.clear();
     }
-    for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_comp, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc,
+    for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc,
                             &ws_rescue, &ws_rmom, &ws_rscale})
         b->release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -755,21 +763,22 @@ void Executor::destroy() {
 }
 
 void Executor::reset_stats() {
-    events_used[0] = events_used[1] = 0;
-    last_kernel_ms = last_rescue_ms = 0.0;
+    events_used[0] = events_used[1] = events_used[2] = 0;
+    last_kernel_ms = last_rescue_ms = last_fields_ms = 0.0;
     last_flops = 0.0;
-    last_launches = last_rescue_points = 0;
+    last_launches = last_rescue_points = last_fields_points = last_fields_bytes = 0;
 }
 
 void Executor::collect_timing() {
-    for (int k = 0; k < 2; ++k) {
+    double* dst[3] = {&last_kernel_ms, &last_rescue_ms, &last_fields_ms};
+    for (int k = 0; k < 3; ++k) {
         double ms = 0.0;
         for (std::size_t e = 0; e < events_used[k]; ++e) {
             float t = 0.f;
             KUMO_HIP(hipEventElapsedTime(&t, events[k][e].first, events[k][e].second));
             ms += t;
         }
-        (k ? last_rescue_ms : last_kernel_ms) = ms;
+        *dst[k] = ms;
     }
 }
 

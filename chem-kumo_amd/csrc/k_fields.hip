@@ -11,8 +11,6 @@
 namespace kumo {
 namespace {
 
-constexpr int kComp = 15; // 13 moments + Laplacian + tau
-constexpr int M_LAP = 13, M_TAU = 14;
 
 struct DDv {
     double v, g[3], t[3], h[6], lap, tau;
@@ -48,49 +46,17 @@ __device__ inline DDv combine(const DDv& a, const DDv& b, double sign, double sc
     return d;
 }
 
-__device__ inline void store_comp(double* m, long long n, long long j, const DDv& d) {
-    m[M_VALUE * n + j] = d.v;
-    for (int a = 0; a < 3; ++a) {
-        m[(M_GX + a) * n + j] = d.g[a];
-        m[(M_TX + a) * n + j] = d.t[a];
+// Component k (0 total, 1 alpha, 2 beta, 3 difference) of a point from its
+// two channels: (alpha, beta) or (total, difference).
+__device__ inline DDv component(const DDv& c0, const DDv& c1, int from_spin, int k) {
+    if (from_spin) {
+        if (k == 1) return c0;
+        if (k == 2) return c1;
+        return combine(c0, c1, k == 0 ? 1.0 : -1.0, 1.0);
     }
-    for (int k = 0; k < 6; ++k)
-        m[(M_HXX + k) * n + j] = d.h[k];
-    m[M_LAP * n + j] = d.lap;
-    m[M_TAU * n + j] = d.tau;
-}
-
-__global__ void combine_kernel(const double* ch, double* comp, long long n, int from_spin, int closed) {
-    const long long j = blockIdx.x * 256ll + threadIdx.x;
-    if (j >= n) return;
-    const DDv c0 = load_channel(ch, n, j);
-    const DDv c1 = closed ? c0 : load_channel(ch + kMoments * n, n, j);
-    const long long stride = kComp * n;
-    if (from_spin) { // channels: alpha, beta
-        store_comp(comp + 1 * stride, n, j, c0);
-        store_comp(comp + 2 * stride, n, j, c1);
-        store_comp(comp + 0 * stride, n, j, combine(c0, c1, 1.0, 1.0));
-        store_comp(comp + 3 * stride, n, j, combine(c0, c1, -1.0, 1.0));
-    } else { // channels: total, spin difference
-        store_comp(comp + 0 * stride, n, j, c0);
-        store_comp(comp + 3 * stride, n, j, c1);
-        store_comp(comp + 1 * stride, n, j, combine(c0, c1, 1.0, 0.5));
-        store_comp(comp + 2 * stride, n, j, combine(c0, c1, -1.0, 0.5));
-    }
-}
-
-__device__ inline DDv load_comp(const double* m, long long n, long long j) {
-    DDv d;
-    d.v = m[M_VALUE * n + j];
-    for (int a = 0; a < 3; ++a) {
-        d.g[a] = m[(M_GX + a) * n + j];
-        d.t[a] = m[(M_TX + a) * n + j];
-    }
-    for (int k = 0; k < 6; ++k)
-        d.h[k] = m[(M_HXX + k) * n + j];
-    d.lap = m[M_LAP * n + j];
-    d.tau = m[M_TAU * n + j];
-    return d;
+    if (k == 0) return c0;
+    if (k == 3) return c1;
+    return combine(c0, c1, k == 1 ? 1.0 : -1.0, 0.5);
 }
 
 __device__ inline double gnorm2(const DDv& d) { return (d.g[0] * d.g[0] + d.g[1] * d.g[1]) + d.g[2] * d.g[2]; }
@@ -174,14 +140,16 @@ __device__ double middle_eigenvalue(const double* h6) {
 __global__ void fields_kernel(FieldsLaunch f) {
     const long long j = blockIdx.x * 256ll + threadIdx.x;
     if (j >= f.n) return;
-    const long long n = f.n, stride = kComp * n;
+    const long long n = f.n;
     const long long o = f.out_offset + j;
+    const DDv c0 = load_channel(f.channels, n, j);
+    const DDv c1 = f.closed ? c0 : load_channel(f.channels + kMoments * n, n, j);
     for (int t = 0; t < f.ntasks; ++t) {
         const FieldTask& task = f.tasks[t];
         const int kind = task.kind;
         // Total ELF / LOL / Tsirelson-Stash forms use both spin channels.
         if ((kind == 6 || kind == 10 || kind == 35 || kind == 36) && task.component == 0) {
-            DDv c[2] = {load_comp(f.moments + 1 * stride, n, j), load_comp(f.moments + 2 * stride, n, j)};
+            DDv c[2] = {component(c0, c1, f.from_spin, 1), component(c0, c1, f.from_spin, 2)};
             double v;
             if (kind == 6) v = elf(c, 2, f.fermi);
             else if (kind == 10) v = lol(c, 2, f.fermi);
@@ -190,12 +158,12 @@ __global__ void fields_kernel(FieldsLaunch f) {
             continue;
         }
         if (kind == 16) { // spin polarization
-            const double tv = f.moments[0 * stride + M_VALUE * n + j];
-            const double sv = f.moments[3 * stride + M_VALUE * n + j];
+            const double tv = component(c0, c1, f.from_spin, 0).v;
+            const double sv = component(c0, c1, f.from_spin, 3).v;
             task.out[0][o] = tv > 0.0 ? sv / tv : 0.0;
             continue;
         }
-        const DDv d = load_comp(f.moments + task.component * stride, n, j);
+        const DDv d = component(c0, c1, f.from_spin, task.component);
         double* out0 = task.out[0];
         switch (kind) {
         case 0: out0[o] = d.v; break;
@@ -422,7 +390,7 @@ void launch_rescue_plan(const double* channel_moments, long long n, int closed, 
                         int block_size, int nblk, int tile, const RescuePlan& r, hipStream_t stream) {
     if (n <= 0) return;
     if (c.rho_which && !c.rho_scale) invalid("density rescue without error scales");
-    const FieldsLaunch f = fields_launch(nullptr, 0, 0, 0, nullptr);
+    const FieldsLaunch f = fields_launch(nullptr, 0, 0, 0, 0, 0, nullptr);
     rescue_flag_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(channel_moments, n, closed, c,
                                                                                   f.fermi, r.flag);
     KUMO_HIP(hipGetLastError());
@@ -450,15 +418,6 @@ void launch_add_into(double* dst, const double* src, long long n, hipStream_t st
 void launch_fill(double* dst, long long n, double value, hipStream_t stream) {
     if (n <= 0) return;
     fill_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(dst, n, value);
-    KUMO_HIP(hipGetLastError());
-}
-
-void launch_combine(const double* channel_moments, double* component_moments, long long n, int from_spin_channels,
-                    int order, int closed_shell, hipStream_t stream) {
-    (void)order;
-    if (n <= 0) return;
-    combine_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(channel_moments, component_moments, n,
-                                                                               from_spin_channels, closed_shell);
     KUMO_HIP(hipGetLastError());
 }
 

@@ -166,8 +166,13 @@ struct FieldTask {
     int kind, component;
     double* out[6];
 };
+// The density components (total, alpha, beta, difference; combine,
+// src/density.cpp:50-60) are formed per point from the two channels' moments
+// and the derived fields computed from them in the same pass.
 struct FieldsLaunch {
-    const double* moments;   // [4 components][kMoments][n]: total, alpha, beta, difference
+    const double* channels;  // [2][kMoments][n]: alpha, beta (from_spin) or total, spin difference
+    int from_spin;           // 1: the channels are alpha and beta
+    int closed;              // 1: closed shell (alpha and beta channels equal; the second is not read)
     long long n;
     long long out_offset;    // index of the batch's first point in the outputs
     int ntasks;
@@ -175,13 +180,9 @@ struct FieldsLaunch {
     double fermi;            // 0.3 (6 pi^2)^(2/3), as src/evaluator.cpp:128-129 computes it
     double rdg_scale;        // 2 (3 pi^2)^(1/3), src/evaluator.cpp:148
 };
-FieldsLaunch fields_launch(const double* moments, long long n, long long out_offset, int ntasks, const FieldTask* tasks);
+FieldsLaunch fields_launch(const double* channels, int from_spin, int closed, long long n, long long out_offset,
+                          int ntasks, const FieldTask* tasks);
 void launch_fields(const FieldsLaunch& f, hipStream_t stream);
-// Channel moments -> component moments: from (alpha, beta) (AO-MO) or
-// (total, spin) (AO-P) build total, alpha, beta, difference
-// (combine, src/density.cpp:50-60).
-void launch_combine(const double* channel_moments, double* component_moments, long long n, int from_spin_channels,
-                    int order, int closed_shell, hipStream_t stream);
 
 // dst[i] += src[i]; dst[i] = value.
 void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream);

@@ -26,13 +26,13 @@ namespace kumo {
 struct Executor {
     hipStream_t stream = nullptr;
     const double* boys_table = nullptr;
-    DeviceBuffer ws_tables, ws_moments, ws_comp, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale;
-    // Events bracketing the contraction launches (index 0) and the FP64
-    // rescue launches (index 1).
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events[2];
-    std::size_t events_used[2] = {0, 0};
-    double last_kernel_ms = 0.0, last_flops = 0.0, last_rescue_ms = 0.0;
-    long long last_launches = 0, last_rescue_points = 0;
+    DeviceBuffer ws_tables, ws_moments, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale;
+    // Events bracketing the contraction launches (index 0), the FP64 rescue
+    // launches (index 1) and the field reductions (index 2).
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
+    std::size_t events_used[3] = {0, 0, 0};
+    double last_kernel_ms = 0.0, last_flops = 0.0, last_rescue_ms = 0.0, last_fields_ms = 0.0;
+    long long last_launches = 0, last_rescue_points = 0, last_fields_points = 0, last_fields_bytes = 0;
     // Capture mode (a batch recorded into a HIP graph): no host
     // synchronisation, no timing events, no readback; small host arrays
     // copied to the device are kept in `keep` so that graph replays read them.

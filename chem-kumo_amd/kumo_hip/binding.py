@@ -65,7 +65,7 @@ EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip
            "kumo_hip_molecule_from_fchk", "kumo_hip_molecule_create", "kumo_hip_molecule_info_get",
            "kumo_hip_molecule_release", "kumo_hip_molecule_blocks", "kumo_hip_molecule_atoms",
            "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
-           "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_batch_create",
+           "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_last_field_stats", "kumo_hip_batch_create",
            "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_release", "kumo_hip_server_create",
            "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release"]
 
@@ -97,6 +97,7 @@ def lib():
         L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_last_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
         L.kumo_hip_last_rescue_stats.argtypes = [vp, vp, vp]
+        L.kumo_hip_last_field_stats.argtypes = [vp, vp, vp, vp]
         L.kumo_hip_batch_create.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_size_t, C.POINTER(_Options), C.c_int32,
                                             vp, vp]
         L.kumo_hip_batch_run.argtypes = [vp, C.c_int32]
@@ -147,8 +148,11 @@ class Context:
         lib().kumo_hip_last_kernel_stats(self.h, C.byref(ms), C.byref(n), C.byref(fl), C.byref(pfl))
         rms, rpts = C.c_double(), C.c_int64()
         lib().kumo_hip_last_rescue_stats(self.h, C.byref(rms), C.byref(rpts))
+        fms, fpts, fbytes = C.c_double(), C.c_int64(), C.c_int64()
+        lib().kumo_hip_last_field_stats(self.h, C.byref(fms), C.byref(fpts), C.byref(fbytes))
         return {"kernel_ms": ms.value, "launches": n.value, "flops": fl.value, "padded_flops": pfl.value,
-                "rescue_ms": rms.value, "rescue_points": rpts.value}
+                "rescue_ms": rms.value, "rescue_points": rpts.value, "fields_ms": fms.value,
+                "fields_points": fpts.value, "fields_bytes": fbytes.value}
 
 
 def grid_struct(grid):

@@ -255,12 +255,20 @@ int kumo_hip_server_release(kumo_hip_server* server);
 int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, int64_t* launches,
                                double* algorithmic_flops, double* padded_flops);
 
-/* Device time (ms) of the last evaluation's FP64 ELF rescue launches and the
+/* Device time (ms) of the last evaluation's FP64 rescue launches and the
  * number of point slots they covered (points in 16-point tiles). With a
- * reduced contraction precision and ELF requested, the points whose ELF the
- * precision leaves uncertain beyond 1e-6 are recomputed in FP64; no
- * counterpart in the reference (its CPU path is FP64 throughout). */
+ * reduced contraction precision, the points whose density (relative error
+ * bound above 5e-7) or ELF (error bound above 3e-6) the precision leaves
+ * uncertain are recomputed in FP64; no counterpart in the reference (its CPU
+ * path is FP64 throughout). */
 int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points);
+
+/* Device time (ms) of the last evaluation's field reductions (the density
+ * components and derived fields formed from the channel moments, store_block
+ * of src/evaluator.cpp:127-353), the points they covered and the bytes they
+ * must move (channel moments read, outputs written): measurement only, no
+ * counterpart in the reference. */
+int kumo_hip_last_field_stats(const kumo_hip_context* ctx, double* fields_ms, int64_t* points, int64_t* bytes);
 
 #ifdef __cplusplus
 }
