@@ -126,6 +126,48 @@ __device__ __attribute__((always_inline)) inline double probe_min_max(const DevS
     return mn;
 }
 
+// Smallest squared distance from c to the block's points (cached in spts),
+// and the first point reaching it. For a grid the block's points lie on at
+// most a few z-line segments, along each of which the squared distance is a
+// convex quadratic in k: only the points within two steps of its clamped
+// minimiser are evaluated (with the point-by-point arithmetic, so the value
+// is the minimum the full scan finds). Listed points are scanned.
+__device__ __attribute__((always_inline)) inline void nearest_point(const PointSource& ps, long long start, int count,
+                                                                  const double* spts, const double (&c)[3],
+                                                                  double& best, int& bj) {
+    best = 1.0e300;
+    bj = 0;
+    const auto probe = [&](int jj) __attribute__((always_inline)) {
+        const double r2 = sq_norm(spts[4 * jj] - c[0], spts[4 * jj + 1] - c[1], spts[4 * jj + 2] - c[2]);
+        if (r2 < best || (r2 == best && jj < bj)) {
+            best = r2;
+            bj = jj;
+        }
+    };
+    if (ps.points) {
+        for (int jj = 0; jj < count; ++jj)
+            probe(jj);
+        return;
+    }
+    const double* s = ps.steps;
+    const double dd = s[6] * s[6] + s[7] * s[7] + s[8] * s[8];
+    const long long end = start + count;
+    for (long long q = start; q < end;) {
+        const long long line = q / ps.nz, k0 = q % ps.nz;
+        const long long k1 = min(ps.nz - 1, k0 + (end - q) - 1);
+        const double fi = static_cast<double>(line / ps.ny), fj = static_cast<double>(line % ps.ny);
+        double a[3];
+        for (int e = 0; e < 3; ++e)
+            a[e] = ps.origin[e] + s[e] * fi + s[3 + e] * fj - c[e];
+        double ks = -(a[0] * s[6] + a[1] * s[7] + a[2] * s[8]) / dd;
+        ks = fmin(fmax(ks, static_cast<double>(k0)), static_cast<double>(k1));
+        const long long kc = llrint(ks);
+        for (long long kk = kc - 2; kk <= kc + 2; ++kk)
+            if (kk >= k0 && kk <= k1) probe(static_cast<int>(line * ps.nz + kk - start));
+        q = (line + 1) * ps.nz;
+    }
+}
+
 // Exclusive prefix sum of v over the workgroup; total receives the sum.
 __device__ __attribute__((always_inline)) inline int block_scan(int v, int* tmp, int& total) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -190,15 +232,10 @@ __global__ void __launch_bounds__(kMaskThreads)
         for (int q = tid; q < nsh; q += kMaskThreads) {
             const int s = m.lorder[q];
             const DevShell& sh = m.shells[s];
-            double best = 1.0e300;
-            int bj = 0;
-            for (int jj = 0; jj < count; ++jj) {
-                const double r2 = sq_norm(spts[4 * jj] - sh.o[0], spts[4 * jj + 1] - sh.o[1], spts[4 * jj + 2] - sh.o[2]);
-                if (r2 < best) {
-                    best = r2;
-                    bj = jj;
-                }
-            }
+            double best;
+            int bj;
+            const double c[3] = {sh.o[0], sh.o[1], sh.o[2]};
+            nearest_point(ps, start, count, spts, c, best, bj);
             int st;
             if (!(best <= sh.cutoff_r2) || best > screen_r2[s]) {
                 st = kDrop;
