@@ -192,7 +192,8 @@ __global__ void __launch_bounds__(kMaskThreads)
     constexpr unsigned PSET = ORDER == 0 ? kPanelsValue : (ORDER == 1 ? kPanelsGrad : kPanelsAll);
     constexpr int NP = popcount10(PSET);
     // LDS: smax[nao] (u64), pts[kMaskThreads][4] (double), then per AO block
-    // bits, state, entry, entry slot (int32), then rows used per chunk.
+    // bits, state, entry, entry slot (int32), then rows used per chunk, then
+    // the lists of the AO blocks left to phases B and C (int32, nshells each).
     extern __shared__ unsigned long long smax[];
     const int nao = m.nao, nsh = m.nshells;
     double* spts = reinterpret_cast<double*>(smax + nao);
@@ -201,8 +202,11 @@ __global__ void __launch_bounds__(kMaskThreads)
     int32_t* sent = sstate + nsh;
     int32_t* sslot = sent + nsh;
     int32_t* sused = sslot + nsh;
+    int32_t* lfull = sused + t.max_chunks + 1;
+    int32_t* lexact = lfull + nsh;
     __shared__ int scan_tmp[2];
     __shared__ int packed[2];
+    __shared__ int nfull, nexact;
 
     const int tid = threadIdx.x, lane = tid & 63;
     const long long blk = first_block + blockIdx.x;
@@ -211,6 +215,13 @@ __global__ void __launch_bounds__(kMaskThreads)
     const bool screen = threshold != 0.0 && count > 0;
     for (int a = tid; a < nao; a += kMaskThreads)
         smax[a] = 0ull;
+    if (tid == 0) nfull = nexact = 0;
+    // AO blocks whose maxima phases B and C evaluate (in any order: the
+    // maxima are order-independent)
+    const auto enlist = [&](int s, int st) __attribute__((always_inline)) {
+        if (st == kFull) lfull[atomicAdd(&nfull, 1)] = s;
+        else if (st == kExact) lexact[atomicAdd(&nexact, 1)] = s;
+    };
     const bool cached = count <= kMaskThreads; // points held in LDS
     if (cached && tid < count) {
         double px, py, pz;
@@ -223,8 +234,11 @@ __global__ void __launch_bounds__(kMaskThreads)
 
     if (screen && !cached) {
         // Blocks of more than kMaskThreads points skip the classification.
-        for (int s = tid; s < nsh; s += kMaskThreads)
-            sstate[s] = m.shells[s].l > 3 ? kExact : kFull;
+        for (int s = tid; s < nsh; s += kMaskThreads) {
+            const int st = m.shells[s].l > 3 ? kExact : kFull;
+            sstate[s] = st;
+            enlist(s, st);
+        }
         __syncthreads();
     }
     if (screen && cached) {
@@ -245,11 +259,13 @@ __global__ void __launch_bounds__(kMaskThreads)
                 st = mn > threshold * (1.0 + kWindow) ? kKeepAll : (sh.l > 3 ? kExact : kFull);
             }
             sstate[s] = st;
+            enlist(s, st);
         }
         __syncthreads();
     }
     if (screen) {
         // B. Fast maxima over the points of the undecided blocks.
+        const int nf = nfull;
         for (int p0 = 0; p0 < count; p0 += kMaskThreads) {
         const bool active = p0 + tid < count;
         double px = 0.0, py = 0.0, pz = 0.0;
@@ -262,9 +278,8 @@ __global__ void __launch_bounds__(kMaskThreads)
                 point_coords(ps, start + p0 + tid, px, py, pz);
             }
         }
-        for (int q = 0; q < nsh; ++q) {
-            const int s = m.lorder[q];
-            if (sstate[s] != kFull) continue;
+        for (int u = 0; u < nf; ++u) {
+            const int s = lfull[u];
             const DevShell& sh = m.shells[s];
             const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
             const double r2 = sq_norm(x, y, z);
@@ -297,23 +312,26 @@ __global__ void __launch_bounds__(kMaskThreads)
         __syncthreads();
         // C. Blocks with an AO near the threshold are decided by the reference
         //    arithmetic.
-        for (int s = tid; s < nsh; s += kMaskThreads) {
-            if (sstate[s] != kFull) continue;
+        for (int u = tid; u < nf; u += kMaskThreads) {
+            const int s = lfull[u];
             const DevShell& sh = m.shells[s];
             bool near = false;
             for (int r = 0; r < sh.count; ++r) {
                 const double v = __longlong_as_double(static_cast<long long>(smax[sh.ao + r]));
                 near = near || fabs(v - threshold) <= kWindow * threshold;
             }
-            if (near) sstate[s] = kExact;
+            if (near) {
+                sstate[s] = kExact;
+                enlist(s, kExact);
+            }
         }
         __syncthreads();
-        for (int s = tid; s < nsh; s += kMaskThreads)
-            if (sstate[s] == kExact) {
-                const DevShell& sh = m.shells[s];
-                for (int r = 0; r < sh.count; ++r)
-                    smax[sh.ao + r] = 0ull;
-            }
+        const int ne = nexact;
+        for (int u = tid; u < ne; u += kMaskThreads) {
+            const DevShell& sh = m.shells[lexact[u]];
+            for (int r = 0; r < sh.count; ++r)
+                smax[sh.ao + r] = 0ull;
+        }
         __syncthreads();
         for (int p0 = 0; p0 < count; p0 += kMaskThreads) {
         const bool active = p0 + tid < count;
@@ -327,9 +345,8 @@ __global__ void __launch_bounds__(kMaskThreads)
                 point_coords(ps, start + p0 + tid, px, py, pz);
             }
         }
-        for (int q = 0; q < nsh; ++q) {
-            const int s = m.lorder[q];
-            if (sstate[s] != kExact) continue;
+        for (int u = 0; u < ne; ++u) {
+            const int s = lexact[u];
             const DevShell& sh = m.shells[s];
             const double x = px - sh.o[0], y = py - sh.o[1], z = pz - sh.o[2];
             const double r2 = sq_norm(x, y, z);
@@ -537,7 +554,7 @@ void launch_masks(const MolView& m, const PointSource& ps, long long npoints, in
                   hipStream_t stream) {
     if (nblk <= 0) return;
     const size_t shmem = static_cast<size_t>(m.nao) * sizeof(unsigned long long) + 4 * kMaskThreads * sizeof(double) +
-                         static_cast<size_t>(m.nshells) * 4 * sizeof(int32_t) +
+                         static_cast<size_t>(m.nshells) * 6 * sizeof(int32_t) +
                          static_cast<size_t>(t.max_chunks + 1) * sizeof(int32_t);
     if (shmem > 64 * 1024) invalid("basis too large for the screening kernel (more than about 6000 AOs)");
     const dim3 grid(static_cast<unsigned>(nblk)), block(kMaskThreads);
