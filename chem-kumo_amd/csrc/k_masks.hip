@@ -78,8 +78,9 @@ __device__ __attribute__((always_inline)) inline void fast_block_max(const DevSh
     });
 }
 
-// Fast smallest (over the block's AOs) largest |panel| at one point, with l
-// a runtime value (lanes evaluate different blocks).
+// Fast smallest (over the block's AOs) largest |panel| at one point. Lanes
+// take AO blocks in l order, so a wave mostly shares l: l <= 3 uses the
+// compile-time evaluation, higher l a runtime loop.
 template <int ORDER, unsigned PSET>
 __device__ __attribute__((always_inline)) inline double probe_min_max(const DevShell& sh, const MolView& m, double x,
                                                                      double y, double z, double r2) {
@@ -87,6 +88,23 @@ __device__ __attribute__((always_inline)) inline double probe_min_max(const DevS
     constexpr int NP = popcount10(PSET);
     double R[3];
     radial_sums<ORDER>(sh, m, r2, R);
+    if (sh.l <= 3) {
+        double amax[kMaxCart];
+#pragma unroll
+        for (int r = 0; r < kMaxCart; ++r)
+            amax[r] = 0.0;
+        switch (sh.l) {
+        case 0: fast_block_max<0, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+        case 1: fast_block_max<1, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+        case 2: fast_block_max<2, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+        default: fast_block_max<3, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
+        }
+        double mn = 1.0e300;
+#pragma unroll
+        for (int r = 0; r < 10; ++r)
+            if (r < sh.count) mn = fmin(mn, amax[r]);
+        return mn;
+    }
     const int L = sh.l, nc = sh.ncart;
     double mn = 1.0e300;
     if (sh.pure) {
