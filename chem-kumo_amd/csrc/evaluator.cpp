@@ -81,6 +81,7 @@ TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
     s.t.chunk_rows = static_cast<int32_t*>(ptr[10]);
     s.t.nchunks = static_cast<int32_t*>(ptr[11]);
     s.t.max_chunks = max_chunks;
+    s.t.lists = 1;
     return s;
 }
 
@@ -410,7 +411,9 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
             const long long first = b0 * bs;
             const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
-            launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, mol->screen_r2(thr, s), ts.t, s);
+            KeptTables kt = ts.t;
+            kt.lists = backend == KUMO_HIP_AO_MO ? 0 : 1;
+            launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, mol->screen_r2(thr, s), kt, s);
             if (backend == KUMO_HIP_AO_MO) {
                 AoMoLaunch a{};
                 a.mol = mv;

@@ -78,31 +78,39 @@ __device__ __attribute__((always_inline)) inline void fast_block_max(const DevSh
     });
 }
 
-// Fast smallest (over the block's AOs) largest |panel| at one point. Lanes
-// take AO blocks in l order, so a wave mostly shares l: l <= 3 uses the
-// compile-time evaluation, higher l a runtime loop.
+// Fast smallest (over the block's AOs) largest |panel| at one point, or any
+// value above `pass` not above it (the caller only asks whether it exceeds
+// pass): the value panel alone is tried first, since an AO whose value
+// already exceeds pass exceeds it over all panels. Lanes take AO blocks in l
+// order, so a wave mostly shares l: l <= 3 uses the compile-time evaluation,
+// higher l a runtime loop.
 template <int ORDER, unsigned PSET>
 __device__ __attribute__((always_inline)) inline double probe_min_max(const DevShell& sh, const MolView& m, double x,
-                                                                     double y, double z, double r2) {
+                                                                     double y, double z, double r2, double pass) {
 #pragma clang fp contract(fast)
     constexpr int NP = popcount10(PSET);
     double R[3];
     radial_sums<ORDER>(sh, m, r2, R);
     if (sh.l <= 3) {
-        double amax[kMaxCart];
-#pragma unroll
-        for (int r = 0; r < kMaxCart; ++r)
-            amax[r] = 0.0;
-        switch (sh.l) {
-        case 0: fast_block_max<0, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
-        case 1: fast_block_max<1, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
-        case 2: fast_block_max<2, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
-        default: fast_block_max<3, ORDER, PSET>(sh, m, x, y, z, R, amax); break;
-        }
         double mn = 1.0e300;
+        const auto probe = [&]<unsigned PS>() __attribute__((always_inline)) {
+            double amax[kMaxCart];
 #pragma unroll
-        for (int r = 0; r < 10; ++r)
-            if (r < sh.count) mn = fmin(mn, amax[r]);
+            for (int r = 0; r < kMaxCart; ++r)
+                amax[r] = 0.0;
+            switch (sh.l) {
+            case 0: fast_block_max<0, ORDER, PS>(sh, m, x, y, z, R, amax); break;
+            case 1: fast_block_max<1, ORDER, PS>(sh, m, x, y, z, R, amax); break;
+            case 2: fast_block_max<2, ORDER, PS>(sh, m, x, y, z, R, amax); break;
+            default: fast_block_max<3, ORDER, PS>(sh, m, x, y, z, R, amax); break;
+            }
+            mn = 1.0e300;
+#pragma unroll
+            for (int r = 0; r < 10; ++r)
+                if (r < sh.count) mn = fmin(mn, amax[r]);
+        };
+        probe.template operator()<kPanelsValue>();
+        if (PSET != kPanelsValue && !(mn > pass)) probe.template operator()<PSET>();
         return mn;
     }
     const int L = sh.l, nc = sh.ncart;
@@ -273,8 +281,9 @@ __global__ void __launch_bounds__(kMaskThreads)
                 st = kDrop;
             } else {
                 const double x = spts[4 * bj] - sh.o[0], y = spts[4 * bj + 1] - sh.o[1], z = spts[4 * bj + 2] - sh.o[2];
-                const double mn = probe_min_max<ORDER, PSET>(sh, m, x, y, z, best);
-                st = mn > threshold * (1.0 + kWindow) ? kKeepAll : (sh.l > 3 ? kExact : kFull);
+                const double pass = threshold * (1.0 + kWindow);
+                const double mn = probe_min_max<ORDER, PSET>(sh, m, x, y, z, best, pass);
+                st = mn > pass ? kKeepAll : (sh.l > 3 ? kExact : kFull);
             }
             sstate[s] = st;
             enlist(s, st);
@@ -415,37 +424,49 @@ __global__ void __launch_bounds__(kMaskThreads)
         sbits[s] = bits;
     }
     __syncthreads();
-    // Flags, the ascending kept AO list and the kept blocks in AO order.
-    uint8_t* kept = t.kept + static_cast<long long>(blockIdx.x) * nao;
-    int32_t* list = t.kept_list + static_cast<long long>(blockIdx.x) * nao;
-    int32_t* ks = t.kept_shell + static_cast<long long>(blockIdx.x) * nsh;
-    uint32_t* kb = t.kept_bits + static_cast<long long>(blockIdx.x) * nsh;
-    int aos = 0, blocks = 0;
-    for (int s0 = 0; s0 < nsh; s0 += kMaskThreads) {
-        const int s = s0 + tid;
-        const uint32_t bits = s < nsh ? sbits[s] : 0u;
-        int tot_a, tot_b;
-        const int pa = aos + block_scan(__popc(bits), scan_tmp, tot_a);
-        const int pb = blocks + block_scan(bits ? 1 : 0, scan_tmp, tot_b);
-        if (s < nsh) {
-            const DevShell& sh = m.shells[s];
-            int w = pa;
-            for (int r = 0; r < sh.count; ++r) {
-                const bool k = (bits >> r) & 1u;
-                kept[sh.ao + r] = k ? 1 : 0;
-                if (k) list[w++] = sh.ao + r;
+    if (!t.lists) {
+        // The AO->MO plan needs only the kept AO count.
+        if (tid == 0) packed[0] = 0;
+        __syncthreads();
+        int cnt = 0;
+        for (int s = tid; s < nsh; s += kMaskThreads)
+            cnt += __popc(sbits[s]);
+        if (cnt) atomicAdd(&packed[0], cnt);
+        __syncthreads();
+        if (tid == 0) t.nkept[blockIdx.x] = packed[0];
+    } else {
+        // Flags, the ascending kept AO list and the kept blocks in AO order.
+        uint8_t* kept = t.kept + static_cast<long long>(blockIdx.x) * nao;
+        int32_t* list = t.kept_list + static_cast<long long>(blockIdx.x) * nao;
+        int32_t* ks = t.kept_shell + static_cast<long long>(blockIdx.x) * nsh;
+        uint32_t* kb = t.kept_bits + static_cast<long long>(blockIdx.x) * nsh;
+        int aos = 0, blocks = 0;
+        for (int s0 = 0; s0 < nsh; s0 += kMaskThreads) {
+            const int s = s0 + tid;
+            const uint32_t bits = s < nsh ? sbits[s] : 0u;
+            int tot_a, tot_b;
+            const int pa = aos + block_scan(__popc(bits), scan_tmp, tot_a);
+            const int pb = blocks + block_scan(bits ? 1 : 0, scan_tmp, tot_b);
+            if (s < nsh) {
+                const DevShell& sh = m.shells[s];
+                int w = pa;
+                for (int r = 0; r < sh.count; ++r) {
+                    const bool k = (bits >> r) & 1u;
+                    kept[sh.ao + r] = k ? 1 : 0;
+                    if (k) list[w++] = sh.ao + r;
+                }
+                if (bits) {
+                    ks[pb] = s;
+                    kb[pb] = bits;
+                }
             }
-            if (bits) {
-                ks[pb] = s;
-                kb[pb] = bits;
-            }
+            aos += tot_a;
+            blocks += tot_b;
         }
-        aos += tot_a;
-        blocks += tot_b;
-    }
-    if (tid == 0) {
-        t.nkept[blockIdx.x] = aos;
-        t.nkshell[blockIdx.x] = blocks;
+        if (tid == 0) {
+            t.nkept[blockIdx.x] = aos;
+            t.nkshell[blockIdx.x] = blocks;
+        }
     }
     // AO->MO entries: kept blocks in descending l.
     int nent = 0;
@@ -465,28 +486,71 @@ __global__ void __launch_bounds__(kMaskThreads)
         nent += tot;
     }
     __syncthreads();
-    // Next-fit packing of the entries into chunks of at most kChunkRows rows.
+    // Next-fit packing of the entries into chunks of at most kChunkRows rows
+    // (each entry has at most kChunkRows rows). With S the exclusive prefix
+    // sum of the entries' rows, a chunk starting at entry q ends before the
+    // first entry i - 1 with S[i] > S[q] + kChunkRows: the chunk starts are
+    // the orbit of 0 under that map, walked by one lane; every entry then
+    // finds its chunk by bisection. (smax, lfull and lexact are free here.)
     int32_t* cb = t.chunk_begin + static_cast<long long>(blockIdx.x) * (t.max_chunks + 1);
-    if (tid == 0) {
-        int nc = 0, rows = kChunkRows, q = 0; // forces a new chunk at the first entry
-        for (; q < nent; ++q) {
-            const int n = sslot[q];
-            if (rows + n > kChunkRows) {
-                if (nc > 0) sused[nc - 1] = rows;
-                if (nc == t.max_chunks) break; // not reached: max_chunks bounds every block
-                cb[nc++] = q;
-                rows = 0;
-            }
-            sslot[q] = ((nc - 1) << 8) | rows;
-            rows += n;
+    int32_t* spre = lfull;                                  // S[q], q < nent
+    int32_t* snext = lexact;                                // next chunk start after a chunk starting at q
+    int32_t* cstart = reinterpret_cast<int32_t*>(smax);     // chunk starts
+    __shared__ int stotal;
+    {
+        int base = 0;
+        for (int q0 = 0; q0 < nent; q0 += kMaskThreads) {
+            const int q = q0 + tid;
+            const int rows = q < nent ? sslot[q] : 0;
+            int tot;
+            const int pre = base + block_scan(rows, scan_tmp, tot);
+            if (q < nent) spre[q] = pre;
+            base += tot;
         }
-        if (nc > 0) sused[nc - 1] = rows;
-        cb[nc] = q;
-        t.nchunks[blockIdx.x] = nc;
-        packed[0] = nc;
-        packed[1] = q;
+        if (tid == 0) stotal = base;
+        __syncthreads();
+        const int total = stotal;
+        for (int q = tid; q < nent; q += kMaskThreads) {
+            const int v = spre[q] + kChunkRows;
+            // first i in [q + 1, nent] with S[i] > v (S[nent] = total)
+            int lo = q + 1, hi = nent;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (spre[mid] > v) hi = mid;
+                else lo = mid + 1;
+            }
+            snext[q] = (lo == nent && !(total > v)) ? nent : lo - 1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int nc = 0, q = 0;
+            while (q < nent) {
+                if (nc == t.max_chunks) break; // not reached: max_chunks bounds every block
+                cstart[nc] = q;
+                cb[nc] = q;
+                const int nq = snext[q];
+                sused[nc] = (nq < nent ? spre[nq] : total) - spre[q];
+                ++nc;
+                q = nq;
+            }
+            cb[nc] = q;
+            t.nchunks[blockIdx.x] = nc;
+            packed[0] = nc;
+            packed[1] = q;
+        }
+        __syncthreads();
+        const int nc = packed[0], qend = packed[1];
+        for (int q = tid; q < qend; q += kMaskThreads) {
+            int lo = 0, hi = nc - 1; // last chunk start <= q
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (cstart[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            sslot[q] = (lo << 16) | ((q - cstart[lo]) << 8) | (spre[q] - spre[cstart[lo]]);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     const int nc = packed[0];
     nent = packed[1];
     int32_t* en = t.ent + static_cast<long long>(blockIdx.x) * nsh;
@@ -496,7 +560,7 @@ __global__ void __launch_bounds__(kMaskThreads)
     for (int q = tid; q < nent; q += kMaskThreads) {
         const int e = sent[q];
         const int s = e & 0xffffff, mode = e >> 24;
-        const int c = sslot[q] >> 8, row0 = sslot[q] & 0xff;
+        const int c = sslot[q] >> 16, row0 = sslot[q] & 0xff;
         const DevShell& sh = m.shells[s];
         const uint32_t bits = sbits[s];
         en[q] = e;

@@ -46,6 +46,9 @@ struct KeptTables {
     int32_t* chunk_rows;
     int32_t* nchunks;       // [nblk]
     int max_chunks;
+    // 1: also write kept, kept_list, kept_shell, kept_bits and nkshell (the
+    // AO-P path and the masks entry point); 0: the AO->MO plan and nkept only.
+    int lists;
 };
 
 // Upper bound on the chunks of any screening block of a basis: next-fit
