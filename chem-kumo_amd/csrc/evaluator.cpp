@@ -401,10 +401,17 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             if (t.kind == kElf) elf_which |= t.component == kDifference ? 1 : bits(t.component);
             if (t.kind == kDensity) rho_which |= bits(t.component);
         }
-        int32_t rescue_tiles = 0;
         FieldTask* dtasks = grow<FieldTask>(ex.ws_tasks, tasks.size());
         upload_async(ex, dtasks, tasks.data(), tasks.size());
-        std::vector<int32_t> nkept(static_cast<std::size_t>(cap));
+        // Per batch: kept AO counts of its blocks, then its rescue tiles,
+        // copied back asynchronously; read after one synchronisation at the
+        // end (or after each batch when a progress callback must see
+        // completed work).
+        const long long nbatches = (b_end - b_first + cap - 1) / cap;
+        const std::size_t slot = static_cast<std::size_t>(cap) + 1;
+        int32_t* hst = ex.capture ? nullptr : ex.host_stats(static_cast<std::size_t>(nbatches) * slot);
+        if (hst) std::fill(hst, hst + static_cast<std::size_t>(nbatches) * slot, 0);
+        const bool stepwise = opt.progress != nullptr;
         double flops = 0.0;
         const int panels = ao_order <= 0 ? 1 : (ao_order == 1 ? 4 : (full_h ? 10 : (diag_h ? 7 : 5)));
         for (long long b0 = b_first; b0 < b_end; b0 += cap) {
@@ -472,7 +479,9 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                     launch_ao_mo(r, s);
                     record(ex, false, 1);
                     launch_rescue_scatter(r.moments, rslots, groups, kRescueTile, bs, rp, chm, count, s);
-                    if (!ex.capture) KUMO_HIP(hipMemcpyAsync(&rescue_tiles, rp.total, 4, hipMemcpyDeviceToHost, s));
+                    if (hst)
+                        KUMO_HIP(hipMemcpyAsync(hst + static_cast<std::size_t>((b0 - b_first) / cap) * slot + cap,
+                                                rp.total, 4, hipMemcpyDeviceToHost, s));
                 }
             } else {
                 AoPLaunch a{};
@@ -519,21 +528,29 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 advance(static_cast<std::size_t>(count));
                 continue;
             }
-            KUMO_HIP(hipMemcpyAsync(nkept.data(), ts.t.nkept, static_cast<std::size_t>(nb) * 4, hipMemcpyDeviceToHost, s));
-            KUMO_HIP(hipStreamSynchronize(s));
-            ex.last_rescue_points += static_cast<long long>(rescue_tiles) * kRescueTile;
-            rescue_tiles = 0;
-            for (int b = 0; b < nb; ++b) {
-                const long long pts_b = std::min<long long>(bs, n - (b0 + b) * static_cast<long long>(bs));
-                const double m_b = nkept[static_cast<std::size_t>(b)];
-                if (backend == KUMO_HIP_AO_MO) {
-                    const double mos = (need_a || closed ? mol->nalpha : 0) + (need_b && !closed ? mol->nbeta : 0);
-                    flops += 2.0 * m_b * mos * panels * static_cast<double>(pts_b);
-                } else {
-                    flops += 2.0 * m_b * m_b * (ao_order == 0 ? 1 : 4) * static_cast<double>(pts_b);
+            int32_t* hb = hst + static_cast<std::size_t>((b0 - b_first) / cap) * slot;
+            KUMO_HIP(hipMemcpyAsync(hb, ts.t.nkept, static_cast<std::size_t>(nb) * 4, hipMemcpyDeviceToHost, s));
+            if (stepwise) sync(ex);
+            advance(static_cast<std::size_t>(count));
+        }
+        if (hst) {
+            sync(ex);
+            for (long long bb = 0; bb < nbatches; ++bb) {
+                const int32_t* hb = hst + static_cast<std::size_t>(bb) * slot;
+                const long long b0 = b_first + bb * cap;
+                const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
+                ex.last_rescue_points += static_cast<long long>(hb[cap]) * kRescueTile;
+                for (int b = 0; b < nb; ++b) {
+                    const long long pts_b = std::min<long long>(bs, n - (b0 + b) * static_cast<long long>(bs));
+                    const double m_b = hb[b];
+                    if (backend == KUMO_HIP_AO_MO) {
+                        const double mos = (need_a || closed ? mol->nalpha : 0) + (need_b && !closed ? mol->nbeta : 0);
+                        flops += 2.0 * m_b * mos * panels * static_cast<double>(pts_b);
+                    } else {
+                        flops += 2.0 * m_b * m_b * (ao_order == 0 ? 1 : 4) * static_cast<double>(pts_b);
+                    }
                 }
             }
-            advance(static_cast<std::size_t>(count));
         }
         ex.last_flops += flops;
     }
@@ -763,6 +780,21 @@ void Executor::destroy() {
         b->release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
+    if (hstats) (void)hipHostFree(hstats);
+    hstats = nullptr;
+    hstats_count = 0;
+}
+
+int32_t* Executor::host_stats(std::size_t count) {
+    if (count > hstats_count) {
+        // (called between evaluations: no copy into the old buffer is pending)
+        if (hstats) (void)hipHostFree(hstats);
+        hstats = nullptr;
+        hstats_count = 0;
+        KUMO_HIP(hipHostMalloc(reinterpret_cast<void**>(&hstats), count * sizeof(int32_t), hipHostMallocDefault));
+        hstats_count = count;
+    }
+    return hstats;
 }
 
 void Executor::reset_stats() {

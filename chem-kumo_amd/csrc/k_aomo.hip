@@ -74,7 +74,6 @@ template <> struct Prec<KUMO_HIP_FP32> {
     using E = float;
     using Acc = f32x4;
     static constexpr int S = 1;
-    static constexpr int T = 32;
     static constexpr int LDK = 36;
     static constexpr int PITCH = 132;
 };
@@ -82,7 +81,6 @@ template <int NSPLIT> struct PrecBf16 {
     using E = uint16_t;
     using Acc = f32x4;
     static constexpr int S = NSPLIT;
-    static constexpr int T = 32;
     static constexpr int LDK = 32;    // swizzled 8-row granules
     static constexpr int PITCH = 132; // 32-bit words per coefficient row pair
 };
@@ -221,7 +219,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     const int G = width / EPG;                             // granules per coefficient row
 
     for (int c = 0; c < nchunks; ++c) {
-        const int e0 = cb[c], e1 = cb[c + 1];
+        const int e1 = cb[c + 1];
         int rows_used;
         {
             const int last = ent[e1 - 1];

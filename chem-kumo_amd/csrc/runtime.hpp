@@ -33,6 +33,11 @@ struct Executor {
     std::size_t events_used[3] = {0, 0, 0};
     double last_kernel_ms = 0.0, last_flops = 0.0, last_rescue_ms = 0.0, last_fields_ms = 0.0;
     long long last_launches = 0, last_rescue_points = 0, last_fields_points = 0, last_fields_bytes = 0;
+    // Pinned host memory for per-batch statistics copied back without
+    // stopping the stream (kept AO counts, rescue tiles).
+    int32_t* hstats = nullptr;
+    std::size_t hstats_count = 0;
+    int32_t* host_stats(std::size_t count);
     // Capture mode (a batch recorded into a HIP graph): no host
     // synchronisation, no timing events, no readback; small host arrays
     // copied to the device are kept in `keep` so that graph replays read them.
