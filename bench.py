@@ -68,6 +68,58 @@ def run_gpu(mol, grid, precision, steps, warmup, torch):
 CHANNELS = ["rho", "grad_x", "grad_y", "grad_z", "lap", "tau", "elf", "lol"]
 
 
+def bench_cube(ctx, mol, grid, out, no_cpu):
+    """Cube files of the scalar fields of the step (rho, lap, tau, ELF, LOL),
+    written from the device outputs (Field::write_cube, src/cube.cpp:70-111):
+    wall time per file, device time of the text kernels and copies; the
+    oracle's iostream writer (one thread) on a bounded sample beside it."""
+    import shutil
+    import tempfile
+    atoms = mol.atoms()
+    tmp = tempfile.mkdtemp(prefix="kumo_cube_")
+    try:
+        files = []
+        for name, c in (("density", 0), ("laplacian", 4), ("tau", 5), ("elf", 6), ("lol", 7)):
+            path = os.path.join(tmp, f"bench_{name}.cube")
+            ctx.write_cube(path, grid, atoms, out[c])
+            st = ctx.cube_stats()
+            files.append(st)
+            os.remove(path)
+        n = files[0]["values"]
+        wall = sum(f["total_ms"] for f in files) / len(files)
+        kern = sum(f["kernel_ms"] for f in files) / len(files)
+        copy = sum(f["copy_ms"] for f in files) / len(files)
+        nbytes = sum(f["bytes"] for f in files) / len(files)
+        hbm = 8.0 * n + nbytes  # samples read, text written
+        res = {"files": len(files), "values_per_file": n, "bytes_per_file": nbytes, "ms_per_file": wall,
+               "values_per_s": n / (wall * 1e-3), "text_kernel_ms": kern, "copy_ms": copy,
+               "text_kernel": {"bound": "hbm", "achieved": hbm / (kern * 1e-3) / 1e9, "peak": 8000.0,
+                               "unit": "GB/s", "frac": hbm / (kern * 1e-3) / 8e12,
+                               "bytes": "8 B per sample read + the text written"}}
+        log(f"Cube writer: {wall:.1f} ms per {n}-sample file ({nbytes / 1e6:.1f} MB), text kernels {kern:.2f} ms, "
+            f"copies {copy:.2f} ms")
+        if not no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from support import oracle
+            counts = np.array([64, 160, 160])
+            m = int(np.prod(counts))
+            sample = {"origin": grid["origin"], "steps": grid["steps"], "counts": counts}
+            vals = out[0][:m].cpu().numpy()
+            path = os.path.join(tmp, "cpu.cube")
+            t = time.perf_counter()
+            rc, msg = oracle.write_cube(path, sample, atoms, vals)
+            dt = time.perf_counter() - t
+            assert rc == 0, msg
+            res["cpu_baseline"] = {"value": m / dt, "unit": "values/s", "cores": 1, "kind": "port",
+                                   "sample": f"the first {m} density samples (64 x 160 x 160 slab) written by "
+                                             "the oracle's iostream restatement of the writer"}
+            res["speedup_vs_cpu"] = (n / (wall * 1e-3)) / (m / dt)
+            log(f"CPU Cube writer: {m / dt:.4g} values/s")
+        return res
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def full_grid_errors(got, ref, torch):
     """Per output channel: max abs error and max abs error relative to the
     channel's largest |value|, rho relative errors above density floors, and
@@ -327,6 +379,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-blocks", type=int, default=1024, help="screening blocks in the CPU sample")
+    ap.add_argument("--no-cube", action="store_true", help="skip the Cube-writer timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
@@ -423,6 +476,8 @@ def main():
             errors["elf_points_off_by_1e-5"] = int(off.sum())
             errors["elf_off_max_rho"] = float(rho[off].max()) if off.any() else None
 
+    cube = None if args.no_cube else bench_cube(ctx, mol, grid, main_run["out"], args.no_cpu)
+
     result = {
         "metric": "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error; p50/p99 latency",
         "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -447,6 +502,7 @@ def main():
             "frac": main_run["fields_bytes"] / (main_run["fields_ms"] * 1e-3) / 8e12 if main_run["fields_ms"] else None,
             "bytes": "channel moments read (2 x 13 x 8 B per point) + output channels written (8 x 8 B per point)"},
         "cpu_baseline": cpu,
+        "cube_writer": cube,
     }
     print(json.dumps(result))
     mol.close()

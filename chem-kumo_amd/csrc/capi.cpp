@@ -68,6 +68,7 @@ int kumo_hip_init(int device, kumo_hip_context** out) {
 int kumo_hip_release(kumo_hip_context* ctx) {
     if (!ctx) return KUMO_HIP_OK;
     ctx->exec.destroy();
+    ctx->cube.release();
     ctx->ws_out.release();
     ctx->boys_table.release();
     delete ctx;
@@ -249,6 +250,46 @@ int kumo_hip_last_field_stats(const kumo_hip_context* ctx, double* fields_ms, in
     if (fields_ms) *fields_ms = ctx->exec.last_fields_ms;
     if (points) *points = ctx->exec.last_fields_points;
     if (bytes) *bytes = ctx->exec.last_fields_bytes;
+    return KUMO_HIP_OK;
+}
+
+int kumo_hip_write_cube(kumo_hip_context* ctx, const char* path, const char* comment, const kumo_hip_grid* grid,
+                        int32_t natoms, const int32_t* atomic_numbers, const double* nuclear_charges,
+                        const double* coordinates, const double* values, int32_t values_on_device) {
+    if (!ctx || !grid) return KUMO_HIP_INVALID_ARGUMENT;
+    return guarded(ctx, [&] {
+        kumo::write_field_cube(ctx, path, comment, *grid, {natoms, atomic_numbers, nuclear_charges, coordinates},
+                               values, values_on_device != 0, 0);
+    });
+}
+
+int kumo_hip_write_orbital_cube(kumo_hip_context* ctx, const char* path, const char* comment,
+                                const kumo_hip_grid* grid, int32_t natoms, const int32_t* atomic_numbers,
+                                const double* nuclear_charges, const double* coordinates, const double* values,
+                                int32_t values_on_device, size_t orbital_index) {
+    if (!ctx || !grid) return KUMO_HIP_INVALID_ARGUMENT;
+    return guarded(ctx, [&] {
+        // Field::write_orbital_cube (src/cube.cpp:257-262).
+        if (orbital_index == 0 || orbital_index > 9999) kumo::invalid("invalid orbital index for Cube");
+        kumo::write_field_cube(ctx, path, comment, *grid, {natoms, atomic_numbers, nuclear_charges, coordinates},
+                               values, values_on_device != 0, orbital_index);
+    });
+}
+
+int kumo_hip_format_scientific(kumo_hip_context* ctx, const double* values, size_t n, int32_t precision,
+                               int32_t exact_only, char* text) {
+    if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
+    return guarded(ctx, [&] { kumo::format_scientific(ctx, values, n, precision, exact_only != 0, text); });
+}
+
+int kumo_hip_last_cube_stats(const kumo_hip_context* ctx, double* total_ms, double* kernel_ms, double* copy_ms,
+                             int64_t* values, int64_t* bytes) {
+    if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
+    if (total_ms) *total_ms = ctx->cube.last_ms;
+    if (kernel_ms) *kernel_ms = ctx->cube.last_kernel_ms;
+    if (copy_ms) *copy_ms = ctx->cube.last_copy_ms;
+    if (values) *values = ctx->cube.last_values;
+    if (bytes) *bytes = ctx->cube.last_bytes;
     return KUMO_HIP_OK;
 }
 

@@ -191,6 +191,21 @@ void launch_fields(const FieldsLaunch& f, hipStream_t stream);
 void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream);
 void launch_fill(double* dst, long long n, double value, hipStream_t stream);
 
+// Cube sample text (Field::write_cube, src/cube.cpp:104-109): values
+// [first, first + count) of a field whose third grid axis has nz points,
+// written as their printf("%13.5e") fields with the line breaks of the file
+// to text, which receives the bytes from offset cube_text_offset(first, nz)
+// of the sample section on.
+long long cube_text_offset(long long i, long long nz);
+void launch_cube_text(const double* values, long long first, long long count, long long nz, long long text_first,
+                      unsigned char* text, hipStream_t stream);
+// printf("%*.*e", P + 8, P, v) in P + 8 bytes per value (0 <= P <= 12);
+// exact_only: every rounding decided by exact big-integer comparison.
+void launch_sci_slots(const double* values, long long n, int precision, int exact_only, char* out,
+                      hipStream_t stream);
+// *bad = number of infinite or NaN values.
+void launch_count_nonfinite(const double* values, long long n, int* bad, hipStream_t stream);
+
 // MO amplitudes (and derivatives) of selected MOs at points:
 // out[mo][component][n] for components value (order 0) or value, gradient,
 // Hessian (xx, xy, xz, yy, yz, zz).

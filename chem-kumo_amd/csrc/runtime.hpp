@@ -57,6 +57,24 @@ struct AtomRec {
 
 } // namespace kumo
 
+namespace kumo {
+// Buffers of the Cube writer: the device copy of host samples, the text of
+// one chunk in HBM and two pinned host buffers (one being written to the
+// file while the next chunk is formatted and copied).
+struct CubeBuffers {
+    DeviceBuffer values, text, flag;
+    char* host[2] = {nullptr, nullptr};
+    std::size_t host_bytes = 0;
+    // Per host buffer: before the text kernel, after it, after the copy.
+    hipEvent_t start[2] = {nullptr, nullptr};
+    hipEvent_t kernel[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    double last_ms = 0.0, last_kernel_ms = 0.0, last_copy_ms = 0.0;
+    long long last_bytes = 0, last_values = 0;
+    void release();
+};
+} // namespace kumo
+
 struct kumo_hip_context {
     int device = 0;
     std::string last_error;
@@ -64,6 +82,7 @@ struct kumo_hip_context {
     kumo::DeviceBuffer boys_table;
     kumo::Executor exec;
     kumo::DeviceBuffer ws_out;
+    kumo::CubeBuffers cube;
 };
 
 struct kumo_hip_molecule {
@@ -149,5 +168,19 @@ void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsA
 
 void screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsArg& pts, int block_size,
                      int order, double threshold, uint8_t* kept);
+
+// Field::write_cube / Field::write_orbital_cube (src/cube.cpp:70-111, 253-262);
+// orbital_index 0 writes a density Cube.
+struct CubeAtoms {
+    int natoms = 0;
+    const int32_t* z = nullptr;
+    const double* charges = nullptr;
+    const double* xyz = nullptr;
+};
+void write_field_cube(kumo_hip_context* ctx, const char* path, const char* comment, const kumo_hip_grid& grid,
+                      const CubeAtoms& atoms, const double* values, bool values_on_device, std::size_t orbital_index);
+// printf("%*.*e", precision + 8, precision, v) of host values into host text.
+void format_scientific(kumo_hip_context* ctx, const double* values, std::size_t n, int precision, bool exact_only,
+                       char* text);
 
 } // namespace kumo

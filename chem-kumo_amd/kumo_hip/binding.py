@@ -67,7 +67,9 @@ EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip
            "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
            "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_last_field_stats", "kumo_hip_batch_create",
            "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_release", "kumo_hip_server_create",
-           "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release"]
+           "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release",
+           "kumo_hip_write_cube", "kumo_hip_write_orbital_cube", "kumo_hip_format_scientific",
+           "kumo_hip_last_cube_stats"]
 
 
 def lib():
@@ -77,6 +79,14 @@ def lib():
         if not os.path.exists(LIBRARY):
             raise RuntimeError(f"{LIBRARY} is missing: build it with `make -C chem-kumo_amd` "
                                "(the HIP backend has no CPU fallback)")
+        # PyTorch-ROCm ships its own HIP runtime under the soname of the
+        # system one (libamdhip64.so.7); the first loaded serves the whole
+        # process. Loading torch first lets device tensors and this library
+        # share one runtime in any import order.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIBRARY)
         vp = C.c_void_p
         L.kumo_hip_init.argtypes = [C.c_int, C.POINTER(vp)]
@@ -108,6 +118,12 @@ def lib():
         L.kumo_hip_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(_RequestInfo)]
         L.kumo_hip_server_query.argtypes = [vp, C.c_uint64, vp]
         L.kumo_hip_server_release.argtypes = [vp]
+        L.kumo_hip_write_cube.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(_Grid), C.c_int32, vp, vp, vp, vp,
+                                          C.c_int32]
+        L.kumo_hip_write_orbital_cube.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(_Grid), C.c_int32, vp, vp,
+                                                  vp, vp, C.c_int32, C.c_size_t]
+        L.kumo_hip_format_scientific.argtypes = [vp, vp, C.c_size_t, C.c_int32, C.c_int32, vp]
+        L.kumo_hip_last_cube_stats.argtypes = [vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -153,6 +169,49 @@ class Context:
         return {"kernel_ms": ms.value, "launches": n.value, "flops": fl.value, "padded_flops": pfl.value,
                 "rescue_ms": rms.value, "rescue_points": rpts.value, "fields_ms": fms.value,
                 "fields_points": fpts.value, "fields_bytes": fbytes.value}
+
+
+    def write_cube(self, path, grid, atoms, values, comment=None, orbital_index=None):
+        """Field::write_cube / write_orbital_cube (src/cube.cpp:70-111, 253-262).
+        atoms: (atomic numbers, nuclear charges or None, coordinates natoms x 3);
+        values: a float64 array in Field order (host), or a device tensor /
+        integer device pointer (kumo_hip_write_cube with values_on_device)."""
+        z = np.ascontiguousarray(atoms[0], dtype=np.int32).ravel()
+        q = None if atoms[1] is None else np.ascontiguousarray(atoms[1], dtype=np.float64).ravel()
+        xyz = np.ascontiguousarray(atoms[2], dtype=np.float64).reshape(-1)
+        if isinstance(values, np.ndarray) or isinstance(values, (list, tuple)):
+            v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+            ptr, on_device, keep = _p(v), 0, v
+        else:
+            ptr = C.c_void_p(values if isinstance(values, int) else values.data_ptr())
+            on_device, keep = 1, values
+        g = grid_struct(grid)
+        c = None if comment is None else comment.encode()
+        p = os.fsencode(path)
+        if orbital_index is None:
+            rc = lib().kumo_hip_write_cube(self.h, p, c, C.byref(g), z.size, _p(z), _p(q), _p(xyz), ptr, on_device)
+        else:
+            rc = lib().kumo_hip_write_orbital_cube(self.h, p, c, C.byref(g), z.size, _p(z), _p(q), _p(xyz), ptr,
+                                                   on_device, int(orbital_index))
+        del keep
+        self.check(rc)
+
+    def format_scientific(self, values, precision, exact_only=False):
+        """The GPU formatter: printf("%*.*e", precision + 8, precision, v) per value."""
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        width = precision + 8
+        out = np.zeros(v.size * width, dtype=np.uint8)
+        self.check(lib().kumo_hip_format_scientific(self.h, _p(v), v.size, precision, 1 if exact_only else 0,
+                                                    _p(out)))
+        text = out.tobytes().decode()
+        return [text[i * width:(i + 1) * width] for i in range(v.size)]
+
+    def cube_stats(self):
+        t, k, c = C.c_double(), C.c_double(), C.c_double()
+        n, b = C.c_int64(), C.c_int64()
+        lib().kumo_hip_last_cube_stats(self.h, C.byref(t), C.byref(k), C.byref(c), C.byref(n), C.byref(b))
+        return {"total_ms": t.value, "kernel_ms": k.value, "copy_ms": c.value, "values": n.value,
+                "bytes": b.value}
 
 
 def grid_struct(grid):
@@ -242,6 +301,15 @@ class Molecule:
         if self.h:
             lib().kumo_hip_molecule_release(self.h)
             self.h = C.c_void_p()
+
+    def atoms(self):
+        """(atomic numbers, nuclear charges, coordinates natoms x 3), as a Field keeps them."""
+        n = self.info["natoms"]
+        z = np.zeros(n, dtype=np.int32)
+        q = np.zeros(n)
+        xyz = np.zeros((n, 3))
+        self.ctx.check(lib().kumo_hip_molecule_atoms(self.h, _p(z), _p(q), _p(xyz)))
+        return z, q, xyz
 
     def blocks(self):
         nb = self.info["nblocks"]

@@ -247,6 +247,38 @@ int kumo_hip_server_query(kumo_hip_server* server, uint64_t request, int32_t* do
 /* Finishes the requests in flight, then stops the workers. */
 int kumo_hip_server_release(kumo_hip_server* server);
 
+/* ---- Cube output ----
+ * Field::write_cube (include/chem/kumo/density.hpp:116-121; src/cube.cpp:70-111,
+ * 253-255): a Gaussian Cube file in bohr, "chem-kumo 2.1.0" as the first line,
+ * `comment` (NULL: "Electron density") as the second, the grid and the atoms
+ * (nuclear_charges NULL: the atomic numbers), then the npoints samples in
+ * Field order (k fastest) as printf("%13.5e") fields, six per line and a line
+ * break after each run along the third axis. The bytes are those of the
+ * reference's writer; the samples are formatted on the GPU. values holds the
+ * samples in host memory, or in device memory (HBM) when values_on_device is
+ * nonzero (e.g. an evaluation's device outputs). A nonfinite sample gives
+ * INVALID_ARGUMENT before the file is opened; a file that cannot be opened or
+ * written gives RUNTIME_ERROR. */
+int kumo_hip_write_cube(kumo_hip_context* ctx, const char* path, const char* comment, const kumo_hip_grid* grid,
+                        int32_t natoms, const int32_t* atomic_numbers, const double* nuclear_charges,
+                        const double* coordinates, const double* values, int32_t values_on_device);
+/* Field::write_orbital_cube (include/chem/kumo/density.hpp:122-128; src/cube.cpp:257-262):
+ * as kumo_hip_write_cube, with a negative atom count and the line "    1" plus
+ * the one-based orbital_index (width 5) after the atoms; comment NULL:
+ * "Molecular orbital amplitude". orbital_index 0 or above 9999 gives
+ * INVALID_ARGUMENT. */
+int kumo_hip_write_orbital_cube(kumo_hip_context* ctx, const char* path, const char* comment,
+                                const kumo_hip_grid* grid, int32_t natoms, const int32_t* atomic_numbers,
+                                const double* nuclear_charges, const double* coordinates, const double* values,
+                                int32_t values_on_device, size_t orbital_index);
+/* The GPU formatter of the Cube writer on its own: n host values written as
+ * printf("%*.*e", precision + 8, precision, v), precision + 8 bytes each (no
+ * terminators), 0 <= precision <= 12. exact_only nonzero decides every
+ * rounding on the exact big-integer path instead of the double-double one
+ * (for testing). No counterpart in the reference. */
+int kumo_hip_format_scientific(kumo_hip_context* ctx, const double* values, size_t n, int32_t precision,
+                               int32_t exact_only, char* text);
+
 /* ---- timing hooks for the benchmark ----
  * Device time (ms) of the last evaluation's dominant contraction kernel,
  * summed over its launches and measured with HIP events on the stream the
@@ -269,6 +301,13 @@ int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, i
  * must move (channel moments read, outputs written): measurement only, no
  * counterpart in the reference. */
 int kumo_hip_last_field_stats(const kumo_hip_context* ctx, double* fields_ms, int64_t* points, int64_t* bytes);
+
+/* The last Cube write of the context: wall time of the call (ms), device
+ * time of the text kernels and of the text copies to host memory (ms, HIP
+ * events on the context's stream), samples and file bytes written:
+ * measurement only, no counterpart in the reference. */
+int kumo_hip_last_cube_stats(const kumo_hip_context* ctx, double* total_ms, double* kernel_ms, double* copy_ms,
+                             int64_t* values, int64_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <fstream>
+#include <iomanip>
 #include <map>
 #include <memory>
 #include <numbers>
@@ -1753,6 +1755,59 @@ void orc_grid_points(const double* origin, const double* steps, const int* count
     PointSet set = make_set(origin, steps, counts, 0, nullptr);
     for (std::int64_t q = 0; q < n; ++q)
         set.point(start + q, out + 3 * q);
+}
+
+// Field::write_cube / write_orbital_cube (src/cube.cpp:70-111, 253-262):
+// the samples must be finite (checked before the file is opened), then the
+// header with std::fixed / setprecision(6) (atom count width 5, negative for
+// an orbital Cube; origin, steps, charges and positions width 13), and the
+// samples with std::scientific / setprecision(5) / setw(13), a line break
+// after every sixth sample of the whole sequence and after the last sample of
+// every run along the third axis.
+int orc_write_field_cube(const char* path, const char* comment, const double* origin, const double* steps,
+                         const int* counts, int natoms, const int* z, const double* charges, const double* xyz,
+                         const double* values, std::int64_t n, int orbital, std::int64_t orbital_index, char* error,
+                         int error_len) {
+    if (orbital && (orbital_index <= 0 || orbital_index > 9999)) {
+        set_error(error, error_len, "invalid orbital index for Cube");
+        return 1;
+    }
+    for (std::int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(values[i])) {
+            set_error(error, error_len, "nonfinite Cube sample");
+            return 1;
+        }
+    std::ofstream out(path);
+    if (!out) {
+        set_error(error, error_len, orbital ? "cannot open orbital cube output" : "cannot open cube output");
+        return 3;
+    }
+    out << "chem-kumo 2.1.0\n" << comment << '\n' << std::fixed << std::setprecision(6);
+    out << std::setw(5) << static_cast<long long>(orbital ? -natoms : natoms);
+    for (int k = 0; k < 3; ++k) out << std::setw(13) << origin[k];
+    out << '\n';
+    for (int a = 0; a < 3; ++a) {
+        out << std::setw(5) << counts[a];
+        for (int k = 0; k < 3; ++k) out << std::setw(13) << steps[3 * a + k];
+        out << '\n';
+    }
+    for (int i = 0; i < natoms; ++i) {
+        out << std::setw(5) << z[i] << std::setw(13) << charges[i];
+        for (int k = 0; k < 3; ++k) out << std::setw(13) << xyz[3 * i + k];
+        out << '\n';
+    }
+    if (orbital) out << std::setw(5) << 1 << std::setw(5) << orbital_index << '\n';
+    out << std::scientific << std::setprecision(5);
+    const std::int64_t run = counts[2];
+    for (std::int64_t i = 1; i <= n; ++i) {
+        out << std::setw(13) << values[i - 1];
+        if (i % 6 == 0 || i % run == 0) out << '\n';
+    }
+    if (!out) {
+        set_error(error, error_len, orbital ? "orbital cube write failed" : "cube write failed");
+        return 3;
+    }
+    return 0;
 }
 
 int orc_threads(void) {

@@ -89,5 +89,12 @@ std::int64_t orc_esp_pair_count(const void* molecule);
 /// Grid point coordinates (src/grid_geometry.hpp:59-68), 3 x n.
 void orc_grid_points(const double* origin, const double* steps, const int* counts,
                      std::int64_t start, std::int64_t n, double* out);
+/// Gaussian Cube file of a field (src/cube.cpp:70-111, 253-262); orbital
+/// nonzero: an orbital Cube with the one-based orbital_index. Returns 0, 1
+/// (invalid argument: nonfinite sample, orbital index) or 3 (file error).
+int orc_write_field_cube(const char* path, const char* comment, const double* origin, const double* steps,
+                         const int* counts, int natoms, const int* z, const double* charges, const double* xyz,
+                         const double* values, std::int64_t n, int orbital, std::int64_t orbital_index, char* error,
+                         int error_len);
 int orc_threads(void);
 }

@@ -51,6 +51,9 @@ def lib():
         L.orc_pure_transform.argtypes = [C.c_int, C.c_void_p]
         L.orc_molecule_destroy.argtypes = [C.c_void_p]
         L.orc_basis_destroy.argtypes = [C.c_void_p]
+        L.orc_write_field_cube.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,
+                                           C.c_int64, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -226,3 +229,24 @@ def grid_points(grid, start=0, n=None):
     out = np.zeros((n, 3))
     lib().orc_grid_points(_ptr(origin), _ptr(steps), _ptr(counts, C.c_int), start, n, _ptr(out))
     return out
+
+
+def write_cube(path, grid, atoms, values, comment=None, orbital_index=None):
+    """Field::write_cube / write_orbital_cube restated (src/cube.cpp:70-111,
+    253-262). atoms: (atomic numbers, nuclear charges, coordinates natoms x 3).
+    Returns (status, message): 0, 1 (invalid argument) or 3 (file error)."""
+    orbital = orbital_index is not None
+    if comment is None:
+        comment = "Molecular orbital amplitude" if orbital else "Electron density"
+    origin = np.ascontiguousarray(grid["origin"], dtype=np.float64)
+    steps = np.ascontiguousarray(np.asarray(grid["steps"], dtype=np.float64).T)
+    counts = np.ascontiguousarray(grid["counts"], dtype=np.int32)
+    z = np.ascontiguousarray(atoms[0], dtype=np.int32).ravel()
+    q = np.ascontiguousarray(atoms[1], dtype=np.float64).ravel()
+    xyz = np.ascontiguousarray(atoms[2], dtype=np.float64).reshape(-1)
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    err = C.create_string_buffer(256)
+    rc = lib().orc_write_field_cube(os.fsencode(path), comment.encode(), _ptr(origin), _ptr(steps),
+                                    _ptr(counts, C.c_int), z.size, _ptr(z, C.c_int), _ptr(q), _ptr(xyz), _ptr(v),
+                                    v.size, 1 if orbital else 0, int(orbital_index or 0), err, 256)
+    return rc, err.value.decode()
