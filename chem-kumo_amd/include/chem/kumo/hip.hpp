@@ -225,6 +225,16 @@ inline kumo_hip_context* context() {
 inline void check(int status) {
     if (status != KUMO_HIP_OK) raise(status);
 }
+inline kumo_hip_grid to_c(const Grid& grid) {
+    kumo_hip_grid g{};
+    for (int a = 0; a < 3; ++a) {
+        g.origin[a] = grid.origin[a];
+        g.counts[a] = grid.counts[a];
+        for (int k = 0; k < 3; ++k)
+            g.steps[3 * a + k] = grid.steps[a][k];
+    }
+    return g;
+}
 } // namespace detail
 
 class Molecule {
@@ -288,13 +298,7 @@ public:
     }
 
     Results evaluate(const Grid& grid, const Request& request) const {
-        kumo_hip_grid g{};
-        for (int a = 0; a < 3; ++a) {
-            g.origin[a] = grid.origin[a];
-            g.counts[a] = grid.counts[a];
-            for (int k = 0; k < 3; ++k)
-                g.steps[3 * a + k] = grid.steps[a][k];
-        }
+        const kumo_hip_grid g = detail::to_c(grid);
         if (grid.counts[0] <= 0 || grid.counts[1] <= 0 || grid.counts[2] <= 0)
             throw std::invalid_argument("invalid or oversized grid");
         return run(request, grid.size(), [&](const kumo_hip_property* p, std::size_t np, const kumo_hip_options* o,
@@ -317,6 +321,36 @@ public:
         Grid grid;
         std::vector<Atom> atoms;
         std::vector<double> values;
+        // Field::write_cube (include/chem/kumo/density.hpp:116-121).
+        void write_cube(const std::string& path, const std::string& comment = "Electron density") const {
+            write(path, comment, 0);
+        }
+        // Field::write_orbital_cube (include/chem/kumo/density.hpp:122-128).
+        void write_orbital_cube(const std::string& path, std::size_t orbital_index,
+                                const std::string& comment = "Molecular orbital amplitude") const {
+            if (orbital_index == 0 || orbital_index > 9999) throw std::invalid_argument("invalid orbital index for Cube");
+            write(path, comment, orbital_index);
+        }
+
+    private:
+        void write(const std::string& path, const std::string& comment, std::size_t orbital_index) const {
+            if (values.size() != grid.size()) throw std::invalid_argument("field size mismatch");
+            const kumo_hip_grid g = detail::to_c(grid);
+            std::vector<int32_t> z;
+            std::vector<double> q, xyz;
+            for (const Atom& a : atoms) {
+                z.push_back(a.atomic_number);
+                q.push_back(a.nuclear_charge);
+                xyz.insert(xyz.end(), a.position.begin(), a.position.end());
+            }
+            const auto n = static_cast<int32_t>(atoms.size());
+            detail::check(orbital_index
+                              ? kumo_hip_write_orbital_cube(detail::context(), path.c_str(), comment.c_str(), &g, n,
+                                                            z.data(), q.data(), xyz.data(), values.data(), 0,
+                                                            orbital_index)
+                              : kumo_hip_write_cube(detail::context(), path.c_str(), comment.c_str(), &g, n, z.data(),
+                                                    q.data(), xyz.data(), values.data(), 0));
+        }
     };
     Field field(const Grid& grid, Property property) const {
         if (channels(property.kind) != 1)

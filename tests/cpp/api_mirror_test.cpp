@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 using namespace chem::kumo;
@@ -52,5 +53,46 @@ int main(int argc, char** argv) {
     std::vector<Vector3> points{{0.0, 0.0, 0.0}, {0.5, 0.2, -0.1}};
     const auto at = evaluator.evaluate(points, Request().add(density::total.value()));
     check(at.get(density::total.value()).component().size() == 2, "evaluation at explicit points");
+
+    // Cube output as tests/atom_charge_cube_test.cpp writes it: an iron atom
+    // (Z 26) with an ECP charge of 16, two points per axis.
+    Grid small;
+    small.counts = {2, 2, 2};
+    const Evaluator::Field iron{small, {{26, 16.0, {0.0, 0.0, 0.0}}}, std::vector<double>(8, 0.5)};
+    const std::string dir = argc > 2 ? argv[2] : "/tmp";
+    iron.write_cube(dir + "/kumo-charge-density-test.cube");
+    iron.write_orbital_cube(dir + "/kumo-charge-orbital-test.cube", 1);
+    for (const char* name : {"/kumo-charge-density-test.cube", "/kumo-charge-orbital-test.cube"}) {
+        std::FILE* f = std::fopen((dir + name).c_str(), "r");
+        char line[256];
+        int natoms = 0, z = 0, count = 0;
+        double charge = 0.0, x = 0.0;
+        bool ok = f != nullptr;
+        for (int i = 0; ok && i < 2; ++i) ok = std::fgets(line, sizeof line, f) != nullptr;
+        ok = ok && std::fscanf(f, "%d %lf %lf %lf", &natoms, &x, &x, &x) == 4;
+        for (int a = 0; ok && a < 3; ++a) ok = std::fscanf(f, "%d %lf %lf %lf", &count, &x, &x, &x) == 4 && count == 2;
+        ok = ok && std::fscanf(f, "%d %lf %lf %lf %lf", &z, &charge, &x, &x, &x) == 5;
+        if (f) std::fclose(f);
+        const bool orbital = std::string(name).find("orbital") != std::string::npos;
+        check(ok && natoms == (orbital ? -1 : 1) && z == 26 && std::abs(charge - 16.0) < 1e-6,
+              orbital ? "orbital Cube keeps Z 26 and charge 16, atom count -1"
+                      : "density Cube keeps Z 26 and charge 16, atom count 1");
+    }
+    threw = false;
+    try {
+        iron.write_orbital_cube(dir + "/kumo-charge-orbital-test.cube", 0);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    check(threw, "orbital index 0 throws std::invalid_argument");
+    Evaluator::Field broken = iron;
+    broken.values[3] = std::nan("");
+    threw = false;
+    try {
+        broken.write_cube(dir + "/kumo-charge-broken-test.cube");
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    check(threw, "a nonfinite sample throws std::invalid_argument");
     return failures ? 1 : 0;
 }
