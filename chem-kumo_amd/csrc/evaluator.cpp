@@ -35,6 +35,92 @@ void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
     if (o.tile_points != 0 && o.tile_points != 16 && o.tile_points != 32) invalid("tile_points must be 0, 16 or 32");
 }
 
+// The checks of OrbitalEnergyFields (src/orbital_energy_fields.hpp:117-190,
+// 229-233), in the reference's order, made before any pass
+// (src/evaluator.cpp:579-596). `what` names the most restrictive group.
+void check_orbital_energy(const kumo_hip_molecule* mol, const Plan& plan, double delta) {
+    const std::string what = plan.orbital_weighted_fukui ? "the orbital-weighted Fukui function or dual descriptor"
+                             : plan.virtual_orbitals     ? "the local electron affinity and related fields"
+                                                         : "the average local ionization energy";
+    if (plan.virtual_orbitals && !mol->unrestricted && mol->nalpha != mol->nbeta)
+        invalid(what + " is not defined for a restricted open-shell calculation");
+    if (plan.orbital_weighted_fukui && (mol->unrestricted || mol->nalpha != mol->nbeta))
+        invalid(what + " is defined only for a restricted closed-shell calculation");
+    if (plan.orbital_weighted_fukui && (!std::isfinite(delta) || delta <= 0.0))
+        invalid(what + ": the Fukui delta must be finite and positive");
+    if (!mol->eligible)
+        invalid(what + " needs occupied MOs that reproduce the FCHK density matrices (an SCF calculation)");
+    const auto& eb = mol->unrestricted ? mol->Eb : mol->Ea;
+    const auto size = [](const std::vector<double>& v) { return static_cast<long long>(v.size()); };
+    if (size(mol->Ea) < mol->nmo_a || (mol->unrestricted && size(mol->Eb) < mol->nmo_b) ||
+        size(mol->Ea) < mol->nalpha || size(eb) < mol->nbeta)
+        invalid(what + " needs the FCHK orbital energies");
+    const int nvir = mol->unrestricted ? std::max(0, mol->nmo_a - mol->nalpha) + std::max(0, mol->nmo_b - mol->nbeta)
+                                       : std::max(0, mol->nmo_a - std::max(mol->nalpha, mol->nbeta));
+    if (plan.virtual_orbitals && nvir == 0) invalid(what + " needs virtual orbitals, and the FCHK has none");
+    if (plan.orbital_weighted_fukui && (mol->nalpha == 0 || mol->nalpha >= mol->nmo_a))
+        invalid(what + " needs occupied and virtual orbitals, and the FCHK lacks one kind");
+}
+
+// Normalized orbital-weighted Fukui weights of energies [first, last)
+// (orbital_weights, src/orbital_energy_fields.hpp:75-96): Gaussians of the
+// distance from mu, exponents shifted by their largest value.
+std::vector<double> fukui_weights(const std::vector<double>& e, int first, int last, double mu, double delta) {
+    std::vector<double> x(static_cast<std::size_t>(last - first)), w(x.size());
+    for (int i = first; i < last; ++i) {
+        const double t = (mu - e[static_cast<std::size_t>(i)]) / delta;
+        x[static_cast<std::size_t>(i - first)] = -t * t;
+    }
+    const double top = *std::max_element(x.begin(), x.end());
+    double sum = 0.0;
+    for (std::size_t i = 0; i < x.size(); ++i) {
+        w[i] = std::exp(x[i] - top);
+        sum += w[i];
+    }
+    for (double& v : w)
+        v /= sum;
+    return w;
+}
+
+// Weights [kOeSums][slots] of the MO slots of the orbital-energy operand in
+// the orbital sums (columns and energies of src/orbital_energy_fields.hpp:
+// 155-192, 219-251): occupation n_i and n_i |e_i| for occupied MOs, 1, e_i
+// and (e_i < 0) w e_i for virtual MOs, the Fukui weights g_i.
+std::vector<double> orbital_energy_weights(const kumo_hip_molecule* mol, const Plan& plan, double delta) {
+    const int ns = mol->oe_slots;
+    std::vector<double> w(static_cast<std::size_t>(kOeSums) * static_cast<std::size_t>(ns), 0.0);
+    const auto at = [&](int sum, int slot) -> double& {
+        return w[static_cast<std::size_t>(sum) * static_cast<std::size_t>(ns) + static_cast<std::size_t>(slot)];
+    };
+    const double attach = mol->unrestricted ? 1.0 : 2.0;
+    for (int q = 0; q < ns; ++q) {
+        const OeSlot& sl = mol->oe_slot[static_cast<std::size_t>(q)];
+        if (sl.kind == 0) continue;
+        const double e = (sl.channel == 1 ? mol->Eb : mol->Ea)[static_cast<std::size_t>(sl.mo)];
+        if (sl.kind == 1) {
+            at(kOeOccDensity, q) = sl.occupation;
+            at(kOeOccEnergy, q) = sl.occupation * std::abs(e);
+        } else {
+            at(kOeVirDensity, q) = 1.0;
+            at(kOeVirEnergy, q) = e;
+            if (e < 0.0) at(kOeAttachment, q) = attach * e;
+        }
+    }
+    if (plan.orbital_weighted_fukui) { // restricted closed shell: alpha MOs only
+        const int homo = mol->nalpha, count = mol->nmo_a;
+        const auto& E = mol->Ea;
+        const double mu = 0.5 * (E[static_cast<std::size_t>(homo - 1)] + E[static_cast<std::size_t>(homo)]);
+        const auto plus = fukui_weights(E, homo, count, mu, delta);
+        const auto minus = fukui_weights(E, 0, homo, mu, delta);
+        for (int q = 0; q < ns; ++q) {
+            const OeSlot& sl = mol->oe_slot[static_cast<std::size_t>(q)];
+            if (sl.kind == 1) at(kOeFukuiMinus, q) = minus[static_cast<std::size_t>(sl.mo)];
+            if (sl.kind == 2) at(kOeFukuiPlus, q) = plus[static_cast<std::size_t>(sl.mo - homo)];
+        }
+    }
+    return w;
+}
+
 struct TableSet {
     KeptTables t;
 };
@@ -296,6 +382,7 @@ Prepared prepare(const kumo_hip_molecule* mol, const PointsArg& pts, const kumo_
     }
     r.plan = make_plan(props, nprops);
     r.backend = choose_backend(opt.backend, mol->eligible, r.plan);
+    if (r.plan.orbital_energies) check_orbital_energy(mol, r.plan, opt.fukui_delta);
     for (std::size_t i = 0; i < nprops; ++i) {
         const auto& p = props[i];
         if (is_orbital_property(p.kind)) {
@@ -320,8 +407,8 @@ Prepared prepare(const kumo_hip_molecule* mol, const PointsArg& pts, const kumo_
         if (on_grid_path(p, r.backend, r.grid) && p.kind == kOrbital) orb_ch[p.channel] = true;
     }
     r.grid_orbital_passes = (orb_ch[0] ? 1 : 0) + (orb_ch[1] ? 1 : 0);
-    r.passes = (r.needed.empty() ? 0 : 1) + (r.plan.esp() ? 1 : 0) + (r.orbital_pass ? 1 : 0) +
-               r.grid_density_passes + r.grid_orbital_passes;
+    r.passes = (r.needed.empty() ? 0 : 1) + (r.plan.esp() ? 1 : 0) + (r.plan.orbital_energies ? 1 : 0) +
+               (r.orbital_pass ? 1 : 0) + r.grid_density_passes + r.grid_orbital_passes;
     return r;
 }
 
@@ -576,6 +663,79 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                                         hipMemcpyDeviceToDevice, s));
     }
 
+    // ---------------------------------------------------------- orbital energies
+    // OrbitalEnergyFields::evaluate (src/orbital_energy_fields.hpp:195-217):
+    // AO values without per-block screening, psi_i of every needed MO in FP64
+    // on the MFMA kernel, the weighted sums per point, then the fields. The
+    // masks use the smallest subnormal as threshold: an AO block is left out
+    // only where every value of the block is zero (beyond the shell cutoff),
+    // which changes no sum; the screening radius is the shell cutoff's.
+    if (plan.orbital_energies) {
+        mol->ensure_orbital_energy(s);
+        const bool virt = plan.virtual_orbitals || plan.orbital_weighted_fukui;
+        const int oslot1 = virt ? mol->oe_slots : mol->oe_occ_slots;
+        const int groups = (oslot1 + 127) / 128;
+        const auto weights = orbital_energy_weights(mol, plan, opt.fukui_delta);
+        const long long b_first = q0 / bs, b_end = (q1 + bs - 1) / bs;
+        const int cap = static_cast<int>(std::min<long long>({b_end - b_first, std::max(1ll, (1ll << 21) / bs), 8192ll}));
+        // weights, then the sums [slot group][kOeSums][cap * bs]
+        const std::size_t wsize = (weights.size() + 31) / 32 * 32;
+        double* dw = grow<double>(ex.ws_oe, wsize + static_cast<std::size_t>(groups) * kOeSums * cap * bs);
+        double* sums = dw + wsize;
+        upload_async(ex, dw, weights.data(), weights.size());
+        std::vector<FieldTask> tasks;
+        for (std::size_t i = 0; i < nprops; ++i)
+            if (is_orbital_energy(props[i].kind)) {
+                FieldTask t{};
+                t.kind = props[i].kind;
+                t.component = props[i].component;
+                t.out[0] = dout[pr.first_channel[i]];
+                tasks.push_back(t);
+            }
+        FieldTask* dtasks = grow<FieldTask>(ex.ws_oe_tasks, tasks.size());
+        upload_async(ex, dtasks, tasks.data(), tasks.size());
+        const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks);
+        const double oe_thr = std::numeric_limits<double>::denorm_min();
+        for (long long b0 = b_first; b0 < b_end; b0 += cap) {
+            const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
+            const long long first = b0 * bs;
+            const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
+            KeptTables kt = ts.t;
+            kt.lists = 0;
+            launch_masks(mv, ps, n, bs, b0, nb, 0, oe_thr, mol->screen_r2(oe_thr, s), kt, s);
+            AoMoLaunch a{};
+            a.mol = mv;
+            a.ps = ps;
+            a.mo = {mol->d_oe_op.as<double>(), nullptr, nullptr, nullptr, mol->operand_rows, mol->oe_slots,
+                    mol->oe_slots};
+            a.tables = ts.t;
+            a.npoints = n;
+            a.batch_first = first;
+            a.batch_points = count;
+            a.block_size = bs;
+            a.first_block = b0;
+            a.nblk = nb;
+            a.order = 0;
+            a.hessian = 0;
+            a.precision = KUMO_HIP_FP64;
+            a.slot0 = 0;
+            a.slot1 = oslot1;
+            a.tile_points = 16;
+            a.moments = sums;
+            a.weights = dw;
+            record(ex, true);
+            launch_ao_mo(a, s);
+            record(ex, false);
+            ++ex.last_launches;
+            for (int g = 1; g < groups; ++g)
+                launch_add_into(sums, sums + static_cast<long long>(g) * kOeSums * count, kOeSums * count, s);
+            launch_orbital_energy_fields(sums, count, first, static_cast<int>(tasks.size()), dtasks, virt ? 1 : 0,
+                                         plan.orbital_weighted_fukui ? 1 : 0, s);
+            if (!ex.capture && opt.progress) sync(ex);
+            advance(static_cast<std::size_t>(count));
+        }
+    }
+
     // ---------------------------------------------------------------- orbitals
     const auto run_orbitals = [&](const std::vector<std::size_t>& keys, int channel, int order) {
         std::vector<int> mos;
@@ -776,7 +936,7 @@ void Executor::destroy() {
         list.clear();
     }
     for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc,
-                            &ws_rescue, &ws_rmom, &ws_rscale})
+                            &ws_rescue, &ws_rmom, &ws_rscale, &ws_oe, &ws_oe_tasks})
         b->release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;

@@ -27,9 +27,11 @@
 // contraction's rounding error that the density rescue tests.
 //
 // MFMA operand layouts (wave64): for 16x16xK the lane l holds A row l % 16
-// and B column l % 16 at a K slice selected by l / 16; the accumulator lane l
-// holds D rows 4 (l / 16) + r, column l % 16. The K slice order is the same
-// for A and B, so the sum over K is exact whatever order the hardware uses.
+// and B column l % 16 at a K slice selected by l / 16; accumulator register r
+// of lane l holds D column l % 16 and row 4 (l / 16) + r for the FP32
+// accumulators, row l / 16 + 4 r for the FP64 one (its vector width is 1).
+// The K slice order is the same for A and B, so the sum over K is exact
+// whatever order the hardware uses.
 // LDS strides are chosen so that the 16 lanes of each K slice hit distinct
 // banks. FP32 and FP64: coefficient rows have a pitch of 132 and 144
 // elements, the Phi row of a point 36 and 34. bf16: coefficient rows are
@@ -140,7 +142,9 @@ template <int PREC, unsigned PSET, int TP> constexpr int occupancy() {
     return popcount10(PSET) > 7 ? 1 : (TP == 16 && PREC != KUMO_HIP_FP64 ? 3 : 2);
 }
 
-template <int PREC, int ORDER, unsigned PSET, int TP>
+// WSUM: weighted mode (orbital-energy sums, FP64 values only): instead of the
+// channel moments, per point the sums over slots i of w_s(i) psi_i^2.
+template <int PREC, int ORDER, unsigned PSET, int TP, bool WSUM = false>
 __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kernel(AoMoLaunch a) {
     using P = Prec<PREC>;
     using E = typename P::E;
@@ -454,6 +458,52 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     }
 
     __syncthreads(); // (a block without kept AOs skips the loop; red aliases Phi)
+    if constexpr (WSUM) {
+        static_assert(ORDER == 0 && PREC == KUMO_HIP_FP64, "weighted sums of FP64 MO values");
+        // Tile rows are MO slots: FP64 accumulator register r of lane l holds
+        // slot slot_lo + 16 mt + l / 16 + 4 r at point column l % 16.
+        const int r16 = lane & 15;
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            const int tile = wave + 4 * t;
+            if (tile >= ntiles) continue;
+            const int mt = tile / NT, nt = tile % NT;
+            double q[kOeSums];
+#pragma unroll
+            for (int k = 0; k < kOeSums; ++k)
+                q[k] = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double v = static_cast<double>(acc[t][0][r]);
+                const double sq = v * v;
+                const double* w = a.weights + slot_lo + mt * 16 + (lane >> 4) + 4 * r;
+#pragma unroll
+                for (int k = 0; k < kOeSums; ++k)
+                    q[k] += w[static_cast<long long>(k) * a.mo.slots] * sq;
+            }
+#pragma unroll
+            for (int k = 0; k < kOeSums; ++k) {
+                q[k] += __shfl_xor(q[k], 16, 64);
+                q[k] += __shfl_xor(q[k], 32, 64);
+            }
+            if (lane < 16)
+#pragma unroll
+                for (int k = 0; k < kOeSums; ++k)
+                    red[(static_cast<size_t>(mt) * kOeSums + k) * T + nt * 16 + r16] = q[k];
+        }
+        __syncthreads();
+        // Sum the m-tiles in order and store the sums of this slot group.
+        for (int item = tid; item < kOeSums * T; item += 256) {
+            const int jj = item % T, k = item / T;
+            if (jj >= nt_valid) continue;
+            double sum = 0.0;
+            for (int mt = 0; mt < MT; ++mt)
+                sum += red[(static_cast<size_t>(mt) * kOeSums + k) * T + jj];
+            const long long point = bstart + p0 + jj - a.batch_first;
+            a.moments[(static_cast<long long>(blockIdx.y) * kOeSums + k) * a.batch_points + point] = sum;
+        }
+        return;
+    }
     if (scale) {
         esl[2 * tid] = es0;
         esl[2 * tid + 1] = es1;
@@ -545,7 +595,8 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     }
 }
 
-template <int PREC, int ORDER, unsigned PSET, int TP> void launch_one(const AoMoLaunch& a, hipStream_t stream) {
+template <int PREC, int ORDER, unsigned PSET, int TP, bool WSUM = false>
+void launch_one(const AoMoLaunch& a, hipStream_t stream) {
     using P = Prec<PREC>;
     constexpr int NP = popcount10(PSET);
     const size_t work = sizeof(typename P::E) * P::S * (static_cast<size_t>(NP) * TP * P::LDK + KC * P::PITCH);
@@ -557,11 +608,11 @@ template <int PREC, int ORDER, unsigned PSET, int TP> void launch_one(const AoMo
     const dim3 grid(static_cast<unsigned>(a.nblk * tiles_per_block), static_cast<unsigned>(groups));
     static bool configured = false;
     if (!configured) {
-        KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&aomo_kernel<PREC, ORDER, PSET, TP>),
+        KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&aomo_kernel<PREC, ORDER, PSET, TP, WSUM>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         configured = true;
     }
-    aomo_kernel<PREC, ORDER, PSET, TP><<<grid, dim3(256), lds, stream>>>(a);
+    aomo_kernel<PREC, ORDER, PSET, TP, WSUM><<<grid, dim3(256), lds, stream>>>(a);
     KUMO_HIP(hipGetLastError());
 }
 
@@ -589,6 +640,12 @@ void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream) {
     if (a.slot0 % 16 || a.slot1 % 16) invalid("MO slot ranges must be multiples of 16");
     static_assert(Prec<KUMO_HIP_FP64>::T == kRescueTile, "rescue tiles are FP64 workgroup tiles");
     if (a.r_tile_blk && a.precision != KUMO_HIP_FP64) invalid("rescue tiles are evaluated in FP64");
+    if (a.weights) {
+        if (a.precision != KUMO_HIP_FP64 || a.order != 0 || a.r_tile_blk)
+            invalid("weighted MO sums are FP64 values without rescue tiles");
+        launch_one<KUMO_HIP_FP64, 0, kPanelsValue, 16, true>(a, stream);
+        return;
+    }
     if (a.tile_points != 0 && a.tile_points != 16 && a.tile_points != 32) invalid("tile points must be 16 or 32");
     switch (a.precision) {
     case KUMO_HIP_FP64: launch_prec<KUMO_HIP_FP64>(a, stream); break;

@@ -3,6 +3,7 @@
 // (src/evaluator.cpp:127-353), in FP64. One thread per point; every output is
 // a coalesced FP64 stream.
 #include "kernels.hpp"
+#include "plan.hpp"
 
 #include "common.hpp"
 
@@ -244,6 +245,41 @@ __global__ void fill_kernel(double* dst, long long n, double v) {
     if (j < n) dst[j] = v;
 }
 
+// store_orbital_energy_block (src/evaluator.cpp:481-520) per point.
+__global__ void orbital_energy_fields_kernel(const double* __restrict__ sums, long long n, long long out_offset,
+                                             int ntasks, const FieldTask* __restrict__ tasks, int virtuals,
+                                             int fukui) {
+    const long long j = blockIdx.x * 256ll + threadIdx.x;
+    if (j >= n) return;
+    const double rho = sums[kOeOccDensity * n + j];
+    const double alie = rho > 0.0 ? sums[kOeOccEnergy * n + j] / rho : 0.0;
+    double lea = 0.0, attachment = 0.0, plus = 0.0, minus = 0.0;
+    if (virtuals) {
+        const double vir = sums[kOeVirDensity * n + j];
+        lea = vir > 0.0 ? -sums[kOeVirEnergy * n + j] / vir : 0.0;
+        attachment = rho > 0.0 ? sums[kOeAttachment * n + j] / rho : 0.0;
+    }
+    if (fukui) {
+        plus = sums[kOeFukuiPlus * n + j];
+        minus = sums[kOeFukuiMinus * n + j];
+    }
+    for (int t = 0; t < ntasks; ++t) {
+        double v;
+        switch (tasks[t].kind) {
+        case kAlie: v = alie; break;
+        case kLea: v = lea; break;
+        case kLeae: v = attachment; break;
+        case kElectronegativity: v = 0.5 * (alie + lea); break;
+        case kHardness: v = 0.5 * (alie - lea); break;
+        case kFukuiPlus: v = plus; break;
+        case kFukuiMinus: v = minus; break;
+        case kFukuiZero: v = 0.5 * (plus + minus); break;
+        default: v = plus - minus; break;
+        }
+        tasks[t].out[0][out_offset + j] = v;
+    }
+}
+
 // FP64 rescue of ill-conditioned ELF values. The Pauli term tau - |grad
 // rho|^2 / (8 rho) of each spin channel is a difference of two positive
 // terms; with a relative error eps on the contraction's moments it carries
@@ -418,6 +454,14 @@ void launch_add_into(double* dst, const double* src, long long n, hipStream_t st
 void launch_fill(double* dst, long long n, double value, hipStream_t stream) {
     if (n <= 0) return;
     fill_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(dst, n, value);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_orbital_energy_fields(const double* sums, long long n, long long out_offset, int ntasks,
+                                  const FieldTask* tasks, int virtuals, int fukui, hipStream_t stream) {
+    if (n <= 0 || ntasks == 0) return;
+    orbital_energy_fields_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(
+        sums, n, out_offset, ntasks, tasks, virtuals, fukui);
     KUMO_HIP(hipGetLastError());
 }
 

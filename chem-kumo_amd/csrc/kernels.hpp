@@ -114,6 +114,20 @@ void launch_rescue_plan(const double* channel_moments, long long n, int closed, 
 void launch_rescue_scatter(const double* rescue_moments, long long rslots, int groups, int tile, int block_size,
                            const RescuePlan& r, double* channel_moments, long long n, hipStream_t stream);
 
+// Per-point orbital sums of the orbital-energy fields (OrbitalEnergySums,
+// src/orbital_energy_fields.hpp:16-60), each sum_i w_s(i) |psi_i|^2 over MO
+// slots i with the weights:
+enum OeSum : int {
+    kOeOccDensity = 0, // n_i (occupied)
+    kOeOccEnergy,      // n_i |e_i| (occupied)
+    kOeVirDensity,     // 1 (virtual)
+    kOeVirEnergy,      // e_i (virtual)
+    kOeAttachment,     // w e_i (virtual, e_i < 0; w = 2 restricted, 1 unrestricted)
+    kOeFukuiPlus,      // g_i (virtual, restricted closed shell)
+    kOeFukuiMinus,     // g_i (occupied, restricted closed shell)
+    kOeSums
+};
+
 struct AoMoLaunch {
     MolView mol;
     PointSource ps;
@@ -134,6 +148,10 @@ struct AoMoLaunch {
     // Reduced precisions, when not null: [2][batch_points] sum_k |Phi_k| ||C_k,c||
     // over the kept operand rows for channel c (slot group 0 only).
     float* rho_scale;
+    // Weighted mode (FP64, order 0), when not null: [kOeSums][mo.slots]
+    // weight of each MO slot in each orbital sum; the moments array then
+    // receives [slot group][kOeSums][batch_points].
+    const double* weights;
     // Rescue mode (FP64 only): workgroup t evaluates the flagged points of
     // tile t of the plan and writes moments at slot t * 16 + j.
     const int32_t* r_tile_blk;
@@ -186,6 +204,14 @@ struct FieldsLaunch {
 FieldsLaunch fields_launch(const double* channels, int from_spin, int closed, long long n, long long out_offset,
                           int ntasks, const FieldTask* tasks);
 void launch_fields(const FieldsLaunch& f, hipStream_t stream);
+
+// The orbital-energy fields (store_orbital_energy_block, src/evaluator.cpp:
+// 481-520) of n points from their sums [kOeSums][n]: ALIE, LEA, LEAE, local
+// electronegativity and hardness (0 where a denominator vanishes), the
+// orbital-weighted Fukui functions f+, f-, f0 and the dual descriptor.
+// virtuals / fukui: the virtual-orbital / Fukui sums were formed.
+void launch_orbital_energy_fields(const double* sums, long long n, long long out_offset, int ntasks,
+                                  const FieldTask* tasks, int virtuals, int fukui, hipStream_t stream);
 
 // dst[i] += src[i]; dst[i] = value.
 void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream);

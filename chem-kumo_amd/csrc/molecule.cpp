@@ -83,6 +83,42 @@ double bf16_value(uint16_t h) {
     return f;
 }
 
+
+// MO operand table rows of MO columns coef(ao, slot), slots 0..ns-1: per AO
+// block the Cartesian rows g[k] (T^T C)[k] (pure transform and component
+// factor folded in), then for pure blocks the plain pure rows C[r].
+template <typename Coef>
+std::vector<double> operand_table(const PreparedBasis& b, std::size_t ns, const Coef& coef) {
+    std::vector<double> op(static_cast<std::size_t>(b.operand_rows) * ns, 0.0);
+    for (std::size_t blk = 0; blk < b.blocks.size(); ++blk) {
+        const HostBlock& hb = b.blocks[blk];
+        const DevShell& d = b.shells[blk];
+        const int nc = hb.ncart();
+        for (int k = 0; k < nc; ++k) {
+            const double gk = component_factor(hb.powers[0], hb.powers[static_cast<std::size_t>(k)]);
+            double* row = op.data() + static_cast<std::size_t>(d.crow + k) * ns;
+            for (std::size_t slot = 0; slot < ns; ++slot) {
+                double v;
+                if (hb.pure.empty()) {
+                    v = coef(hb.offset + k, slot);
+                } else {
+                    v = 0.0;
+                    for (int r = 0; r < hb.count; ++r)
+                        v += hb.pure[static_cast<std::size_t>(r * nc + k)] * coef(hb.offset + r, slot);
+                }
+                row[slot] = gk * v;
+            }
+        }
+        if (!hb.pure.empty())
+            for (int r = 0; r < hb.count; ++r) {
+                double* row = op.data() + static_cast<std::size_t>(d.prow + r) * ns;
+                for (std::size_t slot = 0; slot < ns; ++slot)
+                    row[slot] = coef(hb.offset + r, slot);
+            }
+    }
+    return op;
+}
+
 void upload_device(kumo_hip_molecule& m, hipStream_t s) {
     const auto& b = m.basis;
     m.d_shells = upload(b.shells.data(), b.shells.size(), s);
@@ -160,33 +196,7 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
                 return m.Cb[static_cast<std::size_t>(i) * static_cast<std::size_t>(m.nmo_b) + (slot - static_cast<std::size_t>(a16))];
             return 0.0;
         };
-        std::vector<double> op(nr * ns, 0.0);
-        for (std::size_t blk = 0; blk < b.blocks.size(); ++blk) {
-            const HostBlock& hb = b.blocks[blk];
-            const DevShell& d = b.shells[blk];
-            const int nc = hb.ncart();
-            for (int k = 0; k < nc; ++k) {
-                const double gk = component_factor(hb.powers[0], hb.powers[static_cast<std::size_t>(k)]);
-                double* row = op.data() + static_cast<std::size_t>(d.crow + k) * ns;
-                for (std::size_t slot = 0; slot < ns; ++slot) {
-                    double v;
-                    if (hb.pure.empty()) {
-                        v = coef(hb.offset + k, slot);
-                    } else {
-                        v = 0.0;
-                        for (int r = 0; r < hb.count; ++r)
-                            v += hb.pure[static_cast<std::size_t>(r * nc + k)] * coef(hb.offset + r, slot);
-                    }
-                    row[slot] = gk * v;
-                }
-            }
-            if (!hb.pure.empty())
-                for (int r = 0; r < hb.count; ++r) {
-                    double* row = op.data() + static_cast<std::size_t>(d.prow + r) * ns;
-                    for (std::size_t slot = 0; slot < ns; ++slot)
-                        row[slot] = coef(hb.offset + r, slot);
-                }
-        }
+        const std::vector<double> op = operand_table(b, ns, coef);
         std::vector<float> o32(op.size());
         // bf16 planes: each term is the bf16 rounding of what the previous
         // terms leave (residuals exact in FP64); three terms carry about 24
@@ -427,6 +437,50 @@ const double* kumo_hip_molecule::screen_r2(double threshold, hipStream_t stream)
 // Primitive pairs of the exact ESP: P contracted with the Hermite expansion
 // coefficients of every pair of Cartesian primitives of blocks a <= b
 // (src/electrostatic_potential.cpp:117-229), grouped by L = la + lb.
+void kumo_hip_molecule::ensure_orbital_energy(hipStream_t stream) {
+    using namespace kumo;
+    std::lock_guard<std::mutex> guard(lazy);
+    if (oe_ready) return;
+    std::vector<OeSlot> slots;
+    const auto pad = [&] {
+        while (slots.size() % 16) slots.push_back({0, 0, -1, 0.0});
+    };
+    // Occupied MOs (src/orbital_energy_fields.hpp:160-187): restricted, the
+    // first min(nalpha, nbeta) doubly and the rest up to max(nalpha, nbeta)
+    // singly occupied; unrestricted, each channel's occupied MOs singly.
+    const int nocc_r = std::max(nalpha, nbeta), ndouble = std::min(nalpha, nbeta);
+    if (unrestricted) {
+        for (int i = 0; i < nalpha; ++i) slots.push_back({1, 0, i, 1.0});
+        for (int i = 0; i < nbeta; ++i) slots.push_back({1, 1, i, 1.0});
+    } else {
+        for (int i = 0; i < nocc_r; ++i) slots.push_back({1, 0, i, i < ndouble ? 2.0 : 1.0});
+    }
+    pad();
+    oe_occ_slots = static_cast<int>(slots.size());
+    if (unrestricted) {
+        for (int i = nalpha; i < nmo_a; ++i) slots.push_back({2, 0, i, 0.0});
+        for (int i = nbeta; i < nmo_b; ++i) slots.push_back({2, 1, i, 0.0});
+    } else {
+        for (int i = nocc_r; i < nmo_a; ++i) slots.push_back({2, 0, i, 0.0});
+    }
+    pad();
+    if (slots.empty()) slots.assign(16, OeSlot{0, 0, -1, 0.0});
+    const auto coef = [&](int ao, std::size_t slot) -> double {
+        const OeSlot& q = slots[slot];
+        if (q.kind == 0) return 0.0;
+        const auto& C = q.channel == 1 ? Cb : Ca;
+        const int nmo = q.channel == 1 ? nmo_b : nmo_a;
+        return C[static_cast<std::size_t>(ao) * static_cast<std::size_t>(nmo) + static_cast<std::size_t>(q.mo)];
+    };
+    const std::vector<double> op = operand_table(basis, slots.size(), coef);
+    d_oe_op = upload(op.data(), op.size(), stream);
+    KUMO_HIP(hipStreamSynchronize(stream));
+    oe_slots = static_cast<int>(slots.size());
+    oe_slot = std::move(slots);
+    device_bytes += static_cast<long long>(d_oe_op.bytes());
+    oe_ready = true;
+}
+
 void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
     using namespace kumo;
     std::lock_guard<std::mutex> guard(lazy);

@@ -61,9 +61,27 @@ struct ComponentPlan {
 struct Plan {
     std::array<ComponentPlan, 4> c{};
     bool orbitals = false;
+    // Orbital-energy fields (src/evaluation_plan.hpp:93-95): the occupied
+    // sums, plus the virtual-orbital sums, plus the orbital-weighted Fukui sums.
+    bool orbital_energies = false, virtual_orbitals = false, orbital_weighted_fukui = false;
     int max_order = -1;
     bool esp() const { return c[kTotal].esp; }
 };
+
+// orbital_energy_name (src/evaluation_plan.hpp:112-128).
+inline const char* orbital_energy_name(int kind) {
+    switch (kind) {
+    case kAlie: return "the average local ionization energy";
+    case kLea: return "the local electron affinity";
+    case kLeae: return "the local electron attachment energy";
+    case kElectronegativity: return "the local electronegativity";
+    case kHardness: return "the local hardness";
+    case kFukuiPlus: return "the orbital-weighted Fukui function f+";
+    case kFukuiMinus: return "the orbital-weighted Fukui function f-";
+    case kFukuiZero: return "the orbital-weighted Fukui function f0";
+    default: return "the orbital-weighted dual descriptor";
+    }
+}
 
 inline void require_total(const kumo_hip_property& p, const char* what) {
     if (p.component != kTotal) invalid(std::string(what) + " is defined only for the total density");
@@ -124,10 +142,19 @@ inline Plan make_plan(const kumo_hip_property* props, std::size_t n) {
             plan.c[kTotal].value_input = true;
             c.value_input = true;
             break;
+        case kAlie:
+            require_total(p, orbital_energy_name(p.kind));
+            plan.orbital_energies = true;
+            break;
+        case kLea: case kLeae: case kElectronegativity: case kHardness:
+            require_total(p, orbital_energy_name(p.kind));
+            plan.orbital_energies = plan.virtual_orbitals = true;
+            break;
+        case kFukuiPlus: case kFukuiMinus: case kFukuiZero: case kDualDescriptor:
+            require_total(p, orbital_energy_name(p.kind));
+            plan.orbital_energies = plan.orbital_weighted_fukui = true;
+            break;
         default:
-            if (is_orbital_energy(p.kind))
-                invalid("orbital-energy fields (ALIE, LEA, LEAE, local electronegativity and hardness, "
-                        "orbital-weighted Fukui functions) are not evaluated by the HIP backend");
             invalid("unknown property kind");
         }
     }

@@ -26,7 +26,8 @@ namespace kumo {
 struct Executor {
     hipStream_t stream = nullptr;
     const double* boys_table = nullptr;
-    DeviceBuffer ws_tables, ws_moments, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale;
+    DeviceBuffer ws_tables, ws_moments, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale,
+        ws_oe, ws_oe_tasks;
     // Events bracketing the contraction launches (index 0), the FP64 rescue
     // launches (index 1) and the field reductions (index 2).
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
@@ -53,6 +54,14 @@ struct AtomRec {
     int z;
     double charge;
     double xyz[3];
+};
+
+// One MO slot of the orbital-energy operand table.
+struct OeSlot {
+    int kind;          // 0 padding, 1 occupied, 2 virtual
+    int channel;       // 0 alpha, 1 beta
+    int mo;            // zero-based MO index in its channel
+    double occupation; // n_i of an occupied MO (2 doubly, 1 singly occupied)
 };
 
 } // namespace kumo
@@ -112,6 +121,14 @@ struct kumo_hip_molecule {
     bool esp_ready = false;
     kumo::DeviceBuffer d_esp_pairs, d_esp_coef, d_esp_atoms;
     int esp_npairs = 0;
+    // Orbital-energy operand (OrbitalEnergyFields, src/orbital_energy_fields.hpp:98-192;
+    // built on first use): every MO of the FCHK in FP64, slots [0,
+    // oe_occ_slots) the occupied MOs, then the virtual ones, each range
+    // padded to 16.
+    bool oe_ready = false;
+    kumo::DeviceBuffer d_oe_op;
+    int oe_slots = 0, oe_occ_slots = 0;
+    std::vector<kumo::OeSlot> oe_slot;
     long long device_bytes = 0;
     std::mutex lazy; // guards the data built on first use
 
@@ -123,6 +140,7 @@ struct kumo_hip_molecule {
     const double* p_matrix(int component, hipStream_t stream);
     const double* screen_r2(double threshold, hipStream_t stream);
     void ensure_esp(hipStream_t stream);
+    void ensure_orbital_energy(hipStream_t stream);
 };
 
 namespace kumo {

@@ -139,6 +139,22 @@ struct DensityDescriptor {
     constexpr Property spin_polarization() const { return p(PropertyKind::SpinPolarization); }
     constexpr Property shannon_entropy_density() const { return p(PropertyKind::ShannonEntropyDensity); }
     constexpr Property fisher_information_density() const { return p(PropertyKind::FisherInformationDensity); }
+    // Orbital-energy fields (include/chem/kumo/evaluator.hpp:296-338; Total only).
+    constexpr Property average_local_ionization_energy() const {
+        return p(PropertyKind::AverageLocalIonizationEnergy);
+    }
+    constexpr Property local_electron_affinity() const { return p(PropertyKind::LocalElectronAffinity); }
+    constexpr Property local_electron_attachment_energy() const {
+        return p(PropertyKind::LocalElectronAttachmentEnergy);
+    }
+    constexpr Property local_electronegativity() const { return p(PropertyKind::LocalElectronegativity); }
+    constexpr Property local_hardness() const { return p(PropertyKind::LocalHardness); }
+    constexpr Property orbital_weighted_fukui_plus() const { return p(PropertyKind::OrbitalWeightedFukuiPlus); }
+    constexpr Property orbital_weighted_fukui_minus() const { return p(PropertyKind::OrbitalWeightedFukuiMinus); }
+    constexpr Property orbital_weighted_fukui_zero() const { return p(PropertyKind::OrbitalWeightedFukuiZero); }
+    constexpr Property orbital_weighted_dual_descriptor() const {
+        return p(PropertyKind::OrbitalWeightedDualDescriptor);
+    }
 };
 
 namespace density {

@@ -132,6 +132,15 @@ class _Density:
     def spin_polarization(self): return self._p(PropertyKind.SpinPolarization)
     def shannon_entropy_density(self): return self._p(PropertyKind.ShannonEntropyDensity)
     def fisher_information_density(self): return self._p(PropertyKind.FisherInformationDensity)
+    def average_local_ionization_energy(self): return self._p(PropertyKind.AverageLocalIonizationEnergy)
+    def local_electron_affinity(self): return self._p(PropertyKind.LocalElectronAffinity)
+    def local_electron_attachment_energy(self): return self._p(PropertyKind.LocalElectronAttachmentEnergy)
+    def local_electronegativity(self): return self._p(PropertyKind.LocalElectronegativity)
+    def local_hardness(self): return self._p(PropertyKind.LocalHardness)
+    def orbital_weighted_fukui_plus(self): return self._p(PropertyKind.OrbitalWeightedFukuiPlus)
+    def orbital_weighted_fukui_minus(self): return self._p(PropertyKind.OrbitalWeightedFukuiMinus)
+    def orbital_weighted_fukui_zero(self): return self._p(PropertyKind.OrbitalWeightedFukuiZero)
+    def orbital_weighted_dual_descriptor(self): return self._p(PropertyKind.OrbitalWeightedDualDescriptor)
 
 
 class density:  # namespace density (evaluator.hpp:321-327)

@@ -516,6 +516,7 @@ struct Molecule {
     int n = 0;
     std::vector<double> P, S; // total, spin (row-major n x n)
     std::vector<double> Ca, Cb; // n x nmo, row-major (AO rows, MO columns)
+    std::vector<double> Ea, Eb; // orbital energies
     int nmo_a = 0, nmo_b = 0, nalpha = 0, nbeta = 0;
     bool unrestricted = false;
     bool eligible = false;
@@ -615,8 +616,24 @@ struct ComponentPlan {
 struct Plan {
     std::array<ComponentPlan, 4> c{};
     bool orbitals = false, orbital_energies = false;
+    bool virtual_orbitals = false, orbital_weighted_fukui = false;
     int max_order = -1;
 };
+
+// orbital_energy_name (src/evaluation_plan.hpp:112-128).
+const char* orbital_energy_name(int kind) {
+    switch (kind) {
+    case AverageLocalIonizationEnergy: return "the average local ionization energy";
+    case LocalElectronAffinity: return "the local electron affinity";
+    case LocalElectronAttachmentEnergy: return "the local electron attachment energy";
+    case LocalElectronegativity: return "the local electronegativity";
+    case LocalHardness: return "the local hardness";
+    case OrbitalWeightedFukuiPlus: return "the orbital-weighted Fukui function f+";
+    case OrbitalWeightedFukuiMinus: return "the orbital-weighted Fukui function f-";
+    case OrbitalWeightedFukuiZero: return "the orbital-weighted Fukui function f0";
+    default: return "the orbital-weighted dual descriptor";
+    }
+}
 
 void require_total(const orc_property& p, const char* what) {
     if (p.component != Total) throw std::invalid_argument(std::string(what) + " is defined only for the total density");
@@ -673,9 +690,21 @@ Plan make_plan(const std::vector<orc_property>& props) {
             plan.c[Total].value_input = true;
             c.value_input = true;
             break;
+        case AverageLocalIonizationEnergy:
+            require_total(p, orbital_energy_name(p.kind));
+            plan.orbital_energies = true;
+            break;
+        case LocalElectronAffinity: case LocalElectronAttachmentEnergy: case LocalElectronegativity:
+        case LocalHardness:
+            require_total(p, orbital_energy_name(p.kind));
+            plan.orbital_energies = plan.virtual_orbitals = true;
+            break;
+        case OrbitalWeightedFukuiPlus: case OrbitalWeightedFukuiMinus: case OrbitalWeightedFukuiZero:
+        case OrbitalWeightedDualDescriptor:
+            require_total(p, orbital_energy_name(p.kind));
+            plan.orbital_energies = plan.orbital_weighted_fukui = true;
+            break;
         default:
-            if (is_orbital_energy(p.kind))
-                throw std::invalid_argument("orbital-energy fields are not part of this evaluator");
             throw std::invalid_argument("unknown property kind");
         }
     }
@@ -1395,6 +1424,124 @@ bool from_point_path(const orc_property& p, int backend, bool grid) {
            !is_orbital_energy(p.kind);
 }
 
+// ---------------------------------------------------------------------------
+// Orbital-energy fields: OrbitalEnergyFields (src/orbital_energy_fields.hpp:66-262)
+// and store_orbital_energy_block (src/evaluator.cpp:481-520).
+
+// One weighted set of MO columns: each column a coefficient vector (already
+// scaled by the square root of its occupation or weight) with an energy.
+struct OrbitalColumns {
+    std::vector<std::vector<double>> col;
+    std::vector<double> energy;
+    void add(const std::vector<double>& C, int nmo, int mo, int n, double scale, double e) {
+        std::vector<double> c(static_cast<std::size_t>(n));
+        for (int a = 0; a < n; ++a)
+            c[static_cast<std::size_t>(a)] = scale * C[static_cast<std::size_t>(a) * nmo + mo];
+        col.push_back(std::move(c));
+        energy.push_back(e);
+    }
+};
+
+// Normalized Gaussian weights of the orbital-weighted Fukui functions
+// (orbital_weights, src/orbital_energy_fields.hpp:75-96): exponents shifted
+// by their largest value before exponentiation.
+std::vector<double> fukui_weights(const std::vector<double>& e, double mu, double delta) {
+    std::vector<double> x(e.size()), w(e.size());
+    for (std::size_t i = 0; i < e.size(); ++i) {
+        const double t = (mu - e[i]) / delta;
+        x[i] = -t * t;
+    }
+    const double top = *std::max_element(x.begin(), x.end());
+    double sum = 0.0;
+    for (std::size_t i = 0; i < e.size(); ++i) {
+        w[i] = std::exp(x[i] - top);
+        sum += w[i];
+    }
+    for (double& v : w)
+        v /= sum;
+    return w;
+}
+
+struct OrbitalEnergySet {
+    bool virtuals = false, fukui = false;
+    OrbitalColumns occupied, virt, attachment, plus, minus;
+};
+
+OrbitalEnergySet orbital_energy_set(const Molecule& M, const Plan& plan, double delta) {
+    const std::string what = plan.orbital_weighted_fukui ? "the orbital-weighted Fukui function or dual descriptor"
+                             : plan.virtual_orbitals     ? "the local electron affinity and related fields"
+                                                         : "the average local ionization energy";
+    OrbitalEnergySet s;
+    s.virtuals = plan.virtual_orbitals;
+    s.fukui = plan.orbital_weighted_fukui;
+    const auto bad = [&](const std::string& tail) { throw std::invalid_argument(what + tail); };
+    if (s.virtuals && !M.unrestricted && M.nalpha != M.nbeta) bad(" is not defined for a restricted open-shell calculation");
+    if (s.fukui && (M.unrestricted || M.nalpha != M.nbeta)) bad(" is defined only for a restricted closed-shell calculation");
+    if (s.fukui && (!std::isfinite(delta) || delta <= 0.0)) bad(": the Fukui delta must be finite and positive");
+    if (!M.eligible) bad(" needs occupied MOs that reproduce the FCHK density matrices (an SCF calculation)");
+    const int nmo_b = M.unrestricted ? M.nmo_b : M.nmo_a;
+    if (M.Ea.size() < static_cast<std::size_t>(M.nmo_a) ||
+        (M.unrestricted && M.Eb.size() < static_cast<std::size_t>(nmo_b)) ||
+        M.Ea.size() < static_cast<std::size_t>(M.nalpha) ||
+        (M.unrestricted ? M.Eb : M.Ea).size() < static_cast<std::size_t>(M.nbeta))
+        bad(" needs the FCHK orbital energies");
+    const auto channel = [&](const std::vector<double>& C, int nmo, const std::vector<double>& e, int occupied,
+                             int doubly, double weight) {
+        for (int i = 0; i < nmo; ++i) {
+            const double ei = e[static_cast<std::size_t>(i)];
+            if (i < occupied) {
+                s.occupied.add(C, nmo, i, M.n, std::sqrt(i < doubly ? 2.0 : 1.0), std::abs(ei));
+            } else {
+                s.virt.add(C, nmo, i, M.n, 1.0, ei);
+                if (ei < 0.0) s.attachment.add(C, nmo, i, M.n, std::sqrt(weight), ei);
+            }
+        }
+    };
+    if (M.unrestricted) {
+        channel(M.Ca, M.nmo_a, M.Ea, M.nalpha, 0, 1.0);
+        channel(M.Cb, M.nmo_b, M.Eb, M.nbeta, 0, 1.0);
+    } else {
+        channel(M.Ca, M.nmo_a, M.Ea, std::max(M.nalpha, M.nbeta), std::min(M.nalpha, M.nbeta), 2.0);
+    }
+    if (s.virtuals && s.virt.col.empty()) bad(" needs virtual orbitals, and the FCHK has none");
+    if (s.fukui) {
+        const int count = M.nmo_a, homo = M.nalpha;
+        if (homo == 0 || homo >= count) bad(" needs occupied and virtual orbitals, and the FCHK lacks one kind");
+        const double mu = 0.5 * (M.Ea[static_cast<std::size_t>(homo - 1)] + M.Ea[static_cast<std::size_t>(homo)]);
+        const auto columns = [&](int first, int last, OrbitalColumns& out) {
+            const std::vector<double> range(M.Ea.begin() + first, M.Ea.begin() + last);
+            const auto w = fukui_weights(range, mu, delta);
+            for (int i = first; i < last; ++i)
+                if (w[static_cast<std::size_t>(i - first)] > 0.0)
+                    out.add(M.Ca, M.nmo_a, i, M.n, std::sqrt(w[static_cast<std::size_t>(i - first)]), 0.0);
+        };
+        columns(homo, count, s.plus);
+        columns(0, homo, s.minus);
+    }
+    return s;
+}
+
+// Weighted square sums of one column set at the points of a block:
+// density[t] = sum_i (c_i . phi_t)^2, energy[t] = sum_i e_i (c_i . phi_t)^2.
+void column_sums(const OrbitalColumns& oc, const Panels& phi, int n, int size, std::vector<double>& density,
+                 std::vector<double>* energy) {
+    density.assign(static_cast<std::size_t>(size), 0.0);
+    if (energy) energy->assign(static_cast<std::size_t>(size), 0.0);
+    for (int t = 0; t < size; ++t) {
+        double d = 0.0, e = 0.0;
+        for (std::size_t i = 0; i < oc.col.size(); ++i) {
+            double v = 0.0;
+            for (int a = 0; a < n; ++a)
+                v += oc.col[i][static_cast<std::size_t>(a)] * phi.at(a, t);
+            const double sq = v * v;
+            d += sq;
+            e += oc.energy[i] * sq;
+        }
+        density[static_cast<std::size_t>(t)] = d;
+        if (energy) (*energy)[static_cast<std::size_t>(t)] = e;
+    }
+}
+
 template <typename Body> void for_blocks(std::int64_t n, int bs, const Body& body) {
     const std::int64_t nb = n / bs + (n % bs != 0);
     std::exception_ptr err;
@@ -1419,6 +1566,8 @@ struct Field {
 void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_property>& props, const orc_options& opt,
               double** outputs, int* backend_used) {
     if (opt.block_size < 1 || opt.block_size > 65536) throw std::invalid_argument("evaluation block_size must be in [1, 65536]");
+    if (!std::isfinite(opt.fukui_delta) || opt.fukui_delta <= 0.0)
+        throw std::invalid_argument("fukui_delta must be finite and positive");
     if (!std::isfinite(opt.screening_threshold) || opt.screening_threshold < 0.0)
         throw std::invalid_argument("screening threshold must be finite and nonnegative");
     if (opt.backend == AO_MO && !M.eligible)
@@ -1440,6 +1589,9 @@ void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_prop
             off += static_cast<std::size_t>(channels(props[i].kind));
         }
     }
+    // The orbital-energy columns, checked before any pass (src/evaluator.cpp:579-596).
+    OrbitalEnergySet oes;
+    if (plan.orbital_energies) oes = orbital_energy_set(M, plan, opt.fukui_delta);
     // Orbital checks (src/evaluator.cpp:618-632).
     for (const auto& f : fields)
         if (is_orbital_property(f.key.kind)) {
@@ -1476,6 +1628,58 @@ void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_prop
                 }
             });
         }
+    // Orbital-energy pass (src/evaluator.cpp:676-682): AO values without
+    // per-block screening (the shell cutoff only), the column sums, then the
+    // fields of store_orbital_energy_block.
+    if (plan.orbital_energies) {
+        for_blocks(n, bs, [&](std::int64_t start, int size) {
+            Scratch sc;
+            std::vector<double> pts;
+            set.block(start, size, pts);
+            Panels phi;
+            ao_panels(*M.basis, size, pts.data(), 0, phi, sc);
+            std::vector<double> od, oe, vd, ve, unused, ae, fp, fm;
+            column_sums(oes.occupied, phi, M.n, size, od, &oe);
+            if (oes.virtuals) {
+                column_sums(oes.virt, phi, M.n, size, vd, &ve);
+                column_sums(oes.attachment, phi, M.n, size, unused, &ae);
+            }
+            if (oes.fukui) {
+                column_sums(oes.plus, phi, M.n, size, fp, nullptr);
+                column_sums(oes.minus, phi, M.n, size, fm, nullptr);
+            }
+            for (const auto& f : fields) {
+                if (!is_orbital_energy(f.key.kind)) continue;
+                for (int t = 0; t < size; ++t) {
+                    const std::size_t q = static_cast<std::size_t>(t);
+                    const double rho = od[q];
+                    const double alie = rho > 0.0 ? oe[q] / rho : 0.0;
+                    double lea = 0.0, att = 0.0, plus = 0.0, minus = 0.0;
+                    if (oes.virtuals) {
+                        lea = vd[q] > 0.0 ? -ve[q] / vd[q] : 0.0;
+                        att = rho > 0.0 ? ae[q] / rho : 0.0;
+                    }
+                    if (oes.fukui) {
+                        plus = fp[q];
+                        minus = fm[q];
+                    }
+                    double v;
+                    switch (f.key.kind) {
+                    case AverageLocalIonizationEnergy: v = alie; break;
+                    case LocalElectronAffinity: v = lea; break;
+                    case LocalElectronAttachmentEnergy: v = att; break;
+                    case LocalElectronegativity: v = 0.5 * (alie + lea); break;
+                    case LocalHardness: v = 0.5 * (alie - lea); break;
+                    case OrbitalWeightedFukuiPlus: v = plus; break;
+                    case OrbitalWeightedFukuiMinus: v = minus; break;
+                    case OrbitalWeightedFukuiZero: v = 0.5 * (plus + minus); break;
+                    default: v = plus - minus; break;
+                    }
+                    f.out[0][start + t] = v;
+                }
+            }
+        });
+    }
     // Orbital pass (src/evaluator.cpp:688-730): derivatives on grids, all at points.
     std::vector<Field> orbital_fields;
     int dorder = 0;
@@ -1668,6 +1872,11 @@ void* orc_molecule_create(void* basis, int natoms, const int* atomic_numbers, co
 }
 
 void orc_molecule_destroy(void* m) { delete static_cast<Molecule*>(m); }
+void orc_molecule_set_energies(void* m, int n_alpha, const double* e_alpha, int n_beta, const double* e_beta) {
+    auto* M = static_cast<Molecule*>(m);
+    M->Ea.assign(e_alpha, e_alpha + (e_alpha ? n_alpha : 0));
+    M->Eb.assign(e_beta, e_beta + (e_beta ? n_beta : 0));
+}
 int orc_molecule_eligible(const void* m) { return static_cast<const Molecule*>(m)->eligible ? 1 : 0; }
 
 static PointSet make_set(const double* origin, const double* steps, const int* counts, std::int64_t npoints,

@@ -35,6 +35,7 @@ struct orc_options {
     int block_size;             // 1..65536 (reference default 128)
     int backend;                // 0 AUTO, 1 AO_P, 2 AO_MO
     double screening_threshold; // reference default 1e-12
+    double fukui_delta;         // hartree, > 0 (reference default 0.1)
 };
 
 /// Basis from the raw FCHK shell records (src/basis.cpp:21-120).
@@ -63,6 +64,9 @@ void* orc_molecule_create(void* basis, int natoms, const int* atomic_numbers,
                           int nmo_beta, const double* c_beta, int nalpha, int nbeta,
                           char* error, int error_len);
 void orc_molecule_destroy(void* molecule);
+/// SCF orbital energies of each channel (FCHK "Alpha/Beta Orbital Energies").
+void orc_molecule_set_energies(void* molecule, int n_alpha, const double* e_alpha, int n_beta,
+                               const double* e_beta);
 int orc_molecule_eligible(const void* molecule);
 
 /// Points: a grid (origin[3], steps[9] column-major as Eigen Matrix3d,

@@ -28,7 +28,12 @@ MULTIWFN = ["h2o_elf", "h2o_rdg", "h2o_sign_lambda2_rho", "o2_elf", "h2o_esp", "
             "h2o_iri", "h2o_dori", "h2o_kinetic_k", "o2_kinetic_k", "h2o_potential_energy", "h2o_energy_density",
             "o2_spin_polarization", "h2o_shannon", "h2o_fisher", "h2o_elf_ts", "h2o_lol_ts", "h2o_kinetic_g",
             "h2o_kinetic_g_x", "h2o_kinetic_k_x", "h2o_potential_kinetic_ratio", "h2o_energy_density_per_electron",
-            "h2o_kinetic_g_per_electron"]
+            "h2o_kinetic_g_per_electron",
+            # orbital-energy fields (tests/multiwfn_reference_test.cpp:395-430, 485-486)
+            "acrolein_alie", "o2_alie", "acrolein_lea", "o2_lea", "acrolein_leae", "o2_leae",
+            "acrolein_local_electronegativity", "acrolein_local_hardness", "acrolein_ow_fukui_plus",
+            "acrolein_ow_fukui_minus", "acrolein_ow_fukui_zero", "acrolein_ow_dual_descriptor", "h2o_ow_fukui_plus",
+            "h2o_ow_fukui_minus"]
 DENSITY = [("o2_density", "o2", 0), ("o2_alpha", "o2", 1), ("o2_beta", "o2", 2), ("o2_spin", "o2", 3),
            ("acrolein_density", "acrolein", 0)]
 MO = [("o2_alpha_mo_3", "o2", 0, 3), ("o2_alpha_mo_9", "o2", 0, 9), ("o2_alpha_mo_10", "o2", 0, 10),

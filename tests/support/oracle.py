@@ -17,7 +17,8 @@ class Prop(C.Structure):
 
 
 class Options(C.Structure):
-    _fields_ = [("block_size", C.c_int), ("backend", C.c_int), ("screening_threshold", C.c_double)]
+    _fields_ = [("block_size", C.c_int), ("backend", C.c_int), ("screening_threshold", C.c_double),
+                ("fukui_delta", C.c_double)]
 
 
 def _ptr(a, t=C.c_double):
@@ -50,6 +51,7 @@ def lib():
         L.orc_grid_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
         L.orc_pure_transform.argtypes = [C.c_int, C.c_void_p]
         L.orc_molecule_destroy.argtypes = [C.c_void_p]
+        L.orc_molecule_set_energies.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_basis_destroy.argtypes = [C.c_void_p]
         L.orc_write_field_cube.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,
@@ -153,13 +155,17 @@ class Molecule:
                                            d.nalpha, d.nbeta, err, 512)
         if not self.h:
             raise RuntimeError(err.value.decode())
+        ea = keep(getattr(d, "e_alpha", None))
+        eb = keep(getattr(d, "e_beta", None))
+        lib().orc_molecule_set_energies(self.h, 0 if ea is None else ea.size, _ptr(ea), 0 if eb is None else eb.size,
+                                        _ptr(eb))
         self.eligible = bool(lib().orc_molecule_eligible(self.h))
 
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_molecule_destroy(self.h)
 
-    def evaluate(self, props, grid=None, points=None, block_size=128, backend=0, threshold=1e-12):
+    def evaluate(self, props, grid=None, points=None, block_size=128, backend=0, threshold=1e-12, fukui_delta=0.1):
         """props: list of (kind, component, channel, orbital_index).
         Returns (list of arrays channels x n, backend_used)."""
         from kumo_hip.api import channels
@@ -182,7 +188,7 @@ class Molecule:
         arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
         used = C.c_int(0)
         err = C.create_string_buffer(512)
-        opt = Options(block_size, backend, threshold)
+        opt = Options(block_size, backend, threshold, fukui_delta)
         rc = lib().orc_evaluate(self.h, _ptr(origin), _ptr(steps), _ptr(counts, C.c_int), n, _ptr(pts), P,
                                 len(props), C.byref(opt), arr, C.byref(used), err, 512)
         if rc == 1:

@@ -24,8 +24,9 @@ from scipy import integrate, special
 from kumo_hip.api import Backend, DensityComponent as DC, PropertyKind as K
 from support import oracle
 from support.fchk import FchkData
-from support.reference_cases import (DENSITY_CUBES, FCHK, MO_CUBES, MULTIWFN, check_density_cube, check_mo_cube,
-                                     check_multiwfn, load_sample)
+from support.reference_cases import (DENSITY_CUBES, FCHK, MO_CUBES, MULTIWFN, ORBITAL_ENERGY, ORBITAL_WEIGHTED,
+                                     check_density_cube, check_mo_cube, check_multiwfn, check_orbital_weighted,
+                                     load_sample)
 
 _MOLS = {}
 
@@ -180,6 +181,52 @@ def test_oracle_against_multiwfn(case):
     def ev(grid, props):
         return mol.evaluate(props, grid=grid)[0]
     check_multiwfn(case, ev, mol.data.xyz)
+
+
+# ------------------------------------------------- orbital-energy fields
+@pytest.mark.parametrize("case", ORBITAL_ENERGY, ids=[c[0] for c in ORBITAL_ENERGY])
+def test_oracle_orbital_energy_fields_against_multiwfn(case):
+    mol = molecule(case[1])
+
+    def ev(grid, props):
+        return mol.evaluate(props, grid=grid)[0]
+    check_multiwfn(case, ev, mol.data.xyz)
+
+
+@pytest.mark.parametrize("case", ORBITAL_WEIGHTED, ids=[c[0] for c in ORBITAL_WEIGHTED])
+def test_oracle_orbital_weighted_fukui_against_multiwfn(case):
+    mol = molecule(case[1])
+
+    def ev(grid, props):
+        return mol.evaluate(props, grid=grid)[0]
+    check_orbital_weighted(case, ev, np.asarray(mol.data.e_alpha), mol.data.nalpha)
+
+
+def test_oracle_orbital_energy_identities_and_errors():
+    """chi = (ALIE + LEA) / 2 and eta = (ALIE - LEA) / 2 with ALIE > 0 where
+    rho > 0; f0 = (f+ + f-) / 2, dual = f+ - f-; f+ and f- integrate to about
+    one electron (each set of weights is normalized); the errors of
+    src/orbital_energy_fields.hpp:117-135 for an open shell and a nonpositive
+    delta (src/evaluator.cpp:532-533)."""
+    mol = molecule("acrolein")
+    grid = {"origin": [-6.0, -6.0, -4.0], "steps": np.eye(3) * 0.4, "counts": [31, 31, 21]}
+    kinds = [K.AverageLocalIonizationEnergy, K.LocalElectronAffinity, K.LocalElectronegativity, K.LocalHardness,
+             K.OrbitalWeightedFukuiPlus, K.OrbitalWeightedFukuiMinus, K.OrbitalWeightedFukuiZero,
+             K.OrbitalWeightedDualDescriptor, K.Density]
+    o, _ = mol.evaluate([(k, 0, 0, 0) for k in kinds], grid=grid)
+    alie, lea, chi, eta, fp, fm, f0, dual, rho = (x[0] for x in o)
+    assert np.allclose(chi, 0.5 * (alie + lea), rtol=1e-14, atol=1e-16)
+    assert np.allclose(eta, 0.5 * (alie - lea), rtol=1e-14, atol=1e-16)
+    assert np.all(alie[rho > 1e-30] > 0)
+    assert np.array_equal(f0, 0.5 * (fp + fm)) and np.array_equal(dual, fp - fm)
+    dv = 0.4 ** 3
+    assert abs(fp.sum() * dv - 1.0) < 0.05 and abs(fm.sum() * dv - 1.0) < 0.05
+    with pytest.raises(ValueError, match="defined only for a restricted closed-shell calculation"):
+        molecule("o2").evaluate([(K.OrbitalWeightedFukuiPlus, 0, 0, 0)], points=np.zeros((1, 3)))
+    with pytest.raises(ValueError, match="fukui_delta must be finite and positive"):
+        mol.evaluate([(K.AverageLocalIonizationEnergy, 0, 0, 0)], points=np.zeros((1, 3)), fukui_delta=0.0)
+    with pytest.raises(ValueError, match="the local electron affinity is defined only for the total density"):
+        mol.evaluate([(K.LocalElectronAffinity, 1, 0, 0)], points=np.zeros((1, 3)))
 
 
 # ---------------------------------------------------------------- invariants
