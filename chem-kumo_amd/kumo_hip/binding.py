@@ -13,6 +13,7 @@ LIBRARY = os.path.join(HERE, "libkumo_hip.so")
 _lib = None
 
 STATUS = {1: ValueError, 2: IndexError, 3: RuntimeError, 4: RuntimeError, 5: RuntimeError}
+HARTREE_IN_KCAL = 627.5094740631  # include/chem/kumo/surface.hpp:17
 
 
 class _Property(C.Structure):
@@ -51,6 +52,16 @@ class _Info(C.Structure):
                 ("spin_relative_error", C.c_double), ("device_bytes", C.c_int64)]
 
 
+class _SurfaceStats(C.Structure):
+    _fields_ = [("area", C.c_double), ("positive_area", C.c_double), ("negative_area", C.c_double),
+                ("nonpolar_area", C.c_double), ("minimum", C.c_double), ("maximum", C.c_double),
+                ("minimum_vertex", C.c_int64), ("maximum_vertex", C.c_int64), ("average", C.c_double),
+                ("positive_average", C.c_double), ("negative_average", C.c_double),
+                ("positive_variance", C.c_double), ("negative_variance", C.c_double),
+                ("total_variance", C.c_double), ("balance", C.c_double), ("charge_separation", C.c_double),
+                ("polarity_index", C.c_double), ("local_minima", C.c_int64), ("local_maxima", C.c_int64)]
+
+
 class _ServerOptions(C.Structure):
     _fields_ = [("workers", C.c_int32), ("reserved", C.c_int32), ("hbm_budget", C.c_int64),
                 ("chunk_points", C.c_int64)]
@@ -69,7 +80,11 @@ EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip
            "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_release", "kumo_hip_server_create",
            "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release",
            "kumo_hip_write_cube", "kumo_hip_write_orbital_cube", "kumo_hip_format_scientific",
-           "kumo_hip_last_cube_stats"]
+           "kumo_hip_last_cube_stats", "kumo_hip_density_isosurface", "kumo_hip_electrostatic_potential_surface",
+           "kumo_hip_local_ionization_energy_surface", "kumo_hip_map_on_density_isosurface",
+           "kumo_hip_average_local_ionization_energy", "kumo_hip_surface_sizes", "kumo_hip_surface_mesh",
+           "kumo_hip_surface_field", "kumo_hip_surface_release", "kumo_hip_surface_statistics",
+           "kumo_hip_write_ply"]
 
 
 def lib():
@@ -124,6 +139,20 @@ def lib():
                                                   vp, vp, C.c_int32, C.c_size_t]
         L.kumo_hip_format_scientific.argtypes = [vp, vp, C.c_size_t, C.c_int32, C.c_int32, vp]
         L.kumo_hip_last_cube_stats.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.kumo_hip_density_isosurface.argtypes = [vp, vp, C.c_double, C.c_double, vp]
+        L.kumo_hip_electrostatic_potential_surface.argtypes = [vp, vp, C.c_double, C.c_double, C.c_double, vp]
+        L.kumo_hip_local_ionization_energy_surface.argtypes = [vp, vp, C.c_double, C.c_double, vp]
+        L.kumo_hip_map_on_density_isosurface.argtypes = [vp, vp, vp, C.c_size_t, C.c_double, C.c_double,
+                                                         C.POINTER(_Options), vp]
+        L.kumo_hip_average_local_ionization_energy.argtypes = [vp, vp, vp, C.c_size_t, vp]
+        L.kumo_hip_surface_sizes.argtypes = [vp, vp, vp, vp, vp]
+        L.kumo_hip_surface_mesh.argtypes = [vp, vp, vp]
+        L.kumo_hip_surface_field.argtypes = [vp, C.c_int32, vp, C.POINTER(_SurfaceStats), vp, vp]
+        L.kumo_hip_surface_release.argtypes = [vp]
+        L.kumo_hip_surface_statistics.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, vp, C.c_double,
+                                                  C.POINTER(_SurfaceStats), vp, vp, vp, vp]
+        L.kumo_hip_write_ply.argtypes = [vp, C.c_char_p, vp, C.c_int64, vp, C.c_int64, C.c_int32, vp, vp,
+                                         C.c_int32, vp]
         _lib = L
     return _lib
 
@@ -214,6 +243,71 @@ class Context:
                 "bytes": b.value}
 
 
+    def surface_statistics(self, vertices, triangles, values, polar_threshold=10.0 / HARTREE_IN_KCAL):
+        """surface_statistics (src/surface.cpp:418-529) with the mesh's area and
+        volume (TriangleMesh::area / volume)."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.int64).reshape(-1, 3)
+        x = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        if x.size != v.shape[0]:
+            raise ValueError("surface values do not match the mesh vertices")
+        st = _SurfaceStats()
+        area, volume = C.c_double(), C.c_double()
+        self.check(lib().kumo_hip_surface_statistics(self.h, _p(v), v.shape[0], _p(t), t.shape[0], _p(x),
+                                                     polar_threshold, C.byref(st), None, None, C.byref(area),
+                                                     C.byref(volume)))
+        mn = np.zeros(st.local_minima, dtype=np.int64)
+        mx = np.zeros(st.local_maxima, dtype=np.int64)
+        self.check(lib().kumo_hip_surface_statistics(self.h, _p(v), v.shape[0], _p(t), t.shape[0], _p(x),
+                                                     polar_threshold, C.byref(st), _p(mn), _p(mx), None, None))
+        return _stats_dict(st, mn, mx), area.value, volume.value
+
+    def write_ply(self, path, vertices, triangles, fields=(), comments=()):
+        """write_ply (src/surface.cpp:701-720); fields: (name, values) pairs."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, dtype=np.int64).reshape(-1, 3)
+        vals = [np.ascontiguousarray(x, dtype=np.float64).ravel() for _, x in fields]
+        for x in vals:
+            if x.size != v.shape[0]:
+                raise ValueError("PLY properties need one value per vertex")
+        names = (C.c_char_p * max(1, len(vals)))(*[n.encode() for n, _ in fields])
+        ptrs = (C.c_void_p * max(1, len(vals)))(*[x.ctypes.data for x in vals])
+        cmts = (C.c_char_p * max(1, len(comments)))(*[c.encode() for c in comments])
+        self.check(lib().kumo_hip_write_ply(self.h, os.fsencode(path), _p(v), v.shape[0], _p(t), t.shape[0],
+                                            len(vals), names, ptrs, len(comments), cmts))
+
+
+def _stats_dict(st, minima, maxima):
+    d = {f: getattr(st, f) for f, _ in _SurfaceStats._fields_}
+    d["local_minima"], d["local_maxima"] = minima, maxima
+    return d
+
+
+class Surface:
+    """A density isosurface (kumo_hip_surface): vertices (n x 3, bohr),
+    triangles (m x 3), and per field its vertex values and statistics."""
+
+    def __init__(self, ctx, handle):
+        nv, nt, nf, be = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        L = lib()
+        L.kumo_hip_surface_sizes(handle, C.byref(nv), C.byref(nt), C.byref(nf), C.byref(be))
+        self.vertices = np.zeros((nv.value, 3))
+        self.triangles = np.zeros((nt.value, 3), dtype=np.int64)
+        L.kumo_hip_surface_mesh(handle, _p(self.vertices), _p(self.triangles))
+        self.values, self.statistics = [], []
+        for f in range(nf.value):
+            st = _SurfaceStats()
+            L.kumo_hip_surface_field(handle, f, None, C.byref(st), None, None)
+            vals = np.zeros(nv.value)
+            mn = np.zeros(st.local_minima, dtype=np.int64)
+            mx = np.zeros(st.local_maxima, dtype=np.int64)
+            L.kumo_hip_surface_field(handle, f, _p(vals), C.byref(st), _p(mn), _p(mx))
+            self.values.append(vals)
+            self.statistics.append(_stats_dict(st, mn, mx))
+        self.backend_used = be.value
+        L.kumo_hip_surface_release(handle)
+
+
 def grid_struct(grid):
     g = _Grid()
     o = np.asarray(grid["origin"], dtype=np.float64)
@@ -301,6 +395,36 @@ class Molecule:
         if self.h:
             lib().kumo_hip_molecule_release(self.h)
             self.h = C.c_void_p()
+
+    def _surface(self, call):
+        h = C.c_void_p()
+        self.ctx.check(call(C.byref(h)))
+        return Surface(self.ctx, h)
+
+    def density_isosurface(self, isovalue=1e-3, spacing=0.25):
+        return self._surface(lambda out: lib().kumo_hip_density_isosurface(self.ctx.h, self.h, isovalue, spacing,
+                                                                            out))
+
+    def electrostatic_potential_surface(self, isovalue=1e-3, spacing=0.25, potential_tolerance=1e-8):
+        return self._surface(lambda out: lib().kumo_hip_electrostatic_potential_surface(
+            self.ctx.h, self.h, isovalue, spacing, potential_tolerance, out))
+
+    def local_ionization_energy_surface(self, isovalue=1e-3, spacing=0.25):
+        return self._surface(lambda out: lib().kumo_hip_local_ionization_energy_surface(
+            self.ctx.h, self.h, isovalue, spacing, out))
+
+    def map_on_density_isosurface(self, props, isovalue=1e-3, spacing=0.25, **opts):
+        P = (_Property * max(1, len(props)))(*[_Property(*map(int, p)) for p in props])
+        o = options(**opts)
+        return self._surface(lambda out: lib().kumo_hip_map_on_density_isosurface(
+            self.ctx.h, self.h, P, len(props), isovalue, spacing, C.byref(o), out))
+
+    def average_local_ionization_energy(self, points):
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(pts.shape[0])
+        self.ctx.check(lib().kumo_hip_average_local_ionization_energy(self.ctx.h, self.h, _p(pts), pts.shape[0],
+                                                                      _p(out)))
+        return out
 
     def atoms(self):
         """(atomic numbers, nuclear charges, coordinates natoms x 3), as a Field keeps them."""

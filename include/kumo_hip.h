@@ -279,6 +279,74 @@ int kumo_hip_write_orbital_cube(kumo_hip_context* ctx, const char* path, const c
 int kumo_hip_format_scientific(kumo_hip_context* ctx, const double* values, size_t n, int32_t precision,
                                int32_t exact_only, char* text);
 
+/* ---- molecular surfaces (include/chem/kumo/surface.hpp) ----
+ * A kumo_hip_surface holds a closed triangle mesh of a total-density
+ * isosurface (vertices in bohr, triangles counter-clockwise seen from
+ * outside) and zero or more per-vertex fields, each with its statistics. The
+ * density on the construction grid, its value and gradient at the vertices
+ * during their placement, and the mapped fields are evaluated on the GPU. */
+typedef struct kumo_hip_surface kumo_hip_surface;
+
+/* SurfaceStatistics (include/chem/kumo/surface.hpp:36-74); the local extrema
+ * lists are returned by kumo_hip_surface_field / kumo_hip_surface_statistics. */
+typedef struct kumo_hip_surface_stats {
+    double area, positive_area, negative_area, nonpolar_area;
+    double minimum, maximum;
+    int64_t minimum_vertex, maximum_vertex;
+    double average, positive_average, negative_average;
+    double positive_variance, negative_variance, total_variance;
+    double balance, charge_separation, polarity_index;
+    int64_t local_minima, local_maxima; /* counts */
+} kumo_hip_surface_stats;
+
+/* density_isosurface (include/chem/kumo/surface.hpp:92-106, src/surface.cpp:
+ * 150-331, 341-369): no fields. */
+int kumo_hip_density_isosurface(kumo_hip_context* ctx, const kumo_hip_molecule* mol, double isovalue,
+                                double spacing, kumo_hip_surface** surface);
+/* electrostatic_potential_surface (include/chem/kumo/surface.hpp:108-131,
+ * src/surface.cpp:612-662): field 0 the potential (exact at every tolerance),
+ * statistics with the polar threshold 10 kcal/mol. */
+int kumo_hip_electrostatic_potential_surface(kumo_hip_context* ctx, const kumo_hip_molecule* mol, double isovalue,
+                                             double spacing, double potential_tolerance,
+                                             kumo_hip_surface** surface);
+/* local_ionization_energy_surface (include/chem/kumo/surface.hpp:155-173,
+ * src/surface.cpp:594-610): field 0 the average local ionization energy. */
+int kumo_hip_local_ionization_energy_surface(kumo_hip_context* ctx, const kumo_hip_molecule* mol, double isovalue,
+                                             double spacing, kumo_hip_surface** surface);
+/* map_on_density_isosurface (include/chem/kumo/surface.hpp:175-195,
+ * src/surface.cpp:664-699): one field per distinct one-component property,
+ * statistics with polar threshold 0; options as for an evaluation. */
+int kumo_hip_map_on_density_isosurface(kumo_hip_context* ctx, const kumo_hip_molecule* mol,
+                                       const kumo_hip_property* properties, size_t nproperties, double isovalue,
+                                       double spacing, const kumo_hip_options* options,
+                                       kumo_hip_surface** surface);
+/* average_local_ionization_energy at n points (include/chem/kumo/surface.hpp:
+ * 133-153, src/surface.cpp:563-592). */
+int kumo_hip_average_local_ionization_energy(kumo_hip_context* ctx, const kumo_hip_molecule* mol,
+                                             const double* points, size_t npoints, double* out);
+/* Sizes of a surface; backend_used of map_on_density_isosurface (else -1). */
+int kumo_hip_surface_sizes(const kumo_hip_surface* surface, int64_t* nvertices, int64_t* ntriangles,
+                           int32_t* nfields, int32_t* backend_used);
+/* The mesh: 3 * nvertices coordinates, 3 * ntriangles vertex indices. */
+int kumo_hip_surface_mesh(const kumo_hip_surface* surface, double* vertices, int64_t* triangles);
+/* Field `field`: nvertices values, its statistics, and the local minima /
+ * maxima (stats.local_minima / local_maxima entries each); NULL skips. */
+int kumo_hip_surface_field(const kumo_hip_surface* surface, int32_t field, double* values,
+                           kumo_hip_surface_stats* stats, int64_t* local_minima, int64_t* local_maxima);
+int kumo_hip_surface_release(kumo_hip_surface* surface);
+/* surface_statistics (include/chem/kumo/surface.hpp:76-84) of vertex values
+ * on a caller's mesh; the extrema arrays may be NULL (call once for the
+ * counts). area and volume: TriangleMesh::area / volume (may be NULL). */
+int kumo_hip_surface_statistics(kumo_hip_context* ctx, const double* vertices, int64_t nvertices,
+                                const int64_t* triangles, int64_t ntriangles, const double* values,
+                                double polar_threshold, kumo_hip_surface_stats* stats, int64_t* local_minima,
+                                int64_t* local_maxima, double* area, double* volume);
+/* write_ply (include/chem/kumo/surface.hpp:197-210, src/surface.cpp:701-720):
+ * ASCII PLY with per-vertex fields named names[f]. */
+int kumo_hip_write_ply(kumo_hip_context* ctx, const char* path, const double* vertices, int64_t nvertices,
+                       const int64_t* triangles, int64_t ntriangles, int32_t nfields, const char* const* names,
+                       const double* const* values, int32_t ncomments, const char* const* comments);
+
 /* ---- timing hooks for the benchmark ----
  * Device time (ms) of the last evaluation's dominant contraction kernel,
  * summed over its launches and measured with HIP events on the stream the

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <fstream>
 #include <iomanip>
 #include <map>
@@ -1766,6 +1767,367 @@ void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_prop
     }
 }
 
+// ---------------------------------------------------------------------------
+// Molecular surfaces: src/surface.cpp (isosurface extraction :150-331, density
+// surface :341-369, statistics :377-529).
+
+struct OMesh {
+    std::vector<std::array<double, 3>> v;
+    std::vector<std::array<std::int64_t, 3>> t;
+};
+
+using ScalarFn = std::function<void(const std::vector<std::array<double, 3>>&, std::vector<double>&,
+                                    std::vector<std::array<double, 3>>&)>;
+
+struct GridSpec {
+    std::array<double, 3> origin;
+    std::array<double, 9> steps; // column a = steps[3a .. 3a + 2]
+    std::array<int, 3> counts;
+    std::array<double, 3> at(std::int64_t q) const {
+        PointSet ps;
+        ps.origin = origin.data();
+        ps.steps = steps.data();
+        ps.counts = counts.data();
+        std::array<double, 3> r;
+        ps.point(q, r.data());
+        return r;
+    }
+    std::int64_t index(int i, int j, int k) const {
+        return (static_cast<std::int64_t>(i) * counts[1] + j) * counts[2] + k;
+    }
+};
+
+bool reaches_boundary(const GridSpec& g, const std::vector<double>& f, double iso) {
+    for (int i = 0; i < g.counts[0]; ++i)
+        for (int j = 0; j < g.counts[1]; ++j)
+            for (int k = 0; k < g.counts[2]; ++k) {
+                const bool edge = i == 0 || j == 0 || k == 0 || i == g.counts[0] - 1 || j == g.counts[1] - 1 ||
+                                  k == g.counts[2] - 1;
+                if (edge && f[static_cast<std::size_t>(g.index(i, j, k))] >= iso) return true;
+            }
+    return false;
+}
+
+std::array<double, 3> sub3(const std::array<double, 3>& a, const std::array<double, 3>& b) {
+    return {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+}
+std::array<double, 3> cross3(const std::array<double, 3>& a, const std::array<double, 3>& b) {
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+double dot3(const std::array<double, 3>& a, const std::array<double, 3>& b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+OMesh isosurface(const GridSpec& g, const std::vector<double>& f, double iso, const ScalarFn& fn) {
+    if (g.counts[0] < 2 || g.counts[1] < 2 || g.counts[2] < 2)
+        throw std::invalid_argument("isosurface grid needs at least two points along each axis");
+    if (f.size() != static_cast<std::size_t>(g.counts[0]) * g.counts[1] * g.counts[2])
+        throw std::invalid_argument("isosurface samples do not match the grid");
+    if (!std::isfinite(iso)) throw std::invalid_argument("isovalue must be finite");
+    for (double x : f)
+        if (!std::isfinite(x)) throw std::invalid_argument("isosurface samples must be finite");
+    if (reaches_boundary(g, f, iso))
+        throw std::invalid_argument("the isosurface reaches the grid boundary, so it would not be closed");
+    // Cells in Field order, six tetrahedra each (corners 0, e_a, e_a + e_b, 7
+    // for the permutations (a, b, c)); vertices numbered at first use of their
+    // edge, the edge named by its lower end and its direction.
+    static const int tets[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7}, {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
+    std::map<std::pair<std::int64_t, int>, std::int64_t> edge_vertex;
+    std::vector<std::pair<std::int64_t, std::int64_t>> ends; // (inside, outside) per vertex
+    std::vector<std::vector<std::int64_t>> polys;
+    for (int i = 0; i + 1 < g.counts[0]; ++i)
+        for (int j = 0; j + 1 < g.counts[1]; ++j)
+            for (int k = 0; k + 1 < g.counts[2]; ++k) {
+                std::int64_t idx[8];
+                bool in[8];
+                int count = 0;
+                for (int m = 0; m < 8; ++m) {
+                    idx[m] = g.index(i + (m & 1), j + (m >> 1 & 1), k + (m >> 2 & 1));
+                    in[m] = f[static_cast<std::size_t>(idx[m])] >= iso;
+                    count += in[m] ? 1 : 0;
+                }
+                if (count == 0 || count == 8) continue;
+                for (const auto& tet : tets) {
+                    std::vector<int> ins, outs;
+                    for (int c = 0; c < 4; ++c) (in[tet[c]] ? ins : outs).push_back(tet[c]);
+                    if (ins.empty() || outs.empty()) continue;
+                    std::vector<std::pair<int, int>> corners;
+                    if (ins.size() == 1)
+                        corners = {{ins[0], outs[0]}, {ins[0], outs[1]}, {ins[0], outs[2]}};
+                    else if (outs.size() == 1)
+                        corners = {{ins[0], outs[0]}, {ins[1], outs[0]}, {ins[2], outs[0]}};
+                    else
+                        corners = {{ins[0], outs[0]}, {ins[0], outs[1]}, {ins[1], outs[1]}, {ins[1], outs[0]}};
+                    std::vector<std::array<double, 3>> mid;
+                    for (const auto& [a, b] : corners) {
+                        const auto pa = g.at(idx[a]), pb = g.at(idx[b]);
+                        mid.push_back({0.5 * (pa[0] + pb[0]), 0.5 * (pa[1] + pb[1]), 0.5 * (pa[2] + pb[2])});
+                    }
+                    const auto normal = corners.size() == 3 ? cross3(sub3(mid[1], mid[0]), sub3(mid[2], mid[0]))
+                                                            : cross3(sub3(mid[2], mid[0]), sub3(mid[3], mid[1]));
+                    std::array<double, 3> dir{0, 0, 0};
+                    for (int c : outs) {
+                        const auto p = g.at(idx[c]);
+                        for (int a = 0; a < 3; ++a) dir[a] += p[a] / static_cast<double>(outs.size());
+                    }
+                    for (int c : ins) {
+                        const auto p = g.at(idx[c]);
+                        for (int a = 0; a < 3; ++a) dir[a] -= p[a] / static_cast<double>(ins.size());
+                    }
+                    if (dot3(normal, dir) < 0) std::reverse(corners.begin(), corners.end());
+                    std::vector<std::int64_t> poly;
+                    for (const auto& [a, b] : corners) {
+                        const int lo = std::min(a, b), hi = std::max(a, b);
+                        const auto key = std::make_pair(idx[lo], hi ^ lo);
+                        auto it = edge_vertex.find(key);
+                        if (it == edge_vertex.end()) {
+                            it = edge_vertex.emplace(key, static_cast<std::int64_t>(ends.size())).first;
+                            ends.push_back({idx[a], idx[b]});
+                        }
+                        poly.push_back(it->second);
+                    }
+                    polys.push_back(poly);
+                }
+            }
+    // Vertex placement: t in [0, 1] along (inside, outside), linear start,
+    // Newton steps on f - iso kept inside the bracket, else bisection.
+    const std::size_t nv = ends.size();
+    std::vector<double> t(nv), lo(nv, 0.0), hi(nv, 1.0);
+    std::vector<char> done(nv, 0);
+    for (std::size_t v = 0; v < nv; ++v) {
+        const double a = f[static_cast<std::size_t>(ends[v].first)], b = f[static_cast<std::size_t>(ends[v].second)];
+        t[v] = std::min(1.0, std::max(0.0, (a - iso) / (a - b)));
+    }
+    const auto place = [&](std::size_t v) {
+        const auto a = g.at(ends[v].first), b = g.at(ends[v].second);
+        return std::array<double, 3>{a[0] + t[v] * (b[0] - a[0]), a[1] + t[v] * (b[1] - a[1]), a[2] + t[v] * (b[2] - a[2])};
+    };
+    for (int round = 0;; ++round) {
+        std::vector<std::size_t> act;
+        for (std::size_t v = 0; v < nv; ++v)
+            if (!done[v]) act.push_back(v);
+        if (act.empty()) break;
+        if (round == 200) throw std::runtime_error("isosurface vertices did not converge");
+        std::vector<std::array<double, 3>> pts;
+        for (std::size_t v : act) pts.push_back(place(v));
+        std::vector<double> val(act.size(), 0.0);
+        std::vector<std::array<double, 3>> grad(act.size(), {0.0, 0.0, 0.0});
+        fn(pts, val, grad);
+        for (std::size_t p = 0; p < act.size(); ++p) {
+            const std::size_t v = act[p];
+            const double r = val[p] - iso;
+            if (!std::isfinite(r) || !std::isfinite(grad[p][0]) || !std::isfinite(grad[p][1]) ||
+                !std::isfinite(grad[p][2]))
+                throw std::runtime_error("isosurface function is not finite");
+            if (r == 0.0) {
+                done[v] = 1;
+                continue;
+            }
+            if (r > 0.0) lo[v] = t[v];
+            else hi[v] = t[v];
+            const auto d = sub3(g.at(ends[v].second), g.at(ends[v].first));
+            const double len = std::sqrt(dot3(d, d));
+            double next = t[v] - r / dot3(grad[p], d);
+            if (!(next > lo[v] && next < hi[v])) next = 0.5 * (lo[v] + hi[v]);
+            const double moved = std::abs(next - t[v]) * len;
+            t[v] = next;
+            if (!(moved > 1e-10 && (hi[v] - lo[v]) * len > 1e-10)) done[v] = 1;
+        }
+    }
+    OMesh m;
+    for (std::size_t v = 0; v < nv; ++v) m.v.push_back(place(v));
+    for (const auto& q : polys) {
+        if (q.size() == 3) {
+            m.t.push_back({q[0], q[1], q[2]});
+            continue;
+        }
+        const auto d02 = sub3(m.v[static_cast<std::size_t>(q[0])], m.v[static_cast<std::size_t>(q[2])]);
+        const auto d13 = sub3(m.v[static_cast<std::size_t>(q[1])], m.v[static_cast<std::size_t>(q[3])]);
+        if (dot3(d02, d02) <= dot3(d13, d13)) {
+            m.t.push_back({q[0], q[1], q[2]});
+            m.t.push_back({q[0], q[2], q[3]});
+        } else {
+            m.t.push_back({q[0], q[1], q[3]});
+            m.t.push_back({q[1], q[2], q[3]});
+        }
+    }
+    return m;
+}
+
+// Grid::around (src/grid.cpp:15-38).
+GridSpec grid_around(const Molecule& M, double margin, double spacing) {
+    std::array<double, 3> lo{M.atoms[0].pos[0], M.atoms[0].pos[1], M.atoms[0].pos[2]}, hi = lo;
+    for (const auto& a : M.atoms)
+        for (int k = 0; k < 3; ++k) {
+            lo[static_cast<std::size_t>(k)] = std::min(lo[static_cast<std::size_t>(k)], a.pos[k]);
+            hi[static_cast<std::size_t>(k)] = std::max(hi[static_cast<std::size_t>(k)], a.pos[k]);
+        }
+    GridSpec g{};
+    for (int k = 0; k < 3; ++k) {
+        const auto u = static_cast<std::size_t>(k);
+        g.origin[u] = lo[u] - margin;
+        g.counts[u] = std::max(2, static_cast<int>(std::ceil((hi[u] - lo[u] + 2 * margin) / spacing) + 1));
+        g.steps[4 * u] = spacing;
+    }
+    return g;
+}
+
+// The density isosurface: AO-P grid densities (block 128, screening 1e-12)
+// and the density with its gradient at the vertices (AO-P, order 1).
+OMesh density_isosurface(const Molecule& M, double iso, double spacing) {
+    if (!std::isfinite(iso) || iso < 1e-8) throw std::invalid_argument("density isovalue must be finite and at least 1e-8 e/bohr^3");
+    if (!std::isfinite(spacing) || spacing <= 0.0) throw std::invalid_argument("surface grid spacing must be positive and finite");
+    const orc_options opt{128, AO_P, 1e-12, 0.1};
+    const ScalarFn fn = [&](const std::vector<std::array<double, 3>>& pts, std::vector<double>& val,
+                            std::vector<std::array<double, 3>>& grad) {
+        std::vector<double> flat;
+        for (const auto& p : pts) flat.insert(flat.end(), p.begin(), p.end());
+        PointSet set;
+        set.pts = flat.data();
+        set.n = static_cast<std::int64_t>(pts.size());
+        std::vector<double> gx(pts.size()), gy(pts.size()), gz(pts.size());
+        double* outs[4] = {val.data(), gx.data(), gy.data(), gz.data()};
+        int used = 0;
+        evaluate(M, set, {{Density, Total, 0, 0}, {Gradient, Total, 0, 0}}, opt, outs, &used);
+        for (std::size_t p = 0; p < pts.size(); ++p) grad[p] = {gx[p], gy[p], gz[p]};
+    };
+    for (double margin = 4.0; margin <= 16.0; margin += 2.0) {
+        const GridSpec g = grid_around(M, margin, spacing);
+        std::vector<double> rho(static_cast<std::size_t>(g.counts[0]) * g.counts[1] * g.counts[2]);
+        PointSet set;
+        set.origin = g.origin.data();
+        set.steps = g.steps.data();
+        set.counts = g.counts.data();
+        set.n = static_cast<std::int64_t>(rho.size());
+        double* out = rho.data();
+        int used = 0;
+        evaluate(M, set, {{Density, Total, 0, 0}}, opt, &out, &used);
+        if (reaches_boundary(g, rho, iso)) continue;
+        return isosurface(g, rho, iso, fn);
+    }
+    throw std::runtime_error("the density isosurface extends more than 16 bohr beyond the atoms; use a larger isovalue");
+}
+
+// Politzer-Murray statistics of vertex values on a mesh, from the linear
+// interpolation over each triangle.
+struct OStats {
+    double area = 0, pos_area = 0, neg_area = 0, nonpolar = 0, minimum = 0, maximum = 0;
+    std::int64_t min_vertex = 0, max_vertex = 0;
+    double average = 0, pos_avg = 0, neg_avg = 0, pos_var = 0, neg_var = 0, tot_var = 0, nu = 0, pi = 0, mpi = 0;
+    std::vector<std::int64_t> minima, maxima;
+};
+
+// Pieces of a triangle where the linear function >= level (upper) or < level.
+void pieces(const std::array<std::array<double, 3>, 3>& x, const std::array<double, 3>& w, double level, bool upper,
+            const std::function<void(double, double, double, double)>& piece) {
+    std::vector<std::array<double, 3>> px;
+    std::vector<double> pw;
+    for (int i = 0; i < 3; ++i) {
+        const int j = (i + 1) % 3;
+        const bool in_i = upper ? w[static_cast<std::size_t>(i)] >= level : w[static_cast<std::size_t>(i)] < level;
+        const bool in_j = upper ? w[static_cast<std::size_t>(j)] >= level : w[static_cast<std::size_t>(j)] < level;
+        if (in_i) {
+            px.push_back(x[static_cast<std::size_t>(i)]);
+            pw.push_back(w[static_cast<std::size_t>(i)]);
+        }
+        if (in_i != in_j) {
+            const double s = (level - w[static_cast<std::size_t>(i)]) / (w[static_cast<std::size_t>(j)] - w[static_cast<std::size_t>(i)]);
+            const auto& a = x[static_cast<std::size_t>(i)];
+            const auto& b = x[static_cast<std::size_t>(j)];
+            px.push_back({a[0] + s * (b[0] - a[0]), a[1] + s * (b[1] - a[1]), a[2] + s * (b[2] - a[2])});
+            pw.push_back(level);
+        }
+    }
+    for (std::size_t i = 1; i + 1 < px.size(); ++i) {
+        const auto c = cross3(sub3(px[i], px[0]), sub3(px[i + 1], px[0]));
+        piece(0.5 * std::sqrt(dot3(c, c)), pw[0], pw[i], pw[i + 1]);
+    }
+}
+
+OStats statistics(const std::vector<std::array<double, 3>>& xv, const std::vector<std::array<std::int64_t, 3>>& tri,
+                  const std::vector<double>& w, double polar) {
+    for (double x : w)
+        if (!std::isfinite(x)) throw std::invalid_argument("surface values must be finite");
+    if (!std::isfinite(polar) || polar < 0.0) throw std::invalid_argument("polar threshold must be finite and nonnegative");
+    for (const auto& t : tri)
+        for (auto v : t)
+            if (v < 0 || v >= static_cast<std::int64_t>(xv.size())) throw std::invalid_argument("surface triangle refers to a missing vertex");
+    OStats s;
+    const auto corners = [&](const std::array<std::int64_t, 3>& t, std::array<std::array<double, 3>, 3>& x,
+                             std::array<double, 3>& v) {
+        for (int c = 0; c < 3; ++c) {
+            x[static_cast<std::size_t>(c)] = xv[static_cast<std::size_t>(t[static_cast<std::size_t>(c)])];
+            v[static_cast<std::size_t>(c)] = w[static_cast<std::size_t>(t[static_cast<std::size_t>(c)])];
+        }
+    };
+    const auto avg3 = [](double a, double b, double c) { return (a + b + c) / 3.0; };
+    double ip = 0, in = 0, above = 0, below = 0;
+    for (const auto& t : tri) {
+        std::array<std::array<double, 3>, 3> x;
+        std::array<double, 3> v;
+        corners(t, x, v);
+        pieces(x, v, 0.0, true, [&](double a, double p, double q, double r) { s.pos_area += a; ip += a * avg3(p, q, r); });
+        pieces(x, v, 0.0, false, [&](double a, double p, double q, double r) { s.neg_area += a; in += a * avg3(p, q, r); });
+        pieces(x, v, polar, true, [&](double a, double, double, double) { above += a; });
+        pieces(x, v, -polar, false, [&](double a, double, double, double) { below += a; });
+    }
+    s.area = s.pos_area + s.neg_area;
+    if (!(s.area > 0.0) || !std::isfinite(s.area)) throw std::invalid_argument("surface area must be positive");
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    s.average = (ip + in) / s.area;
+    s.pos_avg = s.pos_area > 0 ? ip / s.pos_area : nan;
+    s.neg_avg = s.neg_area > 0 ? in / s.neg_area : nan;
+    s.mpi = (ip - in) / s.area;
+    s.nonpolar = std::max(0.0, s.area - above - below);
+    double sp = 0, sn = 0, dev = 0;
+    const auto sq = [](double a, double p, double q, double r, double m) {
+        const double u = p - m, v = q - m, x = r - m;
+        return a / 6.0 * (u * u + v * v + x * x + u * v + v * x + x * u);
+    };
+    for (const auto& t : tri) {
+        std::array<std::array<double, 3>, 3> x;
+        std::array<double, 3> v;
+        corners(t, x, v);
+        if (s.pos_area > 0) pieces(x, v, 0.0, true, [&](double a, double p, double q, double r) { sp += sq(a, p, q, r, s.pos_avg); });
+        if (s.neg_area > 0) pieces(x, v, 0.0, false, [&](double a, double p, double q, double r) { sn += sq(a, p, q, r, s.neg_avg); });
+        pieces(x, v, s.average, true, [&](double a, double p, double q, double r) { dev += a * (avg3(p, q, r) - s.average); });
+        pieces(x, v, s.average, false, [&](double a, double p, double q, double r) { dev += a * (s.average - avg3(p, q, r)); });
+    }
+    s.pos_var = s.pos_area > 0 ? sp / s.pos_area : 0.0;
+    s.neg_var = s.neg_area > 0 ? sn / s.neg_area : 0.0;
+    s.tot_var = s.pos_var + s.neg_var;
+    s.nu = s.tot_var > 0 ? s.pos_var * s.neg_var / (s.tot_var * s.tot_var) : nan;
+    s.pi = dev / s.area;
+    s.min_vertex = std::min_element(w.begin(), w.end()) - w.begin();
+    s.max_vertex = std::max_element(w.begin(), w.end()) - w.begin();
+    s.minimum = w[static_cast<std::size_t>(s.min_vertex)];
+    s.maximum = w[static_cast<std::size_t>(s.max_vertex)];
+    // Extrema: strictly below (above) every vertex within two edges.
+    std::vector<std::set<std::int64_t>> nb(xv.size());
+    for (const auto& t : tri)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                if (a != b) nb[static_cast<std::size_t>(t[static_cast<std::size_t>(a)])].insert(t[static_cast<std::size_t>(b)]);
+    for (std::size_t i = 0; i < xv.size(); ++i) {
+        if (nb[i].empty()) continue;
+        bool is_min = true, is_max = true;
+        for (auto j : nb[i]) {
+            std::vector<std::int64_t> ring{j};
+            ring.insert(ring.end(), nb[static_cast<std::size_t>(j)].begin(), nb[static_cast<std::size_t>(j)].end());
+            for (auto k : ring) {
+                if (k == static_cast<std::int64_t>(i)) continue;
+                if (!(w[i] < w[static_cast<std::size_t>(k)])) is_min = false;
+                if (!(w[i] > w[static_cast<std::size_t>(k)])) is_max = false;
+            }
+        }
+        if (is_min) s.minima.push_back(static_cast<std::int64_t>(i));
+        if (is_max) s.maxima.push_back(static_cast<std::int64_t>(i));
+    }
+    std::stable_sort(s.minima.begin(), s.minima.end(), [&](std::int64_t a, std::int64_t b) { return w[static_cast<std::size_t>(a)] < w[static_cast<std::size_t>(b)]; });
+    std::stable_sort(s.maxima.begin(), s.maxima.end(), [&](std::int64_t a, std::int64_t b) { return w[static_cast<std::size_t>(a)] > w[static_cast<std::size_t>(b)]; });
+    return s;
+}
+
 void set_error(char* buf, int len, const char* msg) {
     if (buf && len > 0) {
         std::strncpy(buf, msg, static_cast<std::size_t>(len - 1));
@@ -2017,6 +2379,94 @@ int orc_write_field_cube(const char* path, const char* comment, const double* or
         return 3;
     }
     return 0;
+}
+
+void* orc_isosurface(const double* origin, const double* steps, const int* counts, const double* samples,
+                     double isovalue, orc_scalar_fn fn, void* user, int* status, char* error, int error_len) {
+    *status = 0;
+    try {
+        GridSpec g{};
+        for (int k = 0; k < 3; ++k) {
+            g.origin[static_cast<std::size_t>(k)] = origin[k];
+            g.counts[static_cast<std::size_t>(k)] = counts[k];
+        }
+        for (int k = 0; k < 9; ++k) g.steps[static_cast<std::size_t>(k)] = steps[k];
+        const std::size_t n = static_cast<std::size_t>(counts[0]) * counts[1] * counts[2];
+        const std::vector<double> f(samples, samples + n);
+        const ScalarFn call = [&](const std::vector<std::array<double, 3>>& pts, std::vector<double>& val,
+                                  std::vector<std::array<double, 3>>& grad) {
+            std::vector<double> flat, g3(3 * pts.size());
+            for (const auto& p : pts) flat.insert(flat.end(), p.begin(), p.end());
+            fn(flat.data(), static_cast<std::int64_t>(pts.size()), val.data(), g3.data(), user);
+            for (std::size_t q = 0; q < pts.size(); ++q) grad[q] = {g3[3 * q], g3[3 * q + 1], g3[3 * q + 2]};
+        };
+        return new OMesh(isosurface(g, f, isovalue, call));
+    } catch (const std::invalid_argument& e) {
+        set_error(error, error_len, e.what());
+        *status = 1;
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        *status = 3;
+    }
+    return nullptr;
+}
+
+void* orc_density_isosurface(const void* molecule, double isovalue, double spacing, int* status, char* error,
+                             int error_len) {
+    *status = 0;
+    try {
+        return new OMesh(density_isosurface(*static_cast<const Molecule*>(molecule), isovalue, spacing));
+    } catch (const std::invalid_argument& e) {
+        set_error(error, error_len, e.what());
+        *status = 1;
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        *status = 3;
+    }
+    return nullptr;
+}
+
+void orc_mesh_sizes(const void* mesh, std::int64_t* nv, std::int64_t* nt) {
+    const auto* m = static_cast<const OMesh*>(mesh);
+    *nv = static_cast<std::int64_t>(m->v.size());
+    *nt = static_cast<std::int64_t>(m->t.size());
+}
+
+void orc_mesh_data(const void* mesh, double* vertices, std::int64_t* triangles) {
+    const auto* m = static_cast<const OMesh*>(mesh);
+    for (std::size_t v = 0; v < m->v.size(); ++v)
+        for (int k = 0; k < 3; ++k) vertices[3 * v + static_cast<std::size_t>(k)] = m->v[v][static_cast<std::size_t>(k)];
+    for (std::size_t t = 0; t < m->t.size(); ++t)
+        for (int k = 0; k < 3; ++k) triangles[3 * t + static_cast<std::size_t>(k)] = m->t[t][static_cast<std::size_t>(k)];
+}
+
+void orc_mesh_free(void* mesh) { delete static_cast<OMesh*>(mesh); }
+
+int orc_surface_statistics(const double* vertices, std::int64_t nv, const std::int64_t* triangles, std::int64_t nt,
+                           const double* values, double polar, double* stats, std::int64_t* counts,
+                           std::int64_t* minima, std::int64_t* maxima, char* error, int error_len) {
+    try {
+        std::vector<std::array<double, 3>> xv(static_cast<std::size_t>(nv));
+        for (std::int64_t v = 0; v < nv; ++v) xv[static_cast<std::size_t>(v)] = {vertices[3 * v], vertices[3 * v + 1], vertices[3 * v + 2]};
+        std::vector<std::array<std::int64_t, 3>> tri(static_cast<std::size_t>(nt));
+        for (std::int64_t t = 0; t < nt; ++t) tri[static_cast<std::size_t>(t)] = {triangles[3 * t], triangles[3 * t + 1], triangles[3 * t + 2]};
+        const OStats s = statistics(xv, tri, std::vector<double>(values, values + nv), polar);
+        const double out[17] = {s.area, s.pos_area, s.neg_area, s.nonpolar, s.minimum, s.maximum,
+                                static_cast<double>(s.min_vertex), static_cast<double>(s.max_vertex), s.average,
+                                s.pos_avg, s.neg_avg, s.pos_var, s.neg_var, s.tot_var, s.nu, s.pi, s.mpi};
+        std::copy(out, out + 17, stats);
+        counts[0] = static_cast<std::int64_t>(s.minima.size());
+        counts[1] = static_cast<std::int64_t>(s.maxima.size());
+        if (minima) std::copy(s.minima.begin(), s.minima.end(), minima);
+        if (maxima) std::copy(s.maxima.begin(), s.maxima.end(), maxima);
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        set_error(error, error_len, e.what());
+        return 1;
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return 3;
+    }
 }
 
 int orc_threads(void) {

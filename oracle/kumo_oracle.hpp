@@ -100,5 +100,27 @@ int orc_write_field_cube(const char* path, const char* comment, const double* or
                          const int* counts, int natoms, const int* z, const double* charges, const double* xyz,
                          const double* values, std::int64_t n, int orbital, std::int64_t orbital_index, char* error,
                          int error_len);
+/// Scalar function for isosurfaces: values (n) and gradients (3 per point) at n points.
+typedef void (*orc_scalar_fn)(const double* points, std::int64_t n, double* values, double* gradients, void* user);
+/// extract_isosurface (src/surface.cpp:150-331) of grid samples; the mesh is
+/// kept in an opaque handle (orc_mesh_*). Returns null with a message on error.
+void* orc_isosurface(const double* origin, const double* steps, const int* counts, const double* samples,
+                     double isovalue, orc_scalar_fn fn, void* user, int* status, char* error, int error_len);
+/// density_isosurface (src/surface.cpp:341-369, 531-533) from the oracle's densities.
+void* orc_density_isosurface(const void* molecule, double isovalue, double spacing, int* status, char* error,
+                             int error_len);
+void orc_mesh_sizes(const void* mesh, std::int64_t* nvertices, std::int64_t* ntriangles);
+void orc_mesh_data(const void* mesh, double* vertices, std::int64_t* triangles);
+void orc_mesh_free(void* mesh);
+/// surface_statistics (src/surface.cpp:418-529): stats[17] in the order area,
+/// positive_area, negative_area, nonpolar_area, minimum, maximum,
+/// minimum_vertex, maximum_vertex, average, positive_average,
+/// negative_average, positive_variance, negative_variance, total_variance,
+/// balance, charge_separation, polarity_index; counts[2] the numbers of local
+/// minima and maxima (their vertices written when the arrays are not null).
+int orc_surface_statistics(const double* vertices, std::int64_t nvertices, const std::int64_t* triangles,
+                           std::int64_t ntriangles, const double* values, double polar_threshold, double* stats,
+                           std::int64_t* counts, std::int64_t* minima, std::int64_t* maxima, char* error,
+                           int error_len);
 int orc_threads(void);
 }

@@ -3,8 +3,9 @@ test data (tests/data of chem-kumo 2.1.0). Run in a container where the
 reference is mounted:  python tests/golden/make_golden.py [/root/reference]
 
 * fchk/: the FCHK inputs, copied byte for byte (o2, acrolein, water def2-TZVP).
-* multiwfn/: Multiwfn 2026.9.1 Cube files of derived fields, copied byte for
-  byte (provenance: the reference's tests/REFERENCE_CUBES.md).
+* multiwfn/: Multiwfn 2026.9.1 Cube files of derived fields and its surface
+  analysis reports, copied byte for byte (provenance: the reference's
+  tests/REFERENCE_CUBES.md).
 * cubegen/*.npz: Gaussian 09 cubegen density and MO cubes, which are 3-10 MB of
   text each, reduced to the grid header, a deterministic sample of points
   (every STRIDE-th point plus every point whose magnitude exceeds KEEP) with
@@ -34,6 +35,9 @@ MULTIWFN = ["h2o_elf", "h2o_rdg", "h2o_sign_lambda2_rho", "o2_elf", "h2o_esp", "
             "acrolein_local_electronegativity", "acrolein_local_hardness", "acrolein_ow_fukui_plus",
             "acrolein_ow_fukui_minus", "acrolein_ow_fukui_zero", "acrolein_ow_dual_descriptor", "h2o_ow_fukui_plus",
             "h2o_ow_fukui_minus"]
+# Multiwfn quantitative surface analyses (tests/multiwfn_surface_reference_test.cpp, meson.build:746-782).
+SURFACES = ["h2o_esp", "o2_esp", "acrolein_esp", "h2o_alie", "o2_alie", "acrolein_alie", "acrolein_leae", "o2_leae",
+            "acrolein_lea"]
 DENSITY = [("o2_density", "o2", 0), ("o2_alpha", "o2", 1), ("o2_beta", "o2", 2), ("o2_spin", "o2", 3),
            ("acrolein_density", "acrolein", 0)]
 MO = [("o2_alpha_mo_3", "o2", 0, 3), ("o2_alpha_mo_9", "o2", 0, 9), ("o2_alpha_mo_10", "o2", 0, 10),
@@ -59,6 +63,9 @@ def main():
         shutil.copyfile(os.path.join(data, f), os.path.join(HERE, "fchk", os.path.basename(f)))
     for f in MULTIWFN:
         shutil.copyfile(os.path.join(data, "multiwfn", f + ".cube.gz"), os.path.join(HERE, "multiwfn", f + ".cube.gz"))
+    for f in SURFACES:
+        shutil.copyfile(os.path.join(data, "multiwfn", f + "_surface.txt"),
+                        os.path.join(HERE, "multiwfn", f + "_surface.txt"))
     for name, mol, comp in DENSITY:
         sample(read_cube(os.path.join(data, name + ".cube")), os.path.join(HERE, "cubegen", name + ".npz"),
                molecule=mol, component=comp)

@@ -21,6 +21,13 @@ class Options(C.Structure):
                 ("fukui_delta", C.c_double)]
 
 
+SCALAR_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                        C.c_void_p)
+STATS = ["area", "positive_area", "negative_area", "nonpolar_area", "minimum", "maximum", "minimum_vertex",
+         "maximum_vertex", "average", "positive_average", "negative_average", "positive_variance",
+         "negative_variance", "total_variance", "balance", "charge_separation", "polarity_index"]
+
+
 def _ptr(a, t=C.c_double):
     return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
 
@@ -53,6 +60,16 @@ def lib():
         L.orc_molecule_destroy.argtypes = [C.c_void_p]
         L.orc_molecule_set_energies.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_basis_destroy.argtypes = [C.c_void_p]
+        L.orc_isosurface.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, SCALAR_FN,
+                                     C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_isosurface.restype = C.c_void_p
+        L.orc_density_isosurface.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_density_isosurface.restype = C.c_void_p
+        L.orc_mesh_sizes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_mesh_data.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_mesh_free.argtypes = [C.c_void_p]
+        L.orc_surface_statistics.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
         L.orc_write_field_cube.argtypes = [C.c_char_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,
                                            C.c_int64, C.c_char_p, C.c_int]
@@ -256,3 +273,85 @@ def write_cube(path, grid, atoms, values, comment=None, orbital_index=None):
                                     _ptr(counts, C.c_int), z.size, _ptr(z, C.c_int), _ptr(q), _ptr(xyz), _ptr(v),
                                     v.size, 1 if orbital else 0, int(orbital_index or 0), err, 256)
     return rc, err.value.decode()
+
+
+def _raise(status, msg):
+    raise (ValueError if status == 1 else RuntimeError)(msg)
+
+
+def _mesh(h):
+    nv, nt = C.c_int64(), C.c_int64()
+    lib().orc_mesh_sizes(h, C.byref(nv), C.byref(nt))
+    v = np.zeros((nv.value, 3))
+    t = np.zeros((nt.value, 3), dtype=np.int64)
+    lib().orc_mesh_data(h, _ptr(v), _ptr(t, C.c_int64))
+    lib().orc_mesh_free(h)
+    return v, t
+
+
+def isosurface(grid, samples, isovalue, fn):
+    """extract_isosurface restated (src/surface.cpp:150-331). fn(points n x 3)
+    -> (values n, gradients n x 3). Returns (vertices, triangles)."""
+    origin = np.ascontiguousarray(grid["origin"], dtype=np.float64)
+    steps = np.ascontiguousarray(np.asarray(grid["steps"], dtype=np.float64).T)
+    counts = np.ascontiguousarray(grid["counts"], dtype=np.int32)
+    f = np.ascontiguousarray(samples, dtype=np.float64).ravel()
+    if f.size != int(np.prod(counts.astype(np.int64))):
+        raise ValueError("isosurface samples do not match the grid")
+
+    def call(pts, n, values, grads, user):
+        p = np.ctypeslib.as_array(pts, shape=(n, 3))
+        v, g = fn(p)
+        np.ctypeslib.as_array(values, shape=(n,))[:] = v
+        np.ctypeslib.as_array(grads, shape=(n, 3))[:] = g
+    cb = SCALAR_FN(call)
+    status = C.c_int(0)
+    err = C.create_string_buffer(256)
+    h = lib().orc_isosurface(_ptr(origin), _ptr(steps), _ptr(counts, C.c_int), _ptr(f), isovalue, cb, None,
+                             C.byref(status), err, 256)
+    if not h:
+        _raise(status.value, err.value.decode())
+    return _mesh(h)
+
+
+def density_isosurface(mol, isovalue=1e-3, spacing=0.25):
+    status = C.c_int(0)
+    err = C.create_string_buffer(256)
+    h = lib().orc_density_isosurface(mol.h, isovalue, spacing, C.byref(status), err, 256)
+    if not h:
+        _raise(status.value, err.value.decode())
+    return _mesh(h)
+
+
+def surface_statistics(vertices, triangles, values, polar_threshold=10.0 / 627.5094740631):
+    v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, dtype=np.int64).reshape(-1, 3)
+    w = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    if w.size != v.shape[0]:
+        raise ValueError("surface values do not match the mesh vertices")
+    st = np.zeros(17)
+    counts = np.zeros(2, dtype=np.int64)
+    err = C.create_string_buffer(256)
+    rc = lib().orc_surface_statistics(_ptr(v), v.shape[0], _ptr(t, C.c_int64), t.shape[0], _ptr(w), polar_threshold,
+                                      _ptr(st), _ptr(counts, C.c_int64), None, None, err, 256)
+    if rc:
+        _raise(rc, err.value.decode())
+    mn = np.zeros(counts[0], dtype=np.int64)
+    mx = np.zeros(counts[1], dtype=np.int64)
+    lib().orc_surface_statistics(_ptr(v), v.shape[0], _ptr(t, C.c_int64), t.shape[0], _ptr(w), polar_threshold,
+                                 _ptr(st), _ptr(counts, C.c_int64), _ptr(mn, C.c_int64), _ptr(mx, C.c_int64), err, 256)
+    d = dict(zip(STATS, st.tolist()))
+    d["minimum_vertex"], d["maximum_vertex"] = int(d["minimum_vertex"]), int(d["maximum_vertex"])
+    d["local_minima"], d["local_maxima"] = mn, mx
+    return d
+
+
+def mesh_area(v, t):
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    return float(np.sum(0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)))
+
+
+def mesh_volume(v, t):
+    c0 = v.mean(axis=0)
+    a, b, c = v[t[:, 0]] - c0, v[t[:, 1]] - c0, v[t[:, 2]] - c0
+    return float(np.sum(np.einsum("ij,ij->i", a, np.cross(b, c))) / 6.0)

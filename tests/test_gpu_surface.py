@@ -1,0 +1,195 @@
+"""Molecular surfaces through the C-ABI (kumo_hip_*_surface and friends):
+the density isosurface built from GPU grid densities, its vertices placed on
+the GPU-evaluated density, the ESP, ALIE and mapped properties at the vertices
+on the GPU, and the statistics.
+
+* Against the reference's external known answers: Multiwfn's quantitative
+  surface analyses of water, O2 and acrolein with the tolerances of
+  tests/multiwfn_surface_reference_test.cpp.
+* Against the oracle: the same mesh (vertex and triangle counts, triangles
+  identical, vertices within 1e-9 bohr), the values within 1e-9 of the largest
+  magnitude, the statistics within 1e-8 relative; surface_statistics and
+  TriangleMesh area/volume on the reference test's octahedron (exact) and on
+  the oracle's meshes; write_ply against the text of src/surface.cpp:701-720.
+* The reference's errors (src/surface.cpp:335-340, 563-571, 664-678).
+"""
+import numpy as np
+import pytest
+
+from kumo_hip.api import PropertyKind as K
+from support import oracle
+from support.fchk import FchkData
+from support.reference_cases import FCHK
+from support.surface_cases import MAPPED, SURFACE_CASES, check_surface, read_report
+
+pytestmark = pytest.mark.gpu
+
+_OR, _GPU = {}, {}
+KCAL = 627.5094740631
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from kumo_hip.binding import Context
+    c = Context(0)
+    yield c
+    for m in _GPU.values():
+        m.close()
+    _GPU.clear()
+    c.close()
+
+
+def omol(name):
+    if name not in _OR:
+        _OR[name] = oracle.Molecule(FchkData(FCHK[name]))
+    return _OR[name]
+
+
+def gmol(ctx, name):
+    from kumo_hip.binding import Molecule
+    if name not in _GPU:
+        print(f"[gpu] load {name}", flush=True)
+        _GPU[name] = Molecule(ctx, fchk_path=FCHK[name])
+    return _GPU[name]
+
+
+def gpu_surface(ctx, name, mode):
+    m = gmol(ctx, name)
+    kind = mode.replace("-no-extrema", "")
+    if kind == "esp":
+        s = m.electrostatic_potential_surface()
+    elif kind == "alie":
+        s = m.local_ionization_energy_surface()
+    else:
+        s = m.map_on_density_isosurface([(MAPPED[kind][0], 0, 0, 0)], isovalue=MAPPED[kind][1])
+    return s, {"vertices": s.vertices, "triangles": s.triangles, "values": s.values[0], "stats": s.statistics[0]}
+
+
+def oracle_surface(name, mode):
+    mol = omol(name)
+    kind = mode.replace("-no-extrema", "")
+    iso = MAPPED[kind][1] if kind in MAPPED else 1e-3
+    v, t = oracle.density_isosurface(mol, iso, 0.25)
+    prop = {"esp": K.ElectrostaticPotential, "alie": K.AverageLocalIonizationEnergy}.get(kind) or MAPPED[kind][0]
+    values = mol.evaluate([(prop, 0, 0, 0)], points=v)[0][0][0]
+    stats = oracle.surface_statistics(v, t, values, 0.0 if kind in MAPPED else 10.0 / KCAL)
+    return {"vertices": v, "triangles": t, "values": values, "stats": stats}
+
+
+STAT_KEYS = ["area", "positive_area", "negative_area", "nonpolar_area", "minimum", "maximum", "average",
+             "positive_average", "negative_average", "positive_variance", "negative_variance", "total_variance",
+             "balance", "charge_separation", "polarity_index"]
+
+
+def same_surface(g, o, label, extrema=("local_minima", "local_maxima"), vtol=1e-9, rel=1e-8):
+    assert g["vertices"].shape == o["vertices"].shape, f"{label}: vertex counts"
+    assert np.array_equal(g["triangles"], o["triangles"]), f"{label}: triangles"
+    dv = float(np.abs(g["vertices"] - o["vertices"]).max())
+    assert dv <= vtol, f"{label}: vertices differ by {dv:.3g} bohr"
+    scale = float(np.abs(o["values"]).max())
+    dval = float(np.abs(g["values"] - o["values"]).max())
+    assert dval <= 1e-9 * scale, f"{label}: values differ by {dval:.3g}"
+    for k in STAT_KEYS:
+        a, b = g["stats"][k], o["stats"][k]
+        if np.isnan(b):
+            assert np.isnan(a), f"{label}: {k}"
+            continue
+        assert abs(a - b) <= rel * max(abs(b), 1e-12), f"{label}: {k} {a!r} vs {b!r}"
+    # Local extrema on a nearly flat field (the ESP of O2, on rings; the LEAE
+    # maxima, zeros on the nodal plane) turn on rounding-level differences
+    # between neighbours; the lists are compared where the reference compares
+    # them (tests/multiwfn_surface_reference_test.cpp:264-269, 296-298, 345-347).
+    for k in extrema:
+        assert np.array_equal(g["stats"][k], o["stats"][k]), f"{label}: {k}"
+
+
+@pytest.mark.parametrize("case", SURFACE_CASES, ids=[c[1] for c in SURFACE_CASES])
+def test_surfaces_against_multiwfn_and_oracle(ctx, case):
+    name, report, mode = case
+    print(f"[gpu] surface {report}", flush=True)
+    s, g = gpu_surface(ctx, name, mode)
+    area, volume = oracle.mesh_area(g["vertices"], g["triangles"]), oracle.mesh_volume(g["vertices"], g["triangles"])
+    worst = check_surface(g, mode, read_report(report), report, volume)
+    assert max(w for w, _ in worst) <= 1.0
+    kind = mode.replace("-no-extrema", "")
+    compared = () if mode.endswith("-no-extrema") else (
+        ("local_minima", "local_maxima") if kind == "esp" else ("local_minima",))
+    same_surface(g, oracle_surface(name, mode), report, extrema=compared)
+    assert g["stats"]["area"] == pytest.approx(area, rel=1e-12)
+
+
+def test_density_isosurface_and_statistics_entry_points(ctx):
+    m = gmol(ctx, "water")
+    print("[gpu] density isosurface, water, spacing 0.3", flush=True)
+    s = m.density_isosurface(isovalue=2e-3, spacing=0.3)
+    v, t = oracle.density_isosurface(omol("water"), 2e-3, 0.3)
+    assert s.values == [] and s.statistics == []
+    assert np.array_equal(s.triangles, t) and float(np.abs(s.vertices - v).max()) <= 1e-9
+    # surface_statistics on a caller's mesh, with TriangleMesh area / volume.
+    x = s.vertices @ np.array([0.2, -0.5, 0.8])
+    st, area, volume = ctx.surface_statistics(s.vertices, s.triangles, x, 0.3)
+    ref = oracle.surface_statistics(s.vertices, s.triangles, x, 0.3)
+    for k in STAT_KEYS:
+        assert st[k] == pytest.approx(ref[k], rel=1e-12, abs=1e-15), k
+    assert np.array_equal(st["local_minima"], ref["local_minima"])
+    assert area == pytest.approx(oracle.mesh_area(s.vertices, s.triangles), rel=1e-12)
+    assert volume == pytest.approx(oracle.mesh_volume(s.vertices, s.triangles), rel=1e-12)
+    # the reference test's octahedron: exact answers
+    ov = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=float)
+    ot = np.array([[0, 2, 4], [0, 5, 2], [0, 4, 3], [0, 3, 5], [1, 4, 2], [1, 2, 5], [1, 3, 4], [1, 5, 3]])
+    st, area, volume = ctx.surface_statistics(ov, ot, ov[:, 0], 0.5)
+    r3 = np.sqrt(3.0)
+    assert area == pytest.approx(4 * r3, rel=1e-14) and volume == pytest.approx(4 / 3, rel=1e-14)
+    assert st["positive_variance"] == pytest.approx(1 / 18, rel=1e-14)
+    assert st["nonpolar_area"] == pytest.approx(3 * r3, rel=1e-14)
+    assert list(st["local_minima"]) == [1] and list(st["local_maxima"]) == [0]
+    with pytest.raises(ValueError, match="polar threshold must be finite and nonnegative"):
+        ctx.surface_statistics(ov, ot, ov[:, 0], -1.0)
+
+
+def test_map_surface_and_alie_at_points(ctx):
+    m = gmol(ctx, "acrolein")
+    props = [(K.Density, 0, 0, 0), (K.Elf, 0, 0, 0), (K.Density, 0, 0, 0), (K.AverageLocalIonizationEnergy, 0, 0, 0)]
+    print("[gpu] map on density isosurface, acrolein", flush=True)
+    s = m.map_on_density_isosurface(props, isovalue=1e-3, spacing=0.3, precision=0)
+    assert len(s.values) == 3 and s.backend_used in (1, 2)
+    assert np.allclose(s.values[0], 1e-3, rtol=1e-7)
+    o, _ = omol("acrolein").evaluate([(K.Elf, 0, 0, 0), (K.AverageLocalIonizationEnergy, 0, 0, 0)],
+                                     points=s.vertices)
+    assert float(np.abs(s.values[1] - o[0][0]).max()) <= 1e-9
+    assert float(np.abs(s.values[2] - o[1][0]).max()) <= 1e-9 * float(np.abs(o[1][0]).max())
+    pts = np.random.default_rng(3).normal(size=(300, 3)) * 2.0
+    a = m.average_local_ionization_energy(pts)
+    r, _ = omol("acrolein").evaluate([(K.AverageLocalIonizationEnergy, 0, 0, 0)], points=pts)
+    assert float(np.abs(a - r[0][0]).max()) <= 1e-10 * float(np.abs(r[0][0]).max())
+    with pytest.raises(ValueError, match="undefined where the density vanishes"):
+        m.average_local_ionization_energy(np.array([[1e4, 0.0, 0.0]]))
+    with pytest.raises(ValueError, match="only one-component properties can be mapped"):
+        m.map_on_density_isosurface([(K.Gradient, 0, 0, 0)])
+    with pytest.raises(ValueError, match="no property to map on the surface"):
+        m.map_on_density_isosurface([])
+
+
+def test_errors_and_ply(ctx, tmp_path):
+    m = gmol(ctx, "o2")
+    with pytest.raises(ValueError, match="density isovalue must be finite and at least 1e-8"):
+        m.density_isosurface(isovalue=1e-9)
+    with pytest.raises(ValueError, match="surface grid spacing must be positive and finite"):
+        m.density_isosurface(spacing=0.0)
+    with pytest.raises(ValueError, match="potential tolerance must be finite and nonnegative"):
+        m.electrostatic_potential_surface(potential_tolerance=-1.0)
+    print("[gpu] ESP surface, O2, for the PLY file", flush=True)
+    s = m.electrostatic_potential_surface(spacing=0.4)
+    path = tmp_path / "o2.ply"
+    ctx.write_ply(str(path), s.vertices, s.triangles, [("esp", s.values[0])], ["O2 ESP", "isovalue 0.001"])
+    lines = ["ply", "format ascii 1.0", "comment O2 ESP", "comment isovalue 0.001",
+             f"element vertex {len(s.vertices)}", "property double x", "property double y", "property double z",
+             "property double esp", f"element face {len(s.triangles)}",
+             "property list uchar int vertex_indices", "end_header"]
+    lines += [" ".join("%.9e" % x for x in (*v, e)) for v, e in zip(s.vertices, s.values[0])]
+    lines += ["3 %d %d %d" % tuple(t) for t in s.triangles]
+    assert path.read_text() == "\n".join(lines) + "\n"
+    for bad, msg in ((("x", s.values[0]),), "duplicate PLY property name 'x'"), ((("e-s", s.values[0]),),
+                                                                                   "invalid PLY property name"):
+        with pytest.raises(ValueError, match=msg):
+            ctx.write_ply(str(path), s.vertices, s.triangles, list(bad))
