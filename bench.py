@@ -68,6 +68,88 @@ def run_gpu(mol, grid, precision, steps, warmup, torch):
 CHANNELS = ["rho", "grad_x", "grad_y", "grad_z", "lap", "tau", "elf", "lol"]
 
 
+OE_PROPS = [(K.AverageLocalIonizationEnergy, 0, 0, 0), (K.LocalElectronAffinity, 0, 0, 0),
+            (K.LocalElectronAttachmentEnergy, 0, 0, 0), (K.LocalElectronegativity, 0, 0, 0),
+            (K.LocalHardness, 0, 0, 0)]
+
+
+def bench_orbital_energy(ctx, mol, grid, path, torch, no_cpu):
+    """ALIE, LEA, LEAE, local electronegativity and hardness of the step's
+    molecule on its cube (src/orbital_energy_fields.hpp): every MO of both
+    channels contracted in FP64 on MFMA (weighted-sum epilogue), then the
+    fields; the oracle on a sample of the points beside it."""
+    n = int(np.prod(grid["counts"].astype(np.int64)))
+    out = torch.empty((len(OE_PROPS), n), dtype=torch.float64, device="cuda")
+    ptrs = [out[c].data_ptr() for c in range(len(OE_PROPS))]
+    mol.evaluate_device(OE_PROPS, grid, ptrs)  # warm-up (operand table built on first use)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    mol.evaluate_device(OE_PROPS, grid, ptrs)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    st = ctx.kernel_stats()
+    info = mol.info
+    mos = info["nmo_alpha"] + (info["nmo_beta"] if info["unrestricted"] else 0)
+    flops = 2.0 * info["nbasis"] * mos * n  # unscreened upper bound of the contraction
+    res = {"points": n, "ms": dt * 1e3, "points_per_s": n / dt, "kernel_ms": st["kernel_ms"],
+           "mos": mos, "precision": "FP64",
+           "contraction": {"bound": "mfma", "unit": "TFLOP/s", "peak": 78.6,
+                           "achieved_unscreened_upper": flops / (st["kernel_ms"] * 1e-3) / 1e12,
+                           "note": "2 x AOs x MOs x points, every AO counted (the kept AOs are fewer)"}}
+    log(f"orbital-energy fields: {res['points_per_s']:.4g} points/s ({res['ms']:.0f} ms, kernel "
+        f"{st['kernel_ms']:.0f} ms, {mos} MOs)")
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from support import oracle
+        from support.fchk import FchkData
+        om = oracle.Molecule(FchkData(path))
+        idx = np.arange(0, n, n // 2048)[:2048]
+        pts = oracle.grid_points(grid)[idx]
+        t = time.perf_counter()
+        o, _ = om.evaluate(OE_PROPS, points=pts)
+        cdt = time.perf_counter() - t
+        got = out[:, torch.as_tensor(idx, device="cuda")].cpu().numpy()
+        res["max_abs_error_vs_oracle"] = float(max(np.abs(got[i] - o[i][0]).max() for i in range(len(OE_PROPS))))
+        res["cpu_baseline"] = {"value": len(idx) / cdt, "unit": "points/s", "cores": int(oracle.lib().orc_threads()),
+                               "kind": "port", "sample": f"{len(idx)} points spread over the cube"}
+        log(f"CPU oracle orbital-energy fields: {len(idx) / cdt:.4g} points/s")
+    del out
+    return res
+
+
+def bench_surfaces(ctx, mol, path, no_cpu):
+    """The ESP and ALIE surfaces of the step's molecule (rho = 0.001, 0.25
+    bohr; src/surface.cpp): density grid, vertex placement on the GPU point
+    path, the fields at the vertices, statistics. The oracle builds the same
+    density isosurface beside it."""
+    res = {}
+    for name, call in (("esp", mol.electrostatic_potential_surface), ("alie", mol.local_ionization_energy_surface)):
+        call()  # warm-up
+        t = time.perf_counter()
+        s = call()
+        dt = time.perf_counter() - t
+        res[name] = {"ms": dt * 1e3, "vertices": int(len(s.vertices)), "triangles": int(len(s.triangles)),
+                     "average": s.statistics[0]["average"]}
+        log(f"{name} surface: {dt * 1e3:.0f} ms, {len(s.vertices)} vertices")
+    t = time.perf_counter()
+    s = mol.density_isosurface()
+    res["density_isosurface_ms"] = (time.perf_counter() - t) * 1e3
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from support import oracle
+        from support.fchk import FchkData
+        om = oracle.Molecule(FchkData(path))
+        t = time.perf_counter()
+        v, tr = oracle.density_isosurface(om)
+        cdt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": cdt * 1e3, "unit": "ms per density isosurface",
+                               "cores": int(oracle.lib().orc_threads()), "kind": "port",
+                               "sample": "the same density isosurface (grid densities, vertex placement)"}
+        res["same_mesh_as_oracle"] = bool(np.array_equal(tr, s.triangles))
+        log(f"CPU oracle density isosurface: {cdt * 1e3:.0f} ms (GPU {res['density_isosurface_ms']:.0f} ms)")
+    return res
+
+
 def bench_cube(ctx, mol, grid, out, no_cpu):
     """Cube files of the scalar fields of the step (rho, lap, tau, ELF, LOL),
     written from the device outputs (Field::write_cube, src/cube.cpp:70-111):
@@ -380,6 +462,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-blocks", type=int, default=1024, help="screening blocks in the CPU sample")
     ap.add_argument("--no-cube", action="store_true", help="skip the Cube-writer timing")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the orbital-energy field and surface timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
@@ -477,6 +561,9 @@ def main():
             errors["elf_off_max_rho"] = float(rho[off].max()) if off.any() else None
 
     cube = None if args.no_cube else bench_cube(ctx, mol, grid, main_run["out"], args.no_cpu)
+    extras = {} if args.no_extras else {
+        "orbital_energy_fields": bench_orbital_energy(ctx, mol, grid, path, torch, args.no_cpu),
+        "surfaces": bench_surfaces(ctx, mol, path, args.no_cpu)}
 
     result = {
         "metric": "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error; p50/p99 latency",
@@ -503,6 +590,7 @@ def main():
             "bytes": "channel moments read (2 x 13 x 8 B per point) + output channels written (8 x 8 B per point)"},
         "cpu_baseline": cpu,
         "cube_writer": cube,
+        **extras,
     }
     print(json.dumps(result))
     mol.close()

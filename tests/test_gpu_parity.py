@@ -234,8 +234,8 @@ def test_errors_map_to_exceptions(ctx):
         m.evaluate([(K.Density, 0, 0, 0)], points=pts, block_size=0)
     with pytest.raises(ValueError):
         m.evaluate([(K.Density, 0, 0, 0)], points=np.array([[np.nan, 0, 0]]))
-    with pytest.raises(ValueError):
-        m.evaluate([(K.AverageLocalIonizationEnergy, 0, 0, 0)], points=pts)
+    with pytest.raises(ValueError, match="the local electron affinity is defined only for the total density"):
+        m.evaluate([(K.LocalElectronAffinity, DC.Alpha, 0, 0)], points=pts)
     with pytest.raises(RuntimeError):
         m.evaluate([(K.Density, 0, 0, 0)], points=pts, progress=lambda d, t: True)
 
