@@ -649,11 +649,15 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         for (std::size_t i = 0; i < nprops; ++i)
             if (props[i].kind == kEsp) idx = pr.first_channel[i];
         const long long chunk = 1 << 18;
+        std::vector<EspGroup> groups = mol->esp_groups;
+        const int nslices = esp_slices(groups, std::min(chunk, range));
+        double* partial = grow<double>(ex.ws_esp, static_cast<std::size_t>(nslices) *
+                                                      static_cast<std::size_t>(std::min(chunk, range)));
         for (long long p0 = q0; p0 < q1; p0 += chunk) {
             const long long cnt = std::min(chunk, q1 - p0);
-            launch_esp(mol->d_esp_pairs.as<EspPairDev>(), mol->esp_npairs, mol->d_esp_coef.as<double>(),
+            launch_esp(mol->d_esp_pairs.as<EspPairDev>(), groups, mol->d_esp_coef.as<double>(),
                        mol->d_esp_atoms.as<EspAtomDev>(), static_cast<int>(mol->atoms.size()), ex.boys_table, ps, p0,
-                       cnt, dout[idx] + p0, s);
+                       cnt, partial, dout[idx] + p0, s);
             sync(ex);
             advance(static_cast<std::size_t>(cnt));
         }
@@ -936,7 +940,7 @@ void Executor::destroy() {
         list.clear();
     }
     for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc,
-                            &ws_rescue, &ws_rmom, &ws_rscale, &ws_oe, &ws_oe_tasks})
+                            &ws_rescue, &ws_rmom, &ws_rscale, &ws_oe, &ws_oe_tasks, &ws_esp})
         b->release();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;

@@ -3,8 +3,12 @@
 // with the McMurchie-Davidson Hermite integrals R from the Boys function
 // (src/boys_function.cpp:47-76: table + 7-term Taylor below T = 40, closed
 // form and upward recursion above). One thread per point; the primitive
-// pairs are grouped by L = la + lb on the host so that each group runs a
-// recursion whose size is known at compile time and stays in registers.
+// pairs are grouped by L = la + lb on the host and each group runs its own
+// kernel, whose recursion size is known at compile time and whose register
+// budget is its own (a high-L group does not lower the occupancy of the
+// low-L ones). A group's pairs are cut into slices (grid rows) so that a few
+// thousand points still fill the GPU; each slice writes its partial sum per
+// point and a final kernel adds the slices in a fixed order (deterministic).
 #include "kernels.hpp"
 
 #include "common.hpp"
@@ -15,11 +19,16 @@ namespace kumo {
 namespace {
 
 constexpr int kBoysOrders = 39, kBoysPoints = 801;
+constexpr int kEspThreads = 128;
 
-__device__ inline void boys(const double* __restrict__ table, double T, int top, double* v) {
+// F_0..F_top(T) into v[0..top]. Inlined: with a constant top the loops
+// unroll and v stays in registers.
+__device__ __attribute__((always_inline)) inline void boys(const double* __restrict__ table, double T, int top,
+                                                           double* v) {
     const double decay = exp(-T);
     if (T >= 40.0) {
         v[0] = 0.5 * sqrt(3.141592653589793 / T);
+#pragma unroll
         for (int m = 0; m < top; ++m)
             v[m + 1] = ((2.0 * m + 1.0) * v[m] - decay) / (2.0 * T);
         return;
@@ -28,11 +37,13 @@ __device__ inline void boys(const double* __restrict__ table, double T, int top,
     const double delta = T - point * 0.05;
     const double* row = table + point * kBoysOrders;
     double sum = 0.0, factor = 1.0;
+#pragma unroll
     for (int k = 0; k < 7; ++k) {
         sum += row[top + k] * factor;
         factor *= -delta / (k + 1.0);
     }
     v[top] = sum;
+#pragma unroll
     for (int m = top; m > 0; --m)
         v[m - 1] = (2.0 * T * v[m] + decay) / (2.0 * m - 1.0);
 }
@@ -48,9 +59,14 @@ __host__ __device__ constexpr int hindex(int t, int u, int v) {
     return pos + (s - t - u);
 }
 
+// sum_k c_k R^0_k of one pair. The recursion from order n + 1 to order n of
+// the auxiliary index runs in place over one array: the entries of total
+// degree s at order n need those of degrees s - 1 and s - 2 at order n + 1,
+// so the degrees are rewritten from the highest down.
 template <int L>
-__device__ inline double pair_sum(const EspPairDev& pr, const double* __restrict__ coef,
-                                  const double* __restrict__ table, double x, double y, double z) {
+__device__ __attribute__((always_inline)) inline double pair_sum(const EspPairDev& pr, const double* __restrict__ coef,
+                                                                 const double* __restrict__ table, double x, double y,
+                                                                 double z) {
     constexpr int NH = hcount(L);
     const double pc[3] = {pr.qx - x, pr.qy - y, pr.qz - z};
     double seeds[L + 1];
@@ -61,17 +77,11 @@ __device__ inline double pair_sum(const EspPairDev& pr, const double* __restrict
         power *= -2.0 * pr.p;
         seeds[n] *= power;
     }
-    double a[NH], b[NH];
-    double* prev = a;
-    double* cur = b;
+    double r[NH];
 #pragma unroll
     for (int n = L; n >= 0; --n) {
-        double* tmp = prev;
-        prev = cur;
-        cur = tmp;
-        cur[0] = seeds[n];
 #pragma unroll
-        for (int s = 1; s <= L - n; ++s)
+        for (int s = L - n; s >= 1; --s)
 #pragma unroll
             for (int t = s; t >= 0; --t)
 #pragma unroll
@@ -79,39 +89,51 @@ __device__ inline double pair_sum(const EspPairDev& pr, const double* __restrict
                     const int v = s - t - u;
                     double value;
                     if (t > 0) {
-                        value = pc[0] * prev[hindex(t - 1, u, v)];
-                        if (t > 1) value += (t - 1) * prev[hindex(t - 2, u, v)];
+                        value = pc[0] * r[hindex(t - 1, u, v)];
+                        if (t > 1) value += (t - 1) * r[hindex(t - 2, u, v)];
                     } else if (u > 0) {
-                        value = pc[1] * prev[hindex(t, u - 1, v)];
-                        if (u > 1) value += (u - 1) * prev[hindex(t, u - 2, v)];
+                        value = pc[1] * r[hindex(t, u - 1, v)];
+                        if (u > 1) value += (u - 1) * r[hindex(t, u - 2, v)];
                     } else {
-                        value = pc[2] * prev[hindex(t, u, v - 1)];
-                        if (v > 1) value += (v - 1) * prev[hindex(t, u, v - 2)];
+                        value = pc[2] * r[hindex(t, u, v - 1)];
+                        if (v > 1) value += (v - 1) * r[hindex(t, u, v - 2)];
                     }
-                    cur[hindex(t, u, v)] = value;
+                    r[hindex(t, u, v)] = value;
                 }
+        r[0] = seeds[n];
     }
     const double* c = coef + pr.first;
     double sum = 0.0;
 #pragma unroll
     for (int k = 0; k < NH; ++k)
-        sum += c[k] * cur[k];
+        sum += c[k] * r[k];
     return sum;
 }
 
+// Partial sums of slice blockIdx.y of the pairs [first_pair, last_pair) of
+// one L at n points: partial[blockIdx.y * n + j].
 template <int L>
-__device__ double group_sum(const EspPairDev* pairs, int first, int last, const double* coef, const double* table,
-                            double x, double y, double z) {
+__global__ void __launch_bounds__(kEspThreads)
+    esp_pairs_kernel(const EspPairDev* __restrict__ pairs, int first_pair, int last_pair, int per_slice,
+                     const double* __restrict__ coef, const double* __restrict__ table, PointSource ps,
+                     long long first, long long n, double* __restrict__ partial) {
+    const long long j = blockIdx.x * static_cast<long long>(kEspThreads) + threadIdx.x;
+    if (j >= n) return;
+    double x, y, z;
+    point_coords(ps, first + j, x, y, z);
+    const int b = first_pair + static_cast<int>(blockIdx.y) * per_slice;
+    const int e = min(last_pair, b + per_slice);
     double s = 0.0;
-    for (int i = first; i < last; ++i)
+    for (int i = b; i < e; ++i)
         s += pair_sum<L>(pairs[i], coef, table, x, y, z);
-    return s;
+    partial[blockIdx.y * n + j] = s;
 }
 
-__global__ void __launch_bounds__(128) esp_kernel(const EspPairDev* pairs, int npairs, const double* coef,
-                                                  const EspAtomDev* atoms, int natoms, const double* table,
-                                                  PointSource ps, long long first, long long n, double* out) {
-    const long long j = blockIdx.x * 128ll + threadIdx.x;
+// out = nuclear - (sum of the slices in order).
+__global__ void __launch_bounds__(kEspThreads)
+    esp_finish_kernel(const double* __restrict__ partial, int nslices, const EspAtomDev* __restrict__ atoms,
+                      int natoms, PointSource ps, long long first, long long n, double* __restrict__ out) {
+    const long long j = blockIdx.x * static_cast<long long>(kEspThreads) + threadIdx.x;
     if (j >= n) return;
     double x, y, z;
     point_coords(ps, first + j, x, y, z);
@@ -124,28 +146,18 @@ __global__ void __launch_bounds__(128) esp_kernel(const EspPairDev* pairs, int n
         nuclear += at.charge / d;
     }
     double total = 0.0;
-    int i = 0;
-    while (i < npairs) {
-        const int L = pairs[i].L;
-        int e = i;
-        while (e < npairs && pairs[e].L == L)
-            ++e;
-        switch (L) {
-        case 0: total += group_sum<0>(pairs, i, e, coef, table, x, y, z); break;
-        case 1: total += group_sum<1>(pairs, i, e, coef, table, x, y, z); break;
-        case 2: total += group_sum<2>(pairs, i, e, coef, table, x, y, z); break;
-        case 3: total += group_sum<3>(pairs, i, e, coef, table, x, y, z); break;
-        case 4: total += group_sum<4>(pairs, i, e, coef, table, x, y, z); break;
-        case 5: total += group_sum<5>(pairs, i, e, coef, table, x, y, z); break;
-        case 6: total += group_sum<6>(pairs, i, e, coef, table, x, y, z); break;
-        case 7: total += group_sum<7>(pairs, i, e, coef, table, x, y, z); break;
-        case 8: total += group_sum<8>(pairs, i, e, coef, table, x, y, z); break;
-        case 9: total += group_sum<9>(pairs, i, e, coef, table, x, y, z); break;
-        default: total += group_sum<10>(pairs, i, e, coef, table, x, y, z); break;
-        }
-        i = e;
-    }
+    for (int sl = 0; sl < nslices; ++sl)
+        total += partial[sl * n + j];
     out[j] = nuclear - total;
+}
+
+template <int L>
+void launch_group(const EspPairDev* pairs, const EspGroup& g, const double* coef, const double* table,
+                  const PointSource& ps, long long first, long long n, double* partial, hipStream_t stream) {
+    const int per = (g.last - g.first + g.slices - 1) / g.slices;
+    const dim3 grid(static_cast<unsigned>((n + kEspThreads - 1) / kEspThreads), static_cast<unsigned>(g.slices));
+    esp_pairs_kernel<L><<<grid, kEspThreads, 0, stream>>>(pairs, g.first, g.last, per, coef, table, ps, first, n,
+                                                          partial);
 }
 
 __global__ void boys_kernel(const double* table, const double* T, long long n, int m, double* out) {
@@ -159,12 +171,51 @@ __global__ void boys_kernel(const double* table, const double* T, long long n, i
 
 } // namespace
 
-void launch_esp(const EspPairDev* pairs, int npairs, const double* coef, const EspAtomDev* atoms, int natoms,
-                const double* boys_table, const PointSource& ps, long long first, long long n, double* out,
-                hipStream_t stream) {
+int esp_slices(std::vector<EspGroup>& groups, long long n) {
+    // About 8 waves per SIMD over the 1024 SIMDs of the GPU; at least 16
+    // pairs per slice, at most 64 slices per group.
+    const long long blocks = (n + kEspThreads - 1) / kEspThreads;
+    const long long want = std::max<long long>(1, (4096 + blocks - 1) / std::max<long long>(1, blocks));
+    int total = 0;
+    for (auto& g : groups) {
+        const long long np = g.last - g.first;
+        g.slices = static_cast<int32_t>(std::max<long long>(1, std::min({want, 64ll, (np + 15) / 16})));
+        // no empty slice: ceil(np / slices) pairs each
+        const long long per = (np + g.slices - 1) / g.slices;
+        g.slices = static_cast<int32_t>(std::max<long long>(1, (np + per - 1) / per));
+        total += g.slices;
+    }
+    return total;
+}
+
+void launch_esp(const EspPairDev* pairs, const std::vector<EspGroup>& groups, const double* coef,
+                const EspAtomDev* atoms, int natoms, const double* boys_table, const PointSource& ps, long long first,
+                long long n, double* partial, double* out, hipStream_t stream) {
     if (n <= 0) return;
-    esp_kernel<<<static_cast<unsigned>((n + 127) / 128), 128, 0, stream>>>(pairs, npairs, coef, atoms, natoms,
-                                                                          boys_table, ps, first, n, out);
+    double* dst = partial;
+    int nslices = 0;
+    for (const EspGroup& g : groups) {
+        if (g.last <= g.first) continue;
+        if (g.L < 0 || g.L > 2 * kMaxL || g.slices < 1) invalid("invalid electrostatic potential pair group");
+        switch (g.L) {
+        case 0: launch_group<0>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 1: launch_group<1>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 2: launch_group<2>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 3: launch_group<3>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 4: launch_group<4>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 5: launch_group<5>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 6: launch_group<6>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 7: launch_group<7>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 8: launch_group<8>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        case 9: launch_group<9>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        default: launch_group<10>(pairs, g, coef, boys_table, ps, first, n, dst, stream); break;
+        }
+        KUMO_HIP(hipGetLastError());
+        dst += static_cast<long long>(g.slices) * n;
+        nslices += g.slices;
+    }
+    esp_finish_kernel<<<static_cast<unsigned>((n + kEspThreads - 1) / kEspThreads), kEspThreads, 0, stream>>>(
+        partial, nslices, atoms, natoms, ps, first, n, out);
     KUMO_HIP(hipGetLastError());
 }
 

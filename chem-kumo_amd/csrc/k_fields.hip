@@ -64,10 +64,12 @@ __device__ inline double gnorm2(const DDv& d) { return (d.g[0] * d.g[0] + d.g[1]
 
 
 // elf(), lol(), gradient_expansion_locator() of src/evaluator.cpp:127-215
-// over one or two channels.
-__device__ double elf(const DDv* c, int nc, double kFermi) {
+// over NC (1 or 2) channels. Inlined with unrolled loops, so that the channel
+// arrays stay in registers.
+template <int NC> __device__ __attribute__((always_inline)) inline double elf(const DDv (&c)[NC], double kFermi) {
     double pauli = 0.0, uniform = 0.0;
-    for (int i = 0; i < nc; ++i) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
         if (!(c[i].v > 0.0)) continue;
         pauli += c[i].tau - gnorm2(c[i]) / (8.0 * c[i].v);
         uniform += kFermi * pow(c[i].v, 5.0 / 3.0);
@@ -77,9 +79,10 @@ __device__ double elf(const DDv* c, int nc, double kFermi) {
     return 1.0 / (1.0 + chi * chi);
 }
 
-__device__ double lol(const DDv* c, int nc, double kFermi) {
+template <int NC> __device__ __attribute__((always_inline)) inline double lol(const DDv (&c)[NC], double kFermi) {
     double tau = 0.0, uniform = 0.0;
-    for (int i = 0; i < nc; ++i) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
         tau += c[i].tau;
         if (c[i].v > 0.0) uniform += kFermi * pow(c[i].v, 5.0 / 3.0);
     }
@@ -88,9 +91,11 @@ __device__ double lol(const DDv* c, int nc, double kFermi) {
     return uniform / (tau + uniform);
 }
 
-__device__ double ge_locator(const DDv* c, int nc, bool is_lol, double kFermi) {
+template <int NC>
+__device__ __attribute__((always_inline)) inline double ge_locator(const DDv (&c)[NC], bool is_lol, double kFermi) {
     double uniform = 0.0, pauli = 0.0, kinetic = 0.0;
-    for (int i = 0; i < nc; ++i) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
         if (!(c[i].v > 0.0)) continue;
         const double ref = kFermi * pow(c[i].v, 5.0 / 3.0);
         const double wz = gnorm2(c[i]) / c[i].v;
@@ -108,23 +113,28 @@ __device__ double ge_locator(const DDv* c, int nc, bool is_lol, double kFermi) {
 }
 
 // Middle eigenvalue of the symmetric Hessian (xx, xy, xz, yy, yz, zz) by
-// cyclic Jacobi rotations in FP64 (the sign decides sign(lambda2) rho).
-__device__ double middle_eigenvalue(const double* h6) {
+// cyclic Jacobi rotations in FP64 (the sign decides sign(lambda2) rho). The
+// rotation loops are unrolled, so that the matrix stays in registers.
+__device__ __attribute__((always_inline)) inline double middle_eigenvalue(const double (&h6)[6]) {
     double a[3][3] = {{h6[0], h6[1], h6[2]}, {h6[1], h6[3], h6[4]}, {h6[2], h6[4], h6[5]}};
     for (int sweep = 0; sweep < 50; ++sweep) {
         const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
         if (off == 0.0) break;
+#pragma unroll
         for (int p = 0; p < 2; ++p)
+#pragma unroll
             for (int q = p + 1; q < 3; ++q) {
                 if (a[p][q] == 0.0) continue;
                 const double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
                 const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                 const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const double akp = a[k][p], akq = a[k][q];
                     a[k][p] = c * akp - s * akq;
                     a[k][q] = s * akp + c * akq;
                 }
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const double apk = a[p][k], aqk = a[q][k];
                     a[p][k] = c * apk - s * aqk;
@@ -138,7 +148,7 @@ __device__ double middle_eigenvalue(const double* h6) {
     return e2;
 }
 
-__global__ void fields_kernel(FieldsLaunch f) {
+__global__ void __launch_bounds__(256) fields_kernel(FieldsLaunch f) {
     const long long j = blockIdx.x * 256ll + threadIdx.x;
     if (j >= f.n) return;
     const long long n = f.n;
@@ -150,11 +160,11 @@ __global__ void fields_kernel(FieldsLaunch f) {
         const int kind = task.kind;
         // Total ELF / LOL / Tsirelson-Stash forms use both spin channels.
         if ((kind == 6 || kind == 10 || kind == 35 || kind == 36) && task.component == 0) {
-            DDv c[2] = {component(c0, c1, f.from_spin, 1), component(c0, c1, f.from_spin, 2)};
+            const DDv c[2] = {component(c0, c1, f.from_spin, 1), component(c0, c1, f.from_spin, 2)};
             double v;
-            if (kind == 6) v = elf(c, 2, f.fermi);
-            else if (kind == 10) v = lol(c, 2, f.fermi);
-            else v = ge_locator(c, 2, kind == 36, f.fermi);
+            if (kind == 6) v = elf(c, f.fermi);
+            else if (kind == 10) v = lol(c, f.fermi);
+            else v = ge_locator(c, kind == 36, f.fermi);
             task.out[0][o] = v;
             continue;
         }
@@ -164,7 +174,8 @@ __global__ void fields_kernel(FieldsLaunch f) {
             task.out[0][o] = tv > 0.0 ? sv / tv : 0.0;
             continue;
         }
-        const DDv d = component(c0, c1, f.from_spin, task.component);
+        const DDv one[1] = {component(c0, c1, f.from_spin, task.component)};
+        const DDv& d = one[0];
         double* out0 = task.out[0];
         switch (kind) {
         case 0: out0[o] = d.v; break;
@@ -172,7 +183,7 @@ __global__ void fields_kernel(FieldsLaunch f) {
         case 2: for (int k = 0; k < 6; ++k) task.out[k][o] = d.h[k]; break;
         case 3: out0[o] = d.lap; break;
         case 4: out0[o] = d.tau; break;
-        case 6: out0[o] = elf(&d, 1, f.fermi); break;
+        case 6: out0[o] = elf(one, f.fermi); break;
         case 7: { // reduced gradient
             const double scale = f.rdg_scale;
             double v = 100.0;
@@ -185,9 +196,9 @@ __global__ void fields_kernel(FieldsLaunch f) {
             break;
         }
         case 8: out0[o] = middle_eigenvalue(d.h) < 0.0 ? -d.v : d.v; break;
-        case 10: out0[o] = lol(&d, 1, f.fermi); break;
-        case 35: out0[o] = ge_locator(&d, 1, false, f.fermi); break;
-        case 36: out0[o] = ge_locator(&d, 1, true, f.fermi); break;
+        case 10: out0[o] = lol(one, f.fermi); break;
+        case 35: out0[o] = ge_locator(one, false, f.fermi); break;
+        case 36: out0[o] = ge_locator(one, true, f.fermi); break;
         case 11: { // IRI
             double v = 100.0;
             if (d.v > 0.0) {

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace kumo {
 
@@ -248,9 +249,18 @@ struct EspPairDev {
 struct EspAtomDev {
     double x, y, z, charge;
 };
-void launch_esp(const EspPairDev* pairs, int npairs, const double* coef, const EspAtomDev* atoms, int natoms,
-                const double* boys_table, const PointSource& ps, long long first, long long n, double* out,
-                hipStream_t stream);
+// The pairs [first, last) of one L = la + lb, evaluated in `slices` contiguous
+// slices (each writes a partial sum per point).
+struct EspGroup {
+    int32_t L, first, last, slices;
+};
+// Sets the slices of every group for n points; returns their total (the rows
+// of the partial-sum workspace, n doubles each).
+int esp_slices(std::vector<EspGroup>& groups, long long n);
+// out[j] = V at point first + j, j < n; partial holds esp_slices(groups, n) x n doubles.
+void launch_esp(const EspPairDev* pairs, const std::vector<EspGroup>& groups, const double* coef,
+                const EspAtomDev* atoms, int natoms, const double* boys_table, const PointSource& ps, long long first,
+                long long n, double* partial, double* out, hipStream_t stream);
 void launch_boys(const double* table, const double* T, long long n, int m, double* out, hipStream_t stream);
 
 } // namespace kumo

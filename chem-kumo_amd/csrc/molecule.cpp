@@ -605,8 +605,12 @@ void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
     }
     std::vector<EspPairDev> pairs;
     std::vector<double> coef;
+    esp_groups.clear();
     for (std::size_t L = 0; L < groups.size(); ++L) {
         const int base = static_cast<int>(coef.size());
+        if (!groups[L].empty())
+            esp_groups.push_back({static_cast<int32_t>(L), static_cast<int32_t>(pairs.size()),
+                                  static_cast<int32_t>(pairs.size() + groups[L].size()), 1});
         for (auto p : groups[L]) {
             p.first += base;
             pairs.push_back(p);

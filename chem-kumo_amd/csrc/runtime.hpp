@@ -27,7 +27,7 @@ struct Executor {
     hipStream_t stream = nullptr;
     const double* boys_table = nullptr;
     DeviceBuffer ws_tables, ws_moments, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale,
-        ws_oe, ws_oe_tasks;
+        ws_oe, ws_oe_tasks, ws_esp;
     // Events bracketing the contraction launches (index 0), the FP64 rescue
     // launches (index 1) and the field reductions (index 2).
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
@@ -121,6 +121,7 @@ struct kumo_hip_molecule {
     bool esp_ready = false;
     kumo::DeviceBuffer d_esp_pairs, d_esp_coef, d_esp_atoms;
     int esp_npairs = 0;
+    std::vector<kumo::EspGroup> esp_groups; // pair ranges by L
     // Orbital-energy operand (OrbitalEnergyFields, src/orbital_energy_fields.hpp:98-192;
     // built on first use): every MO of the FCHK in FP64, slots [0,
     // oe_occ_slots) the occupied MOs, then the virtual ones, each range
