@@ -14,12 +14,14 @@
 #include <array>
 #include <cmath>
 #include <cstddef>
+#include <filesystem>
 #include <functional>
 #include <map>
 #include <memory>
 #include <span>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace chem::kumo {
@@ -430,5 +432,277 @@ private:
     const Molecule* molecule_;
     EvaluationOptions options_;
 };
+
+// ---- molecular surfaces (include/chem/kumo/surface.hpp) ----------------
+
+inline constexpr double hartree_in_kcal_per_mol = 627.5094740631;  // include/chem/kumo/surface.hpp:18
+inline constexpr double hartree_in_electronvolt = 27.211386245988; // include/chem/kumo/surface.hpp:20
+inline constexpr double minimum_density_isovalue = 1e-8;           // include/chem/kumo/surface.hpp:90
+
+// Closed, oriented triangle mesh (include/chem/kumo/surface.hpp:25-34):
+// vertices in bohr, triangles counter-clockwise seen from outside.
+struct TriangleMesh {
+    std::vector<Vector3> vertices;
+    std::vector<std::array<std::size_t, 3>> triangles;
+
+    double area() const { return measure().first; }
+    double volume() const { return measure().second; }
+
+private:
+    std::pair<double, double> measure() const {
+        double area = 0.0, volume = 0.0;
+        for (const auto& t : triangles) {
+            const Vector3 &a = vertices.at(t[0]), &b = vertices.at(t[1]), &c = vertices.at(t[2]);
+            const Vector3 u{b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v{c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            const Vector3 n{u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+            area += 0.5 * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            // divergence theorem: (1/6) a . ((b - a) x (c - a)) = (1/6) a . (b x c)
+            volume += (a[0] * n[0] + a[1] * n[1] + a[2] * n[2]) / 6.0;
+        }
+        return {area, volume};
+    }
+};
+
+// Politzer-Murray statistics of a function on a surface
+// (include/chem/kumo/surface.hpp:36-74).
+struct SurfaceStatistics {
+    double area = 0.0, positive_area = 0.0, negative_area = 0.0, nonpolar_area = 0.0;
+    double minimum = 0.0, maximum = 0.0;
+    std::size_t minimum_vertex = 0, maximum_vertex = 0;
+    double average = 0.0, positive_average = 0.0, negative_average = 0.0;
+    double positive_variance = 0.0, negative_variance = 0.0, total_variance = 0.0;
+    double balance = 0.0, charge_separation = 0.0, polarity_index = 0.0;
+    std::vector<std::size_t> local_minima, local_maxima;
+};
+
+struct SurfaceOptions { // include/chem/kumo/surface.hpp:108-116
+    double isovalue = 1e-3;
+    double spacing = 0.25;
+    double potential_tolerance = 1e-8; // the HIP path evaluates the potential exactly
+};
+
+struct PotentialSurface { // include/chem/kumo/surface.hpp:118-123
+    TriangleMesh mesh;
+    std::vector<double> potential;
+    SurfaceStatistics statistics;
+};
+
+struct IonizationEnergySurface { // include/chem/kumo/surface.hpp:156-166
+    TriangleMesh mesh;
+    std::vector<double> ionization_energy;
+    SurfaceStatistics statistics;
+};
+
+struct MappedSurface { // include/chem/kumo/surface.hpp:176-182
+    TriangleMesh mesh;
+    std::vector<Property> properties;
+    std::vector<std::vector<double>> values;
+    std::vector<SurfaceStatistics> statistics;
+    Backend backend_used = Backend::AO_P;
+};
+
+namespace detail {
+inline SurfaceStatistics to_statistics(const kumo_hip_surface_stats& c, const std::vector<int64_t>& minima,
+                                       const std::vector<int64_t>& maxima) {
+    SurfaceStatistics s;
+    s.area = c.area;
+    s.positive_area = c.positive_area;
+    s.negative_area = c.negative_area;
+    s.nonpolar_area = c.nonpolar_area;
+    s.minimum = c.minimum;
+    s.maximum = c.maximum;
+    s.minimum_vertex = static_cast<std::size_t>(c.minimum_vertex);
+    s.maximum_vertex = static_cast<std::size_t>(c.maximum_vertex);
+    s.average = c.average;
+    s.positive_average = c.positive_average;
+    s.negative_average = c.negative_average;
+    s.positive_variance = c.positive_variance;
+    s.negative_variance = c.negative_variance;
+    s.total_variance = c.total_variance;
+    s.balance = c.balance;
+    s.charge_separation = c.charge_separation;
+    s.polarity_index = c.polarity_index;
+    s.local_minima.assign(minima.begin(), minima.end());
+    s.local_maxima.assign(maxima.begin(), maxima.end());
+    return s;
+}
+
+inline std::vector<int64_t> flat_triangles(const TriangleMesh& mesh) {
+    std::vector<int64_t> t;
+    t.reserve(3 * mesh.triangles.size());
+    for (const auto& tri : mesh.triangles)
+        for (std::size_t k : tri)
+            t.push_back(static_cast<int64_t>(k));
+    return t;
+}
+
+inline std::vector<double> flat_vertices(const TriangleMesh& mesh) {
+    std::vector<double> v;
+    v.reserve(3 * mesh.vertices.size());
+    for (const auto& x : mesh.vertices)
+        v.insert(v.end(), x.begin(), x.end());
+    return v;
+}
+
+// A kumo_hip_surface read into host structures: the mesh, then each field's
+// values and statistics.
+class SurfaceHandle {
+public:
+    template <typename Make> explicit SurfaceHandle(Make&& make) {
+        kumo_hip_surface* s = nullptr;
+        check(make(&s));
+        ptr_.reset(s);
+        int64_t nv = 0, nt = 0;
+        check(kumo_hip_surface_sizes(s, &nv, &nt, &nfields_, &backend_));
+        std::vector<double> v(static_cast<std::size_t>(3 * nv));
+        std::vector<int64_t> t(static_cast<std::size_t>(3 * nt));
+        check(kumo_hip_surface_mesh(s, v.data(), t.data()));
+        for (int64_t i = 0; i < nv; ++i)
+            mesh_.vertices.push_back({v[3 * i], v[3 * i + 1], v[3 * i + 2]});
+        for (int64_t i = 0; i < nt; ++i)
+            mesh_.triangles.push_back({static_cast<std::size_t>(t[3 * i]), static_cast<std::size_t>(t[3 * i + 1]),
+                                       static_cast<std::size_t>(t[3 * i + 2])});
+    }
+    const TriangleMesh& mesh() const noexcept { return mesh_; }
+    int fields() const noexcept { return nfields_; }
+    int backend() const noexcept { return backend_; }
+    std::pair<std::vector<double>, SurfaceStatistics> field(int f) const {
+        std::vector<double> values(mesh_.vertices.size());
+        kumo_hip_surface_stats c{};
+        check(kumo_hip_surface_field(ptr_.get(), f, values.data(), &c, nullptr, nullptr));
+        std::vector<int64_t> mins(static_cast<std::size_t>(c.local_minima)), maxs(static_cast<std::size_t>(c.local_maxima));
+        check(kumo_hip_surface_field(ptr_.get(), f, nullptr, &c, mins.data(), maxs.data()));
+        return {std::move(values), to_statistics(c, mins, maxs)};
+    }
+
+private:
+    std::unique_ptr<kumo_hip_surface, int (*)(kumo_hip_surface*)> ptr_{nullptr, kumo_hip_surface_release};
+    TriangleMesh mesh_;
+    int32_t nfields_ = 0, backend_ = -1;
+};
+} // namespace detail
+
+// surface_statistics (include/chem/kumo/surface.hpp:76-84).
+inline SurfaceStatistics surface_statistics(const TriangleMesh& mesh, std::span<const double> values,
+                                            double polar_threshold = 10.0 / hartree_in_kcal_per_mol) {
+    if (values.size() != mesh.vertices.size())
+        throw std::invalid_argument("surface statistics need one value per vertex");
+    const auto v = detail::flat_vertices(mesh);
+    const auto t = detail::flat_triangles(mesh);
+    kumo_hip_surface_stats c{};
+    const auto call = [&](int64_t* mins, int64_t* maxs) {
+        detail::check(kumo_hip_surface_statistics(detail::context(), v.data(),
+                                                  static_cast<int64_t>(mesh.vertices.size()), t.data(),
+                                                  static_cast<int64_t>(mesh.triangles.size()), values.data(),
+                                                  polar_threshold, &c, mins, maxs, nullptr, nullptr));
+    };
+    call(nullptr, nullptr);
+    std::vector<int64_t> mins(static_cast<std::size_t>(c.local_minima)), maxs(static_cast<std::size_t>(c.local_maxima));
+    call(mins.data(), maxs.data());
+    return detail::to_statistics(c, mins, maxs);
+}
+
+// density_isosurface (include/chem/kumo/surface.hpp:92-106).
+inline TriangleMesh density_isosurface(const Molecule& molecule, double isovalue = 1e-3, double spacing = 0.25) {
+    const detail::SurfaceHandle h([&](kumo_hip_surface** s) {
+        return kumo_hip_density_isosurface(detail::context(), molecule.handle(), isovalue, spacing, s);
+    });
+    return h.mesh();
+}
+
+// electrostatic_potential_surface (include/chem/kumo/surface.hpp:125-131).
+inline PotentialSurface electrostatic_potential_surface(const Molecule& molecule, const SurfaceOptions& options = {}) {
+    const detail::SurfaceHandle h([&](kumo_hip_surface** s) {
+        return kumo_hip_electrostatic_potential_surface(detail::context(), molecule.handle(), options.isovalue,
+                                                        options.spacing, options.potential_tolerance, s);
+    });
+    auto [values, stats] = h.field(0);
+    return {h.mesh(), std::move(values), std::move(stats)};
+}
+
+// average_local_ionization_energy (include/chem/kumo/surface.hpp:133-153).
+inline std::vector<double> average_local_ionization_energy(const Molecule& molecule, std::span<const Vector3> points) {
+    std::vector<double> xyz;
+    xyz.reserve(3 * points.size());
+    for (const auto& p : points)
+        xyz.insert(xyz.end(), p.begin(), p.end());
+    std::vector<double> out(points.size());
+    detail::check(kumo_hip_average_local_ionization_energy(detail::context(), molecule.handle(), xyz.data(),
+                                                           points.size(), out.data()));
+    return out;
+}
+
+// local_ionization_energy_surface (include/chem/kumo/surface.hpp:168-173).
+inline IonizationEnergySurface local_ionization_energy_surface(const Molecule& molecule, double isovalue = 1e-3,
+                                                               double spacing = 0.25) {
+    const detail::SurfaceHandle h([&](kumo_hip_surface** s) {
+        return kumo_hip_local_ionization_energy_surface(detail::context(), molecule.handle(), isovalue, spacing, s);
+    });
+    auto [values, stats] = h.field(0);
+    return {h.mesh(), std::move(values), std::move(stats)};
+}
+
+// map_on_density_isosurface (include/chem/kumo/surface.hpp:184-195): one
+// entry per distinct property, in the order of first occurrence. The options'
+// progress callback is not used on this path.
+inline MappedSurface map_on_density_isosurface(const Molecule& molecule, const std::vector<Property>& properties,
+                                               double isovalue = 1e-3, double spacing = 0.25,
+                                               const EvaluationOptions& options = {}) {
+    MappedSurface out;
+    std::vector<kumo_hip_property> props;
+    for (const auto& p : properties) {
+        bool seen = false;
+        for (const auto& q : out.properties)
+            seen = seen || q == p;
+        if (seen) continue;
+        out.properties.push_back(p);
+        props.push_back({static_cast<int32_t>(p.kind), static_cast<int32_t>(p.component),
+                         static_cast<int32_t>(p.channel), static_cast<int32_t>(p.orbital_index)});
+    }
+    kumo_hip_options o;
+    kumo_hip_default_options(&o);
+    o.threads = options.threads;
+    o.block_size = static_cast<int32_t>(options.block_size);
+    o.backend = static_cast<int32_t>(options.backend);
+    o.precision = static_cast<int32_t>(options.precision);
+    o.screening_threshold = options.screening_threshold;
+    o.potential_tolerance = options.potential_tolerance;
+    o.fukui_delta = options.fukui_delta;
+    const detail::SurfaceHandle h([&](kumo_hip_surface** s) {
+        return kumo_hip_map_on_density_isosurface(detail::context(), molecule.handle(), props.data(), props.size(),
+                                                  isovalue, spacing, &o, s);
+    });
+    out.mesh = h.mesh();
+    for (int f = 0; f < h.fields(); ++f) {
+        auto [values, stats] = h.field(f);
+        out.values.push_back(std::move(values));
+        out.statistics.push_back(std::move(stats));
+    }
+    out.backend_used = static_cast<Backend>(h.backend());
+    return out;
+}
+
+// write_ply (include/chem/kumo/surface.hpp:197-210).
+inline void write_ply(const std::filesystem::path& path, const TriangleMesh& mesh,
+                      const std::vector<std::pair<std::string, std::vector<double>>>& fields,
+                      const std::vector<std::string>& comments = {}) {
+    const auto v = detail::flat_vertices(mesh);
+    const auto t = detail::flat_triangles(mesh);
+    std::vector<const char*> names, notes;
+    std::vector<const double*> values;
+    for (const auto& [name, vals] : fields) {
+        if (vals.size() != mesh.vertices.size())
+            throw std::invalid_argument("PLY property '" + name + "' needs one value per vertex");
+        names.push_back(name.c_str());
+        values.push_back(vals.data());
+    }
+    for (const auto& c : comments)
+        notes.push_back(c.c_str());
+    const std::string p = path.string();
+    detail::check(kumo_hip_write_ply(detail::context(), p.c_str(), v.data(), static_cast<int64_t>(mesh.vertices.size()),
+                                     t.data(), static_cast<int64_t>(mesh.triangles.size()),
+                                     static_cast<int32_t>(fields.size()), names.data(), values.data(),
+                                     static_cast<int32_t>(comments.size()), notes.data()));
+}
 
 } // namespace chem::kumo

@@ -94,5 +94,53 @@ int main(int argc, char** argv) {
         threw = true;
     }
     check(threw, "a nonfinite sample throws std::invalid_argument");
+
+    // Molecular surfaces (include/chem/kumo/surface.hpp) as a reference user
+    // calls them.
+    const PotentialSurface esp = electrostatic_potential_surface(molecule);
+    std::printf("esp_surface vertices=%zu triangles=%zu area=%.17g minimum=%.17g\n", esp.mesh.vertices.size(),
+                esp.mesh.triangles.size(), esp.statistics.area, esp.statistics.minimum);
+    check(esp.potential.size() == esp.mesh.vertices.size(), "one potential value per surface vertex");
+    check(std::abs(esp.statistics.area - esp.mesh.area()) <= 1e-12 * esp.mesh.area(),
+          "statistics area equals TriangleMesh::area");
+    check(esp.mesh.volume() > 0.0, "the isosurface encloses a positive volume");
+    check(!esp.statistics.local_minima.empty() && esp.statistics.local_minima.size() <= esp.mesh.vertices.size(),
+          "ESP surface has local minima");
+    const TriangleMesh mesh = density_isosurface(molecule);
+    check(mesh.vertices.size() == esp.mesh.vertices.size() && mesh.triangles == esp.mesh.triangles,
+          "density_isosurface is the ESP surface's mesh");
+    const IonizationEnergySurface alie = local_ionization_energy_surface(molecule);
+    check(alie.statistics.negative_area == 0.0 && alie.statistics.minimum > 0.0, "ALIE is positive on the surface");
+    const auto at_points = average_local_ionization_energy(molecule, points);
+    check(at_points.size() == 2 && at_points[0] > 0.0, "average_local_ionization_energy at points");
+    const MappedSurface mapped =
+        map_on_density_isosurface(molecule, {density::total.value(), density::total.elf(), density::total.value()});
+    bool on_iso = mapped.values.size() == 2 && mapped.properties.size() == 2;
+    for (double v : on_iso ? mapped.values[0] : std::vector<double>{})
+        on_iso = on_iso && std::abs(v - 1e-3) <= 1e-9;
+    check(on_iso, "mapped density equals the isovalue; a repeated property is evaluated once");
+    // The reference test's octahedron with V = x: exact statistics.
+    TriangleMesh octahedron;
+    octahedron.vertices = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    octahedron.triangles = {{0, 2, 4}, {0, 5, 2}, {0, 4, 3}, {0, 3, 5}, {1, 4, 2}, {1, 2, 5}, {1, 3, 4}, {1, 5, 3}};
+    const std::vector<double> x{1, -1, 0, 0, 0, 0};
+    const SurfaceStatistics oct = surface_statistics(octahedron, x, 0.5);
+    check(std::abs(oct.area - 4.0 * std::sqrt(3.0)) < 1e-13 && std::abs(octahedron.volume() - 4.0 / 3.0) < 1e-14 &&
+              std::abs(oct.positive_variance - 1.0 / 18.0) < 1e-15 && oct.local_minima == std::vector<std::size_t>{1} &&
+              oct.local_maxima == std::vector<std::size_t>{0},
+          "octahedron statistics are exact");
+    write_ply(dir + "/kumo-surface-test.ply", esp.mesh, {{"esp", esp.potential}}, {"water ESP"});
+    std::FILE* ply = std::fopen((dir + "/kumo-surface-test.ply").c_str(), "r");
+    char head[64] = {0};
+    const bool ply_ok = ply && std::fgets(head, sizeof head, ply) && std::string(head) == "ply\n";
+    if (ply) std::fclose(ply);
+    check(ply_ok, "write_ply writes an ASCII PLY file");
+    threw = false;
+    try {
+        (void)density_isosurface(molecule, 1e-9);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    check(threw, "an isovalue below 1e-8 throws std::invalid_argument");
     return failures ? 1 : 0;
 }

@@ -2,7 +2,8 @@
 kumo/hip.hpp) compiled into tests/cpp/api_mirror_test and run on the GPU:
 Molecule::from_fchk, Grid::around, Evaluator::evaluate on a grid and at
 points, backend_used(), Field::write_cube / write_orbital_cube (the case of
-tests/atom_charge_cube_test.cpp), and the exception types of the reference."""
+tests/atom_charge_cube_test.cpp), the molecular surface functions of
+include/chem/kumo/surface.hpp, and the exception types of the reference."""
 import os
 import subprocess
 
@@ -33,3 +34,10 @@ def test_cpp_api_mirror(tmp_path):
     got = float([x for x in r.stdout.splitlines() if x.startswith("density_sum=")][0].split()[0].split("=")[1])
     ref, _ = oracle.Molecule(d).evaluate([(0, 0, 0, 0)], grid=grid)
     assert abs(got - ref[0][0].sum()) <= 1e-12 * ref[0][0].sum()
+    # The ESP surface through the mirror: the oracle's mesh of the same
+    # isosurface (rho = 0.001, 0.25 bohr) and its area.
+    line = [x for x in r.stdout.splitlines() if x.startswith("esp_surface ")][0].split()
+    vals = dict(kv.split("=") for kv in line[1:])
+    v, t = oracle.density_isosurface(oracle.Molecule(d), 1e-3, 0.25)
+    assert (int(vals["vertices"]), int(vals["triangles"])) == (len(v), len(t))
+    assert float(vals["area"]) == pytest.approx(oracle.mesh_area(v, t), rel=1e-9)
