@@ -460,7 +460,8 @@ def main():
     ap.add_argument("--precision", default="BF16X6", choices=list(PEAK))
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-blocks", type=int, default=1024, help="screening blocks in the CPU sample")
+    ap.add_argument("--cpu-blocks", type=int, default=0,
+                    help="screening blocks in the CPU sample (0: every block of the grid)")
     ap.add_argument("--no-cube", action="store_true", help="skip the Cube-writer timing")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the orbital-energy field and surface timings")
@@ -525,21 +526,38 @@ def main():
                 "mfma_products_per_flop": PRODUCTS[args.precision]}
 
     errors, cpu = None, None
-    blocks = np.arange(0, n // 128, max(1, (n // 128) // args.cpu_blocks))[:args.cpu_blocks]
+    nblocks = n // 128
+    want = args.cpu_blocks if args.cpu_blocks > 0 else nblocks
+    blocks = np.arange(0, nblocks, max(1, nblocks // want))[:want]
     sample_idx = (blocks[:, None] * 128 + np.arange(128)[None, :]).ravel()
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from support import oracle
         from support.fchk import FchkData
         om = oracle.Molecule(FchkData(path))
-        pts = oracle.grid_points(grid)[sample_idx]
+        all_pts = oracle.grid_points(grid)
+        pts = all_pts[sample_idx]
+        whole = len(blocks) == nblocks
         t = time.perf_counter()
         o, _ = om.evaluate(PROPS, points=pts, block_size=128)
         dt = time.perf_counter() - t
         cpu = {"value": len(pts) / dt, "unit": "grid points/s", "cores": int(oracle.lib().orc_threads()),
-               "kind": "port", "sample": f"{len(blocks)} screening blocks of 128 points spread over the 160^3 grid "
-                                         f"({len(pts)} points), same molecule and properties, FP64"}
-        log(f"CPU oracle: {cpu['value']:.4g} points/s on {cpu['cores']} threads")
+               "kind": "port",
+               "sample": ("every point of the 160^3 grid" if whole else
+                          f"{len(blocks)} screening blocks of 128 points spread over the 160^3 grid ({len(pts)} "
+                          "points)") + ", same molecule and properties, FP64, 128-point blocks"}
+        # the same restatement on one host thread, over 256 blocks spread over the grid
+        threads = int(oracle.lib().orc_threads())
+        oracle.lib().orc_set_threads(1)
+        b1 = np.arange(0, nblocks, max(1, nblocks // 256))[:256]
+        p1 = all_pts[(b1[:, None] * 128 + np.arange(128)[None, :]).ravel()]
+        t = time.perf_counter()
+        om.evaluate(PROPS, points=p1, block_size=128)
+        cpu["single_thread"] = {"value": len(p1) / (time.perf_counter() - t), "unit": "grid points/s",
+                                "sample": f"{len(b1)} screening blocks of 128 points spread over the grid"}
+        oracle.lib().orc_set_threads(threads)
+        log(f"CPU oracle: {cpu['value']:.4g} points/s on {cpu['cores']} threads, "
+            f"{cpu['single_thread']['value']:.4g} on one")
         got = main_run["out"][:, torch.as_tensor(sample_idx, device="cuda")].cpu().numpy()
         rho = o[0][0]
         sig = rho >= 1e-6

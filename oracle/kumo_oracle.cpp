@@ -2476,4 +2476,12 @@ int orc_threads(void) {
     return 1;
 #endif
 }
+
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}
 }

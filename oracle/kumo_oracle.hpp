@@ -123,4 +123,6 @@ int orc_surface_statistics(const double* vertices, std::int64_t nvertices, const
                            std::int64_t* counts, std::int64_t* minima, std::int64_t* maxima, char* error,
                            int error_len);
 int orc_threads(void);
+// Threads of the following calls (OpenMP); nonpositive keeps the current count.
+void orc_set_threads(int threads);
 }
