@@ -52,7 +52,7 @@ int kumo_hip_init(int device, kumo_hip_context** out) {
         KUMO_HIP(hipSetDevice(device));
         ctx->device = device;
         ctx->exec.create(device, nullptr);
-        const auto table = kumo::boys_table_host();
+        const auto table = kumo::boys_device_table();
         ctx->boys_table = kumo::upload(table.data(), table.size(), ctx->exec.stream);
         KUMO_HIP(hipStreamSynchronize(ctx->exec.stream));
         ctx->exec.boys_table = ctx->boys_table.as<double>();

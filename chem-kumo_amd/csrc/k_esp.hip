@@ -18,13 +18,13 @@
 namespace kumo {
 namespace {
 
-constexpr int kBoysOrders = 39, kBoysPoints = 801;
 constexpr int kEspThreads = 128;
 
-// F_0..F_top(T) into v[0..top]. Inlined: with a constant top the loops
-// unroll and v stays in registers.
-__device__ __attribute__((always_inline)) inline void boys(const double* __restrict__ table, double T, int top,
-                                                           double* v) {
+// F_0..F_top(T) into v[0..top] in the reference's arithmetic
+// (src/boys_function.cpp:47-76). Row i of `table` (stride doubles apart)
+// holds F_top .. F_top+6 at i * 0.05 from offset `first`.
+__device__ __attribute__((always_inline)) inline void boys(const double* __restrict__ table, int stride, int first,
+                                                           double T, int top, double* v) {
     const double decay = exp(-T);
     if (T >= 40.0) {
         v[0] = 0.5 * sqrt(3.141592653589793 / T);
@@ -35,17 +35,54 @@ __device__ __attribute__((always_inline)) inline void boys(const double* __restr
     }
     const int point = static_cast<int>(llround(T / 0.05));
     const double delta = T - point * 0.05;
-    const double* row = table + point * kBoysOrders;
+    const double* row = table + point * stride + first;
     double sum = 0.0, factor = 1.0;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-        sum += row[top + k] * factor;
+        sum += row[k] * factor;
         factor *= -delta / (k + 1.0);
     }
     v[top] = sum;
 #pragma unroll
     for (int m = top; m > 0; --m)
         v[m - 1] = (2.0 * T * v[m] + decay) / (2.0 * m - 1.0);
+}
+
+// boys() for the pair loop of the ESP kernels: the same scheme (the table
+// point nearest to T and a 7-term Taylor series below T = 40, the closed form
+// and upward recursion above), with every division by a constant, and the
+// repeated division by 2T, replaced by a product with the reciprocal and the
+// series summed by Horner's rule. The values agree with boys() to a few ulp;
+// the loop then needs one FP64 division (at T >= 40) instead of up to 7 + L.
+template <int TOP>
+__device__ __attribute__((always_inline)) inline void boys_esp(const double* __restrict__ rows, double T,
+                                                               double (&v)[TOP + 1]) {
+    constexpr double kInvFactorial[7] = {1.0, 1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0};
+    if (T >= 40.0) {
+        const double inv = 1.0 / T;
+        v[0] = 0.5 * sqrt(3.141592653589793 * inv);
+        if constexpr (TOP > 0) {
+            const double decay = exp(-T), half = 0.5 * inv;
+#pragma unroll
+            for (int m = 0; m < TOP; ++m)
+                v[m + 1] = ((2.0 * m + 1.0) * v[m] - decay) * half;
+        }
+        return;
+    }
+    const int point = static_cast<int>(llround(T * 20.0)); // nearest table point (step 0.05)
+    const double x = point * 0.05 - T;
+    const double* row = rows + point * kBoysRowL;
+    double sum = row[6] * kInvFactorial[6];
+#pragma unroll
+    for (int k = 5; k >= 0; --k)
+        sum = sum * x + row[k] * kInvFactorial[k];
+    v[TOP] = sum;
+    if constexpr (TOP > 0) {
+        const double decay = exp(-T), t2 = 2.0 * T;
+#pragma unroll
+        for (int m = TOP; m > 0; --m)
+            v[m - 1] = (t2 * v[m] + decay) * (1.0 / (2.0 * m - 1.0));
+    }
 }
 
 __host__ __device__ constexpr int hcount(int s) { return (s + 1) * (s + 2) * (s + 3) / 6; }
@@ -70,7 +107,9 @@ __device__ __attribute__((always_inline)) inline double pair_sum(const EspPairDe
     constexpr int NH = hcount(L);
     const double pc[3] = {pr.qx - x, pr.qy - y, pr.qz - z};
     double seeds[L + 1];
-    boys(table, pr.p * ((pc[0] * pc[0] + pc[1] * pc[1]) + pc[2] * pc[2]), L, seeds);
+    // the orders L .. L + 6 of this L (kBoysRowL doubles per point)
+    boys_esp<L>(table + kBoysOrders * kBoysPoints + L * kBoysPoints * kBoysRowL,
+                pr.p * ((pc[0] * pc[0] + pc[1] * pc[1]) + pc[2] * pc[2]), seeds);
     double power = 1.0;
 #pragma unroll
     for (int n = 1; n <= L; ++n) {
@@ -164,7 +203,7 @@ __global__ void boys_kernel(const double* table, const double* T, long long n, i
     const long long j = blockIdx.x * 128ll + threadIdx.x;
     if (j >= n) return;
     double v[33];
-    boys(table, T[j], m, v);
+    boys(table, kBoysOrders, m, T[j], m, v);
     for (int k = 0; k <= m; ++k)
         out[j * (m + 1) + k] = v[k];
 }

@@ -241,6 +241,10 @@ void launch_orbitals(const MolView& m, const PointSource& ps, long long first, l
                      hipStream_t stream);
 
 // Electrostatic potential at points (src/electrostatic_potential.cpp:231-361).
+// Boys table on the device: F_m(i * 0.05) for m < kBoysOrders, i < kBoysPoints
+// (row i at i * kBoysOrders), then per pair order L the orders L .. L + 6 of
+// each point in rows of kBoysRowL doubles.
+constexpr int kBoysOrders = 39, kBoysPoints = 801, kBoysRowL = 8;
 struct EspPairDev {
     double p, qx, qy, qz;
     int32_t L;

@@ -406,6 +406,22 @@ std::vector<double> boys_table_host() {
     return t;
 }
 
+// The device copy: the table, then for each ESP pair order L <= 2 kMaxL the
+// seven orders L .. L + 6 of every point, padded to kBoysRowL doubles, so
+// that a kernel of one L reads one short row per point from a table small
+// enough to stay in the vector caches.
+std::vector<double> boys_device_table() {
+    std::vector<double> t = boys_table_host();
+    constexpr int orders = 39, points = 801;
+    t.resize(static_cast<std::size_t>(orders * points + (2 * kMaxL + 1) * points * kBoysRowL), 0.0);
+    for (int L = 0; L <= 2 * kMaxL; ++L)
+        for (int i = 0; i < points; ++i)
+            for (int k = 0; k < 7; ++k)
+                t[static_cast<std::size_t>(orders * points + (L * points + i) * kBoysRowL + k)] =
+                    t[static_cast<std::size_t>(i * orders + L + k)];
+    return t;
+}
+
 } // namespace kumo
 
 const double* kumo_hip_molecule::p_matrix(int component, hipStream_t stream) {

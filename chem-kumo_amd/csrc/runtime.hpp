@@ -149,6 +149,7 @@ namespace kumo {
 std::unique_ptr<kumo_hip_molecule> load_molecule(kumo_hip_context* ctx, const kumo_hip_molecule_desc& d);
 std::unique_ptr<kumo_hip_molecule> load_fchk(kumo_hip_context* ctx, const std::string& path);
 std::vector<double> boys_table_host();
+std::vector<double> boys_device_table();
 
 struct PointsArg {
     const kumo_hip_grid* grid = nullptr;
