@@ -242,6 +242,17 @@ double rho_rescue_eps(int precision) {
 }
 constexpr double kRhoRescueTol = 5e-7; // largest relative density error left to the contraction
 
+// ELF rescue from per-point error scales (BF16X6): the unit e of the error
+// model, calibrated. The chain of bounds (Cauchy-Schwarz over the MOs, the
+// triangle inequality over the operand rows) overestimates the contraction's
+// rounding error, which sums with random signs, by about two orders of
+// magnitude. Over the 4.1e6 points of a 640-AO UHF molecule on a 160^3 cube
+// (ELF rescue alone) e = 3e-9 flagged 9.2 % of the points and left no ELF
+// error above 7.8e-7 (e = 1e-9: 7.2 %, 1.5e-6), where the model over
+// tau + |grad rho|^2 / 8 rho flagged 13.5 % and left 1.04e-6. Other
+// precisions use that model (rescue_eps).
+double elf_scale_eps(int precision) { return precision == KUMO_HIP_BF16X6 ? 3e-9 : 0.0; }
+
 // Autotuner of the AO->MO tile. The points per workgroup (16 or 32) change
 // only how points are shared among workgroups (every point's arithmetic and
 // summation order are the same, so results are identical): the first launch
@@ -528,7 +539,11 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.slot1 = slot1;
                 a.moments = chm;
                 const bool rho_rescue = rho_which && rho_rescue_eps(opt.precision) > 0.0;
-                a.rho_scale = rho_rescue ? grow<float>(ex.ws_rscale, 2 * static_cast<std::size_t>(batch_max)) : nullptr;
+                const bool elf_scaled = elf_which && ao_order >= 1 && elf_scale_eps(opt.precision) > 0.0;
+                // value scales [2][count], then (with gradient panels) gradient scales [2][count]
+                a.rho_scale = rho_rescue || elf_scaled
+                                  ? grow<float>(ex.ws_rscale, 4 * static_cast<std::size_t>(batch_max))
+                                  : nullptr;
                 a.tile_points = opt.tile_points ? opt.tile_points : tuned_tile(ex, mol, a, s);
                 record(ex, true);
                 launch_ao_mo(a, s);
@@ -551,6 +566,8 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                     rc.rho_scale = a.rho_scale;
                     rc.rho_eps = rho_rescue_eps(opt.precision);
                     rc.rho_tol = kRhoRescueTol;
+                    rc.grad_scale = elf_scaled ? a.rho_scale + 2 * count : nullptr;
+                    rc.elf_scale_eps = elf_scale_eps(opt.precision);
                     launch_rescue_plan(chm, count, closed ? 1 : 0, rc, bs, nb, kRescueTile, rp, s);
                     AoMoLaunch r = a;
                     r.precision = KUMO_HIP_FP64;

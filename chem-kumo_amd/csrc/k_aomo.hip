@@ -23,8 +23,9 @@
 // gradient, tau terms, Hessian) and reduces them over the tiles of each
 // spin channel in a fixed order, so the result is independent of batching.
 // Reduced precisions also sum, per point and channel, |Phi_k| times the norm
-// of coefficient row k over the channel's MOs: the scale of the
-// contraction's rounding error that the density rescue tests.
+// of coefficient row k over the channel's MOs, and (with gradient panels)
+// |d/dx Phi_k| + |d/dy Phi_k| + |d/dz Phi_k| times the same norm: the scales
+// of the contraction's rounding error that the rescue tests.
 //
 // MFMA operand layouts (wave64): for 16x16xK the lane l holds A row l % 16
 // and B column l % 16 at a K slice selected by l / 16; accumulator register r
@@ -156,14 +157,16 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     constexpr int CS_PLANE = KC * PITCH;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     // LDS: phi[S][NP][T][LDK], cs[S][KC][PITCH]; after the chunk loop the
-    // same memory holds red[MT][kMoments][T] (double), then esl[256][2]
-    // (float, the threads' error-scale sums).
+    // same memory holds red[MT][kMoments][T] (double), then esl[256][4]
+    // (float, the threads' error-scale sums: value alpha, beta, gradient
+    // alpha, beta).
     E* phi = reinterpret_cast<E*>(smem);
     E* cs = phi + S * PHI_PLANE;
     double* red = reinterpret_cast<double*>(smem);
     float* esl = reinterpret_cast<float*>(smem + sizeof(double) * 8 * kMoments * T);
     const bool scale = PREC != KUMO_HIP_FP64 && a.rho_scale != nullptr; // uniform
-    float es0 = 0.f, es1 = 0.f;
+    float es0 = 0.f, es1 = 0.f, eg0 = 0.f, eg1 = 0.f;
+    constexpr bool kGradScale = ORDER >= 1 && (PSET & 0xfu) == 0xfu; // panels 1-3 are the gradient
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Tile: T points of one screening block, consecutive or (rescue mode)
@@ -313,14 +316,20 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                 const int qq[3] = {(info >> 6) & 7, (info >> 3) & 7, info & 7};
                 cart_component_pw<ORDER, PSET>(qq, x, y, z, R, v);
             };
-            const auto add_scale = [&](int i, double v0) __attribute__((always_inline)) {
+            const auto add_scale = [&](int i, const double (&v)[NP]) __attribute__((always_inline)) {
                 if (!scale) return;
                 const int g = cr[2 * (r0 + i)];
                 if (g < 0) return;
                 const float2 w = a.mo.norm[g];
-                const float av = fabsf(static_cast<float>(v0));
+                const float av = fabsf(static_cast<float>(v[0]));
                 es0 += av * w.x;
                 es1 += av * w.y;
+                if constexpr (kGradScale) {
+                    const float ag = fabsf(static_cast<float>(v[1])) + fabsf(static_cast<float>(v[2])) +
+                                     fabsf(static_cast<float>(v[3]));
+                    eg0 += ag * w.x;
+                    eg1 += ag * w.y;
+                }
             };
             constexpr int PLANE = NP * T * LDK;
             E* d0 = phi + j * LDK + r0;
@@ -332,8 +341,8 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                     double va[NP], vb[NP];
                     eval_row(2 * pi, va);
                     eval_row(2 * pi + 1, vb);
-                    add_scale(2 * pi, va[0]);
-                    add_scale(2 * pi + 1, vb[0]);
+                    add_scale(2 * pi, va);
+                    add_scale(2 * pi + 1, vb);
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
                         uint32_t tt[S];
@@ -351,7 +360,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                 for (int i = 0; i < RPL; ++i) {
                     double v[NP];
                     eval_row(i, v);
-                    add_scale(i, v[0]);
+                    add_scale(i, v);
 #pragma unroll
                     for (int p = 0; p < NP; ++p)
                         st[i][p] = static_cast<float>(v[p]);
@@ -505,8 +514,10 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
         return;
     }
     if (scale) {
-        esl[2 * tid] = es0;
-        esl[2 * tid + 1] = es1;
+        esl[4 * tid] = es0;
+        esl[4 * tid + 1] = es1;
+        esl[4 * tid + 2] = eg0;
+        esl[4 * tid + 3] = eg1;
     }
     // Epilogue: dot products over the MOs of each tile, FP64.
     // Raw sums q: 0 v0.v0, 1-3 v0.va, 4-6 va.va, 7-12 (v0.vac + va.vc).
@@ -587,10 +598,13 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
         double* dst = a.moments + (static_cast<long long>(blockIdx.y) * 2 + ch) * kMoments * a.batch_points;
         dst[k * a.batch_points + point] = f * s;
         if (k == 0 && scale && blockIdx.y == 0) {
-            float e = 0.f;
-            for (int q = jj; q < 256; q += T)
-                e += esl[2 * q + ch];
+            float e = 0.f, eg = 0.f;
+            for (int q = jj; q < 256; q += T) {
+                e += esl[4 * q + ch];
+                eg += esl[4 * q + 2 + ch];
+            }
             a.rho_scale[ch * a.batch_points + point] = e;
+            if (kGradScale) a.rho_scale[(2 + ch) * a.batch_points + point] = eg;
         }
     }
 }
@@ -600,7 +614,7 @@ void launch_one(const AoMoLaunch& a, hipStream_t stream) {
     using P = Prec<PREC>;
     constexpr int NP = popcount10(PSET);
     const size_t work = sizeof(typename P::E) * P::S * (static_cast<size_t>(NP) * TP * P::LDK + KC * P::PITCH);
-    const size_t red = sizeof(double) * 8 * kMoments * TP + sizeof(float) * 2 * 256;
+    const size_t red = sizeof(double) * 8 * kMoments * TP + sizeof(float) * 4 * 256;
     const size_t lds = work > red ? work : red;
     const int tiles_per_block = (a.block_size + TP - 1) / TP;
     const int groups = (a.slot1 - a.slot0 + 127) / 128;

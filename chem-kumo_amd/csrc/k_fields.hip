@@ -318,6 +318,51 @@ __device__ inline bool elf_uncertain(const DDv* c, int nc, double fermi, double 
     return !(smax * dchi <= tol);
 }
 
+// ELF test from each channel's own error scales. With the contraction's
+// error model |d psi_i| <= e a_i and |d d_a psi_i| <= e b_ai (e the model's
+// unit, calibrated by the caller; a_i = sum_k |C_ik Phi_k|, b_ai the same
+// for d_a Phi_k),
+// s >= sqrt(sum_i a_i^2) and w >= sum_a sqrt(sum_i b_ai^2) (the triangle
+// inequality over the operand rows), Cauchy-Schwarz over the MOs gives
+//   |d rho| <= 2 e sqrt(rho) s + (e s)^2,
+//   |d tau| <= e sqrt(2 tau) w + (e w)^2 / 2,
+//   |d grad rho| <= 2 e (sqrt(rho) w + sqrt(3) sqrt(2 tau) s) + 2 e^2 s w.
+// The Pauli term P = tau - |grad rho|^2 / (8 rho) and the uniform-gas term
+// U = fermi rho^(5/3) are bounded over those intervals, chi = |P| / U over
+// theirs, and, ELF = 1 / (1 + chi^2) being monotone in chi, the point is
+// uncertain when the ELF interval is wider than tol on either side.
+__device__ inline bool elf_uncertain_scaled(const DDv* c, const float* s, const float* w, int nc, double fermi,
+                                            double e, double tol) {
+    double P = 0.0, U = 0.0, dP = 0.0, dU = 0.0;
+    bool any = false;
+    for (int i = 0; i < nc; ++i) {
+        const double r = c[i].v;
+        if (!(r > 0.0)) continue;
+        any = true;
+        const double es = e * static_cast<double>(s[i]), ew = e * static_cast<double>(w[i]);
+        const double sr = sqrt(r), st = sqrt(2.0 * fmax(c[i].tau, 0.0));
+        const double drho = 2.0 * sr * es + es * es;
+        if (!(drho < r)) return true;
+        const double dtau = st * ew + 0.5 * ew * ew;
+        const double dg = 2.0 * (sr * ew + 1.7320508075688772 * st * es) + 2.0 * es * ew;
+        const double g2 = gnorm2(c[i]), gn = sqrt(g2), q = g2 / (8.0 * r);
+        const double ghi = gn + dg, glo = fmax(gn - dg, 0.0);
+        const double dq = fmax(ghi * ghi / (8.0 * (r - drho)) - q, q - glo * glo / (8.0 * (r + drho)));
+        P += c[i].tau - q;
+        dP += dtau + dq;
+        const double u = fermi * pow(r, 5.0 / 3.0);
+        U += u;
+        dU += fermi * pow(r + drho, 5.0 / 3.0) - u;
+    }
+    if (!any) return false; // ELF is 0 in every precision
+    if (!(U - dU > 0.0)) return true;
+    const double chi = fabs(P) / U;
+    const double lo = fmax(fabs(P) - dP, 0.0) / (U + dU), hi = (fabs(P) + dP) / (U - dU);
+    const double elf = 1.0 / (1.0 + chi * chi);
+    const double widest = fmax(1.0 / (1.0 + lo * lo) - elf, elf - 1.0 / (1.0 + hi * hi));
+    return !(widest <= tol);
+}
+
 // Relative density error bound of one channel, 2 eps s / sqrt(rho)
 // (0 where the channel holds no density).
 __device__ inline double rho_rel_bound(double rho, double s, double eps) {
@@ -334,9 +379,18 @@ __global__ void rescue_flag_kernel(const double* ch, long long n, int closed, Re
     c[1] = closed ? c[0] : load_channel(ch + kMoments * n, n, j);
     bool f = false;
     const int ew = rc.elf_which;
-    if (ew & 1) f = f || elf_uncertain(c, 2, fermi, rc.elf_eps, rc.elf_tol);
-    if (ew & 2) f = f || elf_uncertain(c, 1, fermi, rc.elf_eps, rc.elf_tol);
-    if (ew & 4) f = f || elf_uncertain(c + 1, 1, fermi, rc.elf_eps, rc.elf_tol);
+    if (rc.grad_scale) {
+        const float sv[2] = {rc.rho_scale[j], closed ? rc.rho_scale[j] : rc.rho_scale[n + j]};
+        const float gv[2] = {rc.grad_scale[j], closed ? rc.grad_scale[j] : rc.grad_scale[n + j]};
+        const double e = rc.elf_scale_eps;
+        if (ew & 1) f = f || elf_uncertain_scaled(c, sv, gv, 2, fermi, e, rc.elf_tol);
+        if (ew & 2) f = f || elf_uncertain_scaled(c, sv, gv, 1, fermi, e, rc.elf_tol);
+        if (ew & 4) f = f || elf_uncertain_scaled(c + 1, sv + 1, gv + 1, 1, fermi, e, rc.elf_tol);
+    } else {
+        if (ew & 1) f = f || elf_uncertain(c, 2, fermi, rc.elf_eps, rc.elf_tol);
+        if (ew & 2) f = f || elf_uncertain(c, 1, fermi, rc.elf_eps, rc.elf_tol);
+        if (ew & 4) f = f || elf_uncertain(c + 1, 1, fermi, rc.elf_eps, rc.elf_tol);
+    }
     if (rc.rho_which) {
         const double sa = rc.rho_scale[j], sb = closed ? sa : rc.rho_scale[n + j];
         const double ba = rho_rel_bound(c[0].v, sa, rc.rho_eps), bb = rho_rel_bound(c[1].v, sb, rc.rho_eps);

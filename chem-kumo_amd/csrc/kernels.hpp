@@ -107,6 +107,12 @@ struct RescueCriteria {
     int rho_which;
     const float* rho_scale; // [2][n] s_alpha, s_beta
     double rho_eps, rho_tol;
+    // When not null, the ELF test bounds each channel's errors from its own
+    // scales: rho_scale ([2][n]) and grad_scale ([2][n], the gradient
+    // scales), with rounding unit elf_scale_eps, instead of elf_eps (tau +
+    // |grad rho|^2 / 8 rho).
+    const float* grad_scale;
+    double elf_scale_eps;
 };
 // Flags the points to recompute and groups them into tiles of `tile` points
 // of one screening block.
@@ -147,7 +153,9 @@ struct AoMoLaunch {
     int tile_points;         // points per workgroup: 16 or 32 (0: 32; FP64 always 16)
     double* moments;         // [2][kMoments][batch_points]
     // Reduced precisions, when not null: [2][batch_points] sum_k |Phi_k| ||C_k,c||
-    // over the kept operand rows for channel c (slot group 0 only).
+    // over the kept operand rows for channel c (slot group 0 only), then, when
+    // the panels include the gradient, [2][batch_points]
+    // sum_k (|d/dx Phi_k| + |d/dy Phi_k| + |d/dz Phi_k|) ||C_k,c||.
     float* rho_scale;
     // Weighted mode (FP64, order 0), when not null: [kOeSums][mo.slots]
     // weight of each MO slot in each orbital sum; the moments array then
