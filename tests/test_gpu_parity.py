@@ -213,6 +213,25 @@ def test_reduced_precision_errors(ctx, precision, rho_tol):
     assert elf_all < 1e-5
 
 
+def test_reduced_precision_spin_elf(ctx):
+    """BF16X6 ELF of each spin channel and of the total on the UHF O2 molecule:
+    the rescue tests each channel's own error scales (the alpha and beta
+    intervals, and both together for the total), so every point stays within
+    1e-5 of the oracle, and the channel density within 1e-6 where rho >= 1e-6."""
+    d = load_sample("o2_density")
+    grid = {"origin": d["origin"] + 1.0, "steps": np.eye(3) * 0.2, "counts": np.array([36, 34, 40])}
+    props = [(K.Elf, DC.Alpha, 0, 0), (K.Elf, DC.Beta, 0, 0), (K.Elf, 0, 0, 0), (K.Density, DC.Alpha, 0, 0)]
+    g, _ = gmol(ctx, "o2").evaluate(props, grid=grid, precision=Precision.BF16X6)
+    o, _ = omol("o2").evaluate(props, grid=grid)
+    errs = [float(np.max(np.abs(g[i][0] - o[i][0]))) for i in range(3)]
+    rho = o[3][0]
+    sig = rho >= 1e-6
+    rel = float(np.max(np.abs(g[3][0] - rho)[sig] / rho[sig]))
+    print(f"[gpu] O2 BF16X6 ELF alpha/beta/total {errs}, alpha density rel {rel:.3g}", flush=True)
+    assert max(errs) < 1e-5
+    assert rel < 1e-6
+
+
 def test_esp_matches_oracle(ctx):
     rng = np.random.default_rng(4)
     pts = rng.normal(size=(400, 3)) * 2.5
