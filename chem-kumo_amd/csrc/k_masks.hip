@@ -212,8 +212,8 @@ __device__ __attribute__((always_inline)) inline int block_scan(int v, int* tmp,
 }
 
 // At least 3 waves per SIMD: the Hessian-order kernel then fits 168 VGPRs
-// (a few spills in its rarely taken exact branch) instead of 217 at 2 waves,
-// and runs 19 % faster on a 640-AO molecule; the lower orders fit anyway.
+// (43 spilled to scratch) instead of 217 at 2 waves, and runs 19 % faster on
+// a 640-AO molecule's 160^3 cube; the lower orders fit in 3 waves anyway.
 template <int ORDER>
 __global__ void __launch_bounds__(kMaskThreads) __attribute__((amdgpu_waves_per_eu(3)))
     mask_kernel(MolView m, PointSource ps, long long npoints, int block_size, long long first_block, double threshold,
