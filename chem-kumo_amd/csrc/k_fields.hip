@@ -433,35 +433,36 @@ __global__ void __launch_bounds__(256) rescue_compact_kernel(const uint8_t* flag
 
 // Tiles of `tile` flagged points: tile t covers points [16 sub, ...) of the
 // flagged list of block tile_blk[t]; total receives the number of tiles.
-__global__ void __launch_bounds__(256) rescue_tiles_kernel(const int32_t* count, int nblk, int tile, int32_t* tile_blk,
-                                                           int32_t* tile_sub, int32_t* total) {
-    __shared__ int wsum[4];
-    __shared__ int base;
+__global__ void __launch_bounds__(1024) rescue_tiles_kernel(const int32_t* count, int nblk, int tile,
+                                                            int32_t* tile_blk, int32_t* tile_sub, int32_t* total) {
+    // Each thread takes a run of consecutive blocks; one scan over the
+    // workgroup of the runs' tile counts gives every run its first tile.
+    __shared__ int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (tid == 0) base = 0;
+    const int per = (nblk + 1023) / 1024;
+    const int b0 = min(nblk, tid * per), b1 = min(nblk, b0 + per);
+    int local = 0;
+    for (int b = b0; b < b1; ++b)
+        local += (count[b] + tile - 1) / tile;
+    int x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
     __syncthreads();
-    for (int b0 = 0; b0 < nblk; b0 += 256) {
-        const int b = b0 + tid;
-        const int nt = b < nblk ? (count[b] + tile - 1) / tile : 0;
-        int x = nt;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        int off = base + x - nt;
-        for (int k = 0; k < w; ++k)
-            off += wsum[k];
+    int off = x - local;
+    for (int k = 0; k < w; ++k)
+        off += wsum[k];
+    for (int b = b0; b < b1; ++b) {
+        const int nt = (count[b] + tile - 1) / tile;
         for (int s2 = 0; s2 < nt; ++s2) {
             tile_blk[off + s2] = b;
             tile_sub[off + s2] = s2;
         }
-        __syncthreads();
-        if (tid == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        __syncthreads();
+        off += nt;
     }
-    if (tid == 0) *total = base;
+    if (tid == 1023) *total = off; // the last run ends at the total
 }
 
 // Writes the rescued channel moments (summed over MO slot groups) over the
@@ -497,7 +498,7 @@ void launch_rescue_plan(const double* channel_moments, long long n, int closed, 
     KUMO_HIP(hipGetLastError());
     rescue_compact_kernel<<<static_cast<unsigned>(nblk), 256, 0, stream>>>(r.flag, n, block_size, r.offs, r.count);
     KUMO_HIP(hipGetLastError());
-    rescue_tiles_kernel<<<1, 256, 0, stream>>>(r.count, nblk, tile, r.tile_blk, r.tile_sub, r.total);
+    rescue_tiles_kernel<<<1, 1024, 0, stream>>>(r.count, nblk, tile, r.tile_blk, r.tile_sub, r.total);
     KUMO_HIP(hipGetLastError());
 }
 
