@@ -247,11 +247,13 @@ constexpr double kRhoRescueTol = 5e-7; // largest relative density error left to
 // triangle inequality over the operand rows) overestimates the contraction's
 // rounding error, which sums with random signs, by about two orders of
 // magnitude. Over the 4.1e6 points of a 640-AO UHF molecule on a 160^3 cube
-// (ELF rescue alone) e = 3e-9 flagged 9.2 % of the points and left no ELF
-// error above 7.8e-7 (e = 1e-9: 7.2 %, 1.5e-6), where the model over
-// tau + |grad rho|^2 / 8 rho flagged 13.5 % and left 1.04e-6. Other
-// precisions use that model (rescue_eps).
-double elf_scale_eps(int precision) { return precision == KUMO_HIP_BF16X6 ? 3e-9 : 0.0; }
+// (ELF rescue alone) e = 1e-9 flagged 7.2 % of the points and left no ELF
+// error above 1.5e-6, half the 3e-6 the contraction may leave (e = 3e-9:
+// 9.2 %, 7.8e-7), where the model over tau + |grad rho|^2 / 8 rho flagged
+// 13.5 % and left 1.04e-6; with the density rescue the largest ELF error was
+// 6.8e-7 on that cube and 1.1e-6 over a 256-molecule batch. Other precisions
+// use that model (rescue_eps).
+double elf_scale_eps(int precision) { return precision == KUMO_HIP_BF16X6 ? 1e-9 : 0.0; }
 
 // Autotuner of the AO->MO tile. The points per workgroup (16 or 32) change
 // only how points are shared among workgroups (every point's arithmetic and
