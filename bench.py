@@ -468,7 +468,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
-              "screening blocks across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "screening blocks across GPUs, each writing its slice of the field, with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
     if args.config == 2:
         args.molecules = args.molecules or 256
