@@ -52,17 +52,24 @@ __device__ __attribute__((always_inline)) inline void boys(const double* __restr
 // point nearest to T and a 7-term Taylor series below T = 40, the closed form
 // and upward recursion above), with every division by a constant, and the
 // repeated division by 2T, replaced by a product with the reciprocal and the
-// series summed by Horner's rule. The values agree with boys() to a few ulp;
-// the loop then needs one FP64 division (at T >= 40) instead of up to 7 + L.
+// series summed by Horner's rule, and above T = 40 the reciprocal square
+// root refined by Newton steps. The values agree with boys() to a few ulp;
+// the loop then needs no FP64 division or square root (boys(): up to 7 + L
+// divisions and a square root).
 template <int TOP>
 __device__ __attribute__((always_inline)) inline void boys_esp(const double* __restrict__ rows, double T,
                                                                double (&v)[TOP + 1]) {
     constexpr double kInvFactorial[7] = {1.0, 1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0};
     if (T >= 40.0) {
-        const double inv = 1.0 / T;
-        v[0] = 0.5 * sqrt(3.141592653589793 * inv);
+        // 1/sqrt(T) from the hardware estimate and two Newton steps (each
+        // squares the relative error), then F_0 = (sqrt(pi) / 2) / sqrt(T)
+        // and 1 / (2T) without a division or a square root.
+        double y = __builtin_amdgcn_rsq(T);
+        y = y * (1.5 - 0.5 * T * y * y);
+        y = y * (1.5 - 0.5 * T * y * y);
+        v[0] = 0.88622692545275801 * y;
         if constexpr (TOP > 0) {
-            const double decay = exp(-T), half = 0.5 * inv;
+            const double decay = exp(-T), half = 0.5 * (y * y);
 #pragma unroll
             for (int m = 0; m < TOP; ++m)
                 v[m + 1] = ((2.0 * m + 1.0) * v[m] - decay) * half;
