@@ -170,6 +170,7 @@ __global__ void __launch_bounds__(kEspThreads)
     const int b = first_pair + static_cast<int>(blockIdx.y) * per_slice;
     const int e = min(last_pair, b + per_slice);
     double s = 0.0;
+#pragma unroll(L <= 3 ? 2 : 1) // two pairs in flight where registers allow
     for (int i = b; i < e; ++i)
         s += pair_sum<L>(pairs[i], coef, table, x, y, z);
     partial[blockIdx.y * n + j] = s;
