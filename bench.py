@@ -1,23 +1,29 @@
 """Benchmark of the HIP FCHK->grid evaluator on one MI355X.
 
-Workload (one step): a 40-atom molecule (16 C/N/O + 24 H, >= 1.8 bohr apart),
-def2-TZVP-shaped pure shells (640 AOs), unrestricted 60 alpha / 59 beta
-occupied MOs, evaluated on a 160^3 cube at 0.15 bohr (4 096 000 points):
-total rho, grad rho, lap rho, tau, ELF and LOL, AUTO backend (AO-MO),
-screening 1e-12, block 128. Molecule tables and MO coefficients are resident
-in HBM and the outputs are written to HBM, so the timed region is the device
-work of one evaluate() call per step.
+Headline workload (one step): 256 synthetic molecules of 3-60 atoms (40 %
+C/N/O, the rest H, >= 1.8 bohr apart), 6-31G*- or def2-TZVP-shaped bases
+chosen 50/50, each on its own cube of edge 40-120 points at 0.2 bohr
+(about 1.66e8 points in all): total rho, grad rho, lap rho, tau and ELF,
+AUTO backend (AO-MO), screening 1e-12, blocks of 128 points. Every item is
+issued into one HIP graph (recorded once, over 4 streams, tile shapes picked
+per molecule by the autotuner during the warm-up) and a step is one replay
+of that graph. Molecule tables, MO coefficients and outputs are resident in
+HBM, so the timed region is device work only.
 
-Printed: one JSON line with points/s of the mixed-precision path (default
-split-bf16 x6 with FP32 accumulation), the FP64 GPU path on the same
-workload, the error of the mixed path against the FP64 oracle on a sample of
-screening blocks, the roofline of the dominant contraction kernel (HIP events
-on the library's stream) and the oracle (FP64 CPU restatement, OpenMP) timed
-on a bounded sample of the same blocks.
+Printed: one JSON line with points/s of the mixed-precision path (split-bf16
+x6 MFMA with FP32 accumulation, FP64 rescue of uncertain points), the FP64
+GPU path on the same workload, the error of every property against the FP64
+oracle on sampled screening blocks of every molecule and against the FP64
+GPU path on every point, the roofline of the dominant contraction kernel
+(HIP events on the streams it runs on), the oracle timed on a bounded sample
+on the host cores, and two sub-records: the single 640-AO molecule on a
+160^3 cube (the per-precision comparison) and a serving load (p50/p99
+latency). --config 1, 3 or 4 runs one of those workloads on its own.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -30,12 +36,109 @@ from kumo_hip.api import Precision, PropertyKind as K, channels  # noqa: E402
 
 PROPS = [(K.Density, 0, 0, 0), (K.Gradient, 0, 0, 0), (K.Laplacian, 0, 0, 0), (K.Tau, 0, 0, 0), (K.Elf, 0, 0, 0),
          (K.Lol, 0, 0, 0)]
+PROPS_BATCH = [(K.Density, 0, 0, 0), (K.Gradient, 0, 0, 0), (K.Laplacian, 0, 0, 0), (K.Tau, 0, 0, 0),
+               (K.Elf, 0, 0, 0)]
+BATCH_CHANNELS = ["rho", "grad_x", "grad_y", "grad_z", "lap", "tau", "elf"]
 PEAK = {"FP64": 78.6e12, "FP32": 157.3e12, "BF16X3": 2.5e15, "BF16X6": 2.5e15}
 PRODUCTS = {"FP64": 1, "FP32": 1, "BF16X3": 3, "BF16X6": 6}
+METRIC = "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error; p50/p99 latency"
+# ELF is compared where rho >= this floor: below it the Pauli term
+# tau - |grad rho|^2 / 8 rho is a rounding of tau even in FP64 (one AO
+# dominates every MO of a channel, so it vanishes in exact arithmetic) and two
+# FP64 formulations of the same algorithm disagree (by up to ~1 below
+# rho ~ 1e-20, by less than 1e-6 above 1e-14).
+ELF_FLOOR = 1e-14
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """CPU model and core counts of the host (lscpu), and the threads the
+    oracle uses."""
+    info = {"model": None, "sockets": None, "cores_per_socket": None, "threads_per_core": None,
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        f = {k.strip(): v.strip() for k, v in f.items()}
+        info.update(model=f.get("Model name"), sockets=f.get("Socket(s)"),
+                    cores_per_socket=f.get("Core(s) per socket"), threads_per_core=f.get("Thread(s) per core"))
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
+    return info
+
+
+def _config2_molecule(arg):
+    """One molecule of the batched workload (worker process: one BLAS thread)."""
+    i, (na, fam, edge) = arg
+    from threadpoolctl import threadpool_limits
+    from kumo_hip import synthetic
+    with threadpool_limits(1):
+        seed = 1000 + i
+        while True:  # redraw the rare exponent sets that are numerically linearly dependent
+            try:
+                return synthetic.batch_molecule(seed, na, fam, edge)
+            except ValueError:
+                seed += 100000
+
+
+def _config4_molecule(arg):
+    i, na = arg
+    from threadpoolctl import threadpool_limits
+    from kumo_hip import synthetic
+    with threadpool_limits(1):
+        seed = 2000 + i
+        while True:
+            try:
+                return synthetic.batch_molecule(seed, na, "tzvp", 40)[0]
+            except ValueError:
+                seed += 100000
+
+
+def parallel_map(fn, items):
+    """Builds synthetic molecules in worker processes (at most 16, the host
+    share of one GPU); called before any GPU work in this process."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing
+    workers = max(1, min(16, len(items), (os.cpu_count() or 4)))
+    with ProcessPoolExecutor(max_workers=workers, mp_context=multiprocessing.get_context("fork")) as ex:
+        return list(ex.map(fn, items, chunksize=1))
+
+
+def config2_specs(n):
+    rng = np.random.default_rng(2)
+    return [(int(rng.integers(3, 61)), "pople" if rng.random() < 0.5 else "tzvp", int(rng.integers(40, 121)))
+            for _ in range(n)]
+
+
+def errors_vs_oracle(gets, refs, floor=ELF_FLOOR):
+    """Max errors of the BATCH_CHANNELS outputs against the oracle over the
+    sampled points of every item: rho relative where rho >= 1e-6, gradient,
+    Laplacian and tau relative to the channel's largest |value| in the item,
+    ELF absolute where rho >= floor (and >= 1e-4)."""
+    e = {"rho_max_rel": 0.0, "grad_max_rel_to_max": 0.0, "lap_max_rel_to_max": 0.0, "tau_max_rel_to_max": 0.0,
+         "elf_max_abs": 0.0, "elf_max_abs_rho_ge_1e-4": 0.0, "points": 0, "points_below_elf_floor": 0}
+    for got, ref in zip(gets, refs):
+        rho = ref[0]
+        sig = rho >= 1e-6
+        if sig.any():
+            e["rho_max_rel"] = max(e["rho_max_rel"], float(np.max(np.abs(got[0] - rho)[sig] / rho[sig])))
+        for key, chans in (("grad_max_rel_to_max", (1, 2, 3)), ("lap_max_rel_to_max", (4,)),
+                           ("tau_max_rel_to_max", (5,))):
+            for c in chans:
+                scale = np.max(np.abs(ref[c]))
+                if scale > 0:
+                    e[key] = max(e[key], float(np.max(np.abs(got[c] - ref[c])) / scale))
+        d = np.abs(got[6] - ref[6])
+        if (rho >= floor).any():
+            e["elf_max_abs"] = max(e["elf_max_abs"], float(d[rho >= floor].max()))
+        if (rho >= 1e-4).any():
+            e["elf_max_abs_rho_ge_1e-4"] = max(e["elf_max_abs_rho_ge_1e-4"], float(d[rho >= 1e-4].max()))
+        e["points"] += int(rho.size)
+        e["points_below_elf_floor"] += int((rho < floor).sum())
+    return e
 
 
 def run_gpu(mol, grid, precision, steps, warmup, torch):
@@ -218,92 +321,153 @@ def full_grid_errors(got, ref, torch):
     return out
 
 
-PROPS_BATCH = [(K.Density, 0, 0, 0), (K.Gradient, 0, 0, 0), (K.Laplacian, 0, 0, 0), (K.Tau, 0, 0, 0),
-               (K.Elf, 0, 0, 0)]
+def roofline_record(st, precision, traffic=None):
+    """Roofline of the dominant contraction kernel from a timed run's launch
+    statistics: algorithmic FLOPs per launch / average launch duration."""
+    kernel_s = st["kernel_ms"] * 1e-3
+    achieved = st["flops"] / kernel_s if kernel_s > 0 else 0.0
+    n = max(1, st["launches"])
+    return {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK[precision], "traffic": traffic, "kernel": "aomo_kernel",
+            "launches": st["launches"], "avg_launch_ms": st["kernel_ms"] / n,
+            "algorithmic_flops_per_launch": st["flops"] / n, "mfma_products_per_flop": PRODUCTS[precision],
+            "flops": "2 x kept AOs x occupied MOs x panels x points, summed over screening blocks"}
 
 
-def bench_batch(args):
-    """256 synthetic molecules (3-60 atoms, 6-31G*- or def2-TZVP-shaped, each
-    on its own cube of edge 40-120 at 0.2 bohr), rho, grad, lap, tau, ELF,
-    evaluated as one batch over 4 streams and replayed from a HIP graph."""
-    import torch
-    from kumo_hip import synthetic
+def load_traffic(name):
+    """HBM bytes per launch of the dominant kernel, from the FETCH_SIZE and
+    WRITE_SIZE counter runs summarised under profiles/ (null if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("traffic_bytes_per_launch")
+
+
+def bench_config2(args, built, torch):
+    """The headline: the batched 256-molecule workload, one graph replay per
+    step, mixed precision beside the FP64 GPU path."""
     from kumo_hip.binding import Batch, Context, Molecule
-    rng = np.random.default_rng(2)
-    specs = [(int(rng.integers(3, 61)), "pople" if rng.random() < 0.5 else "tzvp", int(rng.integers(40, 121)))
-             for _ in range(args.molecules)]
+    smols = [b[0] for b in built]
+    grids = [b[1] for b in built]
     ctx = Context(0)
-    t = time.perf_counter()
-    smols, grids = [], []
-    for i, (na, fam, edge) in enumerate(specs):
-        seed = 1000 + i
-        while True:  # redraw the rare exponent sets that are numerically linearly dependent
-            try:
-                sm, grid = synthetic.batch_molecule(seed, na, fam, edge)
-                break
-            except ValueError:
-                seed += 100000
-        smols.append(sm)
-        grids.append(grid)
     mols = [Molecule(ctx, data=sm) for sm in smols]
     counts = [int(np.prod(g["counts"].astype(np.int64))) for g in grids]
     total = sum(counts)
-    log(f"{len(mols)} molecules ({sum(m.info['nbasis'] for m in mols)} AOs in all), {total} points, "
-        f"ready in {time.perf_counter() - t:.1f}s")
+    nao = sum(m.info["nbasis"] for m in mols)
+    log(f"config 2: {len(mols)} molecules ({nao} AOs in all), {total} points, streams {args.streams}")
     nch = sum(channels(p[0]) for p in PROPS_BATCH)
-    res = {}
-    outs = {}
+    res, outs, stats = {}, {}, {}
     for prec in (args.precision, "FP64"):
+        main = prec == args.precision
         o = [torch.empty((nch, n), dtype=torch.float64, device="cuda") for n in counts]
         ptrs = [[t_[c].data_ptr() for c in range(nch)] for t_ in o]
-        b = Batch(ctx, mols, grids, PROPS_BATCH, ptrs, streams=4, precision=int(Precision[prec]))
-        for _ in range(max(1, args.warmup)):
+        b = Batch(ctx, mols, grids, PROPS_BATCH, ptrs, streams=args.streams, precision=int(Precision[prec]))
+        t = time.perf_counter()
+        for _ in range(max(1, args.warmup if main else 1)):
             b.run(graph=True)
-        ms = [b.run(graph=True) for _ in range(args.steps)]
+        log(f"{prec}: warm-up (workspaces, autotuned tiles, graph) {time.perf_counter() - t:.1f}s")
+        steps = args.steps if main else max(1, min(args.steps, args.fp64_steps))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run(graph=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
         eager = b.run(graph=False)
-        res[prec] = {"value": total / (min(ms) * 1e-3), "ms_per_step": min(ms), "ms_median": float(np.median(ms)),
+        st = b.timed_run()
+        res[prec] = {"value": total * steps / dt, "ms_per_step": 1e3 * dt / steps, "steps": steps,
                      "ms_without_graph": eager}
-        log(f"{prec}: {res[prec]['value']:.4g} points/s, {min(ms):.1f} ms/step (graph), {eager:.1f} ms without")
+        stats[prec] = st
+        log(f"{prec}: {res[prec]['value']:.4g} points/s, {res[prec]['ms_per_step']:.1f} ms/step (graph), "
+            f"{eager:.1f} ms without; contraction {st['kernel_ms']:.1f} ms in {st['launches']} launches, "
+            f"rescue {st['rescue_ms']:.1f} ms, fields {st['fields_ms']:.1f} ms")
         outs[prec] = o
         b.close()
-    rel, elf = 0.0, 0.0
+    # every point against the FP64 GPU path
+    vs64 = {"rho_max_rel_rho_ge_1e-6": 0.0, "elf_max_abs": 0.0, "elf_max_abs_all_points": 0.0,
+            "grad_max_rel_to_max": 0.0, "lap_max_rel_to_max": 0.0, "tau_max_rel_to_max": 0.0}
     for a, r in zip(outs[args.precision], outs["FP64"]):
         rho = r[0]
         sel = rho >= 1e-6
         if bool(sel.any()):
-            rel = max(rel, float(((a[0] - rho).abs()[sel] / rho[sel]).max()))
-        elf = max(elf, float((a[6] - r[6]).abs().max()))
-    cpu = None
+            vs64["rho_max_rel_rho_ge_1e-6"] = max(vs64["rho_max_rel_rho_ge_1e-6"],
+                                                  float(((a[0] - rho).abs()[sel] / rho[sel]).max()))
+        d = (a[6] - r[6]).abs()
+        vs64["elf_max_abs_all_points"] = max(vs64["elf_max_abs_all_points"], float(d.max()))
+        fl = rho >= ELF_FLOOR
+        if bool(fl.any()):
+            vs64["elf_max_abs"] = max(vs64["elf_max_abs"], float(d[fl].max()))
+        for key, chans in (("grad_max_rel_to_max", (1, 2, 3)), ("lap_max_rel_to_max", (4,)),
+                           ("tau_max_rel_to_max", (5,))):
+            for c in chans:
+                m = float(r[c].abs().max())
+                if m > 0:
+                    vs64[key] = max(vs64[key], float((a[c] - r[c]).abs().max()) / m)
+    vs64["vs"] = f"FP64 GPU path, every point; ELF where rho >= {ELF_FLOOR:g} (and at all points)"
+    # sampled screening blocks of every molecule against the oracle
+    errors, cpu = None, None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from support import oracle
-        pts_done, secs = 0, 0.0
-        for i in range(0, len(mols), max(1, len(mols) // 16)):
-            om = oracle.Molecule(smols[i])
-            pts = oracle.grid_points(grids[i])
-            sel = np.arange(0, len(pts) // 128, max(1, len(pts) // 128 // 32))[:32]
-            idx = (sel[:, None] * 128 + np.arange(128)[None, :]).ravel()
-            t0 = time.perf_counter()
-            om.evaluate(PROPS_BATCH, points=pts[idx], block_size=128)
-            secs += time.perf_counter() - t0
-            pts_done += len(idx)
-        cpu = {"value": pts_done / secs, "unit": "grid points/s", "cores": int(oracle.lib().orc_threads()),
-               "kind": "port", "sample": f"32 screening blocks from each of 16 of the molecules ({pts_done} points)"}
-        log(f"CPU oracle: {cpu['value']:.4g} points/s")
-    out = {"metric": "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error (batched)",
-           "value": res[args.precision]["value"], "unit": "grid points/s", "n_gpus": 1,
-           "ms_per_step": res[args.precision]["ms_per_step"], "higher_is_better": True,
-           "dtype": args.precision.lower(), "data": "synthetic (seeded molecules built in memory)",
-           "config": {"workload": f"{len(mols)} molecules of 3-60 atoms (6-31G*/def2-TZVP shapes 50/50), cubes of "
-                                  "edge 40-120 at 0.2 bohr, rho/grad/lap/tau/ELF, one graph replay per step",
-                      "points": total, "streams": 4},
-           "fp64_gpu": res["FP64"], "speedup_vs_fp64_gpu": res[args.precision]["value"] / res["FP64"]["value"],
-           "errors_vs_fp64_gpu": {"rho_max_rel_rho_ge_1e-6": rel, "elf_max_abs": elf},
-           "cpu_baseline": cpu}
-    print(json.dumps(out))
+        rng = np.random.default_rng(3)
+        gets, refs, secs, npts = [], [], 0.0, 0
+        per = args.oracle_blocks
+        for i, (sm, grid) in enumerate(zip(smols, grids)):
+            nb = (counts[i] + 127) // 128
+            centre = (counts[i] // 2) // 128
+            blocks = np.unique(np.concatenate([[centre], rng.integers(0, nb, per - 1)]))
+            idx = (blocks[:, None] * 128 + np.arange(128)[None, :]).ravel()
+            idx = idx[idx < counts[i]]
+            om = oracle.Molecule(sm)
+            pts = oracle.grid_points(grid)[idx]
+            t = time.perf_counter()
+            o, _ = om.evaluate(PROPS_BATCH, points=pts, block_size=128)
+            secs += time.perf_counter() - t
+            npts += len(idx)
+            refs.append(np.concatenate([np.atleast_2d(x) for x in o]))
+            gets.append(outs[args.precision][i][:, torch.as_tensor(idx, device="cuda")].cpu().numpy())
+        errors = errors_vs_oracle(gets, refs)
+        errors["vs"] = (f"FP64 oracle at {per} screening blocks of 128 points of each of the {len(mols)} molecules "
+                        f"(the centre block and random ones); ELF where rho >= {ELF_FLOOR:g}")
+        threads = int(oracle.lib().orc_threads())
+        cpu = {"value": npts / secs, "unit": "grid points/s", "cores": threads, "kind": "port",
+               "sample": f"{npts} points: {per} screening blocks of 128 points of each of the {len(mols)} molecules, "
+                         "same properties, FP64, OpenMP over blocks", "host": host_cpu()}
+        log(f"oracle: {cpu['value']:.4g} points/s on {threads} threads; errors {errors}")
+    st = stats[args.precision]
+    traffic = load_traffic("traffic_aomo_bf16x6_config2.json") if args.precision == "BF16X6" else None
+    out = {
+        "metric": METRIC, "value": res[args.precision]["value"], "unit": "grid points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": res[args.precision]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16" if args.precision.startswith("BF16") else args.precision.lower(),
+        "data": "synthetic (seeded molecules built in memory; random-exponent bases, S-orthonormal random MOs)",
+        "config": {"workload": f"{len(mols)} molecules of 3-60 atoms (6-31G*/def2-TZVP shapes 50/50), cubes of edge "
+                               "40-120 at 0.2 bohr; rho, grad rho, lap rho, tau, ELF; one HIP-graph replay per step",
+                   "molecules": len(mols), "points": total, "aos": nao, "streams": args.streams,
+                   "precision": args.precision, "block_size": 128, "backend": "AUTO(AO-MO)",
+                   "parallelism": "single GPU"},
+        "fp64_gpu": res["FP64"],
+        "speedup_vs_fp64_gpu": res[args.precision]["value"] / res["FP64"]["value"],
+        "errors": errors,
+        "errors_vs_fp64_gpu": vs64,
+        "rescue": {"points": st["rescue_points"], "fraction": st["rescue_points"] / total,
+                   "fp64_kernel_ms": st["rescue_ms"], "criteria": "density or ELF uncertain at the precision"},
+        "roofline": roofline_record(st, args.precision, traffic),
+        "roofline_fp64_path": roofline_record(stats["FP64"], "FP64"),
+        "field_reductions": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "kernel_ms": st["fields_ms"],
+                             "bytes": st["fields_bytes"],
+                             "achieved": st["fields_bytes"] / (st["fields_ms"] * 1e-3) / 1e9 if st["fields_ms"] else None},
+        "timed_run": {"ms": st["ms"], "note": "the launch statistics come from one run without the graph, every "
+                                              "contraction launch bracketed by HIP events on its stream"},
+        "cpu_baseline": cpu,
+    }
     for m in mols:
         m.close()
+    del outs
     ctx.close()
+    return out
 
 
 def bench_esp(args):
@@ -345,29 +509,86 @@ def bench_esp(args):
     ctx.close()
 
 
-def bench_serving(args):
-    """Serving load: resident synthetic molecules of 10-150 atoms
-    (def2-TZVP-shaped), Poisson arrivals of point batches (40 %), lines of 512
-    points (30 %), 256 x 256 planes (20 %) and cubes of edge 64-128 (10 %),
-    properties drawn from rho, lap rho, ELF, RDG + sign(lambda2) rho and an MO
-    amplitude; p50 / p99 latency per request kind and aggregate points/s."""
-    from kumo_hip import synthetic
+def bench_config1(args, c1, torch, sub=True):
+    """The 640-AO UHF molecule on a 160^3 cube: rho, grad, lap, tau, ELF and
+    LOL in the mixed precision and in FP64, errors against the oracle on
+    sampled blocks and against the FP64 GPU path on every point."""
+    from kumo_hip.binding import Context, Molecule
+    path, smol, grid = c1
+    ctx = Context(0)
+    mol = Molecule(ctx, fchk_path=path)
+    assert mol.info["ao_mo_eligible"] == 1, "synthetic MOs must reproduce P"
+    n = int(np.prod(grid["counts"].astype(np.int64)))
+    steps = min(args.steps, 10) if sub else args.steps
+    warmup = min(args.warmup, 2) if sub else args.warmup
+    main_run = run_gpu(mol, grid, int(Precision[args.precision]), steps, warmup, torch)
+    value = main_run["points"] / main_run["seconds"]
+    log(f"config 1 {args.precision}: {value:.4g} points/s, {1e3 * main_run['seconds'] / steps:.1f} ms/step")
+    r = run_gpu(mol, grid, int(Precision.FP64), 1, 1, torch)
+    fp64 = {"value": r["points"] / r["seconds"], "ms_per_step": 1e3 * r["seconds"], "kernel_ms": r["kernel_ms"],
+            "tflops": r["flops"] / (r["kernel_ms"] * 1e-3) / 1e12 if r["kernel_ms"] > 0 else None}
+    log(f"config 1 FP64: {fp64['value']:.4g} points/s")
+    ref_out = r["out"]
+    del r
+    full = full_grid_errors(main_run["out"], ref_out, torch)
+    errors, cpu = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from support import oracle
+        from support.fchk import FchkData
+        om = oracle.Molecule(FchkData(path))
+        nblocks = n // 128
+        want = 256 if sub else (args.cpu_blocks or nblocks)
+        blocks = np.arange(0, nblocks, max(1, nblocks // want))[:want]
+        idx = (blocks[:, None] * 128 + np.arange(128)[None, :]).ravel()
+        pts = oracle.grid_points(grid)[idx]
+        t = time.perf_counter()
+        o, _ = om.evaluate(PROPS, points=pts, block_size=128)
+        dt = time.perf_counter() - t
+        cpu = {"value": len(pts) / dt, "unit": "grid points/s", "cores": int(oracle.lib().orc_threads()),
+               "kind": "port", "sample": f"{len(blocks)} screening blocks of 128 points spread over the 160^3 grid"}
+        got = main_run["out"][:, torch.as_tensor(idx, device="cuda")].cpu().numpy()
+        g64 = ref_out[:, torch.as_tensor(idx, device="cuda")].cpu().numpy()
+        ref = np.concatenate([np.atleast_2d(x) for x in o])  # rho, gx, gy, gz, lap, tau, elf, lol
+        errors = errors_vs_oracle([got[:7]], [ref[:7]])
+        errors["lol_max_abs"] = float(np.max(np.abs(got[7] - ref[7])))
+        errors["fp64_gpu"] = errors_vs_oracle([g64[:7]], [ref[:7]])
+        rho = ref[0]
+        off = np.abs(g64[6] - ref[6]) > 1e-5
+        errors["fp64_gpu"]["elf_points_off_by_1e-5_all_rho"] = int(off.sum())
+        errors["fp64_gpu"]["elf_off_max_rho"] = float(rho[off].max()) if off.any() else None
+        errors["vs"] = f"FP64 oracle at {len(idx)} points ({len(blocks)} blocks); ELF where rho >= {ELF_FLOOR:g}"
+    kernel_s = main_run["kernel_ms"] * 1e-3
+    st = {"kernel_ms": main_run["kernel_ms"], "launches": main_run["launches"], "flops": main_run["flops"]}
+    traffic = load_traffic("traffic_aomo_bf16x6_tzvp640.json") if args.precision == "BF16X6" else None
+    res = {"workload": "40-atom def2-TZVP-shaped UHF (640 AOs, 60a/59b) on a 160^3 cube at 0.15 bohr: rho, grad, "
+                       "lap, tau, ELF, LOL", "points": n, "precision": args.precision, "value": value,
+           "unit": "grid points/s", "ms_per_step": 1e3 * main_run["seconds"] / steps, "steps": steps,
+           "fp64_gpu": fp64, "speedup_vs_fp64_gpu": value / fp64["value"], "errors": errors,
+           "errors_full_grid": dict(full, vs="FP64 GPU path, all points"),
+           "rescue": {"points": main_run["rescue_points"], "fraction": main_run["rescue_points"] / n,
+                      "fp64_kernel_ms": main_run["rescue_ms"]},
+           "roofline": roofline_record(st, args.precision, traffic) if kernel_s > 0 else None,
+           "cpu_baseline": cpu}
+    if sub:
+        del main_run, ref_out
+        mol.close()
+        ctx.close()
+        return res
+    return res, ctx, mol, main_run
+
+
+def bench_config4(args, smols, rate, seconds):
+    """Serving load over resident molecules: Poisson arrivals of point
+    batches (40 %), lines of 512 points (30 %), 256 x 256 planes (20 %) and
+    cubes of edge 64-128 (10 %); properties drawn from rho, lap rho, ELF,
+    RDG + sign(lambda2) rho and an MO amplitude; p50 / p99 latency per kind
+    and aggregate points/s."""
     from kumo_hip.binding import Context, Molecule, Server
     rng = np.random.default_rng(4)
     ctx = Context(0)
-    t = time.perf_counter()
-    smols = []
-    for i in range(args.molecules):
-        na, seed = int(rng.integers(10, 151)), 2000 + i
-        while True:  # redraw the rare exponent sets that are numerically linearly dependent
-            try:
-                smols.append(synthetic.batch_molecule(seed, na, "tzvp", 40)[0])
-                break
-            except ValueError:
-                seed += 100000
     mols = [Molecule(ctx, data=sm) for sm in smols]
-    log(f"{len(mols)} resident molecules ({sum(m.info['nbasis'] for m in mols)} AOs in all) "
-        f"ready in {time.perf_counter() - t:.1f}s")
+    log(f"config 4: {len(mols)} resident molecules ({sum(m.info['nbasis'] for m in mols)} AOs in all)")
     props_sets = [[(K.Density, 0, 0, 0)], [(K.Laplacian, 0, 0, 0)], [(K.Elf, 0, 0, 0)],
                   [(K.ReducedGradient, 0, 0, 0), (K.SignLambda2Density, 0, 0, 0)], None]
     srv = Server(ctx, workers=args.workers, chunk_points=args.chunk)
@@ -376,9 +597,9 @@ def bench_serving(args):
         srv.submit(m, [(K.Elf, 0, 0, 0), (K.SignLambda2Density, 0, 0, 0)], points=sm.xyz,
                    precision=int(Precision.BF16X6)).wait()
     results = []
-    for rate in args.rates:
-        n_req = int(rate * args.seconds)
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    for r in ([rate] if np.isscalar(rate) else rate):
+        n_req = int(r * seconds)
+        arrivals = np.cumsum(rng.exponential(1.0 / r, n_req))
         reqs = []
         for _ in range(n_req):
             i = int(rng.integers(len(mols)))
@@ -423,197 +644,114 @@ def bench_serving(args):
             npts += n
         wall = time.perf_counter() - t0
         allv = [x for v in lat.values() for x in v]
-        entry = {"rate_per_s": rate, "requests": n_req, "seconds": wall, "points_per_s": npts / wall,
+        entry = {"rate_per_s": r, "requests": n_req, "seconds": wall, "points_per_s": npts / wall,
                  "p50_ms": float(np.percentile(allv, 50)), "p99_ms": float(np.percentile(allv, 99))}
         for kind, v in lat.items():
             entry[kind] = {"n": len(v), "p50_ms": float(np.percentile(v, 50)), "p99_ms": float(np.percentile(v, 99))}
         small = lat.get("points", []) + lat.get("line", [])
         entry["point_line_p99_ms"] = float(np.percentile(small, 99)) if small else None
-        log(f"rate {rate}/s: p50 {entry['p50_ms']:.1f} ms, p99 {entry['p99_ms']:.1f} ms, point/line p99 "
+        log(f"config 4 rate {r}/s: p50 {entry['p50_ms']:.1f} ms, p99 {entry['p99_ms']:.1f} ms, point/line p99 "
             f"{entry['point_line_p99_ms']:.1f} ms, {entry['points_per_s']:.4g} points/s")
         results.append(entry)
     out = {"metric": "serving latency p50/p99 and aggregate points/s", "unit": "ms",
-           "config": {"workload": f"{len(mols)} resident def2-TZVP-shaped molecules of 10-150 atoms; Poisson "
-                                  "arrivals 40 % points (1-1000), 30 % lines (512), 20 % planes (256^2), 10 % cubes "
-                                  "(64^3-128^3); rho, lap, ELF, RDG+sign(l2)rho or MO amplitude; BF16X6",
-                      "workers": args.workers, "chunk_points": args.chunk},
-           "loads": results}
-    print(json.dumps(out))
+           "workload": f"{len(mols)} resident def2-TZVP-shaped molecules of 10-150 atoms; Poisson arrivals 40 % points "
+                       "(1-1000), 30 % lines (512), 20 % planes (256^2), 10 % cubes (64^3-128^3); rho, lap, ELF, "
+                       "RDG+sign(l2)rho or MO amplitude; BF16X6 with its FP64 rescues",
+           "workers": args.workers, "chunk_points": args.chunk, "loads": results}
     srv.close()
     for m in mols:
         m.close()
     ctx.close()
+    return out
+
+
+def build_inputs(args, need):
+    """Every synthetic molecule of the requested workloads, built in worker
+    processes before this process touches the GPU."""
+    from kumo_hip import synthetic
+    t = time.perf_counter()
+    tasks = []
+    if "c2" in need:
+        tasks += [("c2", a) for a in enumerate(config2_specs(args.molecules))]
+    if "c4" in need:
+        rng = np.random.default_rng(4)
+        tasks += [("c4", (i, int(rng.integers(10, 151)))) for i in range(args.serve_molecules)]
+    built = parallel_map(_build_task, tasks) if tasks else []
+    out = {"c2": [b for (k, _), b in zip(tasks, built) if k == "c2"],
+           "c4": [b for (k, _), b in zip(tasks, built) if k == "c4"]}
+    if "c1" in need:
+        out["c1"] = synthetic.cached_fchk(lambda: synthetic.tzvp_cube(seed=1), "tzvp40_seed1")
+    log(f"synthetic inputs ready in {time.perf_counter() - t:.1f}s")
+    return out
+
+
+def _build_task(task):
+    kind, arg = task
+    return _config2_molecule(arg) if kind == "c2" else _config4_molecule(arg)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[1, 2, 3, 4],
-                    help="1: 160^3 cube (headline line); 2: batched molecules; 3: ESP; 4: serving load")
-    ap.add_argument("--molecules", type=int, default=None, help="molecules of configurations 2 and 4")
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4],
+                    help="2: the batched molecules (headline, with configuration 1 and 4 sub-records); "
+                         "1: 160^3 cube; 3: ESP; 4: serving load")
+    ap.add_argument("--molecules", type=int, default=256, help="molecules of configuration 2")
+    ap.add_argument("--serve-molecules", type=int, default=32, help="resident molecules of configuration 4")
+    ap.add_argument("--streams", type=int, default=4, help="streams of the configuration 2 graph")
+    ap.add_argument("--fp64-steps", type=int, default=3, help="timed steps of the FP64 comparison (config 2)")
+    ap.add_argument("--oracle-blocks", type=int, default=4, help="oracle blocks per molecule (config 2)")
     ap.add_argument("--rates", type=float, nargs="*", default=[20.0, 50.0], help="arrival rates (config 4)")
     ap.add_argument("--seconds", type=float, default=10.0, help="arrival window per rate (config 4)")
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--chunk", type=int, default=131072)
     ap.add_argument("--precision", default="BF16X6", choices=list(PEAK))
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle comparisons")
+    ap.add_argument("--no-sub", action="store_true", help="config 2 without the configuration 1 and 4 sub-records")
     ap.add_argument("--cpu-blocks", type=int, default=0,
-                    help="screening blocks in the CPU sample (0: every block of the grid)")
-    ap.add_argument("--no-cube", action="store_true", help="skip the Cube-writer timing")
-    ap.add_argument("--no-extras", action="store_true",
-                    help="skip the orbital-energy field and surface timings")
+                    help="config 1 alone: screening blocks in the CPU sample (0: every block of the grid)")
+    ap.add_argument("--extras", action="store_true",
+                    help="config 1 alone: also the Cube writer, orbital-energy fields and surfaces")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
-              "screening blocks across GPUs, each writing its slice of the field, with no exchange)",
+              "screening blocks or molecules across GPUs, each writing its slice of the fields, with no exchange)",
               file=sys.stderr)
         sys.exit(2)
-    if args.config == 2:
-        args.molecules = args.molecules or 256
-        return bench_batch(args)
     if args.config == 3:
         return bench_esp(args)
-    if args.config == 4:
-        args.molecules = args.molecules or 32
-        return bench_serving(args)
-
+    need = {2: {"c2"} | (set() if args.no_sub else {"c1", "c4"}), 1: {"c1"}, 4: {"c4"}}[args.config]
+    inputs = build_inputs(args, need)
     import torch
-    from kumo_hip import synthetic
-    from kumo_hip.binding import Context, Molecule
-
-    t = time.perf_counter()
-    path, smol, grid = synthetic.cached_fchk(lambda: synthetic.tzvp_cube(seed=1), "tzvp40_seed1")
-    log(f"synthetic molecule {smol.nbasis} AOs ready in {time.perf_counter() - t:.1f}s")
-    ctx = Context(0)
-    mol = Molecule(ctx, fchk_path=path)
-    assert mol.info["ao_mo_eligible"] == 1, "synthetic MOs must reproduce P"
-    n = int(np.prod(grid["counts"].astype(np.int64)))
-
-    log(f"timing {args.precision}: warmup {args.warmup}, steps {args.steps}")
-    main_run = run_gpu(mol, grid, int(Precision[args.precision]), args.steps, args.warmup, torch)
-    value = main_run["points"] / main_run["seconds"]
-    ms_per_step = 1e3 * main_run["seconds"] / args.steps
-    log(f"{args.precision}: {value:.4g} points/s, {ms_per_step:.1f} ms/step, kernel {main_run['kernel_ms']:.1f} ms")
-
-    fp64 = None
-    if not args.no_fp64 and args.precision != "FP64":
-        r = run_gpu(mol, grid, int(Precision.FP64), 1, 1, torch)
-        fp64 = {"value": r["points"] / r["seconds"], "ms_per_step": 1e3 * r["seconds"], "kernel_ms": r["kernel_ms"],
-                "tflops": r["flops"] / (r["kernel_ms"] * 1e-3) / 1e12 if r["kernel_ms"] > 0 else None}
-        log(f"FP64 GPU path: {fp64['value']:.4g} points/s")
-        ref_out = r["out"]
-        del r
-        full_errors = full_grid_errors(main_run["out"], ref_out, torch)
-        log(f"full-grid errors vs FP64 GPU: rho rel (>=1e-6) {full_errors['rho']['max_rel_rho_ge_1e-06']:.3g}, "
-            f"ELF abs {full_errors['elf']['max_abs']:.3g}")
-    else:
-        ref_out, full_errors = None, None
-
-    kernel_s = main_run["kernel_ms"] * 1e-3
-    achieved = main_run["flops"] / kernel_s if kernel_s > 0 else 0.0
-    traffic = None  # HBM bytes per launch, from the FETCH_SIZE / WRITE_SIZE counter runs of this workload
-    tpath = os.path.join(ROOT, "profiles", "traffic_aomo_bf16x6_tzvp640.json")
-    if args.precision == "BF16X6" and os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh)["traffic_bytes_per_launch"]
-    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision], "traffic": traffic,
-                "kernel": "aomo_kernel", "kernel_ms": main_run["kernel_ms"],
-                "launches_per_step": main_run["launches"],
-                "algorithmic_flops_per_step": main_run["flops"],
-                "mfma_products_per_flop": PRODUCTS[args.precision]}
-
-    errors, cpu = None, None
-    nblocks = n // 128
-    want = args.cpu_blocks if args.cpu_blocks > 0 else nblocks
-    blocks = np.arange(0, nblocks, max(1, nblocks // want))[:want]
-    sample_idx = (blocks[:, None] * 128 + np.arange(128)[None, :]).ravel()
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from support import oracle
-        from support.fchk import FchkData
-        om = oracle.Molecule(FchkData(path))
-        all_pts = oracle.grid_points(grid)
-        pts = all_pts[sample_idx]
-        whole = len(blocks) == nblocks
-        t = time.perf_counter()
-        o, _ = om.evaluate(PROPS, points=pts, block_size=128)
-        dt = time.perf_counter() - t
-        cpu = {"value": len(pts) / dt, "unit": "grid points/s", "cores": int(oracle.lib().orc_threads()),
-               "kind": "port",
-               "sample": ("every point of the 160^3 grid" if whole else
-                          f"{len(blocks)} screening blocks of 128 points spread over the 160^3 grid ({len(pts)} "
-                          "points)") + ", same molecule and properties, FP64, 128-point blocks"}
-        # the same restatement on one host thread, over 256 blocks spread over the grid
-        threads = int(oracle.lib().orc_threads())
-        oracle.lib().orc_set_threads(1)
-        b1 = np.arange(0, nblocks, max(1, nblocks // 256))[:256]
-        p1 = all_pts[(b1[:, None] * 128 + np.arange(128)[None, :]).ravel()]
-        t = time.perf_counter()
-        om.evaluate(PROPS, points=p1, block_size=128)
-        cpu["single_thread"] = {"value": len(p1) / (time.perf_counter() - t), "unit": "grid points/s",
-                                "sample": f"{len(b1)} screening blocks of 128 points spread over the grid"}
-        oracle.lib().orc_set_threads(threads)
-        log(f"CPU oracle: {cpu['value']:.4g} points/s on {cpu['cores']} threads, "
-            f"{cpu['single_thread']['value']:.4g} on one")
-        got = main_run["out"][:, torch.as_tensor(sample_idx, device="cuda")].cpu().numpy()
-        rho = o[0][0]
-        sig = rho >= 1e-6
-        rel = float(np.max(np.abs(got[0] - rho)[sig] / rho[sig]))
-        elf_all = float(np.max(np.abs(got[6] - o[4][0])))
-        elf_sig = float(np.max(np.abs(got[6] - o[4][0])[rho >= 1e-4]))
-        errors = {"rho_max_rel": rel, "elf_max_abs": elf_all, "elf_max_abs_rho_ge_1e-4": elf_sig,
-                  "vs": "FP64 oracle on the CPU sample"}
-        if ref_out is not None:
-            g64 = ref_out[:, torch.as_tensor(sample_idx, device="cuda")].cpu().numpy()
-            errors["fp64_gpu_rho_max_rel"] = float(np.max(np.abs(g64[0] - rho)[sig] / rho[sig]))
-            d64 = np.abs(g64[6] - o[4][0])
-            errors["fp64_gpu_elf_max_abs"] = float(d64.max())
-            # ELF where rho is so small that tau - |grad rho|^2 / 8 rho is
-            # rounding noise in FP64 itself: largest rho at which either GPU
-            # path differs from the oracle by more than 1e-5
-            off = (np.abs(got[6] - o[4][0]) > 1e-5) | (d64 > 1e-5)
-            errors["elf_points_off_by_1e-5"] = int(off.sum())
-            errors["elf_off_max_rho"] = float(rho[off].max()) if off.any() else None
-
-    cube = None if args.no_cube else bench_cube(ctx, mol, grid, main_run["out"], args.no_cpu)
-    extras = {} if args.no_extras else {
-        "orbital_energy_fields": bench_orbital_energy(ctx, mol, grid, path, torch, args.no_cpu),
-        "surfaces": bench_surfaces(ctx, mol, path, args.no_cpu)}
-
-    result = {
-        "metric": "grid pts/s for rho,grad rho,lap rho,tau,ELF vs FP64 GPU path at stated error; p50/p99 latency",
-        "value": value, "unit": "grid points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16" if args.precision.startswith("BF16") else args.precision.lower(),
-        "data": "synthetic (seeded 40-atom TZVP-shaped molecule written and read as FCHK)",
-        "config": {"workload": "40-atom def2-TZVP-shaped UHF (640 AOs, 60a/59b) on a 160^3 cube at 0.15 bohr: "
-                               "rho, grad, lap, tau, ELF, LOL",
-                   "points": n, "precision": args.precision, "block_size": 128, "backend": "AUTO(AO-MO)",
-                   "parallelism": "single GPU"},
-        "fp64_gpu": fp64,
-        "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
-        "errors": errors,
-        "errors_full_grid": dict(full_errors or {}, vs="FP64 GPU path, all points") if full_errors else None,
-        "elf_rescue": {"points": main_run["rescue_points"], "fraction": main_run["rescue_points"] / n,
-                       "fp64_kernel_ms": main_run["rescue_ms"], "criteria": "density or ELF uncertain"},
-        "roofline": roofline,
-        "field_reductions": {
-            "bound": "hbm", "unit": "GB/s", "peak": 8000.0, "kernel": "fields_kernel",
-            "kernel_ms": main_run["fields_ms"], "bytes_per_step": main_run["fields_bytes"],
-            "achieved": main_run["fields_bytes"] / (main_run["fields_ms"] * 1e-3) / 1e9 if main_run["fields_ms"] else None,
-            "frac": main_run["fields_bytes"] / (main_run["fields_ms"] * 1e-3) / 8e12 if main_run["fields_ms"] else None,
-            "bytes": "channel moments read (2 x 13 x 8 B per point) + output channels written (8 x 8 B per point)"},
-        "cpu_baseline": cpu,
-        "cube_writer": cube,
-        **extras,
-    }
-    print(json.dumps(result))
-    mol.close()
-    ctx.close()
+    if args.config == 4:
+        print(json.dumps(bench_config4(args, inputs["c4"], args.rates, args.seconds)))
+        return
+    if args.config == 1:
+        res, ctx, mol, main_run = bench_config1(args, inputs["c1"], torch, sub=False)
+        grid, path = inputs["c1"][2], inputs["c1"][0]
+        if args.extras:
+            res["cube_writer"] = bench_cube(ctx, mol, grid, main_run["out"], args.no_cpu)
+            res["orbital_energy_fields"] = bench_orbital_energy(ctx, mol, grid, path, torch, args.no_cpu)
+            res["surfaces"] = bench_surfaces(ctx, mol, path, args.no_cpu)
+        out = {"metric": METRIC, "value": res["value"], "unit": "grid points/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None,
+               "dtype": "bf16" if args.precision.startswith("BF16") else args.precision.lower(),
+               "data": "synthetic (seeded 40-atom TZVP-shaped molecule written and read as FCHK)",
+               "config": {"workload": res.pop("workload"), "points": res["points"], "precision": args.precision,
+                          "block_size": 128, "backend": "AUTO(AO-MO)", "parallelism": "single GPU"}}
+        out.update({k: v for k, v in res.items() if k not in ("value", "ms_per_step", "points")})
+        print(json.dumps(out))
+        mol.close()
+        ctx.close()
+        return
+    out = bench_config2(args, inputs["c2"], torch)
+    if not args.no_sub:
+        out["config1"] = bench_config1(args, inputs["c1"], torch)
+        out["config4"] = bench_config4(args, inputs["c4"], 30.0, 6.0)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -30,6 +30,10 @@ struct kumo_hip_batch {
     std::vector<hipEvent_t> fork, join;
     double last_ms = 0.0;
     bool warmed = false;
+    // Launch statistics of the last timed run (use_graph == 2), summed over
+    // the executors.
+    double kernel_ms = 0.0, flops = 0.0, rescue_ms = 0.0, fields_ms = 0.0;
+    long long launches = 0, rescue_points = 0, fields_bytes = 0;
 
     ~kumo_hip_batch() {
         if (!execs.empty() && execs[0]->stream) (void)hipStreamSynchronize(execs[0]->stream);
@@ -165,7 +169,7 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
                 KUMO_HIP(hipStreamSynchronize(e->stream));
             b->warmed = true;
         }
-        if (use_graph && !b->graph_exec) {
+        if (use_graph == 1 && !b->graph_exec) {
             KUMO_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
             try {
                 b->issue(true, &b->graph_keep);
@@ -177,6 +181,33 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
             }
             KUMO_HIP(hipStreamEndCapture(s0, &b->graph));
             KUMO_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+        }
+        if (use_graph == 2) {
+            // Timed run: every launch issued again without capture, so each
+            // contraction, rescue and field launch is bracketed by events on
+            // its stream and the kept-AO counts (the algorithmic flops) are
+            // read back; the items of a stream then run one after another.
+            KUMO_HIP(hipEventRecord(b->t0, s0));
+            b->issue(false, nullptr);
+            KUMO_HIP(hipEventRecord(b->t1, s0));
+            KUMO_HIP(hipEventSynchronize(b->t1));
+            b->kernel_ms = b->flops = b->rescue_ms = b->fields_ms = 0.0;
+            b->launches = b->rescue_points = b->fields_bytes = 0;
+            for (auto& e : b->execs) {
+                KUMO_HIP(hipStreamSynchronize(e->stream));
+                e->collect_timing();
+                b->kernel_ms += e->last_kernel_ms;
+                b->flops += e->last_flops;
+                b->launches += e->last_launches;
+                b->rescue_ms += e->last_rescue_ms;
+                b->rescue_points += e->last_rescue_points;
+                b->fields_ms += e->last_fields_ms;
+                b->fields_bytes += e->last_fields_bytes;
+            }
+            float ms = 0.f;
+            KUMO_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+            b->last_ms = ms;
+            return;
         }
         KUMO_HIP(hipEventRecord(b->t0, s0));
         if (use_graph) {
@@ -196,6 +227,19 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
 int kumo_hip_batch_last_ms(const kumo_hip_batch* b, double* ms) {
     if (!b || !ms) return KUMO_HIP_INVALID_ARGUMENT;
     *ms = b->last_ms;
+    return KUMO_HIP_OK;
+}
+
+int kumo_hip_batch_kernel_stats(const kumo_hip_batch* b, double* kernel_ms, int64_t* launches, double* flops,
+                                double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes) {
+    if (!b) return KUMO_HIP_INVALID_ARGUMENT;
+    if (kernel_ms) *kernel_ms = b->kernel_ms;
+    if (launches) *launches = b->launches;
+    if (flops) *flops = b->flops;
+    if (rescue_ms) *rescue_ms = b->rescue_ms;
+    if (rescue_points) *rescue_points = b->rescue_points;
+    if (fields_ms) *fields_ms = b->fields_ms;
+    if (fields_bytes) *fields_bytes = b->fields_bytes;
     return KUMO_HIP_OK;
 }
 

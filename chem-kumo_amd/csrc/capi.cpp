@@ -235,6 +235,24 @@ int kumo_hip_boys(kumo_hip_context* ctx, const double* T, size_t n, int32_t m, d
     });
 }
 
+int kumo_hip_boys_esp(kumo_hip_context* ctx, const double* T, size_t n, int32_t top, double* out) {
+    if (!ctx || (n && (!T || !out))) return KUMO_HIP_INVALID_ARGUMENT;
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lock(ctx->mutex);
+        if (top < 0 || top > 2 * kumo::kMaxL) kumo::invalid("the pair-loop Boys function has orders 0 to 10");
+        for (size_t i = 0; i < n; ++i)
+            if (!(T[i] >= 0.0) || !std::isfinite(T[i]))
+                kumo::invalid("Boys function argument must be finite and nonnegative");
+        if (n == 0) return;
+        kumo::DeviceBuffer dt = kumo::upload(T, n, ctx->exec.stream);
+        kumo::DeviceBuffer dout(n * static_cast<std::size_t>(top + 1) * sizeof(double));
+        kumo::launch_boys_esp(ctx->boys_table.as<double>(), dt.as<double>(), static_cast<long long>(n), top,
+                              dout.as<double>(), ctx->exec.stream);
+        KUMO_HIP(hipMemcpyAsync(out, dout.get(), dout.bytes(), hipMemcpyDeviceToHost, ctx->exec.stream));
+        KUMO_HIP(hipStreamSynchronize(ctx->exec.stream));
+    });
+}
+
 int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, int64_t* launches, double* flops,
                                double* padded_flops) {
     if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;

@@ -216,6 +216,19 @@ __global__ void boys_kernel(const double* table, const double* T, long long n, i
         out[j * (m + 1) + k] = v[k];
 }
 
+// boys_esp<TOP> on its own (the pair loop's variant of the Boys function):
+// out[j * (TOP + 1) + m] = F_m(T[j]), m = 0..TOP.
+template <int TOP>
+__global__ void __launch_bounds__(128) boys_esp_kernel(const double* table, const double* T, long long n, double* out) {
+    const long long j = blockIdx.x * 128ll + threadIdx.x;
+    if (j >= n) return;
+    double v[TOP + 1];
+    boys_esp<TOP>(table + kBoysOrders * kBoysPoints + TOP * kBoysPoints * kBoysRowL, T[j], v);
+#pragma unroll
+    for (int k = 0; k <= TOP; ++k)
+        out[j * (TOP + 1) + k] = v[k];
+}
+
 } // namespace
 
 int esp_slices(std::vector<EspGroup>& groups, long long n) {
@@ -263,6 +276,26 @@ void launch_esp(const EspPairDev* pairs, const std::vector<EspGroup>& groups, co
     }
     esp_finish_kernel<<<static_cast<unsigned>((n + kEspThreads - 1) / kEspThreads), kEspThreads, 0, stream>>>(
         partial, nslices, atoms, natoms, ps, first, n, out);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_boys_esp(const double* table, const double* T, long long n, int top, double* out, hipStream_t stream) {
+    if (n <= 0) return;
+    const dim3 grid(static_cast<unsigned>((n + 127) / 128)), block(128);
+    switch (top) {
+    case 0: boys_esp_kernel<0><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 1: boys_esp_kernel<1><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 2: boys_esp_kernel<2><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 3: boys_esp_kernel<3><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 4: boys_esp_kernel<4><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 5: boys_esp_kernel<5><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 6: boys_esp_kernel<6><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 7: boys_esp_kernel<7><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 8: boys_esp_kernel<8><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 9: boys_esp_kernel<9><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    case 10: boys_esp_kernel<10><<<grid, block, 0, stream>>>(table, T, n, out); break;
+    default: invalid("the pair-loop Boys function has orders 0 to 10");
+    }
     KUMO_HIP(hipGetLastError());
 }
 

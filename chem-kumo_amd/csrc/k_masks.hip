@@ -576,10 +576,13 @@ __global__ void __launch_bounds__(kMaskThreads) __attribute__((amdgpu_waves_per_
             for (int i = 0; i < n; ++i) {
                 int32_t* cr = crows + 2 * (c * kChunkRows + row0 + i);
                 cr[0] = (mode == kEntryCart || ((bits >> i) & 1u)) ? first + i : -1;
-                // Phi row: block << 15 | mode << 13 | row << 9 | Cartesian powers (3 bits each)
+                // Phi row: block << 15 | mode << 13 | pure row << 9 | Cartesian powers (3 bits each);
+                // the pure row (< 2 kMaxL + 1 <= 16) is only encoded for pure entries, the
+                // Cartesian row (up to 21 components) only through its powers.
                 const int cart = mode == kEntryPure ? 0 : i;
-                cr[1] = (s << 15) | (mode << 13) | (i << 9) | (kPow[sh.l][cart][0] << 6) | (kPow[sh.l][cart][1] << 3) |
-                        kPow[sh.l][cart][2];
+                const int prow = mode == kEntryPure ? i : 0;
+                cr[1] = (s << 15) | (mode << 13) | (prow << 9) | (kPow[sh.l][cart][0] << 6) |
+                        (kPow[sh.l][cart][1] << 3) | kPow[sh.l][cart][2];
             }
     }
     for (int item = tid; item < nc * kChunkRows; item += kMaskThreads) {

@@ -177,10 +177,12 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
     m.d_rec = upload(rec.data(), rec.size(), s);
     m.d_radw = upload(b.radw.data(), b.radw.size(), s);
     // Occupied-MO operand: alpha slots [0, A16), beta slots [A16, A16 + B16)
-    // (closed shells contract the alpha channel only).
+    // (closed shells contract the alpha channel only). An unrestricted
+    // molecule keeps at least 16 (zero) slots per channel, so a channel with
+    // no electrons is contracted as zeros and gives exactly zero moments.
     const bool closed = !m.unrestricted;
-    const int a16 = (m.nalpha + 15) / 16 * 16;
-    const int b16 = closed ? 0 : (m.nbeta + 15) / 16 * 16;
+    const int a16 = closed ? (m.nalpha + 15) / 16 * 16 : std::max(16, (m.nalpha + 15) / 16 * 16);
+    const int b16 = closed ? 0 : std::max(16, (m.nbeta + 15) / 16 * 16);
     m.alpha_slots = a16;
     m.slots = std::max(16, a16 + b16);
     m.operand_rows = b.operand_rows;

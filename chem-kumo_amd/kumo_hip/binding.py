@@ -76,8 +76,8 @@ EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip
            "kumo_hip_molecule_from_fchk", "kumo_hip_molecule_create", "kumo_hip_molecule_info_get",
            "kumo_hip_molecule_release", "kumo_hip_molecule_blocks", "kumo_hip_molecule_atoms",
            "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
-           "kumo_hip_boys", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_last_field_stats", "kumo_hip_batch_create",
-           "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_release", "kumo_hip_server_create",
+           "kumo_hip_boys", "kumo_hip_boys_esp", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_last_field_stats", "kumo_hip_batch_create",
+           "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_kernel_stats", "kumo_hip_batch_release", "kumo_hip_server_create",
            "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release",
            "kumo_hip_write_cube", "kumo_hip_write_orbital_cube", "kumo_hip_format_scientific",
            "kumo_hip_last_cube_stats", "kumo_hip_density_isosurface", "kumo_hip_electrostatic_potential_surface",
@@ -120,6 +120,7 @@ def lib():
         L.kumo_hip_ao_eval.argtypes = [vp, vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_screening_masks.argtypes = [vp, vp, vp, vp, C.c_size_t, C.c_int32, C.c_int32, C.c_double, vp]
         L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
+        L.kumo_hip_boys_esp.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_last_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
         L.kumo_hip_last_rescue_stats.argtypes = [vp, vp, vp]
         L.kumo_hip_last_field_stats.argtypes = [vp, vp, vp, vp]
@@ -127,6 +128,7 @@ def lib():
                                             vp, vp]
         L.kumo_hip_batch_run.argtypes = [vp, C.c_int32]
         L.kumo_hip_batch_last_ms.argtypes = [vp, vp]
+        L.kumo_hip_batch_kernel_stats.argtypes = [vp] + [vp] * 7
         L.kumo_hip_batch_release.argtypes = [vp]
         L.kumo_hip_server_create.argtypes = [vp, C.POINTER(_ServerOptions), vp]
         L.kumo_hip_server_submit.argtypes = [vp, vp, vp, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(_Options), vp, vp]
@@ -185,6 +187,13 @@ class Context:
         T = np.ascontiguousarray(T, dtype=np.float64).ravel()
         out = np.zeros((T.size, m + 1))
         self.check(lib().kumo_hip_boys(self.h, _p(T), T.size, m, _p(out)))
+        return out
+
+    def boys_esp(self, T, top):
+        """The ESP pair loop's Boys function, orders 0..top (top <= 10)."""
+        T = np.ascontiguousarray(T, dtype=np.float64).ravel()
+        out = np.zeros((T.size, top + 1))
+        self.check(lib().kumo_hip_boys_esp(self.h, _p(T), T.size, top, _p(out)))
         return out
 
     def kernel_stats(self):
@@ -528,6 +537,20 @@ class Batch:
         ms = C.c_double()
         lib().kumo_hip_batch_last_ms(self.h, C.byref(ms))
         return ms.value
+
+    def timed_run(self):
+        """One run without a graph with every contraction, rescue and field
+        launch bracketed by HIP events on its stream; returns the run's device
+        ms and the launch statistics (kumo_hip_batch_kernel_stats)."""
+        self.ctx.check(lib().kumo_hip_batch_run(self.h, 2))
+        ms = C.c_double()
+        lib().kumo_hip_batch_last_ms(self.h, C.byref(ms))
+        km, fl, rms, fms = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        n, rp, fb = C.c_int64(), C.c_int64(), C.c_int64()
+        self.ctx.check(lib().kumo_hip_batch_kernel_stats(self.h, C.byref(km), C.byref(n), C.byref(fl), C.byref(rms),
+                                                         C.byref(rp), C.byref(fms), C.byref(fb)))
+        return {"ms": ms.value, "kernel_ms": km.value, "launches": n.value, "flops": fl.value, "rescue_ms": rms.value,
+                "rescue_points": rp.value, "fields_ms": fms.value, "fields_bytes": fb.value}
 
     def close(self):
         if self.h:

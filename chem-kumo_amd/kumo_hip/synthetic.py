@@ -20,6 +20,10 @@ _CART = {0: [(0, 0, 0)], 1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
          2: [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)],
          3: [(3, 0, 0), (0, 3, 0), (0, 0, 3), (1, 2, 0), (2, 1, 0), (2, 0, 1), (1, 0, 2), (0, 1, 2), (0, 2, 1),
              (1, 1, 1)]}
+# g shells and above: lexicographic in (lx, ly) (zzzz, yzzz, yyzz, ...).
+for _l in (4, 5):
+    _CART[_l] = [(lx, ly, _l - lx - ly) for lx in range(_l + 1) for ly in range(_l - lx + 1)]
+MAX_L = 5
 
 # Shell recipes: (FCHK shell type, primitives). 6-31G*-shaped: Cartesian d.
 POPLE_HEAVY = [(0, 6), (-1, 3), (-1, 1), (2, 1)]
@@ -29,7 +33,7 @@ TZVP_HEAVY = [(0, 6), (0, 2), (0, 1), (0, 1), (0, 1), (1, 4), (1, 1), (1, 1), (-
 TZVP_H = [(0, 3), (0, 1), (0, 1), (1, 1)]
 
 
-_BINOM = np.array([[1, 0, 0, 0], [1, 1, 0, 0], [1, 2, 1, 0], [1, 3, 3, 1]], dtype=float)
+_BINOM = np.array([[math.comb(n, k) for k in range(MAX_L + 1)] for n in range(MAX_L + 1)], dtype=float)
 
 
 def _dfact(n):
@@ -134,11 +138,11 @@ class Basis:
             pa, pb = P - xa, P - xb
             na, nb = pw[A, axis], pw[B, axis]
             pref = np.exp(-a * b / p * (xa - xb) ** 2) * np.sqrt(math.pi / p)
-            pap = [np.ones_like(pa), pa, pa * pa, pa * pa * pa]
-            pbp = [np.ones_like(pb), pb, pb * pb, pb * pb * pb]
+            pap = [pa ** q for q in range(MAX_L + 1)]
+            pbp = [pb ** q for q in range(MAX_L + 1)]
             tot = np.zeros(A.shape)
-            for i in range(4):
-                for j in range(4):
+            for i in range(MAX_L + 1):
+                for j in range(MAX_L + 1):
                     k = i + j
                     if k % 2:
                         continue
@@ -207,22 +211,27 @@ class SyntheticMolecule:
         S = basis.overlap()
         off = 0
         ao = 0
+        scale = np.ones(S.shape[0])  # per AO: the factor its shell's coefficients are scaled by
         for s, t in enumerate(self.shell_types):
             sl = slice(off, off + self.nprim[s])
             off += self.nprim[s]
             if t == -1:
-                self.coefficients[sl] /= math.sqrt(S[ao, ao])
-                self.p_coefficients[sl] /= math.sqrt(S[ao + 1, ao + 1])
+                fs, fp = 1.0 / math.sqrt(S[ao, ao]), 1.0 / math.sqrt(S[ao + 1, ao + 1])
+                self.coefficients[sl] *= fs
+                self.p_coefficients[sl] *= fp
+                scale[ao] = fs
+                scale[ao + 1:ao + 4] = fp
                 ao += 4
             else:
                 l = abs(t)
                 cnt = 2 * l + 1 if t <= -2 else len(_CART[l])
                 # Cartesian d/f components are not all of unit norm; scale by the first (x^l) one.
-                self.coefficients[sl] /= math.sqrt(S[ao, ao])
+                f = 1.0 / math.sqrt(S[ao, ao])
+                self.coefficients[sl] *= f
+                scale[ao:ao + cnt] = f
                 ao += cnt
-        basis = Basis(self.shell_types, self.nprim, self.shell_xyz, self.exponents, self.coefficients,
-                      self.p_coefficients if self.p_coefficients is not None else np.zeros_like(self.exponents))
-        S = basis.overlap()
+        # the overlap of the rescaled functions (every AO of a shell scales alike)
+        S = S * scale[:, None] * scale[None, :]
         self.overlap = S
         self.nbasis = basis.nao
         self.atomic_numbers = np.asarray(z, np.int32)
@@ -373,6 +382,37 @@ def batch_molecule(seed, natoms, family, edge, spacing=0.2):
         (lambda zz: POPLE_HEAVY if zz > 1 else POPLE_H)
     mol = SyntheticMolecule(rng, z, xyz, recipe, na, nb, na != nb)
     return mol, cube_grid((xyz.min(0) + xyz.max(0)) / 2, edge, spacing)
+
+
+# Shells up to h: pure and Cartesian g and h beside s, p, SP and d shells.
+HIGH_L_A = [(0, 4), (1, 2), (-2, 1), (-4, 2), (5, 1)]      # pure g, Cartesian h
+HIGH_L_B = [(0, 3), (-1, 2), (4, 1), (-5, 1)]               # Cartesian g, pure h
+HIGH_L_H = [(0, 2), (1, 1)]
+
+
+def high_l_molecule(seed=7):
+    """Three atoms (O, C, H) 2.2-2.6 bohr apart with g and h shells, pure and
+    Cartesian (74 AOs), unrestricted 8 alpha / 7 beta, and a grid around them
+    at 0.3 bohr with a 2.5 bohr margin."""
+    while True:
+        rng = np.random.default_rng(seed)
+        xyz = np.array([[0.0, 0.0, 0.0], [2.4, 0.3, -0.2], [-0.9, 2.1, 0.6]]) + rng.normal(size=(3, 3)) * 0.1
+        recipes = {8: HIGH_L_A, 6: HIGH_L_B, 1: HIGH_L_H}
+        try:
+            mol = SyntheticMolecule(rng, [8, 6, 1], xyz, lambda zz: recipes[zz], 8, 7, True)
+            return mol, grid_around(xyz, 2.5, 0.3)
+        except ValueError:
+            seed += 1000
+
+
+def no_beta_molecule(seed=5):
+    """Two H atoms 1.4 bohr apart, def2-TZVP-shaped (3s1p each), unrestricted
+    with two alpha and no beta electrons (a triplet): the beta channel is
+    empty."""
+    rng = np.random.default_rng(seed)
+    xyz = np.array([[0.0, 0.0, 0.0], [0.3, 0.2, 1.4]])
+    mol = SyntheticMolecule(rng, [1, 1], xyz, lambda zz: TZVP_H, 2, 0, True)
+    return mol, grid_around(xyz, 3.0, 0.25)
 
 
 def cached_fchk(mol_fn, name, directory=None):

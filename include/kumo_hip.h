@@ -181,6 +181,13 @@ int kumo_hip_screening_masks(kumo_hip_context* ctx, const kumo_hip_molecule* mol
 /* Boys function F_0..F_m(T) on the device (src/boys_function.cpp:47-76),
  * out is n x (m + 1). */
 int kumo_hip_boys(kumo_hip_context* ctx, const double* T, size_t n, int32_t m, double* out);
+/* The Boys function of the ESP pair loop (src/boys_function.cpp:47-76 as the
+ * pair sum of src/electrostatic_potential.cpp:286-330 calls it): the same
+ * table + Taylor / closed-form scheme with constant divisions turned into
+ * products and the square root above T = 40 from the hardware estimate,
+ * orders 0..top (0 <= top <= 10, the pair orders of shells up to h), out is
+ * n x (top + 1). For testing that variant on its own. */
+int kumo_hip_boys_esp(kumo_hip_context* ctx, const double* T, size_t n, int32_t top, double* out);
 
 /* ---- batched evaluation ----
  * A fixed list of (molecule, grid) items sharing one property list and one
@@ -203,6 +210,16 @@ int kumo_hip_batch_create(kumo_hip_context* ctx, size_t nitems, const kumo_hip_m
 int kumo_hip_batch_run(kumo_hip_batch* batch, int32_t use_graph);
 /* Device time (ms) of the last run, measured with HIP events around it. */
 int kumo_hip_batch_last_ms(const kumo_hip_batch* batch, double* ms);
+/* kumo_hip_batch_run with use_graph = 2 issues every launch again without a
+ * graph, bracketing each contraction, FP64 rescue and field-reduction launch
+ * with HIP events on its stream and reading back the kept-AO counts (items of
+ * one stream then run one after another). Afterwards this returns, summed
+ * over the run: the contraction kernels' device time (ms), their launches and
+ * algorithmic FLOPs (2 x kept AOs x MOs x panels x points per screening
+ * block), the rescue launches' time and point slots, and the field
+ * reductions' time and bytes moved. Measurement only. */
+int kumo_hip_batch_kernel_stats(const kumo_hip_batch* batch, double* kernel_ms, int64_t* launches, double* flops,
+                                double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes);
 int kumo_hip_batch_release(kumo_hip_batch* batch);
 
 /* ---- serving loop ----

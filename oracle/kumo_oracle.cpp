@@ -255,8 +255,43 @@ struct Scratch {
     std::vector<double> radial, cart, pw, factors;
 };
 
-// AO values at one point (evaluate_into_impl, src/basis_evaluator.cpp:442-525):
-// Cartesian component = ((x^i y^j) z^k) * sum_p w_kp exp(-a_p r^2).
+// AO values of one block at displacement (x, y, z), r2 = |r|^2, inside its
+// cutoff (evaluate_into_impl, src/basis_evaluator.cpp:442-525): Cartesian
+// component = ((x^i y^j) z^k) * sum_p w_kp exp(-a_p r^2), then the pure
+// transform. Row i of the block is written to out[i * stride].
+template <typename B>
+void block_values(const B& b, double x, double y, double z, double r2, double* out, std::size_t stride, Scratch& s) {
+    for (int p = 0; p < b.nprim(); ++p)
+        s.radial[static_cast<std::size_t>(p)] = std::exp(-b.exps[static_cast<std::size_t>(p)] * r2);
+    double xp[32], yp[32], zp[32];
+    xp[0] = yp[0] = zp[0] = 1.0;
+    for (int n = 1; n <= b.l; ++n) {
+        xp[n] = xp[n - 1] * x;
+        yp[n] = yp[n - 1] * y;
+        zp[n] = zp[n - 1] * z;
+    }
+    for (int k = 0; k < b.ncart(); ++k) {
+        const auto& pw = b.powers[static_cast<std::size_t>(k)];
+        const double poly = (xp[pw.x] * yp[pw.y]) * zp[pw.z];
+        double rs = 0.0;
+        for (int p = 0; p < b.nprim(); ++p)
+            rs += b.w(k, p) * s.radial[static_cast<std::size_t>(p)];
+        s.cart[static_cast<std::size_t>(k)] = poly * rs;
+    }
+    if (b.pure.empty()) {
+        for (int k = 0; k < b.ncart(); ++k)
+            out[static_cast<std::size_t>(k) * stride] = s.cart[static_cast<std::size_t>(k)];
+    } else {
+        for (int r = 0; r < b.count; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < b.ncart(); ++k)
+                acc += b.pure[static_cast<std::size_t>(r * b.ncart() + k)] * s.cart[static_cast<std::size_t>(k)];
+            out[static_cast<std::size_t>(r) * stride] = acc;
+        }
+    }
+}
+
+// AO values at one point (evaluate_into_impl, src/basis_evaluator.cpp:442-525).
 void ao_values_point(const Basis& B, const double* pt, double* out, Scratch& s) {
     s.radial.resize(static_cast<std::size_t>(B.max_prim));
     s.cart.resize(static_cast<std::size_t>(B.max_cart));
@@ -268,35 +303,7 @@ void ao_values_point(const Basis& B, const double* pt, double* out, Scratch& s) 
                 out[b.offset + i] = 0.0;
             continue;
         }
-        for (int p = 0; p < b.nprim(); ++p)
-            s.radial[static_cast<std::size_t>(p)] = std::exp(-b.exps[static_cast<std::size_t>(p)] * r2);
-        double xp[32], yp[32], zp[32];
-        xp[0] = yp[0] = zp[0] = 1.0;
-        for (int n = 1; n <= b.l; ++n) {
-            xp[n] = xp[n - 1] * x;
-            yp[n] = yp[n - 1] * y;
-            zp[n] = zp[n - 1] * z;
-        }
-        for (int k = 0; k < b.ncart(); ++k) {
-            const auto& pw = b.powers[static_cast<std::size_t>(k)];
-            const double poly = (xp[pw.x] * yp[pw.y]) * zp[pw.z];
-            double rs = 0.0;
-            for (int p = 0; p < b.nprim(); ++p)
-                rs += b.w(k, p) * s.radial[static_cast<std::size_t>(p)];
-            s.cart[static_cast<std::size_t>(k)] = poly * rs;
-        }
-        if (b.pure.empty()) {
-            for (int k = 0; k < b.ncart(); ++k)
-                out[b.offset + k] = s.cart[static_cast<std::size_t>(k)];
-        } else {
-            for (int r = 0; r < b.count; ++r) {
-                double acc = 0.0;
-                for (int k = 0; k < b.ncart(); ++k)
-                    acc += b.pure[static_cast<std::size_t>(r * b.ncart() + k)] *
-                           s.cart[static_cast<std::size_t>(k)];
-                out[b.offset + r] = acc;
-            }
-        }
+        block_values(b, x, y, z, r2, out + b.offset, 1, s);
     }
 }
 
@@ -325,12 +332,17 @@ void ao_panels(const Basis& B, int n, const double* pts, int order, Panels& out,
     const int panels = panel_count(order);
     out.resize(B.nao, panels * n);
     if (order == 0) {
-        std::vector<double> col(static_cast<std::size_t>(B.nao));
-        for (int j = 0; j < n; ++j) {
-            ao_values_point(B, pts + 3 * j, col.data(), s);
-            for (int a = 0; a < B.nao; ++a)
-                out.at(a, j) = col[static_cast<std::size_t>(a)];
-        }
+        // blocks outer, points inner: the same per-(block, point) arithmetic
+        // as ao_values_point, written along the rows of the panel matrix
+        s.radial.resize(static_cast<std::size_t>(B.max_prim));
+        s.cart.resize(static_cast<std::size_t>(B.max_cart));
+        for (const auto& b : B.blocks)
+            for (int j = 0; j < n; ++j) {
+                const double* pt = pts + 3 * j;
+                const double x = pt[0] - b.origin[0], y = pt[1] - b.origin[1], z = pt[2] - b.origin[2];
+                const double r2 = sqnorm(x, y, z);
+                if (r2 <= b.cutoff_r2) block_values(b, x, y, z, r2, out.row(b.offset) + j, static_cast<std::size_t>(out.cols), s);
+            }
         return;
     }
     const int deg = B.max_l + 1;
@@ -778,27 +790,55 @@ std::vector<int> significant(const Panels& ao, double threshold) {
 
 // out(i, col) = sum_k A(kept[k], i) * X(kept[k] row, col), A row-major n x lda.
 // Blocked over 4 output rows; the inner loop runs along contiguous columns.
+// out[i][c] = sum over k in kept (in list order) of A[k][i] * X[k][col0 + c]:
+// register tiles of 4 rows x 8 columns (two 4-wide vectors per row) whose
+// accumulators stay in registers over the kept list; each output element is
+// summed in kept order from zero, with separate multiply and add (no fused
+// operations), so the result equals the plain triple loop bit for bit.
+typedef double v4d __attribute__((vector_size(32)));
+
 void project(const double* A, int lda, int m_out, const std::vector<int>& kept, const Panels& X,
              int col0, int ncols, std::vector<double>& out) {
     out.assign(static_cast<std::size_t>(m_out) * static_cast<std::size_t>(ncols), 0.0);
+    const std::size_t nk = kept.size();
+    const int c8 = ncols / 8 * 8;
+    // the four coefficients of each kept row, packed once per row group
+    std::vector<double> pack(4 * nk);
     int i = 0;
     for (; i + 4 <= m_out; i += 4) {
-        double* o0 = out.data() + static_cast<std::size_t>(i) * ncols;
-        double* o1 = o0 + ncols;
-        double* o2 = o1 + ncols;
-        double* o3 = o2 + ncols;
-        for (int k : kept) {
-            const double* a = A + static_cast<std::size_t>(k) * lda + i;
-            const double c0 = a[0], c1 = a[1], c2 = a[2], c3 = a[3];
-            const double* x = X.row(k) + col0;
-            for (int c = 0; c < ncols; ++c) {
-                const double v = x[c];
-                o0[c] += c0 * v;
-                o1[c] += c1 * v;
-                o2[c] += c2 * v;
-                o3[c] += c3 * v;
+        double* o = out.data() + static_cast<std::size_t>(i) * ncols;
+        for (std::size_t q = 0; q < nk; ++q)
+            for (int r = 0; r < 4; ++r)
+                pack[4 * q + static_cast<std::size_t>(r)] = A[static_cast<std::size_t>(kept[q]) * lda + i + r];
+        for (int c = 0; c < c8; c += 8) {
+            v4d acc[4][2] = {};
+            for (std::size_t q = 0; q < nk; ++q) {
+                const int k = kept[q];
+                const double* a = pack.data() + 4 * q;
+                const double* x = X.row(k) + col0 + c;
+                v4d x0, x1;
+                std::memcpy(&x0, x, sizeof(x0));
+                std::memcpy(&x1, x + 4, sizeof(x1));
+                for (int r = 0; r < 4; ++r) {
+                    const v4d ar = {a[r], a[r], a[r], a[r]};
+                    acc[r][0] += ar * x0;
+                    acc[r][1] += ar * x1;
+                }
+            }
+            for (int r = 0; r < 4; ++r) {
+                std::memcpy(o + static_cast<std::size_t>(r) * ncols + c, &acc[r][0], sizeof(v4d));
+                std::memcpy(o + static_cast<std::size_t>(r) * ncols + c + 4, &acc[r][1], sizeof(v4d));
             }
         }
+        for (int c = c8; c < ncols; ++c)
+            for (int r = 0; r < 4; ++r) {
+                double sum = 0.0;
+                for (std::size_t q = 0; q < nk; ++q) {
+                    const int k = kept[q];
+                    sum += A[static_cast<std::size_t>(k) * lda + i + r] * X.row(k)[col0 + c];
+                }
+                o[static_cast<std::size_t>(r) * ncols + c] = sum;
+            }
     }
     for (; i < m_out; ++i) {
         double* o = out.data() + static_cast<std::size_t>(i) * ncols;

@@ -206,11 +206,14 @@ def test_reduced_precision_errors(ctx, precision, rho_tol):
     rel = np.max(np.abs(g[0][0] - rho)[sig] / rho[sig])
     elf_err = np.max(np.abs(g[1][0] - o[1][0])[rho >= 1e-4])
     elf_all = np.max(np.abs(g[1][0] - o[1][0]))
-    print(f"[gpu] precision {int(precision)}: max rel rho {rel:.3g}, ELF abs {elf_err:.3g} (all points {elf_all:.3g})",
-          flush=True)
+    # the Laplacian relative to its largest |value| (no rescue is aimed at it)
+    lap = np.max(np.abs(g[2][0] - o[2][0])) / np.max(np.abs(o[2][0]))
+    print(f"[gpu] precision {int(precision)}: max rel rho {rel:.3g}, ELF abs {elf_err:.3g} (all points {elf_all:.3g}), "
+          f"Laplacian {lap:.3g}", flush=True)
     assert rel < rho_tol
     assert elf_err < 50 * rho_tol
     assert elf_all < 1e-5
+    assert lap < rho_tol
 
 
 def test_reduced_precision_spin_elf(ctx):

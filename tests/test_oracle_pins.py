@@ -261,8 +261,17 @@ def test_screening_changes_little_and_masks_depend_on_order():
     assert np.allclose(scr[0], exact[0], rtol=0, atol=1e-10)
     m0 = mol.masks(128, 0, points=pts)
     m2 = mol.masks(128, 2, points=pts)
-    assert m0.sum() <= m2.sum()
     assert np.all(m2[m0 == 1] == 1)  # a value above threshold keeps the row at every order
+    # derivative panels keep AOs whose values alone are below the threshold
+    assert np.count_nonzero((m2 == 1) & (m0 == 0)) > 0
+    # the definition (src/ao_screening.hpp:35-50) by brute force: the largest
+    # |entry| of the block's value (order 0) or value + gradient + Hessian
+    # panels (order 2) above the threshold
+    for b in range(m0.shape[0]):
+        p = pts[b * 128:(b + 1) * 128]
+        for order, m in ((0, m0), (2, m2)):
+            largest = np.abs(mol.basis.ao(p, order)).max(axis=1)
+            assert np.array_equal((largest > 1e-12).astype(np.uint8), m[b]), (b, order)
     loose, _ = mol.evaluate([(K.Density, 0, 0, 0)], points=pts, threshold=1e-3)
     assert not np.allclose(loose[0], exact[0], rtol=0, atol=1e-10)
 
@@ -292,3 +301,34 @@ def test_errors_match_reference_contract():
         mol.evaluate([(K.Orbital, 0, 0, 10_000)], points=pts)
     with pytest.raises(ValueError):
         mol.evaluate([(K.Density, 0, 0, 0)], points=pts, block_size=0)
+
+
+def test_elf_tail_is_undetermined_in_fp64():
+    """ELF = 1 / (1 + (D / D0)^2) with D = tau - |grad rho|^2 / 8 rho per spin
+    channel (src/evaluator.cpp:127-140). Far from the nuclei one AO dominates
+    every occupied MO of a channel, D vanishes in exact arithmetic and what
+    FP64 computes is a rounding of tau (~1e-16 tau), while D0 / tau ~ rho^(2/3)
+    falls below 1e-16 near rho ~ 1e-20: there ELF is not determined by FP64
+    arithmetic. Shown with the oracle's two FP64 formulations of the
+    reference algorithm (AO-MO: sums over MOs; AO-P: P times the AO panels)
+    on the 640-AO UHF molecule of the 160^3 benchmark cube, at the blocks of
+    its lowest densities: they disagree by more than 0.5 below rho = 1e-20 and
+    agree within 1e-6 where rho >= 1e-14, the floor above which the GPU paths
+    are held to the oracle's ELF."""
+    from kumo_hip import synthetic
+    path, _, grid = synthetic.cached_fchk(lambda: synthetic.tzvp_cube(seed=1), "tzvp40_seed1")
+    mol = oracle.Molecule(FchkData(path))
+    # 300 of the screening blocks that hold points with rho < 1e-14
+    r, _ = mol.evaluate([(K.Density, 0, 0, 0)], grid=grid)
+    blocks = np.unique(np.nonzero(r[0][0] < 1e-14)[0] // 128)
+    blocks = np.sort(np.random.default_rng(0).choice(blocks, size=300, replace=False))
+    idx = (blocks[:, None] * 128 + np.arange(128)[None, :]).ravel()
+    pts = oracle.grid_points(grid)[idx]
+    props = [(K.Density, 0, 0, 0), (K.Elf, 0, 0, 0)]
+    mo, _ = mol.evaluate(props, points=pts, backend=Backend.AO_MO)
+    ap, _ = mol.evaluate(props, points=pts, backend=Backend.AO_P)
+    rho = mo[0][0]
+    d = np.abs(mo[1][0] - ap[1][0])
+    assert np.allclose(rho, ap[0][0], rtol=1e-12, atol=0)
+    assert d[rho < 1e-20].max() > 0.5
+    assert d[rho >= 1e-14].max() < 1e-6
