@@ -182,9 +182,14 @@ def test_pair_loop_boys_matches_oracle(ctx, top):
     T = boys_arguments()
     g = ctx.boys_esp(T, top)
     o = np.array([oracle.boys(t, top) for t in T])
-    # the same scheme with products by reciprocals, Horner's rule and a refined
-    # hardware reciprocal square root: a few units in the last place
+    # The same scheme with products by rounded reciprocals, Horner's rule and a
+    # refined hardware reciprocal square root. The Taylor sum (or the closed
+    # form above T = 40) agrees to about 10 units in the last place; each step
+    # of the recursion to the other orders multiplies by a rounded reciprocal
+    # where the reference divides (one more rounding per step), and the step
+    # errors add up: 12 + 3 x TOP ulp.
     ulp = np.abs(g - o) / np.spacing(np.abs(o))
-    print(f"[gpu] boys_esp top {top}: max {ulp.max():.1f} ulp", flush=True)
+    worst = np.unravel_index(np.argmax(ulp), ulp.shape)
+    print(f"[gpu] boys_esp top {top}: max {ulp.max():.1f} ulp at T = {T[worst[0]]!r}, order {worst[1]}", flush=True)
     assert np.all(np.isfinite(g))
-    assert ulp.max() <= 16
+    assert ulp.max() <= 12 + 3 * top
