@@ -52,6 +52,7 @@ int kumo_hip_init(int device, kumo_hip_context** out) {
         KUMO_HIP(hipSetDevice(device));
         ctx->device = device;
         ctx->exec.create(device, nullptr);
+        kumo::noalloc_scratch(device); // the allocation-free entry point's scratch on this device
         const auto table = kumo::boys_device_table();
         ctx->boys_table = kumo::upload(table.data(), table.size(), ctx->exec.stream);
         KUMO_HIP(hipStreamSynchronize(ctx->exec.stream));

@@ -425,6 +425,33 @@ Prepared prepare(const kumo_hip_molecule* mol, const PointsArg& pts, const kumo_
     return r;
 }
 
+// The workspace sizes of the point pass of run_range, in the order it grows
+// them: screening tables (tables()), channel moments, field tasks, and for a
+// reduced precision the error scales, the rescue plan (rescue_plan()) and the
+// rescue moments; for AO-P the staged panels and products. Each piece is
+// rounded up to the arena's 256-byte alignment in `fixed`.
+void point_pass_bytes(const kumo_hip_molecule* mol, const Prepared& pr, const kumo_hip_options& opt,
+                      std::size_t& fixed, std::size_t& per_block) {
+    const std::size_t nao = static_cast<std::size_t>(mol->n), ns = mol->basis.shells.size();
+    const std::size_t mc = static_cast<std::size_t>(mol->max_chunks), bs = static_cast<std::size_t>(opt.block_size);
+    const int ao_order = pr.grid ? pr.plan.max_order : std::max(1, pr.plan.max_order);
+    fixed = 32 * 256 + static_cast<std::size_t>(pr.props.size()) * sizeof(FieldTask);
+    per_block = 5 * nao + 17 * ns + 4 * (mc + 1) + 8 * mc * kChunkRows + 12; // tables
+    if (pr.backend == KUMO_HIP_AO_MO) {
+        const std::size_t groups = static_cast<std::size_t>((mol->slots + 127) / 128);
+        per_block += groups * 2 * kMoments * bs * 8; // moments
+        if (opt.precision != KUMO_HIP_FP64) {
+            const std::size_t tpb = (bs + kRescueTile - 1) / kRescueTile;
+            per_block += 4 * bs * 4;                                         // error scales
+            per_block += 3 * bs + 4 + 8 * tpb;                               // rescue plan
+            per_block += groups * 2 * kMoments * tpb * kRescueTile * 8;      // rescue moments
+        }
+    } else {
+        const std::size_t pan = ao_order == 0 ? 1 : (ao_order == 1 ? 4 : 10);
+        per_block += 2 * kMoments * bs * 8 + nao * pan * bs * 8 + nao * 4 * bs * 8;
+    }
+}
+
 void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const PointSource& ps,
                const kumo_hip_options& opt, double* const* dout, long long q0, long long q1, const Advance& advance) {
     hipStream_t s = ex.stream;
@@ -460,6 +487,10 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             const long long per_block = static_cast<long long>(nao) * (ao_order == 0 ? 1 : (ao_order == 1 ? 4 : 10)) * bs;
             cap = static_cast<int>(std::max(1ll, std::min<long long>(cap, (512ll << 20) / (8 * per_block))));
         }
+        if (ex.max_batch_blocks > 0) cap = static_cast<int>(std::min<long long>(cap, ex.max_batch_blocks));
+        const double* sr2 = ex.no_alloc ? mol->screen_r2_resident(thr) : mol->screen_r2(thr, s);
+        if (!sr2) invalid("the allocation-free evaluation needs the screening radii of its threshold resident "
+                          "(an evaluation at that threshold through kumo_hip_evaluate_* makes them so)");
         const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks);
         const long long batch_max = static_cast<long long>(cap) * bs;
         int slot0 = 0, slot1 = 0;
@@ -502,7 +533,8 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             if (t.kind == kDensity) rho_which |= bits(t.component);
         }
         FieldTask* dtasks = grow<FieldTask>(ex.ws_tasks, tasks.size());
-        upload_async(ex, dtasks, tasks.data(), tasks.size());
+        if (ex.no_alloc) launch_store_tasks(tasks.data(), static_cast<int>(tasks.size()), dtasks, s);
+        else upload_async(ex, dtasks, tasks.data(), tasks.size());
         // Per batch: kept AO counts of its blocks, then its rescue tiles,
         // copied back asynchronously; read after one synchronisation at the
         // end (or after each batch when a progress callback must see
@@ -520,7 +552,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
             const long long count = std::min<long long>(static_cast<long long>(nb) * bs, q1 - first);
             KeptTables kt = ts.t;
             kt.lists = backend == KUMO_HIP_AO_MO ? 0 : 1;
-            launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, mol->screen_r2(thr, s), kt, s);
+            launch_masks(mv, ps, n, bs, b0, nb, std::max(0, ao_order), thr, sr2, kt, s);
             if (backend == KUMO_HIP_AO_MO) {
                 AoMoLaunch a{};
                 a.mol = mv;

@@ -511,6 +511,27 @@ void launch_rescue_scatter(const double* rescue_moments, long long rslots, int g
     KUMO_HIP(hipGetLastError());
 }
 
+// Field tasks passed by value (kernel arguments are part of the launch, so a
+// caller capturing a graph needs no host buffer to outlive the call).
+struct TaskPack {
+    FieldTask t[kMaxStoredTasks];
+};
+
+__global__ void store_tasks_kernel(TaskPack pack, int n, FieldTask* dst) {
+    const int i = threadIdx.x;
+    if (i < n) dst[i] = pack.t[i];
+}
+
+void launch_store_tasks(const FieldTask* tasks, int n, FieldTask* dst, hipStream_t stream) {
+    if (n <= 0) return;
+    if (n > kMaxStoredTasks) invalid("at most " + std::to_string(kMaxStoredTasks) + " point-path properties per call");
+    TaskPack pack{};
+    for (int i = 0; i < n; ++i)
+        pack.t[i] = tasks[i];
+    store_tasks_kernel<<<1, 64, 0, stream>>>(pack, n, dst);
+    KUMO_HIP(hipGetLastError());
+}
+
 void launch_add_into(double* dst, const double* src, long long n, hipStream_t stream) {
     if (n <= 0) return;
     add_kernel<<<static_cast<unsigned>((n + 255) / 256), 256, 0, stream>>>(dst, src, n);

@@ -274,6 +274,9 @@ void launch_esp(const EspPairDev* pairs, const std::vector<EspGroup>& groups, co
                 const EspAtomDev* atoms, int natoms, const double* boys_table, const PointSource& ps, long long first,
                 long long n, double* partial, double* out, hipStream_t stream);
 void launch_boys(const double* table, const double* T, long long n, int m, double* out, hipStream_t stream);
+// Copies n <= kMaxStoredTasks field tasks to dst through kernel arguments.
+constexpr int kMaxStoredTasks = 16;
+void launch_store_tasks(const FieldTask* tasks, int n, FieldTask* dst, hipStream_t stream);
 // The ESP pair loop's Boys function (orders 0..top, top <= 10) at n arguments.
 void launch_boys_esp(const double* table, const double* T, long long n, int top, double* out, hipStream_t stream);
 

@@ -301,6 +301,9 @@ std::unique_ptr<kumo_hip_molecule> load_molecule(kumo_hip_context* ctx, const ku
     if (d.e_beta && m->nmo_b) m->Eb.assign(d.e_beta, d.e_beta + m->nmo_b);
     assess(*m);
     upload_device(*m, ctx->exec.stream);
+    // the screening radii of the default threshold (1e-12) are resident from
+    // the start, so evaluations at that threshold allocate nothing for them
+    m->screen_r2(1e-12, ctx->exec.stream);
     return m;
 }
 
@@ -450,6 +453,12 @@ const double* kumo_hip_molecule::screen_r2(double threshold, hipStream_t stream)
         KUMO_HIP(hipStreamSynchronize(stream));
     }
     return it->second.as<double>();
+}
+
+const double* kumo_hip_molecule::screen_r2_resident(double threshold) {
+    std::lock_guard<std::mutex> guard(lazy);
+    const auto it = d_screen_r2.find(threshold);
+    return it == d_screen_r2.end() ? nullptr : it->second.as<double>();
 }
 
 // Primitive pairs of the exact ESP: P contracted with the Hermite expansion

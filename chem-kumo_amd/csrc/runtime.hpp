@@ -44,6 +44,16 @@ struct Executor {
     // copied to the device are kept in `keep` so that graph replays read them.
     bool capture = false;
     std::deque<std::vector<char>>* keep = nullptr;
+    // Allocation-free mode: every workspace is a piece of a fixed arena,
+    // batches hold at most max_batch_blocks screening blocks, and small host
+    // tables reach the device as kernel arguments (capture mode is implied).
+    bool no_alloc = false;
+    long long max_batch_blocks = 0;
+    void use_arena(Arena* arena) {
+        for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc, &ws_rescue,
+                                &ws_rmom, &ws_rscale, &ws_oe, &ws_oe_tasks, &ws_esp})
+            b->use_arena(arena);
+    }
     void create(int device, const double* table);
     void destroy();
     void reset_stats();
@@ -140,11 +150,17 @@ struct kumo_hip_molecule {
     }
     const double* p_matrix(int component, hipStream_t stream);
     const double* screen_r2(double threshold, hipStream_t stream);
+    // The squared screening radii for `threshold` if they are resident, else null.
+    const double* screen_r2_resident(double threshold);
     void ensure_esp(hipStream_t stream);
     void ensure_orbital_energy(hipStream_t stream);
 };
 
 namespace kumo {
+
+// Device address of the allocation-free entry point's scratch array on
+// `device`, looked up (hipGetSymbolAddress) on the first call per device.
+void* noalloc_scratch(int device);
 
 std::unique_ptr<kumo_hip_molecule> load_molecule(kumo_hip_context* ctx, const kumo_hip_molecule_desc& d);
 std::unique_ptr<kumo_hip_molecule> load_fchk(kumo_hip_context* ctx, const std::string& path);
@@ -181,6 +197,10 @@ Prepared prepare(const kumo_hip_molecule* mol, const PointsArg& pts, const kumo_
 // request into the device outputs dout[channel][point] (full-length arrays).
 void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const PointSource& ps,
                const kumo_hip_options& opt, double* const* dout, long long q0, long long q1, const Advance& advance);
+// Device bytes the point pass of run_range takes from its workspaces: `fixed`
+// whatever the batch, plus `per_block` for each screening block of a batch.
+void point_pass_bytes(const kumo_hip_molecule* mol, const Prepared& pr, const kumo_hip_options& opt,
+                      std::size_t& fixed, std::size_t& per_block);
 
 void evaluate(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const PointsArg& pts,
               const kumo_hip_property* props, std::size_t nprops, const kumo_hip_options& opt, double* const* outputs,

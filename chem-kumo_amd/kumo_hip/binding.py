@@ -75,7 +75,7 @@ class _RequestInfo(C.Structure):
 EXPORTS = ["kumo_hip_init", "kumo_hip_release", "kumo_hip_last_error", "kumo_hip_default_options",
            "kumo_hip_molecule_from_fchk", "kumo_hip_molecule_create", "kumo_hip_molecule_info_get",
            "kumo_hip_molecule_release", "kumo_hip_molecule_blocks", "kumo_hip_molecule_atoms",
-           "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
+           "kumo_hip_evaluate_grid", "kumo_hip_evaluate_points", "kumo_hip_evaluate_points_noalloc", "kumo_hip_ao_eval", "kumo_hip_screening_masks",
            "kumo_hip_boys", "kumo_hip_boys_esp", "kumo_hip_last_kernel_stats", "kumo_hip_last_rescue_stats", "kumo_hip_last_field_stats", "kumo_hip_batch_create",
            "kumo_hip_batch_run", "kumo_hip_batch_last_ms", "kumo_hip_batch_kernel_stats", "kumo_hip_batch_release", "kumo_hip_server_create",
            "kumo_hip_server_submit", "kumo_hip_server_wait", "kumo_hip_server_query", "kumo_hip_server_release",
@@ -117,6 +117,8 @@ def lib():
         L.kumo_hip_molecule_atoms.argtypes = [vp, vp, vp, vp]
         L.kumo_hip_evaluate_grid.argtypes = [vp, vp, C.POINTER(_Grid), vp, C.c_size_t, C.POINTER(_Options), vp, vp]
         L.kumo_hip_evaluate_points.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(_Options), vp, vp]
+        L.kumo_hip_evaluate_points_noalloc.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(_Options), vp,
+                                                       vp, vp]
         L.kumo_hip_ao_eval.argtypes = [vp, vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_screening_masks.argtypes = [vp, vp, vp, vp, C.c_size_t, C.c_int32, C.c_int32, C.c_double, vp]
         L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
@@ -497,6 +499,22 @@ class Molecule:
                                                 C.byref(used))
         self.ctx.check(rc)
         return outs, used.value
+
+    def evaluate_points_noalloc(self, props, points, npoints, outputs, stream=None, **opts):
+        """kumo_hip_evaluate_points_noalloc: points (3 * npoints doubles) and
+        outputs (one per component) are device pointers; the work is issued on
+        `stream` (a raw hipStream_t, None: the context's stream) and not
+        waited for. Returns the backend used."""
+        P = (_Property * max(1, len(props)))(*[_Property(*map(int, p)) for p in props])
+        o = options(device_outputs=True, **opts)
+        ptrs = [int(x) for x in outputs]
+        arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
+        used = C.c_int32(-1)
+        self.ctx.check(lib().kumo_hip_evaluate_points_noalloc(self.ctx.h, self.h, C.c_void_p(int(points)), int(npoints),
+                                                              P, len(props), C.byref(o), arr,
+                                                              C.c_void_p(int(stream) if stream else 0),
+                                                              C.byref(used)))
+        return used.value
 
     def evaluate_device(self, props, grid, outputs, **opts):
         """Evaluation into device buffers (torch tensors or raw pointers);

@@ -166,6 +166,26 @@ int kumo_hip_evaluate_points(kumo_hip_context* ctx, const kumo_hip_molecule* mol
                              size_t npoints, const kumo_hip_property* properties, size_t nproperties,
                              const kumo_hip_options* options, double* const* outputs, int32_t* backend_used);
 
+/* ---- allocation-free evaluation ----
+ * kumo_hip_evaluate_points without device memory allocation, for callers that
+ * cannot allocate: code under HIP graph capture, tight serving loops. No
+ * counterpart in the reference (it has no device memory). points (3 * npoints
+ * doubles) and the outputs are device pointers; the work is issued on
+ * `stream` (a hipStream_t; NULL: the context's stream) and the call returns
+ * without synchronising it. Density-family and derived-field properties only
+ * (KUMO_HIP_INVALID_ARGUMENT for MO amplitudes, the electrostatic potential
+ * and orbital-energy fields, and for more than 16 properties); the screening
+ * threshold must be the default 1e-12 or one an allocating evaluation of the
+ * molecule has used. Workspaces come from one 64 MiB device array per GPU,
+ * allocated with the library: points beyond one batch of it are evaluated in
+ * several batches, and a molecule whose single screening block does not fit
+ * is rejected with KUMO_HIP_INVALID_ARGUMENT. Calls that overlap on one GPU
+ * share that array and race; results equal kumo_hip_evaluate_points'. */
+int kumo_hip_evaluate_points_noalloc(kumo_hip_context* ctx, const kumo_hip_molecule* mol, const double* points,
+                                     size_t npoints, const kumo_hip_property* properties, size_t nproperties,
+                                     const kumo_hip_options* options, double* const* outputs, void* stream,
+                                     int32_t* backend_used);
+
 /* ---- kernel-level seams used by the parity tests ----
  * BasisEvaluator::evaluate_batch / evaluate_derivatives_batch
  * (include/chem/kumo/basis_evaluator.hpp:99-117): order 0, 1 or 2; out is
