@@ -459,8 +459,9 @@ def bench_config2(args, built, torch):
         "field_reductions": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "kernel_ms": st["fields_ms"],
                              "bytes": st["fields_bytes"],
                              "achieved": st["fields_bytes"] / (st["fields_ms"] * 1e-3) / 1e9 if st["fields_ms"] else None},
-        "timed_run": {"ms": st["ms"], "note": "the launch statistics come from one run without the graph, every "
-                                              "contraction launch bracketed by HIP events on its stream"},
+        "timed_run": {"ms": st["ms"], "note": "launch statistics of one run issued without the graph but, as in the "
+                                              "graph, with its streams' launches overlapping; every contraction launch "
+                                              "bracketed by HIP events on its stream"},
         "cpu_baseline": cpu,
     }
     for m in mols:
@@ -714,7 +715,18 @@ def main():
                     help="config 1 alone: screening blocks in the CPU sample (0: every block of the grid)")
     ap.add_argument("--extras", action="store_true",
                     help="config 1 alone: also the Cube writer, orbital-energy fields and surfaces")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device once (the calibration kernel of the HBM traffic counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
               "screening blocks or molecules across GPUs, each writing its slice of the fields, with no exchange)",

@@ -409,33 +409,32 @@ __device__ __attribute__((always_inline)) inline void radial_sums_rec(const Shel
 //   d/da (P R)      = P_a R0 + a P R1
 //   d2/da2 (P R)    = P_aa R0 + (2 a P_a + P) R1 + a^2 P R2
 //   d2/da db (P R)  = P_ab R0 + (b P_a + a P_b) R1 + a b P R2.
-template <int L, int ORDER, unsigned PSET, typename Emit>
-__device__ __attribute__((always_inline)) inline void cart_components(double x, double y, double z, const double (&R)[3],
-                                                                    Emit&& emit) {
+template <int L, int ORDER, unsigned PSET, typename T = double, typename Emit>
+__device__ __attribute__((always_inline)) inline void cart_components(T x, T y, T z, const T (&R)[3], Emit&& emit) {
 #pragma clang fp contract(fast)
     constexpr int NC = (L + 1) * (L + 2) / 2;
     constexpr int NP = popcount10(PSET);
-    const double c[3] = {x, y, z};
-    double pw[3][L + 1];
+    const T c[3] = {x, y, z};
+    T pw[3][L + 1];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        pw[a][0] = 1.0;
+        pw[a][0] = T(1);
 #pragma unroll
         for (int n = 1; n <= L; ++n)
             pw[a][n] = pw[a][n - 1] * c[a];
     }
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-        double f[3][3];
+        T f[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int q = kPow[L][k][a];
             f[a][0] = pw[a][q];
-            f[a][1] = q >= 1 ? q * pw[a][q >= 1 ? q - 1 : 0] : 0.0;
-            f[a][2] = q >= 2 ? q * (q - 1) * pw[a][q >= 2 ? q - 2 : 0] : 0.0;
+            f[a][1] = q >= 1 ? T(q) * pw[a][q >= 1 ? q - 1 : 0] : T(0);
+            f[a][2] = q >= 2 ? T(q * (q - 1)) * pw[a][q >= 2 ? q - 2 : 0] : T(0);
         }
-        const double P = f[0][0] * f[1][0] * f[2][0];
-        double v[NP];
+        const T P = f[0][0] * f[1][0] * f[2][0];
+        T v[NP];
 #pragma unroll
         for (int i = 0; i < NP; ++i) {
             const int cc = panel_component(PSET, i);
@@ -443,23 +442,23 @@ __device__ __attribute__((always_inline)) inline void cart_components(double x, 
                 v[i] = P * R[0];
             } else if (cc <= 3) {
                 const int a = cc - 1;
-                const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+                const T Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
                 v[i] = Pa * R[0] + c[a] * P * R[1];
             } else if (cc == 4 || cc == 7 || cc == 9) {
                 const int a = cc == 4 ? 0 : (cc == 7 ? 1 : 2);
-                const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
-                const double Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
-                v[i] = Paa * R[0] + (2.0 * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
+                const T Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+                const T Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
+                v[i] = Paa * R[0] + (T(2) * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
             } else if (cc == kPanelLap) {
                 // sum of the three above; sum_a c_a P_a = L P for a monomial of degree L
-                const double Plap = f[0][2] * f[1][0] * f[2][0] + f[0][0] * f[1][2] * f[2][0] + f[0][0] * f[1][0] * f[2][2];
-                v[i] = Plap * R[0] + (2 * L + 3) * P * R[1] + (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) * P * R[2];
+                const T Plap = f[0][2] * f[1][0] * f[2][0] + f[0][0] * f[1][2] * f[2][0] + f[0][0] * f[1][0] * f[2][2];
+                v[i] = Plap * R[0] + T(2 * L + 3) * P * R[1] + (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) * P * R[2];
             } else {
                 const int a = cc == 8 ? 1 : 0, b = cc == 5 ? 1 : 2;
-                const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
-                const double Pb = f[0][0] * (b == 1 ? f[1][1] : f[1][0]) * (b == 2 ? f[2][1] : f[2][0]);
-                const double Pab = (a == 0 ? f[0][1] : f[0][0]) * ((a == 1 || b == 1) ? f[1][1] : f[1][0]) *
-                                   (b == 2 ? f[2][1] : f[2][0]);
+                const T Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
+                const T Pb = f[0][0] * (b == 1 ? f[1][1] : f[1][0]) * (b == 2 ? f[2][1] : f[2][0]);
+                const T Pab = (a == 0 ? f[0][1] : f[0][0]) * ((a == 1 || b == 1) ? f[1][1] : f[1][0]) *
+                              (b == 2 ? f[2][1] : f[2][0]);
                 v[i] = Pab * R[0] + (c[b] * Pa + c[a] * Pb) * R[1] + c[a] * c[b] * P * R[2];
             }
         }
@@ -471,40 +470,37 @@ __device__ __attribute__((always_inline)) inline void cart_components(double x, 
 // runtime values (a loop over components instead of an unrolled body): for
 // the rare high-l blocks and partially kept pure blocks, where unrolling every
 // component would cost registers in every other path of a kernel.
-template <int ORDER, unsigned PSET>
-__device__ __attribute__((always_inline)) inline void cart_component_pw(const int (&qq)[3], double x, double y,
-                                                                      double z, const double (&R)[3],
-                                                                      double (&v)[popcount10(PSET)]);
+template <int ORDER, unsigned PSET, typename T = double>
+__device__ __attribute__((always_inline)) inline void cart_component_pw(const int (&qq)[3], T x, T y, T z,
+                                                                      const T (&R)[3], T (&v)[popcount10(PSET)]);
 
-template <int ORDER, unsigned PSET>
-__device__ __attribute__((always_inline)) inline void cart_component_rt(int L, int k, double x, double y, double z,
-                                                                      const double (&R)[3],
-                                                                      double (&v)[popcount10(PSET)]) {
+template <int ORDER, unsigned PSET, typename T = double>
+__device__ __attribute__((always_inline)) inline void cart_component_rt(int L, int k, T x, T y, T z, const T (&R)[3],
+                                                                      T (&v)[popcount10(PSET)]) {
     const int qq[3] = {kPow[L][k][0], kPow[L][k][1], kPow[L][k][2]};
-    cart_component_pw<ORDER, PSET>(qq, x, y, z, R, v);
+    cart_component_pw<ORDER, PSET, T>(qq, x, y, z, R, v);
 }
 
 // The component with powers (qq[0], qq[1], qq[2]) as runtime values.
-template <int ORDER, unsigned PSET>
-__device__ __attribute__((always_inline)) inline void cart_component_pw(const int (&qq)[3], double x, double y,
-                                                                      double z, const double (&R)[3],
-                                                                      double (&v)[popcount10(PSET)]) {
+template <int ORDER, unsigned PSET, typename T>
+__device__ __attribute__((always_inline)) inline void cart_component_pw(const int (&qq)[3], T x, T y, T z,
+                                                                      const T (&R)[3], T (&v)[popcount10(PSET)]) {
 #pragma clang fp contract(fast)
     constexpr int NP = popcount10(PSET);
-    const double c[3] = {x, y, z};
-    double f[3][3];
+    const T c[3] = {x, y, z};
+    T f[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int q = qq[a];
-        double p2 = 1.0; // c^(q-2)
+        T p2 = T(1); // c^(q-2)
         for (int n = 2; n < q; ++n)
             p2 *= c[a];
-        const double p1 = q >= 2 ? p2 * c[a] : 1.0; // c^(q-1)
-        f[a][0] = q >= 1 ? p1 * c[a] : 1.0;
-        f[a][1] = q >= 1 ? q * p1 : 0.0;
-        f[a][2] = q >= 2 ? q * (q - 1) * p2 : 0.0;
+        const T p1 = q >= 2 ? p2 * c[a] : T(1); // c^(q-1)
+        f[a][0] = q >= 1 ? p1 * c[a] : T(1);
+        f[a][1] = q >= 1 ? T(q) * p1 : T(0);
+        f[a][2] = q >= 2 ? T(q * (q - 1)) * p2 : T(0);
     }
-    const double P = f[0][0] * f[1][0] * f[2][0];
+    const T P = f[0][0] * f[1][0] * f[2][0];
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
         const int cc = panel_component(PSET, i);
@@ -512,24 +508,24 @@ __device__ __attribute__((always_inline)) inline void cart_component_pw(const in
             v[i] = P * R[0];
         } else if (cc <= 3) {
             const int a = cc - 1;
-            const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+            const T Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
             v[i] = Pa * R[0] + c[a] * P * R[1];
         } else if (cc == 4 || cc == 7 || cc == 9) {
             const int a = cc == 4 ? 0 : (cc == 7 ? 1 : 2);
-            const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
-            const double Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
-            v[i] = Paa * R[0] + (2.0 * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
+            const T Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * (a == 2 ? f[2][1] : f[2][0]);
+            const T Paa = (a == 0 ? f[0][2] : f[0][0]) * (a == 1 ? f[1][2] : f[1][0]) * (a == 2 ? f[2][2] : f[2][0]);
+            v[i] = Paa * R[0] + (T(2) * c[a] * Pa + P) * R[1] + c[a] * c[a] * P * R[2];
         } else if (cc == kPanelLap) {
             // sum of the three above; sum_a c_a P_a = (qx + qy + qz) P
-            const double Plap = f[0][2] * f[1][0] * f[2][0] + f[0][0] * f[1][2] * f[2][0] + f[0][0] * f[1][0] * f[2][2];
-            v[i] = Plap * R[0] + (2 * (qq[0] + qq[1] + qq[2]) + 3) * P * R[1] +
+            const T Plap = f[0][2] * f[1][0] * f[2][0] + f[0][0] * f[1][2] * f[2][0] + f[0][0] * f[1][0] * f[2][2];
+            v[i] = Plap * R[0] + T(2 * (qq[0] + qq[1] + qq[2]) + 3) * P * R[1] +
                    (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) * P * R[2];
         } else {
             const int a = cc == 8 ? 1 : 0, b = cc == 5 ? 1 : 2;
-            const double Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
-            const double Pb = f[0][0] * (b == 1 ? f[1][1] : f[1][0]) * (b == 2 ? f[2][1] : f[2][0]);
-            const double Pab = (a == 0 ? f[0][1] : f[0][0]) * ((a == 1 || b == 1) ? f[1][1] : f[1][0]) *
-                               (b == 2 ? f[2][1] : f[2][0]);
+            const T Pa = (a == 0 ? f[0][1] : f[0][0]) * (a == 1 ? f[1][1] : f[1][0]) * f[2][0];
+            const T Pb = f[0][0] * (b == 1 ? f[1][1] : f[1][0]) * (b == 2 ? f[2][1] : f[2][0]);
+            const T Pab = (a == 0 ? f[0][1] : f[0][0]) * ((a == 1 || b == 1) ? f[1][1] : f[1][0]) *
+                          (b == 2 ? f[2][1] : f[2][0]);
             v[i] = Pab * R[0] + (c[b] * Pa + c[a] * Pb) * R[1] + c[a] * c[b] * P * R[2];
         }
     }

@@ -34,6 +34,10 @@ struct kumo_hip_batch {
     // the executors.
     double kernel_ms = 0.0, flops = 0.0, rescue_ms = 0.0, fields_ms = 0.0;
     long long launches = 0, rescue_points = 0, fields_bytes = 0;
+    // Algorithmic FLOPs and rescued points of one run, read back in the first
+    // (uncaptured) run: they do not change between runs.
+    double warm_flops = 0.0;
+    long long warm_rescue_points = 0;
 
     ~kumo_hip_batch() {
         if (!execs.empty() && execs[0]->stream) (void)hipStreamSynchronize(execs[0]->stream);
@@ -165,8 +169,11 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
             // Sizes every workspace and builds the molecules' lazily made
             // tables outside any capture.
             b->issue(false, nullptr);
-            for (auto& e : b->execs)
+            for (auto& e : b->execs) {
                 KUMO_HIP(hipStreamSynchronize(e->stream));
+                b->warm_flops += e->last_flops;
+                b->warm_rescue_points += e->last_rescue_points;
+            }
             b->warmed = true;
         }
         if (use_graph == 1 && !b->graph_exec) {
@@ -183,24 +190,35 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
             KUMO_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
         }
         if (use_graph == 2) {
-            // Timed run: every launch issued again without capture, so each
-            // contraction, rescue and field launch is bracketed by events on
-            // its stream and the kept-AO counts (the algorithmic flops) are
-            // read back; the items of a stream then run one after another.
+            // Timed run: every launch issued again without a graph but, as
+            // in the graph, with no host synchronisation, so the streams'
+            // launches overlap as they do on replay; each contraction, rescue
+            // and field launch is bracketed by events on its stream. The
+            // FLOPs and rescued points are those read back in the first run.
+            b->run_keep.clear();
+            for (auto& e : b->execs)
+                e->events_in_capture = true;
             KUMO_HIP(hipEventRecord(b->t0, s0));
-            b->issue(false, nullptr);
+            try {
+                b->issue(true, &b->run_keep);
+            } catch (...) {
+                for (auto& e : b->execs)
+                    e->events_in_capture = false;
+                throw;
+            }
             KUMO_HIP(hipEventRecord(b->t1, s0));
             KUMO_HIP(hipEventSynchronize(b->t1));
-            b->kernel_ms = b->flops = b->rescue_ms = b->fields_ms = 0.0;
-            b->launches = b->rescue_points = b->fields_bytes = 0;
+            b->kernel_ms = b->rescue_ms = b->fields_ms = 0.0;
+            b->launches = b->fields_bytes = 0;
+            b->flops = b->warm_flops;
+            b->rescue_points = b->warm_rescue_points;
             for (auto& e : b->execs) {
+                e->events_in_capture = false;
                 KUMO_HIP(hipStreamSynchronize(e->stream));
                 e->collect_timing();
                 b->kernel_ms += e->last_kernel_ms;
-                b->flops += e->last_flops;
                 b->launches += e->last_launches;
                 b->rescue_ms += e->last_rescue_ms;
-                b->rescue_points += e->last_rescue_points;
                 b->fields_ms += e->last_fields_ms;
                 b->fields_bytes += e->last_fields_bytes;
             }

@@ -206,7 +206,7 @@ hipEvent_t next_event(Executor& ex, bool start, int kind = 0) {
 }
 
 void record(Executor& ex, bool start, int kind) {
-    if (!ex.capture) KUMO_HIP(hipEventRecord(next_event(ex, start, kind), ex.stream));
+    if (!ex.capture || ex.events_in_capture) KUMO_HIP(hipEventRecord(next_event(ex, start, kind), ex.stream));
 }
 
 // Relative error of the channel moments assumed by the ELF rescue, per
@@ -225,14 +225,18 @@ double rescue_eps(int precision) {
 // 1e-6 the largest error left on the 640-AO cube was 1.8e-7).
 constexpr double kElfRescueTol = 3e-6;
 
-// Density rescue: rounding unit of the contraction (error of channel c about
-// eps sum_i |Psi_i| sum_k |C_ik Phi_k|) for the precisions whose operands
-// carry 24 bits; BF16X3 (16-bit operands) has no density rescue. Over the
-// 4.1e6 points of a 640-AO UHF molecule on a 160^3 cube the BF16X6 error
-// stayed below 1.05 x 2^-24 times the bound (99.99 % below 0.64 x), and with
-// eps = 0.4 x 2^-24 no unflagged point kept a relative error above 7e-7
-// while 5.8 % of the points were flagged.
-constexpr double kRhoEpsBf16x6 = 0.4 / 16777216.0, kRhoEpsFp32 = 0.9 / 16777216.0;
+// Density rescue: unit of the contraction's rounding for the precisions
+// whose operands carry 24 bits (BF16X3, with 16-bit operands, has no density
+// rescue). The kernels accumulate, per point and channel, the scale
+// s_c = sqrt(sum_k Phi_k^2 ||C_k,c||^2): the bf16 MFMA rounds each 8-term K
+// group to about 24 bits with random sign, so the error of Psi_i is about
+// 2^-24 sqrt(sum_k (C_ik Phi_k)^2) and that of rho_c about
+// 2 eps sqrt(rho_c) s_c (the sum over k with absolute values, the worst case,
+// overestimated it by up to the square root of the kept rows). Calibrated on
+// the 1.66e8 points of the 256-molecule batch against the FP64 path:
+// eps = 1.0 x 2^-24 leaves no unflagged point with a relative density error
+// above 7.1e-7 (0.7 x 2^-24: 8.3e-7; 0.4 x 2^-24: 1.5e-6).
+constexpr double kRhoEpsBf16x6 = 1.0 / 16777216.0, kRhoEpsFp32 = 2.25 / 16777216.0;
 double rho_rescue_eps(int precision) {
     switch (precision) {
     case KUMO_HIP_FP32: return kRhoEpsFp32;
@@ -243,17 +247,16 @@ double rho_rescue_eps(int precision) {
 constexpr double kRhoRescueTol = 5e-7; // largest relative density error left to the contraction
 
 // ELF rescue from per-point error scales (BF16X6): the unit e of the error
-// model, calibrated. The chain of bounds (Cauchy-Schwarz over the MOs, the
-// triangle inequality over the operand rows) overestimates the contraction's
-// rounding error, which sums with random signs, by about two orders of
-// magnitude. Over the 4.1e6 points of a 640-AO UHF molecule on a 160^3 cube
-// (ELF rescue alone) e = 1e-9 flagged 7.2 % of the points and left no ELF
-// error above 1.5e-6, half the 3e-6 the contraction may leave (e = 3e-9:
-// 9.2 %, 7.8e-7), where the model over tau + |grad rho|^2 / 8 rho flagged
-// 13.5 % and left 1.04e-6; with the density rescue the largest ELF error was
-// 6.8e-7 on that cube and 1.1e-6 over a 256-molecule batch. Other precisions
-// use that model (rescue_eps).
-double elf_scale_eps(int precision) { return precision == KUMO_HIP_BF16X6 ? 1e-9 : 0.0; }
+// model, calibrated. The model bounds each channel's errors of rho, |grad rho|
+// and tau from the density scale s_c and the gradient scale
+// w_c = sqrt(sum_k |grad Phi_k|^2 ||C_k,c||^2) (Cauchy-Schwarz over the MOs)
+// and carries the intervals through D = tau - |grad rho|^2 / 8 rho to ELF;
+// the bounds compound, so e is fitted: over the 256-molecule batch (1.66e8
+// points, with the density rescue at 1.0 x 2^-24) e = 3e-10 leaves no ELF
+// error above 2.3e-6 where rho >= 1e-14 and flags, with the density, 7.6 % of
+// the points (e = 1e-9: 1.1e-6 and 8.9 %; e = 1e-10: 4.5e-6 and 6.7 %).
+// Other precisions use the model of rescue_eps.
+double elf_scale_eps(int precision) { return precision == KUMO_HIP_BF16X6 ? 3e-10 : 0.0; }
 
 // Autotuner of the AO->MO tile. The points per workgroup (16 or 32) change
 // only how points are shared among workgroups (every point's arithmetic and

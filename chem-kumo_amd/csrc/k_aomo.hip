@@ -22,10 +22,10 @@
 // The epilogue forms the per-point dot products over MOs in FP64 (value,
 // gradient, tau terms, Hessian) and reduces them over the tiles of each
 // spin channel in a fixed order, so the result is independent of batching.
-// Reduced precisions also sum, per point and channel, |Phi_k| times the norm
-// of coefficient row k over the channel's MOs, and (with gradient panels)
-// |d/dx Phi_k| + |d/dy Phi_k| + |d/dz Phi_k| times the same norm: the scales
-// of the contraction's rounding error that the rescue tests.
+// Reduced precisions also form, per point and channel, the 2-norm over the
+// operand rows k of Phi_k times the norm of coefficient row k over the
+// channel's MOs, and (with gradient panels) of |grad Phi_k| times the same
+// norm: the scales of the contraction's rounding error that the rescue tests.
 //
 // MFMA operand layouts (wave64): for 16x16xK the lane l holds A row l % 16
 // and B column l % 16 at a K slice selected by l / 16; accumulator register r
@@ -67,6 +67,7 @@ constexpr int KC = kChunkRows; // 32 operand rows per chunk
 template <int PREC> struct Prec;
 template <> struct Prec<KUMO_HIP_FP64> {
     using E = double;
+    using G = double; // arithmetic of the Phi components
     using Acc = f64x4;
     static constexpr int S = 1;       // split planes
     static constexpr int T = 16;      // points per workgroup
@@ -75,6 +76,7 @@ template <> struct Prec<KUMO_HIP_FP64> {
 };
 template <> struct Prec<KUMO_HIP_FP32> {
     using E = float;
+    using G = double;
     using Acc = f32x4;
     static constexpr int S = 1;
     static constexpr int LDK = 36;
@@ -82,6 +84,7 @@ template <> struct Prec<KUMO_HIP_FP32> {
 };
 template <int NSPLIT> struct PrecBf16 {
     using E = uint16_t;
+    using G = double;
     using Acc = f32x4;
     static constexpr int S = NSPLIT;
     static constexpr int LDK = 32;    // swizzled 8-row granules
@@ -133,6 +136,29 @@ __device__ __attribute__((always_inline)) inline void split_pair(double v0, doub
     for (int k = 1; k < S; ++k) {
         t[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
         if (k + 1 < S) r = f32x2{r.x - __uint_as_float(t[k] << 16), r.y - __uint_as_float(t[k] & 0xffff0000u)};
+    }
+}
+
+// Splits two FP32 values into S bf16 terms, packed in pairs (element 0 in
+// the low half): each term is the bf16 rounding of what the previous terms
+// leave, every residual exact in FP32; three terms carry the 24 bits.
+template <int S>
+__device__ __attribute__((always_inline)) inline void split_pair_f32(float v0, float v1, uint32_t (&t)[S]) {
+    f32x2 r = {v0, v1};
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        t[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+        if (k + 1 < S) r = f32x2{r.x - __uint_as_float(t[k] << 16), r.y - __uint_as_float(t[k] & 0xffff0000u)};
+    }
+}
+template <int S>
+__device__ __attribute__((always_inline)) inline void split_f32(float v, uint16_t (&t)[S]) {
+    float r = v;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const __bf16 h = static_cast<__bf16>(r);
+        t[k] = __builtin_bit_cast(uint16_t, h);
+        if (k + 1 < S) r -= static_cast<float>(h);
     }
 }
 
@@ -225,26 +251,59 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     constexpr int GMAX = 128 / EPG;                        // granules of a 128-slot row (power of two)
     const int G = width / EPG;                             // granules per coefficient row
 
+    // The chunk tables a thread reads (operand rows of its staging items, its
+    // Phi rows' operand rows and row descriptors, the chunk's used rows) are
+    // fetched one chunk ahead, before the MFMA phase, so that their load
+    // latency is not on the next chunk's path.
+    constexpr int NSTAGE = is_bf16<PREC> ? S * (KC / 2) * GMAX / 256 : S * KC * GMAX / 256;
+    constexpr int RPL = KC * T / 256;
+    const int r0 = (tid / T) * RPL;
+    int pg0[NSTAGE], pg1[NSTAGE], prow_g[RPL], prow_info[RPL], next_rows_used = 0;
+    const auto fetch = [&](int cc) __attribute__((always_inline)) {
+        const int32_t* crc = crows + 2 * cc * KC;
+#pragma unroll
+        for (int u = 0; u < NSTAGE; ++u) {
+            const int item = tid + 256 * u;
+            if constexpr (is_bf16<PREC>) {
+                const int q = (item / GMAX) % (KC / 2);
+                pg0[u] = crc[4 * q];
+                pg1[u] = crc[4 * q + 2];
+            } else {
+                pg0[u] = crc[2 * ((item / GMAX) % KC)];
+                pg1[u] = -1;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            prow_g[i] = crc[2 * (r0 + i)];
+            prow_info[i] = crc[2 * (r0 + i) + 1];
+        }
+        const int e1n = cb[cc + 1];
+        const int last = ent[e1n - 1];
+        const DevShell& ls = m.shells[last & 0xffffff];
+        next_rows_used = erow[e1n - 1] + ((last >> 24) == kEntryPure ? ls.count : ls.ncart);
+    };
+    if (nchunks > 0) fetch(0);
+
     for (int c = 0; c < nchunks; ++c) {
-        const int e1 = cb[c + 1];
-        int rows_used;
-        {
-            const int last = ent[e1 - 1];
-            const DevShell& ls = m.shells[last & 0xffffff];
-            rows_used = erow[e1 - 1] + ((last >> 24) == kEntryPure ? ls.count : ls.ncart);
+        const int rows_used = next_rows_used;
+        int g_row[RPL], info_row[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            g_row[i] = prow_g[i];
+            info_row[i] = prow_info[i];
         }
         // 1. Coefficient rows of the chunk (zero past its rows).
-        const int32_t* cr = crows + 2 * c * KC;
         if constexpr (is_bf16<PREC>) {
             // rows 2q and 2q + 1, 8 slots each, interleaved into 32-bit words
 #pragma unroll
-            for (int u = 0; u < S * (KC / 2) * GMAX / 256; ++u) {
+            for (int u = 0; u < NSTAGE; ++u) {
                 const int item = tid + 256 * u;
                 const int gi = item % GMAX;
                 const int q = (item / GMAX) % (KC / 2);
                 const int sp = item / ((KC / 2) * GMAX);
                 if (gi >= G) continue;
-                const int g0 = cr[4 * q], g1 = cr[4 * q + 2];
+                const int g0 = pg0[u], g1 = pg1[u];
                 const E* src = op + sp * op_plane + slot_lo + gi * EPG;
                 uint4 v0 = uint4{0, 0, 0, 0}, v1 = uint4{0, 0, 0, 0};
                 if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
@@ -258,13 +317,13 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < S * KC * GMAX / 256; ++u) {
+            for (int u = 0; u < NSTAGE; ++u) {
                 const int item = tid + 256 * u;
                 const int gi = item % GMAX;
                 const int r = (item / GMAX) % KC;
                 const int sp = item / (KC * GMAX);
                 if (gi >= G) continue;
-                const int grow = cr[2 * r];
+                const int grow = pg0[u];
                 uint4 v = uint4{0, 0, 0, 0};
                 if (grow >= 0)
                     v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
@@ -273,29 +332,37 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
             }
         }
         // 2. Phi: thread (point j, row group) evaluates RPL consecutive rows of
-        //    the chunk at its point (rows past the chunk's end are zero).
+        //    the chunk at its point (rows past the chunk's end are zero), in
+        //    GT arithmetic (FP64: forming the components in FP32 would double
+        //    the reduced precisions' error and the FP64 rescue they need).
         {
-            constexpr int RPL = KC * T / 256;
-            const int r0 = (tid / T) * RPL;
+            using GT = typename P::G;
             int cached = -1;
             bool inside = false;
-            double R[3] = {0.0, 0.0, 0.0}, x = 0.0, y = 0.0, z = 0.0;
-            const auto eval_row = [&](int i, double (&v)[NP]) __attribute__((always_inline)) {
+            GT R[3] = {GT(0), GT(0), GT(0)}, x = GT(0), y = GT(0), z = GT(0);
+            const auto eval_row = [&](int i, GT (&v)[NP]) __attribute__((always_inline)) {
 #pragma unroll
                 for (int p = 0; p < NP; ++p)
-                    v[p] = 0.0;
-                const int info = cr[2 * (r0 + i) + 1];
+                    v[p] = GT(0);
+                const int info = info_row[i];
                 if (info < 0 || j >= nt_valid) return;
                 const int sidx = info >> 15;
                 const ShellRec& rec = m.rec[sidx];
                 if (sidx != cached) {
                     cached = sidx;
-                    x = px - rec.o[0];
-                    y = py - rec.o[1];
-                    z = pz - rec.o[2];
-                    const double r2 = sq_norm(x, y, z);
+                    const double dx = px - rec.o[0], dy = py - rec.o[1], dz = pz - rec.o[2];
+                    const double r2 = sq_norm(dx, dy, dz);
                     inside = r2 <= rec.cutoff_r2;
-                    if (inside) radial_sums_rec<ORDER>(rec, m, r2, R);
+                    if (inside) {
+                        double R64[3];
+                        radial_sums_rec<ORDER>(rec, m, r2, R64);
+#pragma unroll
+                        for (int q = 0; q < 3; ++q)
+                            R[q] = static_cast<GT>(R64[q]);
+                    }
+                    x = static_cast<GT>(dx);
+                    y = static_cast<GT>(dy);
+                    z = static_cast<GT>(dz);
                 }
                 if (!inside) return;
                 if (((info >> 13) & 3) == kEntryPure) {
@@ -303,10 +370,10 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                     const int L = rec.l, kp = (info >> 9) & 15;
                     const double* tg = m.tg + (L * kMaxPure + kp) * kMaxCart;
                     for (int c = 0; c < (L + 1) * (L + 2) / 2; ++c) {
-                        const double w = tg[c];
-                        if (w == 0.0) continue;
-                        double cv[NP];
-                        cart_component_rt<ORDER, PSET>(L, c, x, y, z, R, cv);
+                        const GT w = static_cast<GT>(tg[c]);
+                        if (w == GT(0)) continue;
+                        GT cv[NP];
+                        cart_component_rt<ORDER, PSET, GT>(L, c, x, y, z, R, cv);
 #pragma unroll
                         for (int p = 0; p < NP; ++p)
                             v[p] += w * cv[p];
@@ -314,21 +381,23 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                     return;
                 }
                 const int qq[3] = {(info >> 6) & 7, (info >> 3) & 7, info & 7};
-                cart_component_pw<ORDER, PSET>(qq, x, y, z, R, v);
+                cart_component_pw<ORDER, PSET, GT>(qq, x, y, z, R, v);
             };
-            const auto add_scale = [&](int i, const double (&v)[NP]) __attribute__((always_inline)) {
+            const auto add_scale = [&](int i, const GT (&v)[NP]) __attribute__((always_inline)) {
                 if (!scale) return;
-                const int g = cr[2 * (r0 + i)];
+                const int g = g_row[i];
                 if (g < 0) return;
                 const float2 w = a.mo.norm[g];
-                const float av = fabsf(static_cast<float>(v[0]));
-                es0 += av * w.x;
-                es1 += av * w.y;
+                const float av = static_cast<float>(v[0]);
+                const float wx2 = w.x * w.x, wy2 = w.y * w.y;
+                es0 += (av * av) * wx2;
+                es1 += (av * av) * wy2;
                 if constexpr (kGradScale) {
-                    const float ag = fabsf(static_cast<float>(v[1])) + fabsf(static_cast<float>(v[2])) +
-                                     fabsf(static_cast<float>(v[3]));
-                    eg0 += ag * w.x;
-                    eg1 += ag * w.y;
+                    const float gx = static_cast<float>(v[1]), gy = static_cast<float>(v[2]),
+                                gz = static_cast<float>(v[3]);
+                    const float ag2 = gx * gx + gy * gy + gz * gz;
+                    eg0 += ag2 * wx2;
+                    eg1 += ag2 * wy2;
                 }
             };
             constexpr int PLANE = NP * T * LDK;
@@ -338,7 +407,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                 E* db = phi + phi_bf16_offset(j, r0);
 #pragma unroll
                 for (int pi = 0; pi < RPL / 2; ++pi) {
-                    double va[NP], vb[NP];
+                    GT va[NP], vb[NP];
                     eval_row(2 * pi, va);
                     eval_row(2 * pi + 1, vb);
                     add_scale(2 * pi, va);
@@ -346,7 +415,8 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
                         uint32_t tt[S];
-                        split_pair<S>(va[p], vb[p], tt);
+                        if constexpr (sizeof(GT) == 4) split_pair_f32<S>(va[p], vb[p], tt);
+                        else split_pair<S>(va[p], vb[p], tt);
                         E* d = db + p * T * LDK + 2 * pi;
 #pragma unroll
                         for (int k = 0; k < S; ++k)
@@ -358,7 +428,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                 float st[RPL][NP];
 #pragma unroll
                 for (int i = 0; i < RPL; ++i) {
-                    double v[NP];
+                    GT v[NP];
                     eval_row(i, v);
                     add_scale(i, v);
 #pragma unroll
@@ -374,7 +444,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                 }
             } else {
                 static_assert(RPL == 2, "two FP64 rows per lane");
-                double va[NP], vb[NP];
+                GT va[NP], vb[NP];
                 eval_row(0, va);
                 eval_row(1, vb);
 #pragma unroll
@@ -382,6 +452,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                     *reinterpret_cast<double2*>(d0 + p * T * LDK) = double2{va[p], vb[p]};
             }
         }
+        if (c + 1 < nchunks) fetch(c + 1); // in flight during the MFMA phase
         __syncthreads();
         // 3. MFMA over the chunk's rows. The wave's tiles share one 16-point
         //    column nt and take MO tiles mt = (wave + 4 t) / NT.
@@ -603,8 +674,8 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
                 e += esl[4 * q + ch];
                 eg += esl[4 * q + 2 + ch];
             }
-            a.rho_scale[ch * a.batch_points + point] = e;
-            if (kGradScale) a.rho_scale[(2 + ch) * a.batch_points + point] = eg;
+            a.rho_scale[ch * a.batch_points + point] = sqrtf(e);
+            if (kGradScale) a.rho_scale[(2 + ch) * a.batch_points + point] = sqrtf(eg);
         }
     }
 }

@@ -44,6 +44,9 @@ struct Executor {
     // copied to the device are kept in `keep` so that graph replays read them.
     bool capture = false;
     std::deque<std::vector<char>>* keep = nullptr;
+    // Timing events even in capture mode (an eager issue with no host
+    // synchronisation, whose launches overlap across streams as in a graph).
+    bool events_in_capture = false;
     // Allocation-free mode: every workspace is a piece of a fixed arena,
     // batches hold at most max_batch_blocks screening blocks, and small host
     // tables reach the device as kernel arguments (capture mode is implied).

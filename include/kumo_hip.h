@@ -231,13 +231,15 @@ int kumo_hip_batch_run(kumo_hip_batch* batch, int32_t use_graph);
 /* Device time (ms) of the last run, measured with HIP events around it. */
 int kumo_hip_batch_last_ms(const kumo_hip_batch* batch, double* ms);
 /* kumo_hip_batch_run with use_graph = 2 issues every launch again without a
- * graph, bracketing each contraction, FP64 rescue and field-reduction launch
- * with HIP events on its stream and reading back the kept-AO counts (items of
- * one stream then run one after another). Afterwards this returns, summed
- * over the run: the contraction kernels' device time (ms), their launches and
+ * graph but, as in the graph, without host synchronisation (the streams'
+ * launches overlap as on replay), bracketing each contraction, FP64 rescue
+ * and field-reduction launch with HIP events on its stream. Afterwards this
+ * returns, summed over the run: the contraction kernels' device time (ms,
+ * each launch from its start to its end on its stream), their launches and
  * algorithmic FLOPs (2 x kept AOs x MOs x panels x points per screening
- * block), the rescue launches' time and point slots, and the field
- * reductions' time and bytes moved. Measurement only. */
+ * block, read back in the batch's first run), the rescue launches' time and
+ * point slots, and the field reductions' time and bytes moved. Measurement
+ * only. */
 int kumo_hip_batch_kernel_stats(const kumo_hip_batch* batch, double* kernel_ms, int64_t* launches, double* flops,
                                 double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes);
 int kumo_hip_batch_release(kumo_hip_batch* batch);
