@@ -97,13 +97,16 @@ def _config4_molecule(arg):
                 seed += 100000
 
 
-def parallel_map(fn, items):
+def parallel_map(fn, items, spawn=False):
     """Builds synthetic molecules in worker processes (at most 16, the host
-    share of one GPU); called before any GPU work in this process."""
+    share of one GPU). Forked workers need a parent that has not touched the
+    GPU yet; spawn=True starts fresh interpreters instead (for callers that
+    already use the GPU)."""
     from concurrent.futures import ProcessPoolExecutor
     import multiprocessing
     workers = max(1, min(16, len(items), (os.cpu_count() or 4)))
-    with ProcessPoolExecutor(max_workers=workers, mp_context=multiprocessing.get_context("fork")) as ex:
+    ctx = multiprocessing.get_context("spawn" if spawn else "fork")
+    with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
         return list(ex.map(fn, items, chunksize=1))
 
 

@@ -101,8 +101,10 @@ KUMO_HD inline int compare(const BigInt& a, const BigInt& b) {
 }
 
 // Sign of M 2^E 10^j - (f + 1/2), exactly: compares 2 M 2^(E + j) 5^j with
-// 2 f + 1, negative powers moved to the other side.
-__attribute__((noinline)) KUMO_HD inline int half_compare(uint64_t M, int E, int j, uint64_t f) {
+// 2 f + 1, negative powers moved to the other side. Inlined like every device
+// function here (its big integers then live in the kernel's own private
+// segment; the path is taken only for ties and near ties).
+__attribute__((always_inline)) KUMO_HD inline int half_compare(uint64_t M, int E, int j, uint64_t f) {
     BigInt a, b;
     a.set(2 * M);
     b.set(2 * f + 1);
