@@ -1376,51 +1376,572 @@ double esp_nuclear(const Esp& e, const double* r) {
     return sum;
 }
 
-// hermite_integrals + hermite_recursion + pair_potential (src/electrostatic_potential.cpp:269-338).
-double esp_electronic(const Esp& e, const double* r) {
-    const int side = e.max_L + 1;
-    const std::size_t cube = static_cast<std::size_t>(side * side * side);
-    std::vector<double> prev(cube, 0.0), cur(cube, 0.0), seeds(static_cast<std::size_t>(side));
-    const auto all = triples(e.max_L);
-    const auto off = [&](int t, int u, int v) { return static_cast<std::size_t>((t * side + u) * side + v); };
-    double total = 0.0;
-    for (const auto& pr : e.pairs) {
-        const double pc[3] = {pr.Q[0] - r[0], pr.Q[1] - r[1], pr.Q[2] - r[2]};
-        const int L = pr.L;
-        boys(pr.p * sqnorm(pc[0], pc[1], pc[2]), L, seeds.data());
-        double power = 1.0;
-        for (int n = 1; n <= L; ++n) {
-            power *= -2.0 * pr.p;
-            seeds[static_cast<std::size_t>(n)] *= power;
+std::size_t hcount(int s) { return static_cast<std::size_t>((s + 1) * (s + 2) * (s + 3) / 6); }
+
+// Work arrays of the Hermite recursion: two cubic tables of `side` entries
+// per index, the seeds R^n_000 and the (t, u, v) in order of t + u + v.
+struct HermiteWork {
+    int side = 0;
+    std::vector<double> prev, cur, seeds;
+    std::vector<Triple> all;
+    void prepare(int side_, int top) {
+        if (side == side_ && all.size() >= hcount(top)) return;
+        side = side_;
+        const std::size_t cube = static_cast<std::size_t>(side * side * side);
+        prev.assign(cube, 0.0);
+        cur.assign(cube, 0.0);
+        seeds.assign(static_cast<std::size_t>(top + 1), 0.0);
+        all = triples(top);
+    }
+    std::size_t off(int t, int u, int v) const { return static_cast<std::size_t>((t * side + u) * side + v); }
+};
+
+// hermite_recursion (src/electrostatic_potential.cpp:293-324): R^0_tuv for
+// t + u + v <= L into w.cur at w.off(t, u, v), from w.seeds[n] = R^n_000:
+//   R^n_(t+1,u,v) = t R^(n+1)_(t-1,u,v) + X R^(n+1)_(t,u,v)
+// (first nonzero index of t, u, v), from n = L down to 0.
+void hermite_recursion(const double pc[3], int L, HermiteWork& w) {
+    for (int n = L; n >= 0; --n) {
+        std::swap(w.prev, w.cur);
+        const auto& prev = w.prev;
+        w.cur[0] = w.seeds[static_cast<std::size_t>(n)];
+        const std::size_t entries = hcount(L - n);
+        for (std::size_t k = 1; k < entries; ++k) {
+            const auto& h = w.all[k];
+            double value;
+            if (h.t > 0) {
+                value = pc[0] * prev[w.off(h.t - 1, h.u, h.v)];
+                if (h.t > 1) value += (h.t - 1) * prev[w.off(h.t - 2, h.u, h.v)];
+            } else if (h.u > 0) {
+                value = pc[1] * prev[w.off(h.t, h.u - 1, h.v)];
+                if (h.u > 1) value += (h.u - 1) * prev[w.off(h.t, h.u - 2, h.v)];
+            } else {
+                value = pc[2] * prev[w.off(h.t, h.u, h.v - 1)];
+                if (h.v > 1) value += (h.v - 1) * prev[w.off(h.t, h.u, h.v - 2)];
+            }
+            w.cur[w.off(h.t, h.u, h.v)] = value;
         }
-        for (int n = L; n >= 0; --n) {
-            std::swap(prev, cur);
-            cur[0] = seeds[static_cast<std::size_t>(n)];
-            const std::size_t entries = static_cast<std::size_t>((L - n + 1) * (L - n + 2) * (L - n + 3) / 6);
-            for (std::size_t k = 1; k < entries; ++k) {
-                const auto& h = all[k];
-                double value;
-                if (h.t > 0) {
-                    value = pc[0] * prev[off(h.t - 1, h.u, h.v)];
-                    if (h.t > 1) value += (h.t - 1) * prev[off(h.t - 2, h.u, h.v)];
-                } else if (h.u > 0) {
-                    value = pc[1] * prev[off(h.t, h.u - 1, h.v)];
-                    if (h.u > 1) value += (h.u - 1) * prev[off(h.t, h.u - 2, h.v)];
-                } else {
-                    value = pc[2] * prev[off(h.t, h.u, h.v - 1)];
-                    if (h.v > 1) value += (h.v - 1) * prev[off(h.t, h.u, h.v - 2)];
+    }
+}
+
+// hermite_integrals (src/electrostatic_potential.cpp:269-280): seeds
+// R^n_000 = (-2p)^n F_n(p |PC|^2), then the recursion to order L.
+void hermite_integrals(double p, const double pc[3], int L, HermiteWork& w) {
+    boys(p * sqnorm(pc[0], pc[1], pc[2]), L, w.seeds.data());
+    double power = 1.0;
+    for (int n = 1; n <= L; ++n) {
+        power *= -2.0 * p;
+        w.seeds[static_cast<std::size_t>(n)] *= power;
+    }
+    hermite_recursion(pc, L, w);
+}
+
+// pair_potential (src/electrostatic_potential.cpp:326-338): sum_k c_k R^0_k
+// of one pair at the point r.
+double pair_potential(const Esp& e, const EspPair& pr, const double* r, HermiteWork& w) {
+    const double pc[3] = {pr.Q[0] - r[0], pr.Q[1] - r[1], pr.Q[2] - r[2]};
+    hermite_integrals(pr.p, pc, pr.L, w);
+    const double* c = e.coef.data() + pr.first;
+    const std::size_t nk = hcount(pr.L);
+    double sum = 0.0;
+    for (std::size_t k = 0; k < nk; ++k)
+        sum += c[k] * w.cur[w.off(w.all[k].t, w.all[k].u, w.all[k].v)];
+    return sum;
+}
+
+// electronic (src/electrostatic_potential.cpp:340-348): minus the sum over
+// every pair.
+double esp_electronic(const Esp& e, const double* r) {
+    HermiteWork w;
+    w.prepare(e.max_L + 1, e.max_L);
+    double total = 0.0;
+    for (const auto& pr : e.pairs)
+        total += pair_potential(e, pr, r, w);
+    return -total;
+}
+
+// ---------------------------------------------------------------------------
+// Far-field approximation of the electrostatic potential: error bounds and
+// plans (src/electrostatic_potential.cpp:363-699), cluster evaluation
+// (:701-844) and target clusters (src/potential_clusters.hpp:29-105,
+// src/evaluator.cpp:391-479).
+
+constexpr int kLocalOrder = 6;      // src/electrostatic_potential.hpp:64
+constexpr int kMultipoleOrder = 10; // src/electrostatic_potential.hpp:66
+constexpr int kLevelCount = 33;     // src/electrostatic_potential.hpp:76
+constexpr double kSourceCell = 2.0; // src/electrostatic_potential.hpp:77
+constexpr double kPointCube = 1.6;  // include/chem/kumo/evaluator.hpp:170
+constexpr int kTileSide = 8;        // src/evaluator.cpp:399
+
+// Factor from a stored Hermite coefficient (2 pi / p included) to the weight
+// of its point multipole, sqrt(pi / p) / 2 (src/electrostatic_potential.cpp:31-33).
+double to_multipole(double p) { return 0.5 * std::sqrt(std::numbers::pi / p); }
+
+// G_m(r), m < count: bound on every order-m derivative of erfc(sqrt(p) R) / R
+// for R >= r (src/electrostatic_potential.cpp:385-430), through the upper
+// incomplete gamma function Gamma((m + 1) / 2, p r^2 / 2), advanced from
+// Gamma(1/2, x) = sqrt(pi) erfc(sqrt(x)) and Gamma(1, x) = exp(-x) by
+// Gamma(s + 1, x) = s Gamma(s, x) + x^s exp(-x).
+void erfc_bounds(double p, double r, std::size_t count, double* out) {
+    if (!(r > 0.0)) {
+        std::fill(out, out + count, std::numeric_limits<double>::infinity());
+        return;
+    }
+    const double x = 0.5 * p * r * r;
+    const double decay = std::exp(-x), rx = std::sqrt(x);
+    const double scale = 2.0 / (r * r), rscale = std::sqrt(scale);
+    double g_even = std::sqrt(std::numbers::pi) * std::erfc(rx), x_even = rx, s_even = 0.5;
+    double g_odd = decay, x_odd = x, s_odd = 1.0;
+    double spow = rscale, norm2 = 1.0;
+    constexpr double k = 1.0865; // Cramer's bound on Hermite functions
+    const double pre = 2.0 / std::sqrt(std::numbers::pi) * k * k * k * 0.5;
+    for (std::size_t m = 0; m < count; ++m) {
+        if (m > 0) {
+            norm2 *= 2.0 * static_cast<double>(m);
+            spow *= rscale;
+        }
+        double g;
+        if (m % 2 == 0) {
+            if (m > 0) {
+                g_even = s_even * g_even + x_even * decay;
+                x_even *= x;
+                s_even += 1.0;
+            }
+            g = g_even;
+        } else {
+            if (m > 1) {
+                g_odd = s_odd * g_odd + x_odd * decay;
+                x_odd *= x;
+                s_odd += 1.0;
+            }
+            g = g_odd;
+        }
+        out[m] = pre * std::sqrt(norm2) * spow * g;
+    }
+}
+
+// T_L(d) = sum_(n > K) (n + L)! / n! rho^n / d^(n + L + 1), the Taylor
+// remainder bound of the order-L multipole terms (src/electrostatic_potential.cpp:432-455):
+// infinite unless d > 1.25 rho; summed until the term ratio drops below 0.9,
+// then closed by the geometric tail.
+double taylor_tail(int L, int K, double rho, double d) {
+    if (!(d > 1.25 * rho)) return std::numeric_limits<double>::infinity();
+    const double q = rho / d;
+    double term = 1.0 / std::pow(d, L + 1);
+    for (int j = 1; j <= L; ++j)
+        term *= j;
+    for (int n = 1; n <= K + 1; ++n)
+        term *= (n + L) / static_cast<double>(n) * q;
+    double sum = 0.0;
+    for (int n = K + 1; n < K + 1000; ++n) {
+        sum += term;
+        const double ratio = (n + L + 1) / static_cast<double>(n + 1) * q;
+        if (ratio < 0.9) return sum + term * ratio / (1.0 - ratio);
+        term *= ratio;
+        if (!std::isfinite(term)) break;
+    }
+    return std::numeric_limits<double>::infinity();
+}
+
+// Smallest d in [low, high] with bound(d) <= allowed for a decreasing bound
+// (src/electrostatic_potential.cpp:457-472): `high` doubled up to 1e4 bohr,
+// then bisection to a relative 1e-3 (at most 40 steps), rounded up.
+template <typename Bound> double smallest_distance(const Bound& bound, double low, double high, double allowed) {
+    while (high < 1e4 && !(bound(high) <= allowed))
+        high *= 2.0;
+    if (!(bound(high) <= allowed)) return std::numeric_limits<double>::infinity();
+    for (int it = 0; it < 40 && high - low > 1e-3 * high; ++it) {
+        const double mid = 0.5 * (low + high);
+        if (bound(mid) <= allowed)
+            high = mid;
+        else
+            low = mid;
+    }
+    return high;
+}
+
+// A double rounded up to a float (src/electrostatic_potential.cpp:474-481).
+float round_up_float(double v) {
+    const float f = static_cast<float>(v);
+    return static_cast<double>(f) < v ? std::nextafter(f, std::numeric_limits<float>::infinity()) : f;
+}
+
+// W_L: sum of |e_k| over the Hermite terms of order L of a pair
+// (multipole_weights, src/electrostatic_potential.cpp:484-495).
+void multipole_weights(const Esp& e, const EspPair& pr, std::vector<double>& w) {
+    w.assign(static_cast<std::size_t>(pr.L + 1), 0.0);
+    const double* c = e.coef.data() + pr.first;
+    const double f = to_multipole(pr.p);
+    std::size_t k = 0;
+    for (int order = 0; order <= pr.L; ++order)
+        for (int j = 0; j < (order + 1) * (order + 2) / 2; ++j, ++k)
+            w[static_cast<std::size_t>(order)] += std::abs(c[k]) * f;
+}
+
+struct SourceCluster {
+    double c[3] = {0.0, 0.0, 0.0};
+    double radius = 0.0, min_distance = 0.0;
+    std::vector<std::size_t> pairs;
+    std::vector<double> moments; // (N + 1) blocks of moment_count: moments truncated at |delta| <= m
+};
+
+struct FarPlan {
+    std::size_t npairs = 0;
+    double radius = 0.0, tolerance = 0.0, pair_tolerance = 0.0;
+    double levels[kLevelCount] = {};
+    std::vector<float> level_d2; // npairs x kLevelCount
+    std::vector<SourceCluster> clusters;
+    std::size_t moment_count = 0;
+};
+
+// plan_pair_levels (src/electrostatic_potential.cpp:503-559): the error
+// bound of each pair's local expansion (order K about a cluster of radius
+// rho) at 64 distances from max(1.25 rho, 0.05) to 400 bohr in geometric
+// progression,
+//   sum_L W_L [T_L(d) + G_L(d - rho) + sum_(n<=K) (sqrt(3) rho)^n / n! G_(L+n)(d)],
+// and for each level the first of these distances whose bound is within it.
+void plan_pair_levels(const Esp& e, FarPlan& plan) {
+    constexpr int K = kLocalOrder, G = 64;
+    const double rho = plan.radius;
+    const double first = std::max(1.25 * rho, 0.05);
+    const double ratio = std::pow(400.0 / first, 1.0 / (G - 1));
+    double dist[G], bound[G];
+    for (int g = 0; g < G; ++g)
+        dist[g] = first * std::pow(ratio, g);
+    std::vector<double> w, nearb, farb;
+    for (std::size_t i = 0; i < e.pairs.size(); ++i) {
+        const EspPair& pr = e.pairs[i];
+        multipole_weights(e, pr, w);
+        nearb.assign(static_cast<std::size_t>(pr.L + 1), 0.0);
+        farb.assign(static_cast<std::size_t>(pr.L + K + 1), 0.0);
+        for (int g = 0; g < G; ++g) {
+            const double d = dist[g];
+            erfc_bounds(pr.p, d - rho, nearb.size(), nearb.data());
+            erfc_bounds(pr.p, d, farb.size(), farb.data());
+            double total = 0.0;
+            for (int L = 0; L <= pr.L; ++L) {
+                const double wl = w[static_cast<std::size_t>(L)];
+                if (wl == 0.0) continue;
+                double term = taylor_tail(L, K, rho, d) + nearb[static_cast<std::size_t>(L)];
+                double sc = 1.0;
+                for (int n = 0; n <= K; ++n) {
+                    if (n > 0) sc *= std::sqrt(3.0) * rho / n;
+                    term += sc * farb[static_cast<std::size_t>(L + n)];
                 }
-                cur[off(h.t, h.u, h.v)] = value;
+                total += wl * term;
+            }
+            bound[g] = total;
+        }
+        int g = 0;
+        for (int j = 0; j < kLevelCount; ++j) {
+            while (g < G && !(bound[g] <= plan.levels[j]))
+                ++g;
+            if (g == G) break;
+            plan.level_d2[i * kLevelCount + static_cast<std::size_t>(j)] = round_up_float(dist[g] * dist[g]);
+        }
+    }
+}
+
+// plan_source_clusters (src/electrostatic_potential.cpp:561-699): pairs
+// grouped by the cube of edge 2 bohr holding their centre and by exponent
+// interval [4^k, 4^(k+1)), clusters numbered by first appearance; each with
+// the centre of the bounding box of its pair centres, the largest distance
+// of a pair centre from it, the moments
+//   Q^(m)_nu = sum over pairs and terms alpha_k + delta = nu, |delta| <= m,
+//              of e_k u^delta / delta!   (u = pair centre - cluster centre)
+// for m = 0..N, and the distance beyond which the translation to a target
+// cluster is within a quarter of the tolerance shared among the clusters.
+void plan_source_clusters(const Esp& e, FarPlan& plan) {
+    constexpr int N = kMultipoleOrder;
+    std::map<std::array<long, 4>, std::size_t> index;
+    for (std::size_t i = 0; i < e.pairs.size(); ++i) {
+        const EspPair& pr = e.pairs[i];
+        std::array<long, 4> key{};
+        for (int a = 0; a < 3; ++a)
+            key[static_cast<std::size_t>(a)] = static_cast<long>(std::floor(pr.Q[a] / kSourceCell));
+        key[3] = static_cast<long>(std::floor(std::log2(pr.p) / 2.0));
+        const auto it = index.find(key);
+        std::size_t c;
+        if (it == index.end()) {
+            c = plan.clusters.size();
+            index.emplace(key, c);
+            plan.clusters.emplace_back();
+        } else {
+            c = it->second;
+        }
+        plan.clusters[c].pairs.push_back(i);
+    }
+    const double share = 0.25 * plan.tolerance / static_cast<double>(plan.clusters.size());
+    const int side = e.max_L + N + 1;
+    const auto nu_list = triples(e.max_L + N);
+    plan.moment_count = nu_list.size();
+    std::vector<int> compact(static_cast<std::size_t>(side * side * side), -1);
+    for (std::size_t m = 0; m < nu_list.size(); ++m)
+        compact[static_cast<std::size_t>((nu_list[m].t * side + nu_list[m].u) * side + nu_list[m].v)] = static_cast<int>(m);
+    const auto deltas = triples(N);
+    std::vector<double> w, cw, g;
+    for (auto& cl : plan.clusters) {
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a)
+            lo[a] = hi[a] = e.pairs[cl.pairs.front()].Q[a];
+        double pmin = std::numeric_limits<double>::infinity();
+        int top = 0;
+        for (const std::size_t i : cl.pairs) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], e.pairs[i].Q[a]);
+                hi[a] = std::max(hi[a], e.pairs[i].Q[a]);
+            }
+            pmin = std::min(pmin, e.pairs[i].p);
+            top = std::max(top, e.pairs[i].L);
+        }
+        for (int a = 0; a < 3; ++a)
+            cl.c[a] = 0.5 * (lo[a] + hi[a]);
+        cl.radius = 0.0;
+        cw.assign(static_cast<std::size_t>(top + 1), 0.0);
+        std::vector<double> by_order(static_cast<std::size_t>(N + 1) * plan.moment_count, 0.0);
+        for (const std::size_t i : cl.pairs) {
+            const EspPair& pr = e.pairs[i];
+            const double u[3] = {pr.Q[0] - cl.c[0], pr.Q[1] - cl.c[1], pr.Q[2] - cl.c[2]};
+            cl.radius = std::max(cl.radius, std::sqrt(sqnorm(u[0], u[1], u[2])));
+            multipole_weights(e, pr, w);
+            for (std::size_t o = 0; o < w.size(); ++o)
+                cw[o] += w[o];
+            const double f = to_multipole(pr.p);
+            const double* c = e.coef.data() + pr.first;
+            const std::size_t nk = hcount(pr.L);
+            for (std::size_t k = 0; k < nk; ++k) {
+                const double ek = c[k] * f;
+                if (ek == 0.0) continue;
+                const Triple& al = nu_list[k];
+                for (const Triple& d : deltas) {
+                    double term = ek;
+                    for (int j = 1; j <= d.t; ++j)
+                        term *= u[0] / j;
+                    for (int j = 1; j <= d.u; ++j)
+                        term *= u[1] / j;
+                    for (int j = 1; j <= d.v; ++j)
+                        term *= u[2] / j;
+                    const int nu = compact[static_cast<std::size_t>(((al.t + d.t) * side + al.u + d.u) * side + al.v + d.v)];
+                    by_order[static_cast<std::size_t>(d.t + d.u + d.v) * plan.moment_count + static_cast<std::size_t>(nu)] += term;
+                }
             }
         }
-        const double* c = e.coef.data() + pr.first;
-        const std::size_t nk = static_cast<std::size_t>((L + 1) * (L + 2) * (L + 3) / 6);
-        double sum = 0.0;
-        for (std::size_t k = 0; k < nk; ++k)
-            sum += c[k] * cur[off(all[k].t, all[k].u, all[k].v)];
-        total += sum;
+        cl.moments.assign(by_order.size(), 0.0);
+        for (int m = 0; m <= N; ++m)
+            for (std::size_t nu = 0; nu < plan.moment_count; ++nu) {
+                const std::size_t at = static_cast<std::size_t>(m) * plan.moment_count + nu;
+                cl.moments[at] = (m > 0 ? cl.moments[at - plan.moment_count] : 0.0) + by_order[at];
+            }
+        const double reach = cl.radius + plan.radius;
+        const auto truncation = [&](double D) {
+            double total = 0.0;
+            for (std::size_t o = 0; o < cw.size(); ++o)
+                if (cw[o] > 0.0) total += cw[o] * taylor_tail(static_cast<int>(o), N, reach, D);
+            return total;
+        };
+        g.assign(cw.size(), 0.0);
+        const auto erfc_terms = [&](double r) {
+            erfc_bounds(pmin, r, g.size(), g.data());
+            double total = 0.0;
+            for (std::size_t o = 0; o < cw.size(); ++o)
+                total += cw[o] * g[o];
+            return total;
+        };
+        const double dt = smallest_distance(truncation, 1.25 * reach, 2.5 * reach + 1e-3, share);
+        const double re = smallest_distance(erfc_terms, 0.0, 1.0 / std::sqrt(pmin), share);
+        cl.min_distance = std::max(dt, reach + re);
     }
-    return -total;
+}
+
+// far_field_plan (src/electrostatic_potential.cpp:497-501 and its body
+// above :503): levels pair_tolerance * 10^(-j/2) with pair_tolerance half
+// the tolerance; tolerance 0 plans nothing (every pair exact).
+std::unique_ptr<FarPlan> far_field_plan(const Esp& e, double radius, double tolerance) {
+    if (!std::isfinite(radius) || radius < 0.0) throw std::invalid_argument("cluster radius must be finite and nonnegative");
+    if (!std::isfinite(tolerance) || tolerance < 0.0)
+        throw std::invalid_argument("electrostatic potential tolerance must be finite and nonnegative");
+    auto plan = std::make_unique<FarPlan>();
+    plan->npairs = e.pairs.size();
+    plan->radius = radius;
+    plan->tolerance = tolerance;
+    plan->level_d2.assign(e.pairs.size() * kLevelCount, std::numeric_limits<float>::infinity());
+    if (tolerance == 0.0 || e.pairs.empty()) return plan;
+    plan->pair_tolerance = 0.5 * tolerance;
+    for (int j = 0; j < kLevelCount; ++j)
+        plan->levels[j] = plan->pair_tolerance * std::pow(10.0, -0.5 * j);
+    plan_pair_levels(e, *plan);
+    plan_source_clusters(e, *plan);
+    return plan;
+}
+
+// cluster_plan (src/potential_clusters.hpp:29-35): the radius enlarged by a
+// relative 1e-10 plus 1e-12 bohr for rounding in the coordinates.
+std::unique_ptr<FarPlan> cluster_plan(const Esp& e, double radius, double tolerance) {
+    return far_field_plan(e, radius * (1.0 + 1e-10) + 1e-12, tolerance);
+}
+
+struct FarStats {
+    std::int64_t far_clusters = 0, expanded = 0, near = 0;
+};
+
+// expand_far_field (src/electrostatic_potential.cpp:730-826): the local
+// expansion about `b` (factors (-1)^|beta| / beta! applied) in `local`, from
+// the source clusters at least min_distance away (one multipole-to-local
+// translation each, local_beta += sum_nu Q^(N - |beta|)_nu d^(nu + beta) 1/|X|)
+// and from the remaining pairs whose error levels fit the pair budget
+// (smallest levels first; at the level that fits only in part, pairs in index
+// order); returns the pairs to evaluate exactly.
+std::vector<std::size_t> expand_far_field(const Esp& e, const FarPlan& plan, const double b[3], HermiteWork& w,
+                                          std::vector<double>& local, FarStats& st) {
+    constexpr int N = kMultipoleOrder;
+    const int top = e.max_L + N;
+    w.prepare(top + 1, top);
+    const std::size_t nlocal = hcount(N);
+    local.assign(nlocal, 0.0);
+    std::vector<char> handled(e.pairs.size(), 0);
+    for (const auto& cl : plan.clusters) {
+        const double x[3] = {cl.c[0] - b[0], cl.c[1] - b[1], cl.c[2] - b[2]};
+        if (!(std::sqrt(sqnorm(x[0], x[1], x[2])) >= cl.min_distance)) continue;
+        ++st.far_clusters;
+        // coulomb_derivatives (src/electrostatic_potential.cpp:282-291):
+        // seeds (-1)^n (2n - 1)!! / |X|^(2n + 1).
+        const double inv = 1.0 / std::sqrt(sqnorm(x[0], x[1], x[2])), inv2 = inv * inv;
+        double seed = inv;
+        for (int n = 0; n <= top; ++n) {
+            w.seeds[static_cast<std::size_t>(n)] = seed;
+            seed *= -(2.0 * n + 1.0) * inv2;
+        }
+        hermite_recursion(x, top, w);
+        for (std::size_t bi = 0; bi < nlocal; ++bi) {
+            const Triple& be = w.all[bi];
+            const int m = N - (be.t + be.u + be.v);
+            const double* q = cl.moments.data() + static_cast<std::size_t>(m) * plan.moment_count;
+            const std::size_t terms = hcount(e.max_L + m);
+            double sum = 0.0;
+            for (std::size_t nu = 0; nu < terms; ++nu) {
+                const Triple& a = w.all[nu];
+                sum += q[nu] * w.cur[w.off(a.t + be.t, a.u + be.u, a.v + be.v)];
+            }
+            local[bi] += sum;
+        }
+        for (const std::size_t i : cl.pairs)
+            handled[i] = 1;
+    }
+    std::vector<signed char> level(e.pairs.size(), -1);
+    std::array<std::size_t, kLevelCount> population{};
+    for (std::size_t i = 0; i < e.pairs.size(); ++i) {
+        if (handled[i]) continue;
+        const EspPair& pr = e.pairs[i];
+        const double d2 = sqnorm(pr.Q[0] - b[0], pr.Q[1] - b[1], pr.Q[2] - b[2]);
+        const float* lim = plan.level_d2.data() + i * kLevelCount;
+        int j = -1;
+        while (j + 1 < kLevelCount && d2 >= static_cast<double>(lim[j + 1]))
+            ++j;
+        level[i] = static_cast<signed char>(j);
+        if (j >= 0) ++population[static_cast<std::size_t>(j)];
+    }
+    int cut = kLevelCount;
+    std::size_t partial = 0;
+    double used = 0.0;
+    for (int j = kLevelCount - 1; j >= 0; --j) {
+        const double v = plan.levels[j];
+        const std::size_t n = population[static_cast<std::size_t>(j)];
+        if (used + static_cast<double>(n) * v <= plan.pair_tolerance) {
+            used += static_cast<double>(n) * v;
+            cut = j - 1;
+            continue;
+        }
+        partial = static_cast<std::size_t>(std::floor((plan.pair_tolerance - used) / v));
+        cut = j;
+        break;
+    }
+    std::vector<std::size_t> near;
+    const std::size_t nlow = hcount(kLocalOrder);
+    for (std::size_t i = 0; i < e.pairs.size(); ++i) {
+        if (handled[i]) continue;
+        bool expand = level[i] >= 0 && level[i] > cut;
+        if (!expand && level[i] >= 0 && level[i] == cut && partial > 0) {
+            expand = true;
+            --partial;
+        }
+        if (!expand) {
+            near.push_back(i);
+            continue;
+        }
+        ++st.expanded;
+        const EspPair& pr = e.pairs[i];
+        const double pc[3] = {pr.Q[0] - b[0], pr.Q[1] - b[1], pr.Q[2] - b[2]};
+        hermite_integrals(pr.p, pc, pr.L + kLocalOrder, w);
+        const double* c = e.coef.data() + pr.first;
+        const std::size_t nk = hcount(pr.L);
+        for (std::size_t k = 0; k < nk; ++k) {
+            const Triple& a = w.all[k];
+            for (std::size_t bi = 0; bi < nlow; ++bi) {
+                const Triple& be = w.all[bi];
+                local[bi] += c[k] * w.cur[w.off(a.t + be.t, a.u + be.u, a.v + be.v)];
+            }
+        }
+    }
+    for (std::size_t bi = 0; bi < nlocal; ++bi) {
+        const Triple& be = w.all[bi];
+        double f = ((be.t + be.u + be.v) % 2 == 0) ? 1.0 : -1.0;
+        for (int j = 2; j <= be.t; ++j)
+            f /= j;
+        for (int j = 2; j <= be.u; ++j)
+            f /= j;
+        for (int j = 2; j <= be.v; ++j)
+            f /= j;
+        local[bi] *= f;
+    }
+    st.near += static_cast<std::int64_t>(near.size());
+    return near;
+}
+
+// local_polynomial (src/electrostatic_potential.cpp:828-844): the local
+// expansion at displacement h from its centre.
+double local_polynomial(const std::vector<double>& local, const std::vector<Triple>& all, const double h[3]) {
+    double pw[3][kMultipoleOrder + 1];
+    for (int a = 0; a < 3; ++a) {
+        pw[a][0] = 1.0;
+        for (int k = 1; k <= kMultipoleOrder; ++k)
+            pw[a][k] = pw[a][k - 1] * h[a];
+    }
+    double poly = 0.0;
+    for (std::size_t bi = 0; bi < local.size(); ++bi) {
+        const Triple& be = all[bi];
+        poly += local[bi] * pw[0][be.t] * pw[1][be.u] * pw[2][be.v];
+    }
+    return poly;
+}
+
+// evaluate_cluster (src/electrostatic_potential.cpp:701-728): points within
+// the plan's radius of the centre (with rounding slack), each the nuclear
+// term minus (local polynomial + the exact near pairs).
+void evaluate_cluster(const Esp& e, const FarPlan& plan, std::int64_t n, const double* pts, const double centre[3],
+                      double* out, FarStats& st) {
+    if (plan.level_d2.size() != e.pairs.size() * kLevelCount)
+        throw std::invalid_argument("far-field plan does not match the potential");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(centre[a])) throw std::invalid_argument("cluster centre must be finite");
+    const double cn = std::sqrt(sqnorm(centre[0], centre[1], centre[2]));
+    for (std::int64_t j = 0; j < n; ++j) {
+        const double* r = pts + 3 * j;
+        if (!std::isfinite(r[0]) || !std::isfinite(r[1]) || !std::isfinite(r[2]))
+            throw std::invalid_argument("electrostatic potential point must be finite");
+        const double d = std::sqrt(sqnorm(r[0] - centre[0], r[1] - centre[1], r[2] - centre[2]));
+        if (d > plan.radius + 1e-12 * (plan.radius + cn)) throw std::invalid_argument("cluster point outside the planned radius");
+    }
+    HermiteWork w;
+    std::vector<double> local;
+    const auto near = expand_far_field(e, plan, centre, w, local, st);
+    for (std::int64_t j = 0; j < n; ++j) {
+        const double* r = pts + 3 * j;
+        const double h[3] = {r[0] - centre[0], r[1] - centre[1], r[2] - centre[2]};
+        double sum = local_polynomial(local, w.all, h);
+        for (const std::size_t i : near)
+            sum += pair_potential(e, e.pairs[i], r, w);
+        out[j] = esp_nuclear(e, r) - sum;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1599,6 +2120,105 @@ template <typename Body> void for_blocks(std::int64_t n, int bs, const Body& bod
     if (err) std::rethrow_exception(err);
 }
 
+// The potential on a grid in tiles of up to 8 x 8 x 8 points, each a target
+// cluster about the centre of its box of grid indices, with one plan whose
+// radius covers the largest tile (src/evaluator.cpp:391-443,
+// grid_box_radius src/potential_clusters.hpp:37-49).
+void esp_grid_clusters(const Esp& e, const PointSet& set, double tolerance, double* out) {
+    int tiles[3];
+    double edges[3];
+    for (int a = 0; a < 3; ++a) {
+        edges[a] = std::min(kTileSide, set.counts[a]) - 1;
+        tiles[a] = (set.counts[a] + kTileSide - 1) / kTileSide;
+    }
+    double radius = 0.0;
+    for (int corner = 0; corner < 8; ++corner) {
+        double o[3], v[3];
+        for (int a = 0; a < 3; ++a)
+            o[a] = ((corner >> a) & 1 ? 0.5 : -0.5) * edges[a];
+        for (int a = 0; a < 3; ++a)
+            v[a] = (set.steps[a] * o[0] + set.steps[3 + a] * o[1]) + set.steps[6 + a] * o[2];
+        radius = std::max(radius, std::sqrt(sqnorm(v[0], v[1], v[2])));
+    }
+    const auto plan = cluster_plan(e, radius, tolerance);
+    const std::int64_t ntiles = static_cast<std::int64_t>(tiles[0]) * tiles[1] * tiles[2];
+    const std::int64_t ny = set.counts[1], nz = set.counts[2];
+    for_blocks(ntiles, 1, [&](std::int64_t index, int) {
+        const int t[3] = {static_cast<int>(index / (static_cast<std::int64_t>(tiles[1]) * tiles[2])),
+                          static_cast<int>((index / tiles[2]) % tiles[1]), static_cast<int>(index % tiles[2])};
+        int begin[3], end[3];
+        for (int a = 0; a < 3; ++a) {
+            begin[a] = t[a] * kTileSide;
+            end[a] = std::min(begin[a] + kTileSide, set.counts[a]);
+        }
+        std::vector<double> pts;
+        std::vector<std::int64_t> target;
+        for (int i = begin[0]; i < end[0]; ++i)
+            for (int j = begin[1]; j < end[1]; ++j)
+                for (int k = begin[2]; k < end[2]; ++k) {
+                    const std::int64_t q = (static_cast<std::int64_t>(i) * ny + j) * nz + k;
+                    double r[3];
+                    set.point(q, r);
+                    pts.insert(pts.end(), r, r + 3);
+                    target.push_back(q);
+                }
+        const double f[3] = {0.5 * (begin[0] + end[0] - 1), 0.5 * (begin[1] + end[1] - 1), 0.5 * (begin[2] + end[2] - 1)};
+        double c[3];
+        for (int a = 0; a < 3; ++a)
+            c[a] = set.origin[a] + ((set.steps[a] * f[0] + set.steps[3 + a] * f[1]) + set.steps[6 + a] * f[2]);
+        std::vector<double> v(target.size());
+        FarStats st;
+        evaluate_cluster(e, *plan, static_cast<std::int64_t>(target.size()), pts.data(), c, v.data(), st);
+        for (std::size_t q = 0; q < target.size(); ++q)
+            out[target[q]] = v[q];
+    });
+}
+
+// The potential at arbitrary points grouped by the axis-aligned cube of edge
+// 1.6 bohr holding them, each cube a target cluster about its centre, one
+// plan covering half a cube diagonal (src/evaluator.cpp:445-479, cluster_by
+// src/potential_clusters.hpp:51-76).
+void esp_point_clusters(const Esp& e, std::int64_t n, const double* points, double tolerance, double* out) {
+    using Cell = std::array<std::int64_t, 3>;
+    std::vector<Cell> cells(static_cast<std::size_t>(n));
+    for (std::int64_t q = 0; q < n; ++q)
+        for (int a = 0; a < 3; ++a) {
+            const double index = std::floor(points[3 * q + a] / kPointCube);
+            if (!(std::abs(index) < 1e15)) throw std::invalid_argument("point too far for the electrostatic potential");
+            cells[static_cast<std::size_t>(q)][static_cast<std::size_t>(a)] = static_cast<std::int64_t>(index);
+        }
+    std::vector<std::int64_t> order(static_cast<std::size_t>(n));
+    for (std::int64_t q = 0; q < n; ++q)
+        order[static_cast<std::size_t>(q)] = q;
+    std::stable_sort(order.begin(), order.end(), [&](std::int64_t a, std::int64_t b) {
+        return cells[static_cast<std::size_t>(a)] < cells[static_cast<std::size_t>(b)];
+    });
+    std::vector<std::int64_t> starts;
+    for (std::int64_t q = 0; q < n; ++q)
+        if (q == 0 || cells[static_cast<std::size_t>(order[static_cast<std::size_t>(q)])] !=
+                          cells[static_cast<std::size_t>(order[static_cast<std::size_t>(q - 1)])])
+            starts.push_back(q);
+    starts.push_back(n);
+    const auto plan = cluster_plan(e, 0.5 * std::sqrt(3.0) * kPointCube, tolerance);
+    const std::int64_t nclusters = static_cast<std::int64_t>(starts.size()) - 1;
+    for_blocks(nclusters, 1, [&](std::int64_t cidx, int) {
+        const std::int64_t b0 = starts[static_cast<std::size_t>(cidx)], b1 = starts[static_cast<std::size_t>(cidx + 1)];
+        std::vector<double> pts;
+        for (std::int64_t q = b0; q < b1; ++q) {
+            const double* r = points + 3 * order[static_cast<std::size_t>(q)];
+            pts.insert(pts.end(), r, r + 3);
+        }
+        const Cell& cell = cells[static_cast<std::size_t>(order[static_cast<std::size_t>(b0)])];
+        const double c[3] = {(static_cast<double>(cell[0]) + 0.5) * kPointCube, (static_cast<double>(cell[1]) + 0.5) * kPointCube,
+                             (static_cast<double>(cell[2]) + 0.5) * kPointCube};
+        std::vector<double> v(static_cast<std::size_t>(b1 - b0));
+        FarStats st;
+        evaluate_cluster(e, *plan, b1 - b0, pts.data(), c, v.data(), st);
+        for (std::int64_t q = b0; q < b1; ++q)
+            out[order[static_cast<std::size_t>(q)]] = v[static_cast<std::size_t>(q - b0)];
+    });
+}
+
 struct Field {
     orc_property key;
     double** out;
@@ -1611,6 +2231,8 @@ void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_prop
         throw std::invalid_argument("fukui_delta must be finite and positive");
     if (!std::isfinite(opt.screening_threshold) || opt.screening_threshold < 0.0)
         throw std::invalid_argument("screening threshold must be finite and nonnegative");
+    if (!std::isfinite(opt.potential_tolerance) || opt.potential_tolerance < 0.0)
+        throw std::invalid_argument("potential tolerance must be finite and nonnegative");
     if (opt.backend == AO_MO && !M.eligible)
         throw std::invalid_argument("AO_MO backend requested but orbital representation is not eligible");
     const Plan plan = make_plan(props);
@@ -1657,17 +2279,26 @@ void evaluate(const Molecule& M, const PointSet& set, const std::vector<orc_prop
                 if (from_point_path(f.key, backend, grid)) store_block(f.key.kind, f.key.component, block, size, start, f.out);
         });
     }
-    // Electrostatic potential (exact at every tolerance).
+    // Electrostatic potential pass (src/evaluator.cpp:664-675): target
+    // clusters under the far-field plan; with tolerance 0 every pair is exact
+    // and the values equal the pointwise sum bit for bit (an empty local
+    // expansion adds zero), which is evaluated directly.
     for (const auto& f : fields)
         if (f.key.kind == ElectrostaticPotential) {
             const auto esp = make_esp(M);
-            for_blocks(n, 64, [&](std::int64_t start, int size) {
-                double r[3];
-                for (int j = 0; j < size; ++j) {
-                    set.point(start + j, r);
-                    f.out[0][start + j] = esp_nuclear(*esp, r) + esp_electronic(*esp, r);
-                }
-            });
+            if (opt.potential_tolerance == 0.0) {
+                for_blocks(n, 64, [&](std::int64_t start, int size) {
+                    double r[3];
+                    for (int j = 0; j < size; ++j) {
+                        set.point(start + j, r);
+                        f.out[0][start + j] = esp_nuclear(*esp, r) + esp_electronic(*esp, r);
+                    }
+                });
+            } else if (grid) {
+                esp_grid_clusters(*esp, set, opt.potential_tolerance, f.out[0]);
+            } else {
+                esp_point_clusters(*esp, n, set.pts, opt.potential_tolerance, f.out[0]);
+            }
         }
     // Orbital-energy pass (src/evaluator.cpp:676-682): AO values without
     // per-block screening (the shell cutoff only), the column sums, then the
@@ -2017,7 +2648,7 @@ GridSpec grid_around(const Molecule& M, double margin, double spacing) {
 OMesh density_isosurface(const Molecule& M, double iso, double spacing) {
     if (!std::isfinite(iso) || iso < 1e-8) throw std::invalid_argument("density isovalue must be finite and at least 1e-8 e/bohr^3");
     if (!std::isfinite(spacing) || spacing <= 0.0) throw std::invalid_argument("surface grid spacing must be positive and finite");
-    const orc_options opt{128, AO_P, 1e-12, 0.1};
+    const orc_options opt{128, AO_P, 1e-12, 0.1, 0.0};
     const ScalarFn fn = [&](const std::vector<std::array<double, 3>>& pts, std::vector<double>& val,
                             std::vector<std::array<double, 3>>& grad) {
         std::vector<double> flat;
@@ -2359,6 +2990,75 @@ int orc_esp(const void* molecule, std::int64_t n, const double* points, double* 
 
 std::int64_t orc_esp_pair_count(const void* molecule) {
     return static_cast<std::int64_t>(make_esp(*static_cast<const Molecule*>(molecule))->pairs.size());
+}
+
+void* orc_esp_create(const void* molecule, char* error, int error_len) {
+    try {
+        return make_esp(*static_cast<const Molecule*>(molecule)).release();
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return nullptr;
+    }
+}
+
+void orc_esp_destroy(void* esp) { delete static_cast<Esp*>(esp); }
+
+void* orc_far_plan(const void* esp, double radius, double tolerance, char* error, int error_len) {
+    try {
+        return far_field_plan(*static_cast<const Esp*>(esp), radius, tolerance).release();
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return nullptr;
+    }
+}
+
+void orc_far_plan_destroy(void* plan) { delete static_cast<FarPlan*>(plan); }
+
+void orc_far_plan_info(const void* plan, std::int64_t* sizes, double* values) {
+    const FarPlan& p = *static_cast<const FarPlan*>(plan);
+    sizes[0] = static_cast<std::int64_t>(p.npairs);
+    sizes[1] = std::count_if(p.level_d2.begin(), p.level_d2.end(), [](float d2) { return std::isfinite(d2); });
+    sizes[2] = static_cast<std::int64_t>(p.clusters.size());
+    sizes[3] = static_cast<std::int64_t>(p.moment_count);
+    values[0] = p.pair_tolerance;
+    values[1] = p.radius;
+}
+
+int orc_far_plan_cluster(const void* plan, std::int64_t c, double* centre, double* radius, double* min_distance,
+                         std::int64_t* npairs, double* moments) {
+    const FarPlan& p = *static_cast<const FarPlan*>(plan);
+    if (c < 0 || c >= static_cast<std::int64_t>(p.clusters.size())) return 1;
+    const SourceCluster& cl = p.clusters[static_cast<std::size_t>(c)];
+    for (int a = 0; a < 3; ++a)
+        centre[a] = cl.c[a];
+    *radius = cl.radius;
+    *min_distance = cl.min_distance;
+    *npairs = static_cast<std::int64_t>(cl.pairs.size());
+    if (moments)
+        std::copy(cl.moments.end() - static_cast<std::ptrdiff_t>(p.moment_count), cl.moments.end(), moments);
+    return 0;
+}
+
+int orc_far_plan_levels(const void* plan, std::int64_t i, float* d2) {
+    const FarPlan& p = *static_cast<const FarPlan*>(plan);
+    if (i < 0 || i >= static_cast<std::int64_t>(p.npairs)) return 1;
+    std::copy_n(p.level_d2.begin() + i * kLevelCount, kLevelCount, d2);
+    return 0;
+}
+
+int orc_far_cluster(const void* esp, const void* plan, std::int64_t n, const double* points, const double* centre,
+                    double* out, std::int64_t* stats, char* error, int error_len) {
+    try {
+        FarStats st;
+        evaluate_cluster(*static_cast<const Esp*>(esp), *static_cast<const FarPlan*>(plan), n, points, centre, out, st);
+        stats[0] = st.far_clusters;
+        stats[1] = st.expanded;
+        stats[2] = st.near;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(error, error_len, e.what());
+        return 1;
+    }
 }
 
 void orc_grid_points(const double* origin, const double* steps, const int* counts, std::int64_t start, std::int64_t n,

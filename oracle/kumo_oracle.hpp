@@ -36,6 +36,7 @@ struct orc_options {
     int backend;                // 0 AUTO, 1 AO_P, 2 AO_MO
     double screening_threshold; // reference default 1e-12
     double fukui_delta;         // hartree, > 0 (reference default 0.1)
+    double potential_tolerance; // hartree/e, >= 0 (reference default 1e-8); 0 is exact
 };
 
 /// Basis from the raw FCHK shell records (src/basis.cpp:21-120).
@@ -90,6 +91,29 @@ int orc_boys(double T, int m, double* out);
 /// Exact electrostatic potential at n points (src/electrostatic_potential.cpp:117-361).
 int orc_esp(const void* molecule, std::int64_t n, const double* points, double* out);
 std::int64_t orc_esp_pair_count(const void* molecule);
+/// The potential of a molecule (pairs contracted with P once) for the
+/// far-field functions below; null with a message on error.
+void* orc_esp_create(const void* molecule, char* error, int error_len);
+void orc_esp_destroy(void* esp);
+/// far_field_plan (src/electrostatic_potential.cpp:497-501) for target
+/// clusters of `radius` bohr; null with a message on error.
+void* orc_far_plan(const void* esp, double radius, double tolerance, char* error, int error_len);
+void orc_far_plan_destroy(void* plan);
+/// sizes[0..5]: pairs, finite level entries, source clusters, moment_count,
+/// largest pair order, and the pair tolerance and radius in values[0..1].
+void orc_far_plan_info(const void* plan, std::int64_t* sizes, double* values);
+/// Source cluster c: centre[3], radius, min_distance, its pair count, and
+/// its moments of full order (moment_count values, (t, u, v) ordered by
+/// t + u + v, then t and u descending) when `moments` is not null.
+int orc_far_plan_cluster(const void* plan, std::int64_t c, double* centre, double* radius, double* min_distance,
+                         std::int64_t* npairs, double* moments);
+/// Level distances of pair i (33 squared distances as floats).
+int orc_far_plan_levels(const void* plan, std::int64_t i, float* d2);
+/// evaluate_cluster (src/electrostatic_potential.cpp:701-728) of n points
+/// about `centre`; stats[0..2]: source clusters translated, pairs expanded,
+/// pairs evaluated exactly. Returns 0, or 1 with a message.
+int orc_far_cluster(const void* esp, const void* plan, std::int64_t n, const double* points, const double* centre,
+                    double* out, std::int64_t* stats, char* error, int error_len);
 /// Grid point coordinates (src/grid_geometry.hpp:59-68), 3 x n.
 void orc_grid_points(const double* origin, const double* steps, const int* counts,
                      std::int64_t start, std::int64_t n, double* out);

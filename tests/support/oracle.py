@@ -18,7 +18,7 @@ class Prop(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("block_size", C.c_int), ("backend", C.c_int), ("screening_threshold", C.c_double),
-                ("fukui_delta", C.c_double)]
+                ("fukui_delta", C.c_double), ("potential_tolerance", C.c_double)]
 
 
 SCALAR_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -55,6 +55,18 @@ def lib():
         L.orc_esp.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.orc_esp_pair_count.argtypes = [C.c_void_p]
         L.orc_esp_pair_count.restype = C.c_int64
+        L.orc_esp_create.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.orc_esp_create.restype = C.c_void_p
+        L.orc_esp_destroy.argtypes = [C.c_void_p]
+        L.orc_far_plan.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_char_p, C.c_int]
+        L.orc_far_plan.restype = C.c_void_p
+        L.orc_far_plan_destroy.argtypes = [C.c_void_p]
+        L.orc_far_plan_info.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_far_plan_cluster.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]
+        L.orc_far_plan_levels.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.orc_far_cluster.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_char_p, C.c_int]
         L.orc_grid_points.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
         L.orc_pure_transform.argtypes = [C.c_int, C.c_void_p]
         L.orc_molecule_destroy.argtypes = [C.c_void_p]
@@ -182,7 +194,8 @@ class Molecule:
         if getattr(self, "h", None):
             lib().orc_molecule_destroy(self.h)
 
-    def evaluate(self, props, grid=None, points=None, block_size=128, backend=0, threshold=1e-12, fukui_delta=0.1):
+    def evaluate(self, props, grid=None, points=None, block_size=128, backend=0, threshold=1e-12, fukui_delta=0.1,
+                 potential_tolerance=1e-8):
         """props: list of (kind, component, channel, orbital_index).
         Returns (list of arrays channels x n, backend_used)."""
         from kumo_hip.api import channels
@@ -205,7 +218,7 @@ class Molecule:
         arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
         used = C.c_int(0)
         err = C.create_string_buffer(512)
-        opt = Options(block_size, backend, threshold, fukui_delta)
+        opt = Options(block_size, backend, threshold, fukui_delta, potential_tolerance)
         rc = lib().orc_evaluate(self.h, _ptr(origin), _ptr(steps), _ptr(counts, C.c_int), n, _ptr(pts), P,
                                 len(props), C.byref(opt), arr, C.byref(used), err, 512)
         if rc == 1:
@@ -241,6 +254,73 @@ class Molecule:
 
     def esp_pair_count(self):
         return lib().orc_esp_pair_count(self.h)
+
+    def potential(self):
+        return Potential(self)
+
+
+class Potential:
+    """The oracle's electrostatic potential of a molecule with its far-field
+    plans (src/electrostatic_potential.cpp:363-844)."""
+
+    def __init__(self, mol):
+        self.mol = mol  # keeps the molecule alive
+        err = C.create_string_buffer(512)
+        self.h = lib().orc_esp_create(mol.h, err, 512)
+        if not self.h:
+            raise ValueError(err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_esp_destroy(self.h)
+
+    def plan(self, radius, tolerance):
+        return FarPlan(self, radius, tolerance)
+
+    def cluster(self, points, centre, plan):
+        """evaluate_cluster: (values, (clusters translated, pairs expanded, pairs exact))."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        c = np.ascontiguousarray(centre, dtype=np.float64)
+        out = np.zeros(pts.shape[0])
+        stats = np.zeros(3, dtype=np.int64)
+        err = C.create_string_buffer(512)
+        if lib().orc_far_cluster(self.h, plan.h, pts.shape[0], _ptr(pts), _ptr(c), _ptr(out), _ptr(stats, C.c_int64),
+                                 err, 512):
+            raise ValueError(err.value.decode())
+        return out, tuple(int(x) for x in stats)
+
+
+class FarPlan:
+    def __init__(self, pot, radius, tolerance):
+        self.pot = pot
+        err = C.create_string_buffer(512)
+        self.h = lib().orc_far_plan(pot.h, radius, tolerance, err, 512)
+        if not self.h:
+            raise ValueError(err.value.decode())
+        sizes = np.zeros(6, dtype=np.int64)
+        vals = np.zeros(2)
+        lib().orc_far_plan_info(self.h, _ptr(sizes, C.c_int64), _ptr(vals))
+        self.npairs, self.finite_levels, self.nclusters, self.moment_count = (int(x) for x in sizes[:4])
+        self.pair_tolerance, self.radius = float(vals[0]), float(vals[1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_far_plan_destroy(self.h)
+
+    def cluster_info(self, c):
+        centre = np.zeros(3)
+        radius, mind = C.c_double(0), C.c_double(0)
+        npairs = C.c_int64(0)
+        mom = np.zeros(self.moment_count)
+        assert lib().orc_far_plan_cluster(self.h, c, _ptr(centre), C.byref(radius), C.byref(mind), C.byref(npairs),
+                                          _ptr(mom)) == 0
+        return {"centre": centre, "radius": radius.value, "min_distance": mind.value, "npairs": npairs.value,
+                "moments": mom}
+
+    def levels(self, i):
+        out = np.zeros(33, dtype=np.float32)
+        assert lib().orc_far_plan_levels(self.h, i, _ptr(out, C.c_float)) == 0
+        return out
 
 
 def grid_points(grid, start=0, n=None):
