@@ -163,11 +163,12 @@ def run_gpu(mol, grid, precision, steps, warmup, torch):
         fields_ms += st["fields_ms"]
         fields_bytes += st["fields_bytes"]
         launches = st["launches"]
-        rescue_ms, rescue_points = st["rescue_ms"], st["rescue_points"]
+        rescue_ms, rescue_points, rescue_flagged = st["rescue_ms"], st["rescue_points"], st["rescue_flagged"]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"seconds": dt, "points": n * steps, "kernel_ms": kernel_ms / steps, "flops": flops / steps,
             "launches": launches, "out": out, "rescue_ms": rescue_ms, "rescue_points": rescue_points,
+            "rescue_flagged": rescue_flagged,
             "fields_ms": fields_ms / steps, "fields_bytes": fields_bytes / steps}
 
 
@@ -378,13 +379,16 @@ def bench_config2(args, built, torch):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         eager = b.run(graph=False)
-        st = b.timed_run()
+        # per-kernel figures from a run with the streams chained (no two
+        # launches overlap, so each launch's event interval is its own time)
+        st = b.timed_run(serial=True)
         res[prec] = {"value": total * steps / dt, "ms_per_step": 1e3 * dt / steps, "steps": steps,
-                     "ms_without_graph": eager}
+                     "ms_without_graph": eager, "ms_serial": st["ms"]}
         stats[prec] = st
         log(f"{prec}: {res[prec]['value']:.4g} points/s, {res[prec]['ms_per_step']:.1f} ms/step (graph), "
-            f"{eager:.1f} ms without; contraction {st['kernel_ms']:.1f} ms in {st['launches']} launches, "
-            f"rescue {st['rescue_ms']:.1f} ms, fields {st['fields_ms']:.1f} ms")
+            f"{eager:.1f} ms without, {st['ms']:.1f} ms serial; contraction {st['kernel_ms']:.1f} ms in "
+            f"{st['launches']} launches, rescue {st['rescue_ms']:.1f} ms ({st['rescue_flagged']} points flagged, "
+            f"{st['rescue_points']} slots), fields {st['fields_ms']:.1f} ms")
         outs[prec] = o
         b.close()
     # every point against the FP64 GPU path
@@ -455,16 +459,18 @@ def bench_config2(args, built, torch):
         "speedup_vs_fp64_gpu": res[args.precision]["value"] / res["FP64"]["value"],
         "errors": errors,
         "errors_vs_fp64_gpu": vs64,
-        "rescue": {"points": st["rescue_points"], "fraction": st["rescue_points"] / total,
-                   "fp64_kernel_ms": st["rescue_ms"], "criteria": "density or ELF uncertain at the precision"},
+        "rescue": {"flagged_points": st["rescue_flagged"], "fraction": st["rescue_flagged"] / total,
+                   "tile_slots": st["rescue_points"], "fp64_kernel_ms": st["rescue_ms"],
+                   "criteria": "density or ELF uncertain at the precision"},
+        "kernel_ms_serial": {"contraction": st["kernel_ms"], "rescue": st["rescue_ms"], "fields": st["fields_ms"],
+                             "run": st["ms"], "note": "one run with the streams chained (no overlapping launches), "
+                                                      "every launch bracketed by HIP events on its stream"},
         "roofline": roofline_record(st, args.precision, traffic),
         "roofline_fp64_path": roofline_record(stats["FP64"], "FP64"),
         "field_reductions": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "kernel_ms": st["fields_ms"],
                              "bytes": st["fields_bytes"],
                              "achieved": st["fields_bytes"] / (st["fields_ms"] * 1e-3) / 1e9 if st["fields_ms"] else None},
-        "timed_run": {"ms": st["ms"], "note": "launch statistics of one run issued without the graph but, as in the "
-                                              "graph, with its streams' launches overlapping; every contraction launch "
-                                              "bracketed by HIP events on its stream"},
+
         "cpu_baseline": cpu,
     }
     for m in mols:
@@ -570,8 +576,8 @@ def bench_config1(args, c1, torch, sub=True):
            "unit": "grid points/s", "ms_per_step": 1e3 * main_run["seconds"] / steps, "steps": steps,
            "fp64_gpu": fp64, "speedup_vs_fp64_gpu": value / fp64["value"], "errors": errors,
            "errors_full_grid": dict(full, vs="FP64 GPU path, all points"),
-           "rescue": {"points": main_run["rescue_points"], "fraction": main_run["rescue_points"] / n,
-                      "fp64_kernel_ms": main_run["rescue_ms"]},
+           "rescue": {"flagged_points": main_run["rescue_flagged"], "fraction": main_run["rescue_flagged"] / n,
+                      "tile_slots": main_run["rescue_points"], "fp64_kernel_ms": main_run["rescue_ms"]},
            "roofline": roofline_record(st, args.precision, traffic) if kernel_s > 0 else None,
            "cpu_baseline": cpu}
     if sub:

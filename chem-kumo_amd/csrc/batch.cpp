@@ -33,11 +33,11 @@ struct kumo_hip_batch {
     // Launch statistics of the last timed run (use_graph == 2), summed over
     // the executors.
     double kernel_ms = 0.0, flops = 0.0, rescue_ms = 0.0, fields_ms = 0.0;
-    long long launches = 0, rescue_points = 0, fields_bytes = 0;
+    long long launches = 0, rescue_points = 0, rescue_flagged = 0, fields_bytes = 0;
     // Algorithmic FLOPs and rescued points of one run, read back in the first
     // (uncaptured) run: they do not change between runs.
     double warm_flops = 0.0;
-    long long warm_rescue_points = 0;
+    long long warm_rescue_points = 0, warm_rescue_flagged = 0;
 
     ~kumo_hip_batch() {
         if (!execs.empty() && execs[0]->stream) (void)hipStreamSynchronize(execs[0]->stream);
@@ -54,25 +54,45 @@ struct kumo_hip_batch {
     }
 
     // Issues every item on its executor; the executors' streams are forked
-    // from and joined back to stream 0.
-    void issue(bool capture, std::deque<std::vector<char>>* keep) {
+    // from and joined back to stream 0. Serial: each executor's stream starts
+    // when the previous one's work has ended (items issued executor by
+    // executor), so no two launches overlap.
+    void issue(bool capture, std::deque<std::vector<char>>* keep, bool serial = false) {
         hipStream_t s0 = execs[0]->stream;
-        for (std::size_t k = 1; k < execs.size(); ++k) {
-            KUMO_HIP(hipEventRecord(fork[k], s0));
-            KUMO_HIP(hipStreamWaitEvent(execs[k]->stream, fork[k], 0));
-        }
         for (auto& e : execs) {
             e->capture = capture;
             e->keep = keep;
             e->reset_stats();
         }
         const kumo::Advance none = [](std::size_t) {};
-        for (auto& it : items)
+        const auto run_item = [&](Item& it) {
             kumo::run_range(*execs[static_cast<std::size_t>(it.exec)], it.mol, it.pr, it.ps, opt, it.out.data(), 0,
                             it.pr.n, none);
-        for (std::size_t k = 1; k < execs.size(); ++k) {
-            KUMO_HIP(hipEventRecord(join[k], execs[k]->stream));
-            KUMO_HIP(hipStreamWaitEvent(s0, join[k], 0));
+        };
+        if (serial) {
+            for (std::size_t k = 0; k < execs.size(); ++k) {
+                if (k > 0) {
+                    KUMO_HIP(hipEventRecord(fork[k], execs[k - 1]->stream));
+                    KUMO_HIP(hipStreamWaitEvent(execs[k]->stream, fork[k], 0));
+                }
+                for (auto& it : items)
+                    if (static_cast<std::size_t>(it.exec) == k) run_item(it);
+            }
+            if (execs.size() > 1) {
+                KUMO_HIP(hipEventRecord(join[execs.size() - 1], execs.back()->stream));
+                KUMO_HIP(hipStreamWaitEvent(s0, join[execs.size() - 1], 0));
+            }
+        } else {
+            for (std::size_t k = 1; k < execs.size(); ++k) {
+                KUMO_HIP(hipEventRecord(fork[k], s0));
+                KUMO_HIP(hipStreamWaitEvent(execs[k]->stream, fork[k], 0));
+            }
+            for (auto& it : items)
+                run_item(it);
+            for (std::size_t k = 1; k < execs.size(); ++k) {
+                KUMO_HIP(hipEventRecord(join[k], execs[k]->stream));
+                KUMO_HIP(hipStreamWaitEvent(s0, join[k], 0));
+            }
         }
         for (auto& e : execs) {
             e->capture = false;
@@ -173,6 +193,7 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
                 KUMO_HIP(hipStreamSynchronize(e->stream));
                 b->warm_flops += e->last_flops;
                 b->warm_rescue_points += e->last_rescue_points;
+                b->warm_rescue_flagged += e->last_rescue_flagged;
             }
             b->warmed = true;
         }
@@ -189,18 +210,19 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
             KUMO_HIP(hipStreamEndCapture(s0, &b->graph));
             KUMO_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
         }
-        if (use_graph == 2) {
+        if (use_graph == 2 || use_graph == 3) {
             // Timed run: every launch issued again without a graph but, as
             // in the graph, with no host synchronisation, so the streams'
-            // launches overlap as they do on replay; each contraction, rescue
-            // and field launch is bracketed by events on its stream. The
-            // FLOPs and rescued points are those read back in the first run.
+            // launches overlap as they do on replay (3: the streams chained,
+            // no overlap); each contraction, rescue and field launch is
+            // bracketed by events on its stream. The FLOPs and rescued points
+            // are those read back in the first run.
             b->run_keep.clear();
             for (auto& e : b->execs)
                 e->events_in_capture = true;
             KUMO_HIP(hipEventRecord(b->t0, s0));
             try {
-                b->issue(true, &b->run_keep);
+                b->issue(true, &b->run_keep, use_graph == 3);
             } catch (...) {
                 for (auto& e : b->execs)
                     e->events_in_capture = false;
@@ -212,6 +234,7 @@ int kumo_hip_batch_run(kumo_hip_batch* b, int32_t use_graph) {
             b->launches = b->fields_bytes = 0;
             b->flops = b->warm_flops;
             b->rescue_points = b->warm_rescue_points;
+            b->rescue_flagged = b->warm_rescue_flagged;
             for (auto& e : b->execs) {
                 e->events_in_capture = false;
                 KUMO_HIP(hipStreamSynchronize(e->stream));
@@ -249,7 +272,8 @@ int kumo_hip_batch_last_ms(const kumo_hip_batch* b, double* ms) {
 }
 
 int kumo_hip_batch_kernel_stats(const kumo_hip_batch* b, double* kernel_ms, int64_t* launches, double* flops,
-                                double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes) {
+                                double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes,
+                                int64_t* rescue_flagged) {
     if (!b) return KUMO_HIP_INVALID_ARGUMENT;
     if (kernel_ms) *kernel_ms = b->kernel_ms;
     if (launches) *launches = b->launches;
@@ -258,6 +282,7 @@ int kumo_hip_batch_kernel_stats(const kumo_hip_batch* b, double* kernel_ms, int6
     if (rescue_points) *rescue_points = b->rescue_points;
     if (fields_ms) *fields_ms = b->fields_ms;
     if (fields_bytes) *fields_bytes = b->fields_bytes;
+    if (rescue_flagged) *rescue_flagged = b->rescue_flagged;
     return KUMO_HIP_OK;
 }
 

@@ -312,10 +312,12 @@ int kumo_hip_last_cube_stats(const kumo_hip_context* ctx, double* total_ms, doub
     return KUMO_HIP_OK;
 }
 
-int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points) {
+int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points,
+                               int64_t* rescue_flagged) {
     if (!ctx) return KUMO_HIP_INVALID_ARGUMENT;
     if (rescue_ms) *rescue_ms = ctx->exec.last_rescue_ms;
     if (rescue_points) *rescue_points = ctx->exec.last_rescue_points;
+    if (rescue_flagged) *rescue_flagged = ctx->exec.last_rescue_flagged;
     return KUMO_HIP_OK;
 }
 

@@ -310,7 +310,7 @@ RescuePlan rescue_plan(Executor& ex, long long count, int nblk, int block_size, 
     const std::size_t tiles = static_cast<std::size_t>(nblk) * static_cast<std::size_t>(tiles_per_block);
     const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };
     const std::size_t sizes[] = {static_cast<std::size_t>(count), 2 * pts, 4 * static_cast<std::size_t>(nblk),
-                                 4 * tiles, 4 * tiles, 4};
+                                 4 * tiles, 4 * tiles, 8};
     std::size_t total = 0;
     for (std::size_t x : sizes)
         total += al(x);
@@ -543,7 +543,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         // end (or after each batch when a progress callback must see
         // completed work).
         const long long nbatches = (b_end - b_first + cap - 1) / cap;
-        const std::size_t slot = static_cast<std::size_t>(cap) + 1;
+        const std::size_t slot = static_cast<std::size_t>(cap) + 2;
         int32_t* hst = ex.capture ? nullptr : ex.host_stats(static_cast<std::size_t>(nbatches) * slot);
         if (hst) std::fill(hst, hst + static_cast<std::size_t>(nbatches) * slot, 0);
         const bool stepwise = opt.progress != nullptr;
@@ -622,7 +622,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                     launch_rescue_scatter(r.moments, rslots, groups, kRescueTile, bs, rp, chm, count, s);
                     if (hst)
                         KUMO_HIP(hipMemcpyAsync(hst + static_cast<std::size_t>((b0 - b_first) / cap) * slot + cap,
-                                                rp.total, 4, hipMemcpyDeviceToHost, s));
+                                                rp.total, 8, hipMemcpyDeviceToHost, s));
                 }
             } else {
                 AoPLaunch a{};
@@ -681,6 +681,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 const long long b0 = b_first + bb * cap;
                 const int nb = static_cast<int>(std::min<long long>(cap, b_end - b0));
                 ex.last_rescue_points += static_cast<long long>(hb[cap]) * kRescueTile;
+                ex.last_rescue_flagged += hb[cap + 1];
                 for (int b = 0; b < nb; ++b) {
                     const long long pts_b = std::min<long long>(bs, n - (b0 + b) * static_cast<long long>(bs));
                     const double m_b = hb[b];
@@ -1019,7 +1020,7 @@ void Executor::reset_stats() {
     events_used[0] = events_used[1] = events_used[2] = 0;
     last_kernel_ms = last_rescue_ms = last_fields_ms = 0.0;
     last_flops = 0.0;
-    last_launches = last_rescue_points = last_fields_points = last_fields_bytes = 0;
+    last_launches = last_rescue_points = last_rescue_flagged = last_fields_points = last_fields_bytes = 0;
 }
 
 void Executor::collect_timing() {

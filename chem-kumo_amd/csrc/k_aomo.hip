@@ -169,10 +169,12 @@ template <int PREC, unsigned PSET, int TP> constexpr int occupancy() {
     return popcount10(PSET) > 7 ? 1 : (TP == 16 && PREC != KUMO_HIP_FP64 ? 3 : 2);
 }
 
+// One workgroup tile: T points of a screening block (tile index tx in the
+// grid's x order) or, in rescue mode, rescue tile tx.
 // WSUM: weighted mode (orbital-energy sums, FP64 values only): instead of the
 // channel moments, per point the sums over slots i of w_s(i) psi_i^2.
-template <int PREC, int ORDER, unsigned PSET, int TP, bool WSUM = false>
-__global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kernel(AoMoLaunch a) {
+template <int PREC, int ORDER, unsigned PSET, int TP, bool WSUM>
+__device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch& a, const int tx) {
     using P = Prec<PREC>;
     using E = typename P::E;
     static_assert(TP == 16 || TP == 32, "16 or 32 points per workgroup");
@@ -200,16 +202,15 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     int blk_local, p0 = 0, nt_valid;
     const uint16_t* roff = nullptr;
     if (a.r_tile_blk) {
-        const int t = blockIdx.x;
-        if (t >= *a.r_total) return; // uniform over the workgroup
+        const int t = tx;
         blk_local = a.r_tile_blk[t];
         const int sub = a.r_tile_sub[t];
         nt_valid = min(T, a.r_count[blk_local] - sub * T);
         roff = a.r_offs + static_cast<long long>(blk_local) * a.block_size + sub * T;
     } else {
         const int tiles_per_block = (a.block_size + T - 1) / T;
-        blk_local = blockIdx.x / tiles_per_block;
-        p0 = (blockIdx.x % tiles_per_block) * T;
+        blk_local = tx / tiles_per_block;
+        p0 = (tx % tiles_per_block) * T;
         const long long bs0 = (a.first_block + blk_local) * a.block_size;
         const int bcount = static_cast<int>(min(static_cast<long long>(a.block_size), a.npoints - bs0));
         if (p0 >= bcount) return; // uniform over the workgroup
@@ -665,7 +666,7 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
             for (int mt = mt0; mt < mt1; ++mt)
                 s += red[(static_cast<size_t>(mt) * kMoments + k) * T + jj];
         const double f = k == 0 ? 1.0 : (k <= 3 ? 2.0 : (k <= 6 ? 0.5 : 2.0));
-        const long long point = roff ? static_cast<long long>(blockIdx.x) * T + jj : bstart + p0 + jj - a.batch_first;
+        const long long point = roff ? static_cast<long long>(tx) * T + jj : bstart + p0 + jj - a.batch_first;
         double* dst = a.moments + (static_cast<long long>(blockIdx.y) * 2 + ch) * kMoments * a.batch_points;
         dst[k * a.batch_points + point] = f * s;
         if (k == 0 && scale && blockIdx.y == 0) {
@@ -680,6 +681,27 @@ __global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kerne
     }
 }
 
+// Rescue mode (FP64 only) runs a bounded grid whose workgroups take the
+// rescue tiles in turn, so that the grid need not match the tile count known
+// only on the device.
+template <int PREC, int ORDER, unsigned PSET, int TP, bool WSUM = false>
+__global__ void __launch_bounds__(256, (occupancy<PREC, PSET, TP>())) aomo_kernel(AoMoLaunch a) {
+    if constexpr (PREC == KUMO_HIP_FP64 && !WSUM) {
+        if (a.r_tile_blk) {
+            const int total = *a.r_total; // uniform over the workgroup
+            for (int t = blockIdx.x; t < total; t += gridDim.x) {
+                aomo_tile<PREC, ORDER, PSET, TP, WSUM>(a, t);
+                __syncthreads(); // the next tile reuses the LDS
+            }
+            return;
+        }
+    }
+    aomo_tile<PREC, ORDER, PSET, TP, WSUM>(a, blockIdx.x);
+}
+
+// Workgroups of a rescue launch: enough to fill every CU several times.
+constexpr unsigned kRescueGroups = 2048;
+
 template <int PREC, int ORDER, unsigned PSET, int TP, bool WSUM = false>
 void launch_one(const AoMoLaunch& a, hipStream_t stream) {
     using P = Prec<PREC>;
@@ -689,8 +711,11 @@ void launch_one(const AoMoLaunch& a, hipStream_t stream) {
     const size_t lds = work > red ? work : red;
     const int tiles_per_block = (a.block_size + TP - 1) / TP;
     const int groups = (a.slot1 - a.slot0 + 127) / 128;
-    // (rescue mode: the same bound, every block's points in tiles of T)
-    const dim3 grid(static_cast<unsigned>(a.nblk * tiles_per_block), static_cast<unsigned>(groups));
+    // (rescue mode: at most every block's points in tiles of T, taken in turn
+    // by at most kRescueGroups workgroups)
+    unsigned gx = static_cast<unsigned>(a.nblk * tiles_per_block);
+    if (a.r_tile_blk && gx > kRescueGroups) gx = kRescueGroups;
+    const dim3 grid(gx, static_cast<unsigned>(groups));
     static bool configured = false;
     if (!configured) {
         KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&aomo_kernel<PREC, ORDER, PSET, TP, WSUM>),

@@ -432,18 +432,24 @@ __global__ void __launch_bounds__(256) rescue_compact_kernel(const uint8_t* flag
 }
 
 // Tiles of `tile` flagged points: tile t covers points [16 sub, ...) of the
-// flagged list of block tile_blk[t]; total receives the number of tiles.
+// flagged list of block tile_blk[t]; total[0] receives the number of tiles,
+// total[1] the number of flagged points.
 __global__ void __launch_bounds__(1024) rescue_tiles_kernel(const int32_t* count, int nblk, int tile,
                                                             int32_t* tile_blk, int32_t* tile_sub, int32_t* total) {
     // Each thread takes a run of consecutive blocks; one scan over the
     // workgroup of the runs' tile counts gives every run its first tile.
-    __shared__ int wsum[16];
+    __shared__ int wsum[16], fsum[16];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int per = (nblk + 1023) / 1024;
     const int b0 = min(nblk, tid * per), b1 = min(nblk, b0 + per);
-    int local = 0;
-    for (int b = b0; b < b1; ++b)
+    int local = 0, flagged = 0;
+    for (int b = b0; b < b1; ++b) {
         local += (count[b] + tile - 1) / tile;
+        flagged += count[b];
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        flagged += __shfl_xor(flagged, o, 64);
+    if (lane == 0) fsum[w] = flagged;
     int x = local;
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(x, o, 64);
@@ -462,7 +468,13 @@ __global__ void __launch_bounds__(1024) rescue_tiles_kernel(const int32_t* count
         }
         off += nt;
     }
-    if (tid == 1023) *total = off; // the last run ends at the total
+    if (tid == 1023) total[0] = off; // the last run ends at the total
+    if (tid == 0) {
+        int f = 0;
+        for (int k = 0; k < 16; ++k)
+            f += fsum[k];
+        total[1] = f;
+    }
 }
 
 // Writes the rescued channel moments (summed over MO slot groups) over the

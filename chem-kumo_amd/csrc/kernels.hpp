@@ -91,7 +91,7 @@ struct RescuePlan {
     int32_t* count;      // [blocks]
     int32_t* tile_blk;   // [max tiles]
     int32_t* tile_sub;   // [max tiles]
-    int32_t* total;      // tiles in use
+    int32_t* total;      // [2]: tiles in use, flagged points
 };
 // What a point is rescued for (output bits: 0 total, 1 alpha, 2 beta).
 //  ELF: the channel moments carry a relative error elf_eps; rescue where the

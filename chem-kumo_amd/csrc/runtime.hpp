@@ -33,7 +33,8 @@ struct Executor {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
     std::size_t events_used[3] = {0, 0, 0};
     double last_kernel_ms = 0.0, last_flops = 0.0, last_rescue_ms = 0.0, last_fields_ms = 0.0;
-    long long last_launches = 0, last_rescue_points = 0, last_fields_points = 0, last_fields_bytes = 0;
+    long long last_launches = 0, last_rescue_points = 0, last_rescue_flagged = 0, last_fields_points = 0,
+              last_fields_bytes = 0;
     // Pinned host memory for per-batch statistics copied back without
     // stopping the stream (kept AO counts, rescue tiles).
     int32_t* hstats = nullptr;

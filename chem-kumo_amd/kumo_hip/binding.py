@@ -124,13 +124,13 @@ def lib():
         L.kumo_hip_boys.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_boys_esp.argtypes = [vp, vp, C.c_size_t, C.c_int32, vp]
         L.kumo_hip_last_kernel_stats.argtypes = [vp, vp, vp, vp, vp]
-        L.kumo_hip_last_rescue_stats.argtypes = [vp, vp, vp]
+        L.kumo_hip_last_rescue_stats.argtypes = [vp, vp, vp, vp]
         L.kumo_hip_last_field_stats.argtypes = [vp, vp, vp, vp]
         L.kumo_hip_batch_create.argtypes = [vp, C.c_size_t, vp, vp, vp, C.c_size_t, C.POINTER(_Options), C.c_int32,
                                             vp, vp]
         L.kumo_hip_batch_run.argtypes = [vp, C.c_int32]
         L.kumo_hip_batch_last_ms.argtypes = [vp, vp]
-        L.kumo_hip_batch_kernel_stats.argtypes = [vp] + [vp] * 7
+        L.kumo_hip_batch_kernel_stats.argtypes = [vp] + [vp] * 8
         L.kumo_hip_batch_release.argtypes = [vp]
         L.kumo_hip_server_create.argtypes = [vp, C.POINTER(_ServerOptions), vp]
         L.kumo_hip_server_submit.argtypes = [vp, vp, vp, vp, C.c_size_t, vp, C.c_size_t, C.POINTER(_Options), vp, vp]
@@ -202,12 +202,12 @@ class Context:
         ms, fl, pfl = C.c_double(), C.c_double(), C.c_double()
         n = C.c_int64()
         lib().kumo_hip_last_kernel_stats(self.h, C.byref(ms), C.byref(n), C.byref(fl), C.byref(pfl))
-        rms, rpts = C.c_double(), C.c_int64()
-        lib().kumo_hip_last_rescue_stats(self.h, C.byref(rms), C.byref(rpts))
+        rms, rpts, rfl = C.c_double(), C.c_int64(), C.c_int64()
+        lib().kumo_hip_last_rescue_stats(self.h, C.byref(rms), C.byref(rpts), C.byref(rfl))
         fms, fpts, fbytes = C.c_double(), C.c_int64(), C.c_int64()
         lib().kumo_hip_last_field_stats(self.h, C.byref(fms), C.byref(fpts), C.byref(fbytes))
         return {"kernel_ms": ms.value, "launches": n.value, "flops": fl.value, "padded_flops": pfl.value,
-                "rescue_ms": rms.value, "rescue_points": rpts.value, "fields_ms": fms.value,
+                "rescue_ms": rms.value, "rescue_points": rpts.value, "rescue_flagged": rfl.value, "fields_ms": fms.value,
                 "fields_points": fpts.value, "fields_bytes": fbytes.value}
 
 
@@ -556,19 +556,21 @@ class Batch:
         lib().kumo_hip_batch_last_ms(self.h, C.byref(ms))
         return ms.value
 
-    def timed_run(self):
+    def timed_run(self, serial=False):
         """One run without a graph with every contraction, rescue and field
-        launch bracketed by HIP events on its stream; returns the run's device
-        ms and the launch statistics (kumo_hip_batch_kernel_stats)."""
-        self.ctx.check(lib().kumo_hip_batch_run(self.h, 2))
+        launch bracketed by HIP events on its stream (serial: the streams
+        chained, so no launches overlap); returns the run's device ms and the
+        launch statistics (kumo_hip_batch_kernel_stats)."""
+        self.ctx.check(lib().kumo_hip_batch_run(self.h, 3 if serial else 2))
         ms = C.c_double()
         lib().kumo_hip_batch_last_ms(self.h, C.byref(ms))
         km, fl, rms, fms = C.c_double(), C.c_double(), C.c_double(), C.c_double()
-        n, rp, fb = C.c_int64(), C.c_int64(), C.c_int64()
+        n, rp, fb, rf = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
         self.ctx.check(lib().kumo_hip_batch_kernel_stats(self.h, C.byref(km), C.byref(n), C.byref(fl), C.byref(rms),
-                                                         C.byref(rp), C.byref(fms), C.byref(fb)))
+                                                         C.byref(rp), C.byref(fms), C.byref(fb), C.byref(rf)))
         return {"ms": ms.value, "kernel_ms": km.value, "launches": n.value, "flops": fl.value, "rescue_ms": rms.value,
-                "rescue_points": rp.value, "fields_ms": fms.value, "fields_bytes": fb.value}
+                "rescue_points": rp.value, "rescue_flagged": rf.value, "fields_ms": fms.value,
+                "fields_bytes": fb.value}
 
     def close(self):
         if self.h:

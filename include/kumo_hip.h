@@ -233,15 +233,18 @@ int kumo_hip_batch_last_ms(const kumo_hip_batch* batch, double* ms);
 /* kumo_hip_batch_run with use_graph = 2 issues every launch again without a
  * graph but, as in the graph, without host synchronisation (the streams'
  * launches overlap as on replay), bracketing each contraction, FP64 rescue
- * and field-reduction launch with HIP events on its stream. Afterwards this
- * returns, summed over the run: the contraction kernels' device time (ms,
- * each launch from its start to its end on its stream), their launches and
- * algorithmic FLOPs (2 x kept AOs x MOs x panels x points per screening
- * block, read back in the batch's first run), the rescue launches' time and
- * point slots, and the field reductions' time and bytes moved. Measurement
- * only. */
+ * and field-reduction launch with HIP events on its stream; use_graph = 3
+ * does the same with the streams chained one after another, so that no two
+ * launches overlap and each bracketed duration is the kernel's own.
+ * Afterwards this returns, summed over the run: the contraction kernels'
+ * device time (ms, each launch from its start to its end on its stream),
+ * their launches and algorithmic FLOPs (2 x kept AOs x MOs x panels x points
+ * per screening block, read back in the batch's first run), the rescue
+ * launches' time, point slots (16 per tile) and flagged points, and the field
+ * reductions' time and bytes moved. Measurement only. */
 int kumo_hip_batch_kernel_stats(const kumo_hip_batch* batch, double* kernel_ms, int64_t* launches, double* flops,
-                                double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes);
+                                double* rescue_ms, int64_t* rescue_points, double* fields_ms, int64_t* fields_bytes,
+                                int64_t* rescue_flagged);
 int kumo_hip_batch_release(kumo_hip_batch* batch);
 
 /* ---- serving loop ----
@@ -394,13 +397,15 @@ int kumo_hip_write_ply(kumo_hip_context* ctx, const char* path, const double* ve
 int kumo_hip_last_kernel_stats(const kumo_hip_context* ctx, double* kernel_ms, int64_t* launches,
                                double* algorithmic_flops, double* padded_flops);
 
-/* Device time (ms) of the last evaluation's FP64 rescue launches and the
- * number of point slots they covered (points in 16-point tiles). With a
+/* Device time (ms) of the last evaluation's FP64 rescue launches, the
+ * number of point slots they covered (points in 16-point tiles) and the
+ * number of flagged points among them. With a
  * reduced contraction precision, the points whose density (relative error
  * bound above 5e-7) or ELF (error bound above 3e-6) the precision leaves
  * uncertain are recomputed in FP64; no counterpart in the reference (its CPU
  * path is FP64 throughout). */
-int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points);
+int kumo_hip_last_rescue_stats(const kumo_hip_context* ctx, double* rescue_ms, int64_t* rescue_points,
+                               int64_t* rescue_flagged);
 
 /* Device time (ms) of the last evaluation's field reductions (the density
  * components and derived fields formed from the channel moments, store_block

@@ -71,7 +71,9 @@ def oracle_surface(name, mode):
     iso = MAPPED[kind][1] if kind in MAPPED else 1e-3
     v, t = oracle.density_isosurface(mol, iso, 0.25)
     prop = {"esp": K.ElectrostaticPotential, "alie": K.AverageLocalIonizationEnergy}.get(kind) or MAPPED[kind][0]
-    values = mol.evaluate([(prop, 0, 0, 0)], points=v)[0][0][0]
+    # the GPU sums the potential exactly; the oracle's default tolerance
+    # (1e-8) would take its far-field approximation
+    values = mol.evaluate([(prop, 0, 0, 0)], points=v, potential_tolerance=0.0)[0][0][0]
     stats = oracle.surface_statistics(v, t, values, 0.0 if kind in MAPPED else 10.0 / KCAL)
     return {"vertices": v, "triangles": t, "values": values, "stats": stats}
 
