@@ -86,6 +86,38 @@ __device__ __attribute__((always_inline)) inline void point_coords(const PointSo
     z = ps.origin[2] + ((ps.steps[2] * fi + ps.steps[5] * fj) + ps.steps[8] * fk);
 }
 
+// point_coords of point q0 + d (0 <= d < 2^30): the grid indices of q0 are
+// found once (the 64-bit divisions are the same for every lane of a tile when
+// q0 is uniform) and advanced by d in 32-bit arithmetic; the coordinates are
+// point_coords' bit for bit.
+__device__ __attribute__((always_inline)) inline void point_coords_from(const PointSource& ps, long long q0, int d,
+                                                                      double& x, double& y, double& z) {
+#pragma clang fp contract(off)
+    if (ps.points) {
+        point_coords(ps, q0 + d, x, y, z);
+        return;
+    }
+    const long long plane = ps.ny * ps.nz;
+    const long long i0 = q0 / plane, r0 = q0 - i0 * plane;
+    const long long j0 = r0 / ps.nz, k0 = r0 - j0 * ps.nz;
+    // k0 + d < nz + 2^30 and j0 + dj < ny + 2^30 fit 32 bits when nz, ny < 2^31
+    if (ps.nz >= (1ll << 31) || ps.ny >= (1ll << 31)) {
+        point_coords(ps, q0 + d, x, y, z);
+        return;
+    }
+    const unsigned nz = static_cast<unsigned>(ps.nz), ny = static_cast<unsigned>(ps.ny);
+    const unsigned t = static_cast<unsigned>(k0) + static_cast<unsigned>(d);
+    const unsigned dj = t / nz, k = t - dj * nz;
+    const unsigned tj = static_cast<unsigned>(j0) + dj;
+    const unsigned di = tj / ny, jj = tj - di * ny;
+    const double fi = static_cast<double>(i0 + di);
+    const double fj = static_cast<double>(jj);
+    const double fk = static_cast<double>(k);
+    x = ps.origin[0] + ((ps.steps[0] * fi + ps.steps[3] * fj) + ps.steps[6] * fk);
+    y = ps.origin[1] + ((ps.steps[1] * fi + ps.steps[4] * fj) + ps.steps[7] * fk);
+    z = ps.origin[2] + ((ps.steps[2] * fi + ps.steps[5] * fj) + ps.steps[8] * fk);
+}
+
 __device__ __attribute__((always_inline)) inline double sq_norm(double x, double y, double z) {
 #pragma clang fp contract(off)
     return (x * x + y * y) + z * z;

@@ -50,7 +50,11 @@ struct DevShell {
 // A block's data for the contraction kernels in one contiguous record, so
 // that a lane reads everything it needs for a row with independent loads:
 // centre, cutoff, l and up to kRecPrim primitives (exponent, factored
-// weight radw); blocks with more primitives read exps[] / radw[] instead.
+// weight radw, skip radius); blocks with more primitives read exps[] /
+// radw[] instead. Beyond its skip radius, r^2 > skip[p], primitive p adds
+// less than 1e-40 to each radial sum (|radw| max(1, 4 a^2) exp(-a r^2) <
+// 1e-40), and the wave-uniform contraction kernel leaves its exponential
+// out.
 constexpr int kRecPrim = 8;
 struct ShellRec {
     double o[3];
@@ -58,6 +62,7 @@ struct ShellRec {
     int32_t l, nprim, prim, pad;
     double e[kRecPrim];
     double w[kRecPrim];
+    double skip[kRecPrim];
 };
 
 // Sparse transform entry: pure row `row` += coef * Cartesian component `cart`.

@@ -32,7 +32,8 @@ void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
     if (o.backend == KUMO_HIP_AO_MO && !mol->eligible)
         invalid("AO_MO backend requested but orbital representation is not eligible");
     if (o.threads > 1) invalid("explicit multi-thread control is not implemented in the API adapter");
-    if (o.tile_points != 0 && o.tile_points != 16 && o.tile_points != 32) invalid("tile_points must be 0, 16 or 32");
+    if (o.tile_points != 0 && o.tile_points != 16 && o.tile_points != 32 && o.tile_points != 64)
+        invalid("tile_points must be 0, 16, 32 or 64");
 }
 
 // The checks of OrbitalEnergyFields (src/orbital_energy_fields.hpp:117-190,
@@ -271,6 +272,7 @@ constexpr int kTuneBlocks = 256;
 
 int tuned_tile(Executor& ex, kumo_hip_molecule* mol, AoMoLaunch a, hipStream_t s) {
     if (a.precision == KUMO_HIP_FP64) return 16;
+    if (ao_mo_wave_supported(a)) return 64;
     const int key = (a.precision * 4 + a.order) * 4 + a.hessian;
     {
         std::lock_guard<std::mutex> lock(mol->tune_mutex);
@@ -582,6 +584,8 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                                   ? grow<float>(ex.ws_rscale, 4 * static_cast<std::size_t>(batch_max))
                                   : nullptr;
                 a.tile_points = opt.tile_points ? opt.tile_points : tuned_tile(ex, mol, a, s);
+                if (a.tile_points == 64 && !ao_mo_wave_supported(a))
+                    invalid("tile_points 64 takes the split-bf16 precisions without full Hessians");
                 record(ex, true);
                 launch_ao_mo(a, s);
                 record(ex, false);

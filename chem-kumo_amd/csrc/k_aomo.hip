@@ -53,14 +53,11 @@
 
 #include "../../include/kumo_hip.h"
 #include "common.hpp"
+#include "aomo_common.hpp"
 
 namespace kumo {
 namespace {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef short s16x8 __attribute__((ext_vector_type(8)));
+using namespace aomo;
 
 constexpr int KC = kChunkRows; // 32 operand rows per chunk
 
@@ -95,71 +92,9 @@ template <> struct Prec<KUMO_HIP_BF16X6> : PrecBf16<3> {};
 
 template <int PREC> constexpr bool is_bf16 = PREC == KUMO_HIP_BF16X3 || PREC == KUMO_HIP_BF16X6;
 
-// Moments a contraction can make nonzero: the value; + gradient and tau terms
-// (order 1); + the trace (Laplacian panel), the diagonal or all six Hessian
-// moments (order 2). The others are written as zeros without being summed.
-template <int ORDER, unsigned PSET> __host__ __device__ constexpr bool moment_used(int k) {
-    if (k == 0) return true;
-    if (ORDER == 0) return false;
-    if (k <= 6) return true;
-    if (ORDER == 1) return false;
-    if ((PSET >> kPanelLap) & 1u) return k == 7;
-    if (PSET == kPanelsDiag) return k == 7 || k == 10 || k == 12;
-    return true;
-}
-
 // Element offset of row r (< 32) of point j in a bf16 Phi plane.
 __device__ __attribute__((always_inline)) inline int phi_bf16_offset(int j, int r) {
     return j * 32 + ((((r >> 3) ^ (j >> 2)) & 3) << 3) + (r & 7);
-}
-
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-
-// Splits two FP64 values into S bf16 terms, packed in pairs (element 0 in the
-// low half): t0 = bf16(v); the residual v - t0, exact in FP64, is rounded
-// once to FP32 (2^-33 relative to v) and split into the remaining terms
-// (each FP32 residual exact), so three terms represent v to about 2^-26. The empty asm keeps
-// each FP64->FP32 rounding apart from the bf16 rounding after it (the
-// compiler would fuse them into a slower direct FP64->bf16 rounding;
-// rounding twice is exact here since FP32 has more than twice bf16's
-// precision).
-template <int S>
-__device__ __attribute__((always_inline)) inline void split_pair(double v0, double v1, uint32_t (&t)[S]) {
-    f32x2 f = {static_cast<float>(v0), static_cast<float>(v1)};
-    asm("" : "+v"(f));
-    t[0] = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
-    f32x2 r = {static_cast<float>(v0 - static_cast<double>(__uint_as_float(t[0] << 16))),
-               static_cast<float>(v1 - static_cast<double>(__uint_as_float(t[0] & 0xffff0000u)))};
-    asm("" : "+v"(r));
-#pragma unroll
-    for (int k = 1; k < S; ++k) {
-        t[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
-        if (k + 1 < S) r = f32x2{r.x - __uint_as_float(t[k] << 16), r.y - __uint_as_float(t[k] & 0xffff0000u)};
-    }
-}
-
-// Splits two FP32 values into S bf16 terms, packed in pairs (element 0 in
-// the low half): each term is the bf16 rounding of what the previous terms
-// leave, every residual exact in FP32; three terms carry the 24 bits.
-template <int S>
-__device__ __attribute__((always_inline)) inline void split_pair_f32(float v0, float v1, uint32_t (&t)[S]) {
-    f32x2 r = {v0, v1};
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        t[k] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
-        if (k + 1 < S) r = f32x2{r.x - __uint_as_float(t[k] << 16), r.y - __uint_as_float(t[k] & 0xffff0000u)};
-    }
-}
-template <int S>
-__device__ __attribute__((always_inline)) inline void split_f32(float v, uint16_t (&t)[S]) {
-    float r = v;
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        const __bf16 h = static_cast<__bf16>(r);
-        t[k] = __builtin_bit_cast(uint16_t, h);
-        if (k + 1 < S) r -= static_cast<float>(h);
-    }
 }
 
 // Workgroups per CU the register budget is sized for: 32-point tiles keep
@@ -226,6 +161,7 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
     const MolView& m = a.mol;
     const int nsh = m.nshells;
     const int32_t* ent = a.tables.ent + static_cast<long long>(blk_local) * nsh;
+    const uint32_t* ebits = a.tables.ent_bits + static_cast<long long>(blk_local) * nsh;
     const uint8_t* erow = a.tables.ent_row + static_cast<long long>(blk_local) * nsh;
     const int32_t* cb = a.tables.chunk_begin + static_cast<long long>(blk_local) * (a.tables.max_chunks + 1);
     const int32_t* crows = a.tables.chunk_rows + static_cast<long long>(blk_local) * a.tables.max_chunks * KC * 2;
@@ -280,9 +216,8 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
             prow_info[i] = crc[2 * (r0 + i) + 1];
         }
         const int e1n = cb[cc + 1];
-        const int last = ent[e1n - 1];
-        const DevShell& ls = m.shells[last & 0xffffff];
-        next_rows_used = erow[e1n - 1] + ((last >> 24) == kEntryPure ? ls.count : ls.ncart);
+        const DevShell& ls = m.shells[ent[e1n - 1] & 0xffffff];
+        next_rows_used = erow[e1n - 1] + ls.ncart;
     };
     if (nchunks > 0) fetch(0);
 
@@ -295,42 +230,85 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
             info_row[i] = prow_info[i];
         }
         // 1. Coefficient rows of the chunk (zero past its rows).
+        // Folded rows (partially kept pure blocks, rare) are formed in a second
+        // pass, so that the plain copies keep their loads together.
+        bool deferred = false;
         if constexpr (is_bf16<PREC>) {
             // rows 2q and 2q + 1, 8 slots each, interleaved into 32-bit words
-#pragma unroll
-            for (int u = 0; u < NSTAGE; ++u) {
+            const auto item_bf16 = [&](int u, bool fold) __attribute__((always_inline)) {
                 const int item = tid + 256 * u;
                 const int gi = item % GMAX;
                 const int q = (item / GMAX) % (KC / 2);
                 const int sp = item / ((KC / 2) * GMAX);
-                if (gi >= G) continue;
+                if (gi >= G) return;
                 const int g0 = pg0[u], g1 = pg1[u];
                 const E* src = op + sp * op_plane + slot_lo + gi * EPG;
                 uint4 v0 = uint4{0, 0, 0, 0}, v1 = uint4{0, 0, 0, 0};
-                if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
-                if (g1 >= 0) v1 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
+                if (!fold) {
+                    if (g0 < -1 || g1 < -1) {
+                        deferred = true;
+                        return;
+                    }
+                    if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
+                    if (g1 >= 0) v1 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
+                } else {
+                    if (!(g0 < -1 || g1 < -1)) return;
+                    if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
+                    else if (g0 < -1) v0 = folded_bf16(m, a.mo, ent, ebits, g0, slot_lo + gi * EPG, sp);
+                    if (g1 >= 0) v1 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
+                    else if (g1 < -1) v1 = folded_bf16(m, a.mo, ent, ebits, g1, slot_lo + gi * EPG, sp);
+                }
                 const auto lo = [](uint32_t x, uint32_t y) { return (x & 0xffffu) | (y << 16); };
                 const auto hi = [](uint32_t x, uint32_t y) { return (x >> 16) | (y & 0xffff0000u); };
                 uint4* d = reinterpret_cast<uint4*>(reinterpret_cast<uint32_t*>(cs + sp * CS_PLANE) + q * PITCH +
                                                     gi * EPG);
                 d[0] = uint4{lo(v0.x, v1.x), hi(v0.x, v1.x), lo(v0.y, v1.y), hi(v0.y, v1.y)};
                 d[1] = uint4{lo(v0.z, v1.z), hi(v0.z, v1.z), lo(v0.w, v1.w), hi(v0.w, v1.w)};
-            }
-        } else {
+            };
 #pragma unroll
-            for (int u = 0; u < NSTAGE; ++u) {
+            for (int u = 0; u < NSTAGE; ++u)
+                item_bf16(u, false);
+            if (deferred)
+                for (int u = 0; u < NSTAGE; ++u)
+                    item_bf16(u, true);
+        } else {
+            const auto item_plain = [&](int u, bool fold) __attribute__((always_inline)) {
                 const int item = tid + 256 * u;
                 const int gi = item % GMAX;
                 const int r = (item / GMAX) % KC;
                 const int sp = item / (KC * GMAX);
-                if (gi >= G) continue;
+                if (gi >= G) return;
                 const int grow = pg0[u];
+                if ((grow < -1) != fold) {
+                    deferred = deferred || grow < -1;
+                    return;
+                }
                 uint4 v = uint4{0, 0, 0, 0};
-                if (grow >= 0)
+                if (grow >= 0) {
                     v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
                                                         slot_lo + gi * EPG);
+                } else if (grow < -1) {
+                    double f[EPG];
+                    folded_row<EPG>(m, a.mo, ent, ebits, grow, slot_lo + gi * EPG, f);
+                    if constexpr (PREC == KUMO_HIP_FP64) {
+                        v = __builtin_bit_cast(uint4, double2{f[0], f[1]});
+                    } else {
+                        float4 h;
+                        h.x = static_cast<float>(f[0]);
+                        h.y = static_cast<float>(f[1 % EPG]);
+                        h.z = static_cast<float>(f[2 % EPG]);
+                        h.w = static_cast<float>(f[3 % EPG]);
+                        v = __builtin_bit_cast(uint4, h);
+                    }
+                }
                 *reinterpret_cast<uint4*>(cs + sp * CS_PLANE + r * PITCH + gi * EPG) = v;
-            }
+            };
+#pragma unroll
+            for (int u = 0; u < NSTAGE; ++u)
+                item_plain(u, false);
+            if (deferred)
+                for (int u = 0; u < NSTAGE; ++u)
+                    item_plain(u, true);
         }
         // 2. Phi: thread (point j, row group) evaluates RPL consecutive rows of
         //    the chunk at its point (rows past the chunk's end are zero), in
@@ -366,29 +344,14 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
                     z = static_cast<GT>(dz);
                 }
                 if (!inside) return;
-                if (((info >> 13) & 3) == kEntryPure) {
-                    // pure AO (row index in bits 9-12) from the unscaled components
-                    const int L = rec.l, kp = (info >> 9) & 15;
-                    const double* tg = m.tg + (L * kMaxPure + kp) * kMaxCart;
-                    for (int c = 0; c < (L + 1) * (L + 2) / 2; ++c) {
-                        const GT w = static_cast<GT>(tg[c]);
-                        if (w == GT(0)) continue;
-                        GT cv[NP];
-                        cart_component_rt<ORDER, PSET, GT>(L, c, x, y, z, R, cv);
-#pragma unroll
-                        for (int p = 0; p < NP; ++p)
-                            v[p] += w * cv[p];
-                    }
-                    return;
-                }
                 const int qq[3] = {(info >> 6) & 7, (info >> 3) & 7, info & 7};
                 cart_component_pw<ORDER, PSET, GT>(qq, x, y, z, R, v);
             };
             const auto add_scale = [&](int i, const GT (&v)[NP]) __attribute__((always_inline)) {
                 if (!scale) return;
                 const int g = g_row[i];
-                if (g < 0) return;
-                const float2 w = a.mo.norm[g];
+                if (g == -1) return;
+                const float2 w = g >= 0 ? a.mo.norm[g] : folded_norm(m, a.mo, ent, ebits, g);
                 const float av = static_cast<float>(v[0]);
                 const float wx2 = w.x * w.x, wy2 = w.y * w.y;
                 es0 += (av * av) * wx2;
@@ -756,7 +719,14 @@ void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream) {
         launch_one<KUMO_HIP_FP64, 0, kPanelsValue, 16, true>(a, stream);
         return;
     }
-    if (a.tile_points != 0 && a.tile_points != 16 && a.tile_points != 32) invalid("tile points must be 16 or 32");
+    if (a.tile_points == 64 && ao_mo_wave_supported(a)) {
+        launch_ao_mo_wave(a, stream);
+        return;
+    }
+    // (FP64 launches, the rescue's among them, take 16-point tiles whatever was asked)
+    if (a.tile_points != 0 && a.tile_points != 16 && a.tile_points != 32 &&
+        !(a.tile_points == 64 && a.precision == KUMO_HIP_FP64))
+        invalid("tile points must be 16 or 32, or 64 for the split-bf16 precisions without full Hessians");
     switch (a.precision) {
     case KUMO_HIP_FP64: launch_prec<KUMO_HIP_FP64>(a, stream); break;
     case KUMO_HIP_FP32: launch_prec<KUMO_HIP_FP32>(a, stream); break;

@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(kMaskThreads) __attribute__((amdgpu_waves_per_
             const uint32_t all = sh.count >= 32 ? 0xffffffffu : ((1u << sh.count) - 1u);
             const int mode = bits == all ? kEntryCart : (sh.pure ? kEntryPure : kEntryCartMasked);
             sent[pos] = s | (mode << 24);
-            sslot[pos] = mode == kEntryPure ? sh.count : sh.ncart; // rows, packed below
+            sslot[pos] = sh.ncart; // rows (Cartesian components, also for folded pure blocks), packed below
         }
         nent += tot;
     }
@@ -569,20 +569,19 @@ __global__ void __launch_bounds__(kMaskThreads) __attribute__((amdgpu_waves_per_
         en[q] = e;
         eb[q] = bits;
         er[q] = static_cast<uint8_t>(row0);
-        const int first = mode == kEntryPure ? sh.prow : sh.crow;
-        const int n = mode == kEntryPure ? sh.count : sh.ncart;
-        // Rows of dropped AOs (partially kept blocks) get zero coefficients.
+        // Every entry contributes its Cartesian components: rows of dropped
+        // AOs of a Cartesian block get zero coefficients (-1); the rows of a
+        // partially kept pure block are folded: coefficient row k is the sum
+        // over its kept pure AOs r of T[r][k] g[k] C[r], formed when the
+        // contraction stages the chunk (code -2 - (entry << 5 | k)).
         if (c < t.max_chunks)
-            for (int i = 0; i < n; ++i) {
+            for (int i = 0; i < sh.ncart; ++i) {
                 int32_t* cr = crows + 2 * (c * kChunkRows + row0 + i);
-                cr[0] = (mode == kEntryCart || ((bits >> i) & 1u)) ? first + i : -1;
-                // Phi row: block << 15 | mode << 13 | pure row << 9 | Cartesian powers (3 bits each);
-                // the pure row (< 2 kMaxL + 1 <= 16) is only encoded for pure entries, the
-                // Cartesian row (up to 21 components) only through its powers.
-                const int cart = mode == kEntryPure ? 0 : i;
-                const int prow = mode == kEntryPure ? i : 0;
-                cr[1] = (s << 15) | (mode << 13) | (prow << 9) | (kPow[sh.l][cart][0] << 6) |
-                        (kPow[sh.l][cart][1] << 3) | kPow[sh.l][cart][2];
+                cr[0] = mode == kEntryPure ? kFoldedRow - ((q << 5) | i)
+                                           : ((mode == kEntryCart || ((bits >> i) & 1u)) ? sh.crow + i : -1);
+                // Phi row: block << 15 | Cartesian mode << 13 | Cartesian powers (3 bits each)
+                const int pm = mode == kEntryPure ? kEntryCartMasked : mode;
+                cr[1] = (s << 15) | (pm << 13) | (kPow[sh.l][i][0] << 6) | (kPow[sh.l][i][1] << 3) | kPow[sh.l][i][2];
             }
     }
     for (int item = tid; item < nc * kChunkRows; item += kMaskThreads) {

@@ -22,8 +22,12 @@ constexpr int kBf16Planes = 3; // bf16 terms of the MO operand table
 enum EntryMode : int32_t {
     kEntryCart = 0,       // every AO kept: the block's Cartesian rows (transform folded into C)
     kEntryCartMasked = 1, // Cartesian block, some AOs dropped
-    kEntryPure = 2,       // pure block, some AOs dropped: its pure rows
+    kEntryPure = 2,       // pure block, some AOs dropped: its Cartesian rows with the kept pure
+                          // AOs' coefficients folded when the chunk is staged
 };
+// Operand row code of a folded row: kFoldedRow - (entry << 5 | Cartesian
+// component) (codes below -1; -1 is a zero row).
+constexpr int32_t kFoldedRow = -2;
 
 // Per screening block outputs of the mask kernel.
 struct KeptTables {
@@ -41,9 +45,9 @@ struct KeptTables {
     uint8_t* ent_row;       // [nblk][nshells] first row of the entry in its chunk
     int32_t* chunk_begin;   // [nblk][max_chunks + 1] first entry of each chunk
     // [nblk][max_chunks][kChunkRows][2]: operand table row (-1: zero
-    // coefficients) and the Phi row it multiplies, block << 15 | mode << 13 |
-    // row within the block << 9 | x, y, z powers of a Cartesian row (3 bits
-    // each), or -1 past the chunk's rows.
+    // coefficients; below -1: a folded row, kFoldedRow) and the Phi row it
+    // multiplies, block << 15 | mode << 13 | x, y, z powers of the Cartesian
+    // component (3 bits each), or -1 past the chunk's rows.
     int32_t* chunk_rows;
     int32_t* nchunks;       // [nblk]
     int max_chunks;
@@ -150,7 +154,7 @@ struct AoMoLaunch {
     int hessian;             // 0 none, 1 diagonal, 2 full, 3 trace only (in the xx moment)
     int precision;           // kumo_hip_precision
     int slot0, slot1;
-    int tile_points;         // points per workgroup: 16 or 32 (0: 32; FP64 always 16)
+    int tile_points;         // points per workgroup: 16, 32 or 64 (the wave-uniform kernel); 0: 32; FP64 always 16
     double* moments;         // [2][kMoments][batch_points]
     // Reduced precisions, when not null: [2][batch_points] sum_k |Phi_k| ||C_k,c||
     // over the kept operand rows for channel c (slot group 0 only), then, when
@@ -171,6 +175,12 @@ struct AoMoLaunch {
 };
 void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream);
 constexpr int kRescueTile = 16; // points per rescue tile (the FP64 kernel's tile)
+// The wave-uniform contraction (tile_points 64: 64 points per workgroup, AO
+// blocks generated one per wave at every point of the tile) for the
+// split-bf16 precisions, every panel set but the full Hessian, outside
+// rescue and weighted modes; launch_ao_mo dispatches to it.
+bool ao_mo_wave_supported(const AoMoLaunch& a);
+void launch_ao_mo_wave(const AoMoLaunch& a, hipStream_t stream);
 
 // AO-P contraction (density_derivatives_shared / densities_blocked,
 // src/density.cpp:149-229, 365-444): W = P_kept [Phi Phi_x Phi_y Phi_z] and

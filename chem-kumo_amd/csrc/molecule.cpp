@@ -170,8 +170,13 @@ void upload_device(kumo_hip_molecule& m, hipStream_t s) {
         r.nprim = d.nprim;
         r.prim = d.prim;
         for (int p = 0; p < d.nprim && p < kRecPrim; ++p) {
-            r.e[p] = b.exps[static_cast<std::size_t>(d.prim + p)];
-            r.w[p] = b.radw[static_cast<std::size_t>(d.prim + p)];
+            const double a = b.exps[static_cast<std::size_t>(d.prim + p)];
+            const double w = b.radw[static_cast<std::size_t>(d.prim + p)];
+            r.e[p] = a;
+            r.w[p] = w;
+            // |w| max(1, 4 a^2) exp(-a r^2) < 1e-40 beyond this r^2
+            const double lw = w != 0.0 ? std::log(std::fabs(w) * std::max(1.0, 4.0 * a * a)) : -1e300;
+            r.skip[p] = a > 0.0 ? std::max(0.0, lw + 92.1) / a : 1e300;
         }
     }
     m.d_rec = upload(rec.data(), rec.size(), s);

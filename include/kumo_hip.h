@@ -83,9 +83,12 @@ typedef struct kumo_hip_options {
     double potential_tolerance; /* >= 0; the ESP is evaluated exactly */
     double fukui_delta;         /* > 0 */
     int32_t device_outputs;     /* 1: outputs are device pointers (HBM) */
-    int32_t tile_points;        /* AO->MO workgroup tile in points, 16 or 32; 0: chosen by the
-                                   autotuner (timed once per molecule, precision and derivative
-                                   order). Results do not depend on it. */
+    int32_t tile_points;        /* AO->MO workgroup tile in points: 16 or 32 (the tile kernel:
+                                   identical results for both) or 64 (the wave-uniform kernel of
+                                   the split-bf16 precisions without full Hessians: the same
+                                   products, its own Phi arithmetic); 0: the wave-uniform kernel
+                                   where it applies, else chosen by the autotuner (timed once per
+                                   molecule, precision and derivative order). */
     kumo_hip_progress_fn progress;
     void* progress_user;
 } kumo_hip_options;
