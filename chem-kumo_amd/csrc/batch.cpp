@@ -166,6 +166,7 @@ int kumo_hip_batch_create(kumo_hip_context* ctx, size_t nitems, const kumo_hip_m
         for (int k = 0; k < nstreams; ++k) {
             auto e = std::make_unique<kumo::Executor>();
             e->create(ctx->device, ctx->boys_table.as<double>());
+            e->exact_esp = true;
             b->execs.push_back(std::move(e));
         }
         b->fork.assign(static_cast<std::size_t>(nstreams), nullptr);

@@ -34,6 +34,8 @@ void validate_options(const kumo_hip_options& o, const kumo_hip_molecule* mol) {
     if (o.threads > 1) invalid("explicit multi-thread control is not implemented in the API adapter");
     if (o.tile_points != 0 && o.tile_points != 16 && o.tile_points != 32 && o.tile_points != 64)
         invalid("tile_points must be 0, 16, 32 or 64");
+    if (o.esp_far_field < KUMO_HIP_ESP_AUTO || o.esp_far_field > KUMO_HIP_ESP_EXACT)
+        invalid("unknown electrostatic potential method");
 }
 
 // The checks of OrbitalEnergyFields (src/orbital_energy_fields.hpp:117-190,
@@ -457,6 +459,165 @@ void point_pass_bytes(const kumo_hip_molecule* mol, const Prepared& pr, const ku
     }
 }
 
+// Work (pairs x points) from which the automatic ESP method takes the far
+// field: below it the exact pair sums are as fast.
+constexpr double kFarAutoWork = 2e10;
+
+bool use_far_field(const Executor& ex, const kumo_hip_molecule* mol, const kumo_hip_options& opt, long long npoints) {
+    if (opt.potential_tolerance == 0.0 || opt.esp_far_field == KUMO_HIP_ESP_EXACT || ex.capture || ex.no_alloc ||
+        ex.exact_esp)
+        return false;
+    if (opt.esp_far_field == KUMO_HIP_ESP_FAR) return true;
+    return static_cast<double>(mol->esp_npairs) * static_cast<double>(npoints) >= kFarAutoWork;
+}
+
+// The potential at points [q0, q1) in target clusters under the far-field
+// plan (evaluate_electrostatic_potential, src/evaluator.cpp:391-479): on a
+// grid, tiles of up to 8^3 points about the centres of their boxes of grid
+// indices, one plan whose radius covers the largest tile; at listed points,
+// the axis-aligned cubes of edge 1.6 bohr holding them, about the cube
+// centres, one plan covering half a cube diagonal. A value depends only on
+// its point, its target and the plan, so a range holds the part of each
+// target that falls in it. The targets run in batches sized by the
+// workspace (per target: a near-pair list, a byte per pair and per source
+// cluster, the local expansion).
+void far_field_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const PointSource& ps,
+                     const kumo_hip_options& opt, double* out, long long q0, long long q1, const Advance& advance) {
+    hipStream_t s = ex.stream;
+    std::vector<FarTargetDev> targets;
+    std::vector<long long> in_range; // points of each target in [q0, q1)
+    std::vector<int64_t> idx;
+    double radius = 0.0;
+    if (pr.grid) {
+        const long long ny = ps.ny, nz = ps.nz, nx = pr.n / (ny * nz);
+        const long long counts[3] = {nx, ny, nz};
+        long long tiles[3];
+        double edges[3];
+        for (int a = 0; a < 3; ++a) {
+            edges[a] = static_cast<double>(std::min<long long>(kFarTileSide, counts[a]) - 1);
+            tiles[a] = (counts[a] + kFarTileSide - 1) / kFarTileSide;
+        }
+        // grid_box_radius (src/potential_clusters.hpp:37-49)
+        for (int corner = 0; corner < 8; ++corner) {
+            double o[3], v[3];
+            for (int a = 0; a < 3; ++a)
+                o[a] = ((corner >> a) & 1 ? 0.5 : -0.5) * edges[a];
+            for (int a = 0; a < 3; ++a)
+                v[a] = (ps.steps[a] * o[0] + ps.steps[3 + a] * o[1]) + ps.steps[6 + a] * o[2];
+            radius = std::max(radius, std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]));
+        }
+        const long long t0 = (q0 / (ny * nz)) / kFarTileSide, t1 = ((q1 - 1) / (ny * nz)) / kFarTileSide;
+        for (long long ti = t0; ti <= t1; ++ti)
+            for (long long tj = 0; tj < tiles[1]; ++tj)
+                for (long long tk = 0; tk < tiles[2]; ++tk) {
+                    const long long t[3] = {ti, tj, tk};
+                    FarTargetDev tg{};
+                    int count = 1;
+                    double f[3];
+                    for (int a = 0; a < 3; ++a) {
+                        tg.lo[a] = static_cast<int32_t>(t[a] * kFarTileSide);
+                        tg.hi[a] = static_cast<int32_t>(std::min<long long>(t[a] * kFarTileSide + kFarTileSide, counts[a]));
+                        count *= tg.hi[a] - tg.lo[a];
+                        f[a] = 0.5 * (tg.lo[a] + tg.hi[a] - 1);
+                    }
+                    const long long qmin = (static_cast<long long>(tg.lo[0]) * ny + tg.lo[1]) * nz + tg.lo[2];
+                    const long long qmax = (static_cast<long long>(tg.hi[0] - 1) * ny + (tg.hi[1] - 1)) * nz + (tg.hi[2] - 1);
+                    if (qmax < q0 || qmin >= q1) continue;
+                    for (int a = 0; a < 3; ++a)
+                        tg.c[a] = ps.origin[a] + ((ps.steps[a] * f[0] + ps.steps[3 + a] * f[1]) + ps.steps[6 + a] * f[2]);
+                    tg.count = count;
+                    long long held = count; // points of the tile in [q0, q1)
+                    if (qmin < q0 || qmax >= q1) {
+                        held = 0;
+                        for (int i = tg.lo[0]; i < tg.hi[0]; ++i)
+                            for (int j = tg.lo[1]; j < tg.hi[1]; ++j) {
+                                const long long row = (static_cast<long long>(i) * ny + j) * nz;
+                                held += std::max(0LL, std::min(row + tg.hi[2], q1) - std::max(row + tg.lo[2], q0));
+                            }
+                    }
+                    targets.push_back(tg);
+                    in_range.push_back(held);
+                }
+    } else {
+        // cubes of listed points (src/evaluator.cpp:449-479, cluster_by
+        // src/potential_clusters.hpp:51-76); the points come back from the
+        // device, where the evaluation holds them
+        const long long m = q1 - q0;
+        std::vector<double> host(static_cast<std::size_t>(3 * m));
+        KUMO_HIP(hipMemcpyAsync(host.data(), ps.points + 3 * q0, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        KUMO_HIP(hipStreamSynchronize(s));
+        struct Item {
+            std::array<int64_t, 3> cell;
+            int64_t q;
+        };
+        std::vector<Item> items(static_cast<std::size_t>(m));
+        for (long long j = 0; j < m; ++j) {
+            Item& it = items[static_cast<std::size_t>(j)];
+            for (int a = 0; a < 3; ++a) {
+                const double index = std::floor(host[static_cast<std::size_t>(3 * j + a)] / kFarPointCube);
+                if (!(std::abs(index) < 1e15)) invalid("point too far for the electrostatic potential");
+                it.cell[static_cast<std::size_t>(a)] = static_cast<int64_t>(index);
+            }
+            it.q = q0 + j;
+        }
+        std::sort(items.begin(), items.end(),
+                  [](const Item& a, const Item& b) { return a.cell != b.cell ? a.cell < b.cell : a.q < b.q; });
+        idx.resize(items.size());
+        for (std::size_t j = 0; j < items.size(); ++j) {
+            idx[j] = items[j].q;
+            if (j == 0 || items[j].cell != items[j - 1].cell) {
+                FarTargetDev tg{};
+                for (int a = 0; a < 3; ++a)
+                    tg.c[a] = (static_cast<double>(items[j].cell[static_cast<std::size_t>(a)]) + 0.5) * kFarPointCube;
+                tg.first = static_cast<int32_t>(j);
+                targets.push_back(tg);
+            }
+            ++targets.back().count;
+        }
+        if (items.size() > static_cast<std::size_t>(std::numeric_limits<int32_t>::max()))
+            invalid("too many points in one electrostatic potential range");
+        for (const FarTargetDev& tg : targets)
+            in_range.push_back(tg.count);
+        radius = 0.5 * std::sqrt(3.0) * kFarPointCube;
+    }
+    const FarPlanDev& plan = mol->far_plan(radius, opt.potential_tolerance, s);
+    const std::size_t nt = targets.size();
+    const std::size_t npairs = static_cast<std::size_t>(plan.npairs);
+    constexpr std::size_t kAlign = 256;
+    const auto up = [](std::size_t b) { return (b + kAlign - 1) / kAlign * kAlign; };
+    const std::size_t per_target = 4 * npairs + (npairs + static_cast<std::size_t>(plan.nclusters)) +
+                                   far_hcount(kFarMultipole) * sizeof(double) + 4 + 64;
+    constexpr std::size_t kBudget = std::size_t(1) << 30;
+    const std::size_t batch = std::max<std::size_t>(1, std::min(nt, kBudget / per_target));
+    const std::size_t b_targets = up(nt * sizeof(FarTargetDev)), b_idx = up(std::max<std::size_t>(1, idx.size()) * 8);
+    const std::size_t b_local = up(batch * far_hcount(kFarMultipole) * sizeof(double)), b_nnear = up(batch * 4),
+                      b_near = up(batch * npairs * 4),
+                      b_work = up(batch * (npairs + static_cast<std::size_t>(plan.nclusters)));
+    char* base = grow<char>(ex.ws_far, b_targets + b_idx + b_local + b_nnear + b_near + b_work);
+    auto* d_targets = reinterpret_cast<FarTargetDev*>(base);
+    auto* d_idx = reinterpret_cast<int64_t*>(base + b_targets);
+    auto* d_local = reinterpret_cast<double*>(base + b_targets + b_idx);
+    auto* d_nnear = reinterpret_cast<int32_t*>(base + b_targets + b_idx + b_local);
+    auto* d_near = reinterpret_cast<int32_t*>(base + b_targets + b_idx + b_local + b_nnear);
+    auto* d_work = reinterpret_cast<int8_t*>(base + b_targets + b_idx + b_local + b_nnear + b_near);
+    upload_async(ex, d_targets, targets.data(), nt);
+    if (!idx.empty()) upload_async(ex, d_idx, idx.data(), idx.size());
+    const auto* pairs = mol->d_esp_pairs.as<EspPairDev>();
+    const auto* coef = mol->d_esp_coef.as<double>();
+    for (std::size_t b0 = 0; b0 < nt; b0 += batch) {
+        const int nb = static_cast<int>(std::min(batch, nt - b0));
+        launch_far_targets(plan, pairs, coef, ex.boys_table, d_targets + b0, nb, d_local, d_near, d_nnear, d_work, s);
+        launch_far_points(pairs, mol->esp_groups, coef, ex.boys_table, mol->d_esp_atoms.as<EspAtomDev>(),
+                          static_cast<int>(mol->atoms.size()), ps, q0, q1, d_targets + b0, nb, d_idx, d_local, d_near,
+                          d_nnear, plan.npairs, out, s);
+        sync(ex);
+        long long done = 0;
+        for (std::size_t t = b0; t < b0 + static_cast<std::size_t>(nb); ++t)
+            done += in_range[t];
+        advance(static_cast<std::size_t>(done));
+    }
+}
+
 void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const PointSource& ps,
                const kumo_hip_options& opt, double* const* dout, long long q0, long long q1, const Advance& advance) {
     hipStream_t s = ex.stream;
@@ -707,18 +868,22 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         std::size_t idx = 0;
         for (std::size_t i = 0; i < nprops; ++i)
             if (props[i].kind == kEsp) idx = pr.first_channel[i];
-        const long long chunk = 1 << 18;
-        std::vector<EspGroup> groups = mol->esp_groups;
-        const int nslices = esp_slices(groups, std::min(chunk, range));
-        double* partial = grow<double>(ex.ws_esp, static_cast<std::size_t>(nslices) *
-                                                      static_cast<std::size_t>(std::min(chunk, range)));
-        for (long long p0 = q0; p0 < q1; p0 += chunk) {
-            const long long cnt = std::min(chunk, q1 - p0);
-            launch_esp(mol->d_esp_pairs.as<EspPairDev>(), groups, mol->d_esp_coef.as<double>(),
-                       mol->d_esp_atoms.as<EspAtomDev>(), static_cast<int>(mol->atoms.size()), ex.boys_table, ps, p0,
-                       cnt, partial, dout[idx] + p0, s);
-            sync(ex);
-            advance(static_cast<std::size_t>(cnt));
+        if (use_far_field(ex, mol, opt, range)) {
+            far_field_range(ex, mol, pr, ps, opt, dout[idx], q0, q1, advance);
+        } else {
+            const long long chunk = 1 << 18;
+            std::vector<EspGroup> groups = mol->esp_groups;
+            const int nslices = esp_slices(groups, std::min(chunk, range));
+            double* partial = grow<double>(ex.ws_esp, static_cast<std::size_t>(nslices) *
+                                                          static_cast<std::size_t>(std::min(chunk, range)));
+            for (long long p0 = q0; p0 < q1; p0 += chunk) {
+                const long long cnt = std::min(chunk, q1 - p0);
+                launch_esp(mol->d_esp_pairs.as<EspPairDev>(), groups, mol->d_esp_coef.as<double>(),
+                           mol->d_esp_atoms.as<EspAtomDev>(), static_cast<int>(mol->atoms.size()), ex.boys_table, ps, p0,
+                           cnt, partial, dout[idx] + p0, s);
+                sync(ex);
+                advance(static_cast<std::size_t>(cnt));
+            }
         }
         for (std::size_t i = 0; i < nprops; ++i)
             if (props[i].kind == kEsp && pr.first_channel[i] != idx)

@@ -1,0 +1,496 @@
+// Far-field ESP on the GPU, one workgroup per target cluster
+// (expand_far_field and local_polynomial, src/electrostatic_potential.cpp:701-844).
+//
+// far_targets_kernel, per target cluster (centre b):
+//   1. every pair gets a state: its source cluster is far enough from b (to
+//      be translated), or its error level at |Q - b|^2 (-1 below the first
+//      level's distance); the levels are counted;
+//   2. one thread finds the cut of the pair budget (levels from the smallest
+//      error up, every pair of a level while the budget holds, then the
+//      number of pairs of the first level that fits only in part, taken in
+//      the reference's pair order);
+//   3. the pairs not translated are split, in pair order, into expanded ones
+//      (above the cut, or at it within the partial count) and near ones
+//      (summed exactly at the points), listed from the two ends of the
+//      target's list;
+//   4. each wave takes every fourth far source cluster and every fourth
+//      expanded pair: the Hermite recursion of 1/|X| (cluster) or of the
+//      pair's charge (Boys seeds) runs across the wave in LDS, and each lane
+//      accumulates its local coefficients beta (cluster: sum over nu of
+//      Q^(N - |beta|)_nu R_(nu + beta); pair: sum over its terms of
+//      c_k R_(alpha_k + beta)); the waves' sums are added in a fixed order and
+//      scaled by (-1)^|beta| / beta!.
+// far_poly_kernel and far_near_kernel, per target cluster: each thread one
+// point, nuclear - (local polynomial at r - b + the exact sums of the near
+// pairs).
+#include "far_field.hpp"
+
+#include "common.hpp"
+#include "esp_common.hpp"
+
+namespace kumo {
+namespace {
+using namespace esp;
+
+constexpr int FT = 256;                           // threads per target workgroup
+constexpr int FW = FT / 64;                       // waves
+constexpr int NLOC = far_hcount(kFarMultipole);   // local coefficients of a target (|beta| <= N)
+constexpr int NLOW = far_hcount(kFarLocal);       // of which a pair expansion reaches (|beta| <= K)
+constexpr int RPL = (NLOC + 63) / 64;             // local coefficients per lane
+constexpr int kExpandAtCut = 64;                  // state of a pair of the cut level within the budget
+
+// Position of (t, u, v) in the Hermite order (by t + u + v, then t and u descending).
+__device__ __attribute__((always_inline)) inline int tuv_index(int t, int u, int v) {
+    const int s = t + u + v;
+    return far_hcount(s - 1) + (s - t) * (s - t + 1) / 2 + (s - t - u);
+}
+
+__device__ __attribute__((always_inline)) inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// R^0_tuv for t + u + v <= top from the seeds R^n_000 (seeds[n], n <= top):
+//   R^n_(t+1,u,v) = t R^(n+1)_(t-1,u,v) + X_t R^(n+1)_(t,u,v)
+// (on the first nonzero index), from n = top down to 0, the entries of one
+// order across the wave's lanes; a and b are the wave's two LDS buffers of
+// far_hcount(top) doubles; returns the one holding order 0.
+__device__ double* wave_hermite(const double* seeds, const double x[3], int top, double* a, double* b,
+                                const uchar4* tri, int lane) {
+    double* cur = a;
+    double* prev = b;
+    for (int n = top; n >= 0; --n) {
+        double* t_ = cur;
+        cur = prev;
+        prev = t_;
+        const int entries = far_hcount(top - n);
+        for (int k = lane; k < entries; k += 64) {
+            double value;
+            if (k == 0) {
+                value = seeds[n];
+            } else {
+                const uchar4 h = tri[k];
+                const int t = h.x, u = h.y, v = h.z;
+                if (t > 0) {
+                    value = x[0] * prev[tuv_index(t - 1, u, v)];
+                    if (t > 1) value += (t - 1) * prev[tuv_index(t - 2, u, v)];
+                } else if (u > 0) {
+                    value = x[1] * prev[tuv_index(t, u - 1, v)];
+                    if (u > 1) value += (u - 1) * prev[tuv_index(t, u - 2, v)];
+                } else {
+                    value = x[2] * prev[tuv_index(t, u, v - 1)];
+                    if (v > 1) value += (v - 1) * prev[tuv_index(t, u, v - 2)];
+                }
+            }
+            cur[k] = value;
+        }
+        wave_sync();
+    }
+    return cur;
+}
+
+__global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const EspPairDev* __restrict__ pairs,
+                                                         const double* __restrict__ coef,
+                                                         const double* __restrict__ boys_table,
+                                                         const FarTargetDev* __restrict__ targets,
+                                                         double* __restrict__ local, int32_t* __restrict__ near_list,
+                                                         int32_t* __restrict__ nnear, int8_t* __restrict__ work) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int top = plan.max_L + kFarMultipole;
+    const int H = far_hcount(top);
+    // LDS: tri[H] (uchar4), then per wave two recursion buffers and the seeds
+    // (top + 1 doubles), then the wave sums red[FW][NLOC], the level counts
+    // and the scan words.
+    uchar4* tri = reinterpret_cast<uchar4*>(smem);
+    double* wbuf = reinterpret_cast<double*>(smem + ((H * 4 + 15) / 16) * 16);
+    const int wstride = 2 * H + top + 2;
+    double* buf_a = wbuf + wave * wstride;
+    double* buf_b = buf_a + H;
+    double* seeds = buf_b + H;
+    double* red = wbuf + FW * wstride;
+    int* counts = reinterpret_cast<int*>(red + FW * NLOC);
+    int* scan = counts + kFarLevels + 4; // cut, partial, -, -, then the wave totals [FW]
+    const FarTargetDev tg = targets[blockIdx.x];
+    const double b0 = tg.c[0], b1 = tg.c[1], b2 = tg.c[2];
+    int8_t* state = work + static_cast<long long>(blockIdx.x) * (plan.npairs + plan.nclusters);
+    int8_t* far = state + plan.npairs; // per source cluster: translated (1) or not
+    int32_t* nl = near_list + static_cast<long long>(blockIdx.x) * plan.npairs;
+
+    for (int k = tid; k < H; k += FT) {
+        int s = 0;
+        while (far_hcount(s) <= k)
+            ++s;
+        int r = k - far_hcount(s - 1), t = s;
+        while (r >= s - t + 1) {
+            r -= s - t + 1;
+            --t;
+        }
+        tri[k] = uchar4{static_cast<unsigned char>(t), static_cast<unsigned char>(s - t - r),
+                        static_cast<unsigned char>(r), 0};
+    }
+    if (tid < kFarLevels) counts[tid] = 0;
+    // 0. the source clusters far enough to be translated, decided once so that
+    //    the pairs' states and the translations agree
+    for (int ci = tid; ci < plan.nclusters; ci += FT) {
+#pragma clang fp contract(off) // the decisions' distances rounded as on the host
+        const FarClusterDev& cl = plan.clusters[ci];
+        const double cx = cl.c[0] - b0, cy = cl.c[1] - b1, cz = cl.c[2] - b2;
+        far[ci] = sqrt((cx * cx + cy * cy) + cz * cz) >= cl.min_distance ? 1 : 0;
+    }
+    __syncthreads();
+
+    // 1. states: -2 translated with its source cluster, else the error level
+    for (int i = tid; i < plan.npairs; i += FT) {
+        int lev;
+        if (far[plan.cluster_of[i]]) {
+            lev = -2;
+        } else {
+#pragma clang fp contract(off)
+            const EspPairDev pr = pairs[i];
+            const double dx = pr.qx - b0, dy = pr.qy - b1, dz = pr.qz - b2;
+            const double d2 = (dx * dx + dy * dy) + dz * dz;
+            const float* lim = plan.level_d2 + static_cast<long long>(i) * kFarLevels;
+            lev = -1;
+            while (lev + 1 < kFarLevels && d2 >= static_cast<double>(lim[lev + 1]))
+                ++lev;
+            if (lev >= 0) atomicAdd(&counts[lev], 1);
+        }
+        state[i] = static_cast<int8_t>(lev);
+    }
+    __syncthreads();
+    // 2. the cut of the pair budget
+    if (tid == 0) {
+        int cut = -1, partial = 0;
+        double used = 0.0;
+        for (int j = kFarLevels - 1; j >= 0; --j) {
+            const double v = plan.levels[j];
+            const double n = static_cast<double>(counts[j]);
+            if (used + n * v <= plan.pair_tolerance) {
+                used += n * v;
+                cut = j - 1;
+                continue;
+            }
+            partial = static_cast<int>(floor((plan.pair_tolerance - used) / v));
+            cut = j;
+            break;
+        }
+        scan[0] = cut;
+        scan[1] = partial;
+    }
+    __syncthreads();
+    const int cut = scan[0], partial = scan[1];
+    // 3. the pairs of the cut level expanded within the budget: the first
+    //    `partial` of them in the reference's order, marked kExpandAtCut; then
+    //    near pairs from the front of the list, expanded pairs from the back,
+    //    both in pair order
+    int* wtot = scan + 4; // [FW]
+    const auto count_before = [&](bool pred, int& total) __attribute__((always_inline)) {
+        const unsigned long long bal = __ballot(pred);
+        const int in_wave = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wtot[wave] = __popcll(bal);
+        __syncthreads();
+        int before = 0;
+        total = 0;
+        for (int w = 0; w < FW; ++w) {
+            if (w < wave) before += wtot[w];
+            total += wtot[w];
+        }
+        __syncthreads();
+        return before + in_wave;
+    };
+    if (cut >= 0 && partial > 0) { // uniform over the workgroup
+        int n_cut = 0;
+        for (int base = 0; base < plan.npairs && n_cut < partial; base += FT) {
+            const int k = base + tid;
+            const int i = k < plan.npairs ? plan.ref_order[k] : -1;
+            const bool at_cut = i >= 0 && state[i] == cut;
+            int tot_cut;
+            const int rank = n_cut + count_before(at_cut, tot_cut);
+            if (at_cut && rank < partial) state[i] = kExpandAtCut;
+            n_cut += tot_cut;
+        }
+        __syncthreads();
+    }
+    int n_near = 0, n_exp = 0;
+    for (int base = 0; base < plan.npairs; base += FT) {
+        const int i = base + tid;
+        const int lev = i < plan.npairs ? state[i] : -3;
+        int tot_near, tot_exp;
+        const bool is_exp = lev >= 0 && lev > cut;
+        const bool is_near = !is_exp && lev >= -1;
+        const int pn = count_before(is_near, tot_near);
+        const int pe = count_before(is_exp, tot_exp);
+        if (is_near) nl[n_near + pn] = i;
+        if (is_exp) nl[plan.npairs - 1 - (n_exp + pe)] = i;
+        n_near += tot_near;
+        n_exp += tot_exp;
+    }
+    if (tid == 0) nnear[blockIdx.x] = n_near;
+    __syncthreads(); // the list entries of every wave are written before step 4 reads them
+
+    // 4. the local expansion
+    double acc[RPL];
+#pragma unroll
+    for (int r = 0; r < RPL; ++r)
+        acc[r] = 0.0;
+    for (int ci = wave; ci < plan.nclusters; ci += FW) {
+        if (!far[ci]) continue; // uniform over the wave
+        const FarClusterDev cl = plan.clusters[ci];
+        const double x[3] = {cl.c[0] - b0, cl.c[1] - b1, cl.c[2] - b2};
+        const double d2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+        // seeds (-1)^n (2n - 1)!! / |X|^(2n + 1) (coulomb_derivatives,
+        // src/electrostatic_potential.cpp:282-291)
+        if (lane == 0) {
+            const double inv = 1.0 / sqrt(d2), inv2 = inv * inv;
+            double sd = inv;
+            for (int n = 0; n <= top; ++n) {
+                seeds[n] = sd;
+                sd *= -(2.0 * n + 1.0) * inv2;
+            }
+        }
+        wave_sync();
+        const double* R = wave_hermite(seeds, x, top, buf_a, buf_b, tri, lane);
+        const double* q0 = plan.moments + static_cast<long long>(ci) * (kFarMultipole + 1) * plan.moment_count;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int be = lane + 64 * r;
+            if (be >= NLOC) continue;
+            const uchar4 hb = tri[be];
+            const int m = kFarMultipole - (hb.x + hb.y + hb.z);
+            const double* q = q0 + static_cast<long long>(m) * plan.moment_count;
+            const int terms = far_hcount(plan.max_L + m);
+            double sum = 0.0;
+            for (int nu = 0; nu < terms; ++nu) {
+                const uchar4 ha = tri[nu];
+                sum += q[nu] * R[tuv_index(ha.x + hb.x, ha.y + hb.y, ha.z + hb.z)];
+            }
+            acc[r] += sum;
+        }
+        wave_sync(); // the buffers are rewritten by the next item
+    }
+    const int nexp = n_exp;
+    for (int e = wave; e < nexp; e += FW) {
+        const int i = nl[plan.npairs - 1 - e];
+        const EspPairDev pr = pairs[i];
+        const double x[3] = {pr.qx - b0, pr.qy - b1, pr.qz - b2};
+        const int ord = pr.L + kFarLocal;
+        // seeds (-2p)^n F_n(p |PC|^2) (hermite_integrals, src/electrostatic_potential.cpp:269-280)
+        if (lane == 0) {
+            boys(boys_table, kBoysOrders, ord, pr.p * ((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]), ord, seeds);
+            double power = 1.0;
+            for (int n = 1; n <= ord; ++n) {
+                power *= -2.0 * pr.p;
+                seeds[n] *= power;
+            }
+        }
+        wave_sync();
+        const double* R = wave_hermite(seeds, x, ord, buf_a, buf_b, tri, lane);
+        const double* c = coef + pr.first;
+        const int nk = far_hcount(pr.L);
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const int be = lane + 64 * r;
+            if (be >= NLOW) continue;
+            const uchar4 hb = tri[be];
+            double sum = 0.0;
+            for (int k = 0; k < nk; ++k) {
+                const uchar4 ha = tri[k];
+                sum += c[k] * R[tuv_index(ha.x + hb.x, ha.y + hb.y, ha.z + hb.z)];
+            }
+            acc[r] += sum;
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        const int be = lane + 64 * r;
+        if (be < NLOC) red[wave * NLOC + be] = acc[r];
+    }
+    __syncthreads();
+    for (int be = tid; be < NLOC; be += FT) {
+        double s = 0.0;
+        for (int w = 0; w < FW; ++w)
+            s += red[w * NLOC + be];
+        const uchar4 hb = tri[be];
+        double f = ((hb.x + hb.y + hb.z) & 1) ? -1.0 : 1.0;
+        for (int j = 2; j <= hb.x; ++j)
+            f /= j;
+        for (int j = 2; j <= hb.y; ++j)
+            f /= j;
+        for (int j = 2; j <= hb.z; ++j)
+            f /= j;
+        local[static_cast<long long>(blockIdx.x) * NLOC + be] = s * f;
+    }
+}
+
+// Point k of a target: its index in the point source, or -1 where a grid
+// tile's point lies outside the launch's range [q0, q1).
+__device__ __attribute__((always_inline)) inline long long far_point(const PointSource& ps, const FarTargetDev& tg,
+                                                                     const int64_t* idx, int k, long long q0,
+                                                                     long long q1) {
+    if (ps.points) return idx[tg.first + k];
+    const int ey = tg.hi[1] - tg.lo[1], ez = tg.hi[2] - tg.lo[2];
+    const int di = k / (ey * ez), r = k - di * ey * ez, dj = r / ez, dk = r - dj * ez;
+    const long long q = (static_cast<long long>(tg.lo[0] + di) * ps.ny + (tg.lo[1] + dj)) * ps.nz + (tg.lo[2] + dk);
+    return (q >= q0 && q < q1) ? q : -1;
+}
+
+// The local polynomial and the nuclear term at the points of each target:
+// out = nuclear - sum_beta local_beta (r - b)^beta (the near pairs are
+// subtracted by far_near_kernel, one L after another).
+__global__ void __launch_bounds__(FT) far_poly_kernel(const EspAtomDev* __restrict__ atoms, int natoms, PointSource ps,
+                                                      long long q0, long long q1,
+                                                      const FarTargetDev* __restrict__ targets,
+                                                      const int64_t* __restrict__ idx,
+                                                      const double* __restrict__ local, double* __restrict__ out) {
+    const FarTargetDev tg = targets[blockIdx.x];
+    const double* loc = local + static_cast<long long>(blockIdx.x) * NLOC;
+    for (int k = threadIdx.x; k < tg.count; k += FT) {
+        const long long q = far_point(ps, tg, idx, k, q0, q1);
+        if (q < 0) continue;
+        double x, y, z;
+        point_coords(ps, q, x, y, z);
+        const double h[3] = {x - tg.c[0], y - tg.c[1], z - tg.c[2]};
+        double pw[3][kFarMultipole + 1];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            pw[a][0] = 1.0;
+#pragma unroll
+            for (int j = 1; j <= kFarMultipole; ++j)
+                pw[a][j] = pw[a][j - 1] * h[a];
+        }
+        double poly = 0.0;
+        int be = 0;
+#pragma unroll
+        for (int s = 0; s <= kFarMultipole; ++s)
+#pragma unroll
+            for (int t = s; t >= 0; --t)
+#pragma unroll
+                for (int u = s - t; u >= 0; --u, ++be)
+                    poly += loc[be] * pw[0][t] * pw[1][u] * pw[2][s - t - u];
+        double nuclear = 0.0;
+        for (int a = 0; a < natoms; ++a) {
+            const EspAtomDev at = atoms[a];
+            const double dx = at.x - x, dy = at.y - y, dz = at.z - z;
+            const double d = sqrt((dx * dx + dy * dy) + dz * dz);
+            if (at.charge == 0.0 || d < 1e-8) continue;
+            nuclear += at.charge / d;
+        }
+        out[q] = nuclear - poly;
+    }
+}
+
+// The near pairs of order L (pair indices [lo, hi), a contiguous part of
+// each target's sorted near list), subtracted at the target's points. A
+// target of up to 64 (128) points splits the pairs into 4 (2) slices over
+// the threads, summed in slice order.
+template <int L>
+__global__ void __launch_bounds__(FT) far_near_kernel(const EspPairDev* __restrict__ pairs,
+                                                      const double* __restrict__ coef,
+                                                      const double* __restrict__ boys_table, PointSource ps,
+                                                      long long q0, long long q1,
+                                                      const FarTargetDev* __restrict__ targets,
+                                                      const int64_t* __restrict__ idx,
+                                                      const int32_t* __restrict__ near_list,
+                                                      const int32_t* __restrict__ nnear, int npairs, int lo, int hi,
+                                                      double* __restrict__ out) {
+    __shared__ double part[FT];
+    const FarTargetDev tg = targets[blockIdx.x];
+    const int32_t* nl = near_list + static_cast<long long>(blockIdx.x) * npairs;
+    const int nn = nnear[blockIdx.x];
+    const auto lower = [&](int key) __attribute__((always_inline)) {
+        int a = 0, b = nn;
+        while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (nl[mid] < key) a = mid + 1;
+            else b = mid;
+        }
+        return a;
+    };
+    const int e0 = lower(lo), e1 = lower(hi);
+    if (e0 >= e1) return; // uniform over the workgroup
+    const int span = tg.count <= 64 ? 64 : (tg.count <= 128 ? 128 : FT); // points per pass
+    const int slices = FT / span;
+    const int slice = threadIdx.x / span;
+    for (int k0 = 0; k0 < tg.count; k0 += span) {
+        const int k = k0 + static_cast<int>(threadIdx.x) % span;
+        long long q = -1;
+        double sum = 0.0;
+        if (k < tg.count) q = far_point(ps, tg, idx, k, q0, q1);
+        if (q >= 0) {
+            double x, y, z;
+            point_coords(ps, q, x, y, z);
+            for (int e = e0 + slice; e < e1; e += slices)
+                sum += pair_sum<L>(pairs[nl[e]], coef, boys_table, x, y, z);
+        }
+        if (slices > 1) {
+            part[threadIdx.x] = sum;
+            __syncthreads();
+            if (slice == 0) {
+                for (int sl = 1; sl < slices; ++sl)
+                    sum += part[threadIdx.x + sl * span];
+            }
+            __syncthreads();
+        }
+        if (slice == 0 && q >= 0) out[q] -= sum;
+    }
+}
+
+} // namespace
+
+void launch_far_targets(const FarPlanDev& plan, const EspPairDev* pairs, const double* coef, const double* boys_table,
+                        const FarTargetDev* targets, int ntargets, double* local, int32_t* near, int32_t* nnear,
+                        int8_t* work, hipStream_t stream) {
+    if (ntargets <= 0) return;
+    const int top = plan.max_L + kFarMultipole;
+    const int H = far_hcount(top);
+    if (top + kFarLocal > 255) invalid("far-field expansion order too high");
+    const size_t lds = ((H * 4 + 15) / 16) * 16 + sizeof(double) * (FW * (2 * H + top + 2) + FW * NLOC) +
+                       sizeof(int) * (kFarLevels + 4 + 4 + FW);
+    if (lds > 160 * 1024) invalid("far-field expansion needs more LDS than a workgroup has");
+    static bool configured = false;
+    if (!configured) {
+        KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&far_targets_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        configured = true;
+    }
+    far_targets_kernel<<<ntargets, FT, lds, stream>>>(plan, pairs, coef, boys_table, targets, local, near, nnear,
+                                                      work);
+    KUMO_HIP(hipGetLastError());
+}
+
+void launch_far_points(const EspPairDev* pairs, const std::vector<EspGroup>& groups, const double* coef,
+                       const double* boys_table, const EspAtomDev* atoms, int natoms, const PointSource& ps,
+                       long long q0, long long q1, const FarTargetDev* targets, int ntargets, const int64_t* idx,
+                       const double* local, const int32_t* near, const int32_t* nnear, int npairs, double* out,
+                       hipStream_t stream) {
+    if (ntargets <= 0) return;
+    far_poly_kernel<<<ntargets, FT, 0, stream>>>(atoms, natoms, ps, q0, q1, targets, idx, local, out);
+    KUMO_HIP(hipGetLastError());
+    for (const EspGroup& g : groups) {
+        if (g.last <= g.first) continue;
+#define KUMO_FAR_NEAR(LL)                                                                                             \
+    far_near_kernel<LL><<<ntargets, FT, 0, stream>>>(pairs, coef, boys_table, ps, q0, q1, targets, idx, near, nnear,   \
+                                                     npairs, g.first, g.last, out)
+        switch (g.L) {
+        case 0: KUMO_FAR_NEAR(0); break;
+        case 1: KUMO_FAR_NEAR(1); break;
+        case 2: KUMO_FAR_NEAR(2); break;
+        case 3: KUMO_FAR_NEAR(3); break;
+        case 4: KUMO_FAR_NEAR(4); break;
+        case 5: KUMO_FAR_NEAR(5); break;
+        case 6: KUMO_FAR_NEAR(6); break;
+        case 7: KUMO_FAR_NEAR(7); break;
+        case 8: KUMO_FAR_NEAR(8); break;
+        case 9: KUMO_FAR_NEAR(9); break;
+        case 10: KUMO_FAR_NEAR(10); break;
+        default: invalid("invalid electrostatic potential pair group");
+        }
+#undef KUMO_FAR_NEAR
+        KUMO_HIP(hipGetLastError());
+    }
+}
+
+} // namespace kumo

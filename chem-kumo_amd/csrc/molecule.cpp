@@ -556,6 +556,8 @@ void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
     };
     std::vector<std::vector<EspPairDev>> groups(2 * kMaxL + 1);
     std::vector<std::vector<double>> gcoef(2 * kMaxL + 1);
+    std::vector<std::vector<int32_t>> gorder(2 * kMaxL + 1); // each pair's place in the reference's order
+    int32_t made = 0;
     const auto& blocks = basis.blocks;
     const int nn = n;
     for (std::size_t ia = 0; ia < blocks.size(); ++ia) {
@@ -630,6 +632,7 @@ void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
                     auto& g = groups[static_cast<std::size_t>(L)];
                     auto& gc = gcoef[static_cast<std::size_t>(L)];
                     g.push_back({ex, Q[0], Q[1], Q[2], L, static_cast<int32_t>(gc.size())});
+                    gorder[static_cast<std::size_t>(L)].push_back(made++);
                     for (double c : pc)
                         gc.push_back(scale * c);
                 }
@@ -638,7 +641,10 @@ void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
     std::vector<EspPairDev> pairs;
     std::vector<double> coef;
     esp_groups.clear();
+    esp_ref_order.assign(static_cast<std::size_t>(made), 0);
     for (std::size_t L = 0; L < groups.size(); ++L) {
+        for (std::size_t k = 0; k < gorder[L].size(); ++k)
+            esp_ref_order[static_cast<std::size_t>(gorder[L][k])] = static_cast<int32_t>(pairs.size() + k);
         const int base = static_cast<int>(coef.size());
         if (!groups[L].empty())
             esp_groups.push_back({static_cast<int32_t>(L), static_cast<int32_t>(pairs.size()),
@@ -657,5 +663,46 @@ void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
     d_esp_atoms = upload(at.data(), at.size(), stream);
     esp_npairs = static_cast<int>(pairs.size());
     KUMO_HIP(hipStreamSynchronize(stream));
+    esp_pairs_host = std::move(pairs);
+    esp_coef_host = std::move(coef);
     esp_ready = true;
+}
+
+// cluster_plan (src/potential_clusters.hpp:29-35): the radius enlarged by a
+// relative 1e-10 plus 1e-12 bohr, then far_field_plan; uploaded once per
+// (radius, tolerance).
+const kumo::FarPlanDev& kumo_hip_molecule::far_plan(double radius, double tolerance, hipStream_t stream) {
+    using namespace kumo;
+    ensure_esp(stream);
+    std::lock_guard<std::mutex> guard(lazy);
+    const auto key = std::make_pair(radius, tolerance);
+    auto it = far_plans.find(key);
+    if (it != far_plans.end()) return it->second.dev;
+    const FarPlanHost h =
+        far_field_plan(esp_pairs_host, esp_coef_host, esp_ref_order, radius * (1.0 + 1e-10) + 1e-12, tolerance);
+    FarPlanResident r;
+    r.ref_order = upload(esp_ref_order.data(), esp_ref_order.size(), stream);
+    r.level_d2 = upload(h.level_d2.data(), h.level_d2.size(), stream);
+    r.clusters = upload(h.clusters.data(), h.clusters.size(), stream);
+    r.cluster_pairs = upload(h.cluster_pairs.data(), h.cluster_pairs.size(), stream);
+    r.cluster_of = upload(h.cluster_of.data(), h.cluster_of.size(), stream);
+    r.moments = upload(h.moments.data(), h.moments.size(), stream);
+    KUMO_HIP(hipStreamSynchronize(stream));
+    FarPlanDev& d = r.dev;
+    d.level_d2 = r.level_d2.as<float>();
+    d.clusters = r.clusters.as<FarClusterDev>();
+    d.cluster_pairs = r.cluster_pairs.as<int32_t>();
+    d.cluster_of = r.cluster_of.as<int32_t>();
+    d.moments = r.moments.as<double>();
+    d.ref_order = r.ref_order.as<int32_t>();
+    d.nclusters = static_cast<int>(h.clusters.size());
+    d.npairs = static_cast<int>(esp_pairs_host.size());
+    d.max_L = h.max_L;
+    d.moment_count = h.moment_count;
+    d.pair_tolerance = h.pair_tolerance;
+    for (int j = 0; j < kFarLevels; ++j)
+        d.levels[j] = h.levels[j];
+    device_bytes += static_cast<long long>(r.level_d2.bytes() + r.clusters.bytes() + r.cluster_pairs.bytes() +
+                                           r.cluster_of.bytes() + r.moments.bytes() + r.ref_order.bytes());
+    return far_plans.emplace(key, std::move(r)).first->second.dev;
 }

@@ -6,6 +6,7 @@
 
 #include "basis.hpp"
 #include "common.hpp"
+#include "far_field.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
@@ -27,7 +28,7 @@ struct Executor {
     hipStream_t stream = nullptr;
     const double* boys_table = nullptr;
     DeviceBuffer ws_tables, ws_moments, ws_phi, ws_w, ws_points, ws_tasks, ws_misc, ws_rescue, ws_rmom, ws_rscale,
-        ws_oe, ws_oe_tasks, ws_esp;
+        ws_oe, ws_oe_tasks, ws_esp, ws_far;
     // Events bracketing the contraction launches (index 0), the FP64 rescue
     // launches (index 1) and the field reductions (index 2).
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events[3];
@@ -53,9 +54,11 @@ struct Executor {
     // tables reach the device as kernel arguments (capture mode is implied).
     bool no_alloc = false;
     long long max_batch_blocks = 0;
+    // Exact ESP sums only (batches: the same values eager and graph-captured).
+    bool exact_esp = false;
     void use_arena(Arena* arena) {
         for (DeviceBuffer* b : {&ws_tables, &ws_moments, &ws_phi, &ws_w, &ws_points, &ws_tasks, &ws_misc, &ws_rescue,
-                                &ws_rmom, &ws_rscale, &ws_oe, &ws_oe_tasks, &ws_esp})
+                                &ws_rmom, &ws_rscale, &ws_oe, &ws_oe_tasks, &ws_esp, &ws_far})
             b->use_arena(arena);
     }
     void create(int device, const double* table);
@@ -136,6 +139,19 @@ struct kumo_hip_molecule {
     kumo::DeviceBuffer d_esp_pairs, d_esp_coef, d_esp_atoms;
     int esp_npairs = 0;
     std::vector<kumo::EspGroup> esp_groups; // pair ranges by L
+    std::vector<kumo::EspPairDev> esp_pairs_host; // the pairs and coefficients, for far-field plans
+    std::vector<double> esp_coef_host;
+    // esp_ref_order[k]: the pair that is the k-th in the reference's order
+    // (block pairs a <= b, then primitive pairs), which the far field's
+    // pair budget follows.
+    std::vector<int32_t> esp_ref_order;
+    // Far-field plans per (target radius, tolerance), resident until the
+    // molecule is released.
+    struct FarPlanResident {
+        kumo::DeviceBuffer level_d2, clusters, cluster_pairs, cluster_of, moments, ref_order;
+        kumo::FarPlanDev dev{};
+    };
+    std::map<std::pair<double, double>, FarPlanResident> far_plans;
     // Orbital-energy operand (OrbitalEnergyFields, src/orbital_energy_fields.hpp:98-192;
     // built on first use): every MO of the FCHK in FP64, slots [0,
     // oe_occ_slots) the occupied MOs, then the virtual ones, each range
@@ -157,6 +173,9 @@ struct kumo_hip_molecule {
     // The squared screening radii for `threshold` if they are resident, else null.
     const double* screen_r2_resident(double threshold);
     void ensure_esp(hipStream_t stream);
+    // The far-field plan for target clusters of `radius` bohr (enlarged for
+    // rounding as the reference's cluster_plan does) and `tolerance` > 0.
+    const kumo::FarPlanDev& far_plan(double radius, double tolerance, hipStream_t stream);
     void ensure_orbital_energy(hipStream_t stream);
 };
 

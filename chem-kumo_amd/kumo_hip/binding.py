@@ -31,7 +31,7 @@ class _Options(C.Structure):
     _fields_ = [("threads", C.c_int32), ("block_size", C.c_int32), ("backend", C.c_int32), ("precision", C.c_int32),
                 ("screening_threshold", C.c_double), ("potential_tolerance", C.c_double), ("fukui_delta", C.c_double),
                 ("device_outputs", C.c_int32), ("tile_points", C.c_int32), ("progress", PROGRESS),
-                ("progress_user", C.c_void_p)]
+                ("progress_user", C.c_void_p), ("esp_far_field", C.c_int32)]
 
 
 class _Desc(C.Structure):
@@ -334,7 +334,7 @@ def grid_struct(grid):
 
 def options(block_size=128, backend=Backend.AUTO, precision=Precision.FP64, screening_threshold=1e-12,
             potential_tolerance=1e-8, fukui_delta=0.1, device_outputs=False, progress=None, threads=0,
-            tile_points=0):
+            tile_points=0, esp_far_field=0):
     o = _Options()
     lib().kumo_hip_default_options(C.byref(o))
     o.threads = threads
@@ -346,6 +346,7 @@ def options(block_size=128, backend=Backend.AUTO, precision=Precision.FP64, scre
     o.fukui_delta = fukui_delta
     o.device_outputs = 1 if device_outputs else 0
     o.tile_points = tile_points
+    o.esp_far_field = int(esp_far_field)
     if progress is not None:
         cb = PROGRESS(lambda d, t, u: int(bool(progress(d, t))))
         o.progress = cb

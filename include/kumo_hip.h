@@ -80,7 +80,7 @@ typedef struct kumo_hip_options {
     int32_t backend;            /* enum kumo_hip_backend */
     int32_t precision;          /* enum kumo_hip_precision */
     double screening_threshold; /* >= 0; 0 keeps every AO */
-    double potential_tolerance; /* >= 0; the ESP is evaluated exactly */
+    double potential_tolerance; /* >= 0: error bound of the ESP's far-field approximation (0: exact) */
     double fukui_delta;         /* > 0 */
     int32_t device_outputs;     /* 1: outputs are device pointers (HBM) */
     int32_t tile_points;        /* AO->MO workgroup tile in points: 16 or 32 (the tile kernel:
@@ -91,7 +91,14 @@ typedef struct kumo_hip_options {
                                    molecule, precision and derivative order). */
     kumo_hip_progress_fn progress;
     void* progress_user;
+    int32_t esp_far_field;      /* ESP at potential_tolerance > 0: KUMO_HIP_ESP_AUTO (far field where
+                                   it pays), KUMO_HIP_ESP_FAR (always the far field, as the
+                                   reference) or KUMO_HIP_ESP_EXACT (the exact pair sums, whose
+                                   error is within any tolerance). Batches and
+                                   allocation-free evaluations always sum exactly. */
 } kumo_hip_options;
+
+enum kumo_hip_esp_method { KUMO_HIP_ESP_AUTO = 0, KUMO_HIP_ESP_FAR = 1, KUMO_HIP_ESP_EXACT = 2 };
 
 /* Molecule from raw FCHK records (src/basis.cpp:21-120, src/molecule.cpp:153-180).
  * Matrices are row-major n x n (full, symmetric); MO coefficients as in the
