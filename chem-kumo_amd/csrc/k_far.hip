@@ -51,13 +51,29 @@ __device__ __attribute__((always_inline)) inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// R^0_tuv for t + u + v <= top from the seeds R^n_000 (seeds[n], n <= top):
-//   R^n_(t+1,u,v) = t R^(n+1)_(t-1,u,v) + X_t R^(n+1)_(t,u,v)
-// (on the first nonzero index), from n = top down to 0, the entries of one
-// order across the wave's lanes; a and b are the wave's two LDS buffers of
-// far_hcount(top) doubles; returns the one holding order 0.
+// Recursion recipe of Hermite entry k > 0 (t, u, v): on the first nonzero
+// index a, R^n_k = X_a R^(n+1)_i1 + m R^(n+1)_i2 with i1 the entry one lower
+// in a, i2 two lower (m = index_a - 1, i2 = 0 when m = 0); packed as i1 |
+// i2 << 11 | a << 22 | m << 24.
+__device__ __attribute__((always_inline)) inline uint32_t hermite_recipe(int t, int u, int v) {
+    int a, m, i1, i2;
+    if (t > 0) {
+        a = 0, m = t - 1, i1 = tuv_index(t - 1, u, v), i2 = t > 1 ? tuv_index(t - 2, u, v) : 0;
+    } else if (u > 0) {
+        a = 1, m = u - 1, i1 = tuv_index(t, u - 1, v), i2 = u > 1 ? tuv_index(t, u - 2, v) : 0;
+    } else {
+        a = 2, m = v - 1, i1 = tuv_index(t, u, v - 1), i2 = v > 1 ? tuv_index(t, u, v - 2) : 0;
+    }
+    return static_cast<uint32_t>(i1) | static_cast<uint32_t>(i2) << 11 | static_cast<uint32_t>(a) << 22 |
+           static_cast<uint32_t>(m) << 24;
+}
+
+// R^0_tuv for t + u + v <= top from the seeds R^n_000 (seeds[n], n <= top),
+// from n = top down to 0, the entries of one order across the wave's lanes;
+// a and b are the wave's two LDS buffers of far_hcount(top) doubles; returns
+// the one holding order 0.
 __device__ double* wave_hermite(const double* seeds, const double x[3], int top, double* a, double* b,
-                                const uchar4* tri, int lane) {
+                                const uint32_t* recipe, int lane) {
     double* cur = a;
     double* prev = b;
     for (int n = top; n >= 0; --n) {
@@ -65,29 +81,47 @@ __device__ double* wave_hermite(const double* seeds, const double x[3], int top,
         cur = prev;
         prev = t_;
         const int entries = far_hcount(top - n);
-        for (int k = lane; k < entries; k += 64) {
-            double value;
-            if (k == 0) {
-                value = seeds[n];
-            } else {
-                const uchar4 h = tri[k];
-                const int t = h.x, u = h.y, v = h.z;
-                if (t > 0) {
-                    value = x[0] * prev[tuv_index(t - 1, u, v)];
-                    if (t > 1) value += (t - 1) * prev[tuv_index(t - 2, u, v)];
-                } else if (u > 0) {
-                    value = x[1] * prev[tuv_index(t, u - 1, v)];
-                    if (u > 1) value += (u - 1) * prev[tuv_index(t, u - 2, v)];
-                } else {
-                    value = x[2] * prev[tuv_index(t, u, v - 1)];
-                    if (v > 1) value += (v - 1) * prev[tuv_index(t, u, v - 2)];
-                }
-            }
-            cur[k] = value;
+        if (lane == 0) cur[0] = seeds[n];
+#pragma unroll 4
+        for (int k = lane + 1; k < entries; k += 64) {
+            const uint32_t r = recipe[k];
+            const int ax = (r >> 22) & 3;
+            const double xa = ax == 0 ? x[0] : (ax == 1 ? x[1] : x[2]);
+            const double m = static_cast<double>(r >> 24);
+            cur[k] = xa * prev[r & 2047u] + m * prev[(r >> 11) & 2047u];
         }
         wave_sync();
     }
     return cur;
+}
+
+// sum over the entries alpha of degree <= deg of w[alpha] R[alpha + beta]
+// (w in the Hermite order): for each degree and t the entries run over v
+// ascending, and so do their sums with beta, so the terms come in contiguous
+// runs of w and R.
+__device__ __attribute__((always_inline)) inline double shifted_dot(const double* __restrict__ w,
+                                                                    const double* __restrict__ R, int deg, int tb,
+                                                                    int sb, int vb) {
+    double s0 = 0.0, s1 = 0.0;
+    int k = 0;
+    for (int s = 0; s <= deg; ++s) {
+        const int sp = s + sb;
+        const int dbase = far_hcount(sp - 1);
+        for (int t = s; t >= 0; --t) {
+            const int d = sp - t - tb;
+            const double* r = R + dbase + d * (d + 1) / 2 + vb;
+            const double* ww = w + k;
+            const int len = s - t + 1;
+            int j = 0;
+            for (; j + 1 < len; j += 2) {
+                s0 += ww[j] * r[j];
+                s1 += ww[j + 1] * r[j + 1];
+            }
+            if (j < len) s0 += ww[j] * r[j];
+            k += len;
+        }
+    }
+    return s0 + s1;
 }
 
 __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const EspPairDev* __restrict__ pairs,
@@ -100,11 +134,13 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int top = plan.max_L + kFarMultipole;
     const int H = far_hcount(top);
-    // LDS: tri[H] (uchar4), then per wave two recursion buffers and the seeds
+    // LDS: the recursion recipes recipe[H], the indices tri[NLOC] of the
+    // local coefficients, then per wave two recursion buffers and the seeds
     // (top + 1 doubles), then the wave sums red[FW][NLOC], the level counts
     // and the scan words.
-    uchar4* tri = reinterpret_cast<uchar4*>(smem);
-    double* wbuf = reinterpret_cast<double*>(smem + ((H * 4 + 15) / 16) * 16);
+    uint32_t* recipe = reinterpret_cast<uint32_t*>(smem);
+    uchar4* tri = reinterpret_cast<uchar4*>(smem + ((H * 4 + 15) / 16) * 16);
+    double* wbuf = reinterpret_cast<double*>(smem + ((H * 4 + 15) / 16) * 16 + NLOC * 4);
     const int wstride = 2 * H + top + 2;
     double* buf_a = wbuf + wave * wstride;
     double* buf_b = buf_a + H;
@@ -127,8 +163,10 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
             r -= s - t + 1;
             --t;
         }
-        tri[k] = uchar4{static_cast<unsigned char>(t), static_cast<unsigned char>(s - t - r),
-                        static_cast<unsigned char>(r), 0};
+        if (k > 0) recipe[k] = hermite_recipe(t, s - t - r, r);
+        if (k < NLOC)
+            tri[k] = uchar4{static_cast<unsigned char>(t), static_cast<unsigned char>(s - t - r),
+                            static_cast<unsigned char>(r), 0};
     }
     if (tid < kFarLevels) counts[tid] = 0;
     // 0. the source clusters far enough to be translated, decided once so that
@@ -251,22 +289,17 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
             }
         }
         wave_sync();
-        const double* R = wave_hermite(seeds, x, top, buf_a, buf_b, tri, lane);
+        const double* R = wave_hermite(seeds, x, top, buf_a, buf_b, recipe, lane);
         const double* q0 = plan.moments + static_cast<long long>(ci) * (kFarMultipole + 1) * plan.moment_count;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int be = lane + 64 * r;
             if (be >= NLOC) continue;
             const uchar4 hb = tri[be];
-            const int m = kFarMultipole - (hb.x + hb.y + hb.z);
-            const double* q = q0 + static_cast<long long>(m) * plan.moment_count;
-            const int terms = far_hcount(plan.max_L + m);
-            double sum = 0.0;
-            for (int nu = 0; nu < terms; ++nu) {
-                const uchar4 ha = tri[nu];
-                sum += q[nu] * R[tuv_index(ha.x + hb.x, ha.y + hb.y, ha.z + hb.z)];
-            }
-            acc[r] += sum;
+            const int sb = hb.x + hb.y + hb.z;
+            const int m = kFarMultipole - sb;
+            acc[r] += shifted_dot(q0 + static_cast<long long>(m) * plan.moment_count, R, plan.max_L + m, hb.x, sb,
+                                  hb.z);
         }
         wave_sync(); // the buffers are rewritten by the next item
     }
@@ -286,20 +319,14 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
             }
         }
         wave_sync();
-        const double* R = wave_hermite(seeds, x, ord, buf_a, buf_b, tri, lane);
+        const double* R = wave_hermite(seeds, x, ord, buf_a, buf_b, recipe, lane);
         const double* c = coef + pr.first;
-        const int nk = far_hcount(pr.L);
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const int be = lane + 64 * r;
             if (be >= NLOW) continue;
             const uchar4 hb = tri[be];
-            double sum = 0.0;
-            for (int k = 0; k < nk; ++k) {
-                const uchar4 ha = tri[k];
-                sum += c[k] * R[tuv_index(ha.x + hb.x, ha.y + hb.y, ha.z + hb.z)];
-            }
-            acc[r] += sum;
+            acc[r] += shifted_dot(c, R, pr.L, hb.x, hb.x + hb.y + hb.z, hb.z);
         }
         wave_sync();
     }
@@ -446,8 +473,8 @@ void launch_far_targets(const FarPlanDev& plan, const EspPairDev* pairs, const d
     if (ntargets <= 0) return;
     const int top = plan.max_L + kFarMultipole;
     const int H = far_hcount(top);
-    if (top + kFarLocal > 255) invalid("far-field expansion order too high");
-    const size_t lds = ((H * 4 + 15) / 16) * 16 + sizeof(double) * (FW * (2 * H + top + 2) + FW * NLOC) +
+    if (H > 2048) invalid("far-field expansion order too high"); // recipe indices take 11 bits
+    const size_t lds = ((H * 4 + 15) / 16) * 16 + NLOC * 4 + sizeof(double) * (FW * (2 * H + top + 2) + FW * NLOC) +
                        sizeof(int) * (kFarLevels + 4 + 4 + FW);
     if (lds > 160 * 1024) invalid("far-field expansion needs more LDS than a workgroup has");
     static bool configured = false;

@@ -91,11 +91,13 @@ typedef struct kumo_hip_options {
                                    molecule, precision and derivative order). */
     kumo_hip_progress_fn progress;
     void* progress_user;
-    int32_t esp_far_field;      /* ESP at potential_tolerance > 0: KUMO_HIP_ESP_AUTO (far field where
-                                   it pays), KUMO_HIP_ESP_FAR (always the far field, as the
-                                   reference) or KUMO_HIP_ESP_EXACT (the exact pair sums, whose
-                                   error is within any tolerance). Batches and
-                                   allocation-free evaluations always sum exactly. */
+    int32_t esp_far_field;      /* ESP at potential_tolerance > 0: KUMO_HIP_ESP_FAR takes the
+                                   reference's far field (8^3 grid tiles, 1.6-bohr point cubes;
+                                   within the tolerance of the exact potential);
+                                   KUMO_HIP_ESP_AUTO and KUMO_HIP_ESP_EXACT sum every pair
+                                   exactly, which is within any tolerance and on this GPU the
+                                   faster of the two. Batches and allocation-free evaluations
+                                   always sum exactly. */
 } kumo_hip_options;
 
 enum kumo_hip_esp_method { KUMO_HIP_ESP_AUTO = 0, KUMO_HIP_ESP_FAR = 1, KUMO_HIP_ESP_EXACT = 2 };
