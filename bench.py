@@ -332,7 +332,8 @@ def roofline_record(st, precision, traffic=None):
     achieved = st["flops"] / kernel_s if kernel_s > 0 else 0.0
     n = max(1, st["launches"])
     return {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK[precision], "traffic": traffic, "kernel": "aomo_kernel",
+            "frac": achieved / PEAK[precision], "traffic": traffic,
+            "kernel": "aomo_wave_kernel" if precision in ("BF16X3", "BF16X6") else "aomo_kernel",
             "launches": st["launches"], "avg_launch_ms": st["kernel_ms"] / n,
             "algorithmic_flops_per_launch": st["flops"] / n, "mfma_products_per_flop": PRODUCTS[precision],
             "flops": "2 x kept AOs x occupied MOs x panels x points, summed over screening blocks"}
@@ -443,7 +444,7 @@ def bench_config2(args, built, torch):
                          "same properties, FP64, OpenMP over blocks", "host": host_cpu()}
         log(f"oracle: {cpu['value']:.4g} points/s on {threads} threads; errors {errors}")
     st = stats[args.precision]
-    traffic = load_traffic("traffic_aomo_bf16x6_config2.json") if args.precision == "BF16X6" else None
+    traffic = load_traffic("traffic_aomo_wave_bf16x6_config2.json") if args.precision == "BF16X6" else None
     out = {
         "metric": METRIC, "value": res[args.precision]["value"], "unit": "grid points/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": res[args.precision]["ms_per_step"],
