@@ -138,6 +138,35 @@ __device__ __attribute__((always_inline)) inline uint16_t bf16_term(double x, in
     return h;
 }
 
+// bf16 plane `plane` of 8 FP64 coefficients, as a 16-byte granule of the
+// staged table.
+__device__ __attribute__((always_inline)) inline uint4 bf16_granule(const double (&f)[8], int plane) {
+    uint32_t w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        w[i] = static_cast<uint32_t>(bf16_term(f[2 * i], plane)) |
+               (static_cast<uint32_t>(bf16_term(f[2 * i + 1], plane)) << 16);
+    return uint4{w[0], w[1], w[2], w[3]};
+}
+
+// The row of the fold workspace that code (<= kFoldedWs) names, from slot
+// `slot` on.
+__device__ __attribute__((always_inline)) inline const double* fold_ws_row(const KeptTables& t, int code, int slot) {
+    return t.fold_rows + static_cast<long long>(kFoldedWs - code) * t.fold_slots + slot;
+}
+
+// bf16 plane `plane` of 8 slots of a fold workspace row (16-byte aligned).
+__device__ __attribute__((always_inline)) inline uint4 fold_ws_bf16(const double* p, int plane) {
+    double f[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double2 v = reinterpret_cast<const double2*>(p)[i];
+        f[2 * i] = v.x;
+        f[2 * i + 1] = v.y;
+    }
+    return bf16_granule(f, plane);
+}
+
 // bf16 plane `plane` of a folded row at 8 slots from `slot`, as the 16-byte
 // granule of the staged table.
 __device__ __attribute__((always_inline)) inline uint4 folded_bf16(const MolView& m, const MoOperand& mo,
@@ -145,12 +174,7 @@ __device__ __attribute__((always_inline)) inline uint4 folded_bf16(const MolView
                                                                  int slot, int plane) {
     double f[8];
     folded_row<8>(m, mo, ent, ebits, code, slot, f);
-    uint32_t w[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        w[i] = static_cast<uint32_t>(bf16_term(f[2 * i], plane)) |
-               (static_cast<uint32_t>(bf16_term(f[2 * i + 1], plane)) << 16);
-    return uint4{w[0], w[1], w[2], w[3]};
+    return bf16_granule(f, plane);
 }
 
 } // namespace aomo

@@ -128,9 +128,13 @@ struct TableSet {
     KeptTables t;
 };
 
-TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
+// Tables of nblk screening blocks; fold_per_block > 0 adds the fold
+// workspace (that many folded rows per block, fold_slots wide).
+TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks, int fold_per_block = 0,
+                int fold_slots = 0) {
     const std::size_t b = static_cast<std::size_t>(nblk), ns = static_cast<std::size_t>(nsh);
     const std::size_t mc = static_cast<std::size_t>(max_chunks);
+    const std::size_t fr = b * static_cast<std::size_t>(std::max(0, fold_per_block));
     const std::size_t sizes[] = {
         b * nao,                // kept
         4 * b * nao,            // kept_list
@@ -144,6 +148,11 @@ TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
         4 * b * (mc + 1),       // chunk_begin
         8 * b * mc * kChunkRows, // chunk_rows
         4 * b,                  // nchunks
+        8 * fr * static_cast<std::size_t>(std::max(0, fold_slots)), // fold_rows
+        6 * fr * static_cast<std::size_t>(std::max(0, fold_slots)), // fold_lo
+        8 * fr,                 // fold_norm
+        4 * fr,                 // fold_desc
+        fr ? 4 * b : 0,         // fold_n
     };
     const auto al = [](std::size_t x) { return (x + 255) / 256 * 256; };
     std::size_t total = 0;
@@ -151,8 +160,8 @@ TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
         total += al(x);
     ex.ws_tables.resize(total);
     char* p = ex.ws_tables.as<char>();
-    void* ptr[12];
-    for (int i = 0; i < 12; ++i) {
+    void* ptr[17];
+    for (int i = 0; i < 17; ++i) {
         ptr[i] = p;
         p += al(sizes[i]);
     }
@@ -169,6 +178,14 @@ TableSet tables(Executor& ex, int nblk, int nao, int nsh, int max_chunks) {
     s.t.chunk_begin = static_cast<int32_t*>(ptr[9]);
     s.t.chunk_rows = static_cast<int32_t*>(ptr[10]);
     s.t.nchunks = static_cast<int32_t*>(ptr[11]);
+    s.t.fold_rows = static_cast<double*>(ptr[12]);
+    s.t.fold_lo = ptr[13];
+    s.t.fold_norm = static_cast<float2*>(ptr[14]);
+    s.t.fold_desc = static_cast<int32_t*>(ptr[15]);
+    s.t.fold_n = static_cast<int32_t*>(ptr[16]);
+    s.t.fold_per_block = fr ? fold_per_block : 0;
+    s.t.fold_slots = fold_slots;
+    s.t.fold_cap = static_cast<int>(fr);
     s.t.max_chunks = max_chunks;
     s.t.lists = 1;
     return s;
@@ -655,7 +672,16 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
         const double* sr2 = ex.no_alloc ? mol->screen_r2_resident(thr) : mol->screen_r2(thr, s);
         if (!sr2) invalid("the allocation-free evaluation needs the screening radii of its threshold resident "
                           "(an evaluation at that threshold through kumo_hip_evaluate_* makes them so)");
-        const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks);
+        // Folded rows of partially kept pure blocks (about 2 % of the rows of a
+        // def2-TZVP-shaped basis at 1e-12) are formed once per batch into the
+        // fold workspace: up to 16 per block, within 256 MiB; the others are
+        // formed where each chunk is staged.
+        int fold_rows = 0;
+        if (backend == KUMO_HIP_AO_MO && !ex.no_alloc) {
+            const long long per_row = 14ll * mol->slots + 12;
+            fold_rows = static_cast<int>(std::min<long long>(16, (256ll << 20) / (per_row * cap)));
+        }
+        const auto ts = tables(ex, cap, nao, nsh, mol->max_chunks, fold_rows, mol->slots);
         const long long batch_max = static_cast<long long>(cap) * bs;
         int slot0 = 0, slot1 = 0;
         bool need_a = false, need_b = false;
@@ -723,6 +749,7 @@ void run_range(Executor& ex, kumo_hip_molecule* mol, const Prepared& pr, const P
                 a.ps = ps;
                 a.mo = {mol->d_op64.as<double>(), mol->d_op32.as<float>(), mol->d_opbf.as<uint16_t>(),
                         mol->d_opnorm.as<float2>(), mol->operand_rows, mol->slots, mol->alpha_slots};
+                if (kt.fold_per_block > 0) launch_fold(mv, a.mo, kt, nb, opt.precision, s);
                 a.tables = ts.t;
                 a.npoints = n;
                 a.batch_first = first;

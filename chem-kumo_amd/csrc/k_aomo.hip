@@ -229,10 +229,12 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
             g_row[i] = prow_g[i];
             info_row[i] = prow_info[i];
         }
-        // 1. Coefficient rows of the chunk (zero past its rows).
-        // Folded rows (partially kept pure blocks, rare) are formed in a second
-        // pass, so that the plain copies keep their loads together.
+        // 1. Coefficient rows of the chunk (zero past its rows). Folded rows
+        // (partially kept pure blocks, rare) come from the fold workspace, or
+        // beyond it are formed in a second pass, so that the plain copies keep
+        // their loads together.
         bool deferred = false;
+        const auto entry_fold = [](int g) { return g < -1 && g > kFoldedWs; };
         if constexpr (is_bf16<PREC>) {
             // rows 2q and 2q + 1, 8 slots each, interleaved into 32-bit words
             const auto item_bf16 = [&](int u, bool fold) __attribute__((always_inline)) {
@@ -244,19 +246,32 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
                 const int g0 = pg0[u], g1 = pg1[u];
                 const E* src = op + sp * op_plane + slot_lo + gi * EPG;
                 uint4 v0 = uint4{0, 0, 0, 0}, v1 = uint4{0, 0, 0, 0};
+                const bool late = entry_fold(g0) || entry_fold(g1);
+                if (late != fold) {
+                    deferred = deferred || late;
+                    return;
+                }
                 if (!fold) {
-                    if (g0 < -1 || g1 < -1) {
-                        deferred = true;
-                        return;
-                    }
-                    if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
-                    if (g1 >= 0) v1 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
+                    // operand table rows and fold workspace rows: one load each
+                    const E* ws = static_cast<const E*>(a.tables.fold_lo) +
+                                  static_cast<long long>(sp) * a.tables.fold_cap * a.tables.fold_slots + slot_lo +
+                                  gi * EPG;
+                    const auto row = [&](int g) __attribute__((always_inline)) {
+                        return g >= 0 ? src + static_cast<long long>(g) * a.mo.slots
+                                      : ws + static_cast<long long>(kFoldedWs - g) * a.tables.fold_slots;
+                    };
+                    if (g0 >= 0 || g0 <= kFoldedWs) v0 = *reinterpret_cast<const uint4*>(row(g0));
+                    if (g1 >= 0 || g1 <= kFoldedWs) v1 = *reinterpret_cast<const uint4*>(row(g1));
                 } else {
-                    if (!(g0 < -1 || g1 < -1)) return;
-                    if (g0 >= 0) v0 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
-                    else if (g0 < -1) v0 = folded_bf16(m, a.mo, ent, ebits, g0, slot_lo + gi * EPG, sp);
-                    if (g1 >= 0) v1 = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
-                    else if (g1 < -1) v1 = folded_bf16(m, a.mo, ent, ebits, g1, slot_lo + gi * EPG, sp);
+                    const auto granule = [&](int g) __attribute__((always_inline)) {
+                        if (g >= 0) return *reinterpret_cast<const uint4*>(src + static_cast<long long>(g) * a.mo.slots);
+                        if (g <= kFoldedWs)
+                            return fold_ws_bf16(fold_ws_row(a.tables, g, slot_lo + gi * EPG), sp);
+                        if (g < -1) return folded_bf16(m, a.mo, ent, ebits, g, slot_lo + gi * EPG, sp);
+                        return uint4{0, 0, 0, 0};
+                    };
+                    v0 = granule(g0);
+                    v1 = granule(g1);
                 }
                 const auto lo = [](uint32_t x, uint32_t y) { return (x & 0xffffu) | (y << 16); };
                 const auto hi = [](uint32_t x, uint32_t y) { return (x >> 16) | (y & 0xffff0000u); };
@@ -279,14 +294,20 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
                 const int sp = item / (KC * GMAX);
                 if (gi >= G) return;
                 const int grow = pg0[u];
-                if ((grow < -1) != fold) {
-                    deferred = deferred || grow < -1;
+                if (entry_fold(grow) != fold) {
+                    deferred = deferred || entry_fold(grow);
                     return;
                 }
                 uint4 v = uint4{0, 0, 0, 0};
                 if (grow >= 0) {
                     v = *reinterpret_cast<const uint4*>(op + sp * op_plane + static_cast<long long>(grow) * a.mo.slots +
                                                         slot_lo + gi * EPG);
+                } else if (grow <= kFoldedWs) {
+                    // the fold workspace as staged (FP64 rows for FP64, FP32 rows for FP32)
+                    const long long at = static_cast<long long>(kFoldedWs - grow) * a.tables.fold_slots + slot_lo +
+                                         gi * EPG;
+                    if constexpr (PREC == KUMO_HIP_FP64) v = *reinterpret_cast<const uint4*>(a.tables.fold_rows + at);
+                    else v = *reinterpret_cast<const uint4*>(static_cast<const float*>(a.tables.fold_lo) + at);
                 } else if (grow < -1) {
                     double f[EPG];
                     folded_row<EPG>(m, a.mo, ent, ebits, grow, slot_lo + gi * EPG, f);
@@ -351,7 +372,9 @@ __device__ __attribute__((always_inline)) inline void aomo_tile(const AoMoLaunch
                 if (!scale) return;
                 const int g = g_row[i];
                 if (g == -1) return;
-                const float2 w = g >= 0 ? a.mo.norm[g] : folded_norm(m, a.mo, ent, ebits, g);
+                const float2 w = g >= 0 ? a.mo.norm[g]
+                                        : (g <= kFoldedWs ? a.tables.fold_norm[kFoldedWs - g]
+                                                          : folded_norm(m, a.mo, ent, ebits, g));
                 const float av = static_cast<float>(v[0]);
                 const float wx2 = w.x * w.x, wy2 = w.y * w.y;
                 es0 += (av * av) * wx2;
@@ -706,7 +729,62 @@ template <int PREC> void launch_prec(const AoMoLaunch& a, hipStream_t stream) {
     }
 }
 
+// Folded rows of the fold workspace (one workgroup per screening block):
+// row w = b * fold_per_block + j holds, at every MO slot, the sum over the
+// kept pure AOs r of the entry of T[r][k] C[r] (pure-to-Cartesian transform
+// times the MO coefficients, in r order as folded_row sums them), and the
+// upper bounds of its norms.
+template <int PREC>
+__global__ void __launch_bounds__(256) fold_kernel(MolView m, MoOperand mo, KeptTables t) {
+    const int b = blockIdx.x;
+    const int n = t.fold_n[b];
+    const int32_t* ent = t.ent + static_cast<long long>(b) * m.nshells;
+    const uint32_t* ebits = t.ent_bits + static_cast<long long>(b) * m.nshells;
+    for (int j = 0; j < n; ++j) {
+        const int w = b * t.fold_per_block + j;
+        const int code = t.fold_desc[w];
+        const int c = kFoldedRow - code;
+        const int q = c >> 5, k = c & 31;
+        const DevShell& sh = m.shells[ent[q] & 0xffffff];
+        const uint32_t bits = ebits[q];
+        const double* tg = m.tg + sh.l * kMaxPure * kMaxCart + k;
+        double* out = t.fold_rows + static_cast<long long>(w) * t.fold_slots;
+        for (int slot = threadIdx.x; slot < mo.slots; slot += 256) {
+            double acc = 0.0;
+            for (int r = 0; r < sh.count; ++r) {
+                const double tr = tg[r * kMaxCart];
+                if (!((bits >> r) & 1u) || tr == 0.0) continue;
+                acc += tr * mo.op64[static_cast<long long>(sh.prow + r) * mo.slots + slot];
+            }
+            out[slot] = acc;
+            const long long at = static_cast<long long>(w) * t.fold_slots + slot;
+            if constexpr (PREC == KUMO_HIP_FP32) {
+                static_cast<float*>(t.fold_lo)[at] = static_cast<float>(acc);
+            } else if constexpr (is_bf16<PREC>) {
+                const long long plane = static_cast<long long>(t.fold_cap) * t.fold_slots;
+#pragma unroll
+                for (int sp = 0; sp < kBf16Planes; ++sp)
+                    static_cast<uint16_t*>(t.fold_lo)[sp * plane + at] = bf16_term(acc, sp);
+            }
+        }
+        if (threadIdx.x == 0) t.fold_norm[w] = folded_norm(m, mo, ent, ebits, code);
+    }
+}
+
 } // namespace
+
+void launch_fold(const MolView& m, const MoOperand& mo, const KeptTables& t, int nblk, int precision,
+                 hipStream_t stream) {
+    if (nblk <= 0 || t.fold_per_block <= 0) return;
+    if (t.fold_slots != mo.slots || nblk * t.fold_per_block > t.fold_cap)
+        invalid("fold workspace does not match the launch");
+    switch (precision) {
+    case KUMO_HIP_FP64: fold_kernel<KUMO_HIP_FP64><<<nblk, 256, 0, stream>>>(m, mo, t); break;
+    case KUMO_HIP_FP32: fold_kernel<KUMO_HIP_FP32><<<nblk, 256, 0, stream>>>(m, mo, t); break;
+    default: fold_kernel<KUMO_HIP_BF16X6><<<nblk, 256, 0, stream>>>(m, mo, t); break;
+    }
+    KUMO_HIP(hipGetLastError());
+}
 
 void launch_ao_mo(const AoMoLaunch& a, hipStream_t stream) {
     if (a.nblk <= 0 || a.slot1 <= a.slot0) return;

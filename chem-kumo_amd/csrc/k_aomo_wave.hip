@@ -162,9 +162,15 @@ __device__ __attribute__((always_inline)) inline void load_next(const AoMoLaunch
         const int q = (item / GMAX) % KQ;
         const int sp = item / (KQ * GMAX);
         const uint16_t* src = op + sp * op_plane + slot_lo + gi * 8;
-        const int g0 = max(wop[2 * q], 0), g1 = max(wop[2 * q + 1], 0);
-        nx.cv[u][0] = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g0) * a.mo.slots);
-        nx.cv[u][1] = *reinterpret_cast<const uint4*>(src + static_cast<long long>(g1) * a.mo.slots);
+        const uint16_t* ws = static_cast<const uint16_t*>(a.tables.fold_lo) +
+                             static_cast<long long>(sp) * a.tables.fold_cap * a.tables.fold_slots + slot_lo + gi * 8;
+        // operand table rows, or fold workspace rows as staged
+        const auto row = [&](int g) __attribute__((always_inline)) {
+            return g <= kFoldedWs ? ws + static_cast<long long>(kFoldedWs - g) * a.tables.fold_slots
+                                  : src + static_cast<long long>(max(g, 0)) * a.mo.slots;
+        };
+        nx.cv[u][0] = *reinterpret_cast<const uint4*>(row(wop[2 * q]));
+        nx.cv[u][1] = *reinterpret_cast<const uint4*>(row(wop[2 * q + 1]));
     }
     nx.nrm = a.mo.norm[max(wop[tid & (KC - 1)], 0)];
     // ring indices of chunk cn + 1: lanes of wave 0 (entries), wave 1 (first
@@ -179,8 +185,8 @@ __device__ __attribute__((always_inline)) inline void load_next(const AoMoLaunch
 }
 
 // Writes what load_next loaded: chunk cn's records, coefficients (into cs)
-// and norms, forming folded rows (partially kept pure blocks, rare) here, and
-// chunk cn + 1's ring indices.
+// and norms, taking folded rows (partially kept pure blocks, rare) from the
+// fold workspace or forming them here, and chunk cn + 1's ring indices.
 template <int S>
 __device__ __attribute__((always_inline)) inline void store_next(const AoMoLaunch& a, const Plan& pl, int cn,
                                                                int nent, int nchunks, int e1, int e2,
@@ -199,9 +205,10 @@ __device__ __attribute__((always_inline)) inline void store_next(const AoMoLaunc
         const int sp = item / (KQ * GMAX);
         if (gi >= G) continue;
         const int g0 = wop[2 * q], g1 = wop[2 * q + 1];
-        uint4 v0 = g0 >= 0 ? nx.cv[u][0] : uint4{0, 0, 0, 0}, v1 = g1 >= 0 ? nx.cv[u][1] : uint4{0, 0, 0, 0};
-        if (g0 < -1) v0 = folded_bf16(a.mol, a.mo, ent, ebits, g0, slot_lo + gi * 8, sp);
-        if (g1 < -1) v1 = folded_bf16(a.mol, a.mo, ent, ebits, g1, slot_lo + gi * 8, sp);
+        const auto loaded = [](int g) { return g >= 0 || g <= kFoldedWs; };
+        uint4 v0 = loaded(g0) ? nx.cv[u][0] : uint4{0, 0, 0, 0}, v1 = loaded(g1) ? nx.cv[u][1] : uint4{0, 0, 0, 0};
+        if (g0 < -1 && g0 > kFoldedWs) v0 = folded_bf16(a.mol, a.mo, ent, ebits, g0, slot_lo + gi * 8, sp);
+        if (g1 < -1 && g1 > kFoldedWs) v1 = folded_bf16(a.mol, a.mo, ent, ebits, g1, slot_lo + gi * 8, sp);
         const auto lo = [](uint32_t x, uint32_t y) { return (x & 0xffffu) | (y << 16); };
         const auto hi = [](uint32_t x, uint32_t y) { return (x >> 16) | (y & 0xffff0000u); };
         uint4* d = reinterpret_cast<uint4*>(cs + (sp * KQ + q) * CP + gi * 8);
@@ -211,7 +218,8 @@ __device__ __attribute__((always_inline)) inline void store_next(const AoMoLaunc
     if (tid >= WTHR - KC) {
         const int r = tid - (WTHR - KC);
         const int g = wop[r];
-        pl.nrm[r] = g < -1 ? folded_norm(a.mol, a.mo, ent, ebits, g) : (g >= 0 ? nx.nrm : float2{0.f, 0.f});
+        pl.nrm[r] = g <= kFoldedWs ? a.tables.fold_norm[kFoldedWs - g]
+                    : (g < -1 ? folded_norm(a.mol, a.mo, ent, ebits, g) : (g >= 0 ? nx.nrm : float2{0.f, 0.f}));
     }
     if (cn + 1 < nchunks && tid < 3 * 64) {
         const int l = tid & 63, w = tid >> 6;

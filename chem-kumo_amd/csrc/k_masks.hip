@@ -556,6 +556,23 @@ __global__ void __launch_bounds__(kMaskThreads) __attribute__((amdgpu_waves_per_
     }
     const int nc = packed[0];
     nent = packed[1];
+    // Fold workspace rows of the partially kept pure entries: the exclusive
+    // prefix sum of their Cartesian components, in entry order; the first
+    // t.fold_per_block go to the workspace (snext is free here).
+    int32_t* sfold = snext;
+    if (t.fold_per_block > 0) {
+        int base = 0;
+        for (int q0 = 0; q0 < nent; q0 += kMaskThreads) {
+            const int q = q0 + tid;
+            const int rows = q < nent && (sent[q] >> 24) == kEntryPure ? m.shells[sent[q] & 0xffffff].ncart : 0;
+            int tot;
+            const int pre = base + block_scan(rows, scan_tmp, tot);
+            if (q < nent) sfold[q] = pre;
+            base += tot;
+        }
+        if (tid == 0) t.fold_n[blockIdx.x] = min(base, t.fold_per_block);
+        __syncthreads();
+    }
     int32_t* en = t.ent + static_cast<long long>(blockIdx.x) * nsh;
     uint32_t* eb = t.ent_bits + static_cast<long long>(blockIdx.x) * nsh;
     uint8_t* er = t.ent_row + static_cast<long long>(blockIdx.x) * nsh;
@@ -572,13 +589,26 @@ __global__ void __launch_bounds__(kMaskThreads) __attribute__((amdgpu_waves_per_
         // Every entry contributes its Cartesian components: rows of dropped
         // AOs of a Cartesian block get zero coefficients (-1); the rows of a
         // partially kept pure block are folded: coefficient row k is the sum
-        // over its kept pure AOs r of T[r][k] g[k] C[r], formed when the
-        // contraction stages the chunk (code -2 - (entry << 5 | k)).
+        // over its kept pure AOs r of T[r][k] g[k] C[r], formed by
+        // launch_fold into the fold workspace (code kFoldedWs - workspace
+        // row) or, past its rows, when the contraction stages the chunk
+        // (code kFoldedRow - (entry << 5 | k)).
         if (c < t.max_chunks)
             for (int i = 0; i < sh.ncart; ++i) {
                 int32_t* cr = crows + 2 * (c * kChunkRows + row0 + i);
-                cr[0] = mode == kEntryPure ? kFoldedRow - ((q << 5) | i)
-                                           : ((mode == kEntryCart || ((bits >> i) & 1u)) ? sh.crow + i : -1);
+                if (mode == kEntryPure) {
+                    const int code = kFoldedRow - ((q << 5) | i);
+                    const int j = t.fold_per_block > 0 ? sfold[q] + i : t.fold_per_block;
+                    if (j < t.fold_per_block) {
+                        const int w = static_cast<int>(blockIdx.x) * t.fold_per_block + j;
+                        t.fold_desc[w] = code;
+                        cr[0] = kFoldedWs - w;
+                    } else {
+                        cr[0] = code;
+                    }
+                } else {
+                    cr[0] = (mode == kEntryCart || ((bits >> i) & 1u)) ? sh.crow + i : -1;
+                }
                 // Phi row: block << 15 | Cartesian mode << 13 | Cartesian powers (3 bits each)
                 const int pm = mode == kEntryPure ? kEntryCartMasked : mode;
                 cr[1] = (s << 15) | (pm << 13) | (kPow[sh.l][i][0] << 6) | (kPow[sh.l][i][1] << 3) | kPow[sh.l][i][2];

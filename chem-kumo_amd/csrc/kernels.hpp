@@ -26,8 +26,11 @@ enum EntryMode : int32_t {
                           // AOs' coefficients folded when the chunk is staged
 };
 // Operand row code of a folded row: kFoldedRow - (entry << 5 | Cartesian
-// component) (codes below -1; -1 is a zero row).
+// component) (codes below -1; -1 is a zero row), formed where the chunk is
+// staged; or, for the rows the fold workspace holds, kFoldedWs - (workspace
+// row), formed once per batch by launch_fold (codes <= kFoldedWs).
 constexpr int32_t kFoldedRow = -2;
+constexpr int32_t kFoldedWs = -(1 << 29);
 
 // Per screening block outputs of the mask kernel.
 struct KeptTables {
@@ -51,6 +54,19 @@ struct KeptTables {
     int32_t* chunk_rows;
     int32_t* nchunks;       // [nblk]
     int max_chunks;
+    // Fold workspace: up to fold_per_block folded rows per block (0: none;
+    // the rows beyond it are formed at staging), row b * fold_per_block + j
+    // of block b: its FP64 coefficients over every MO slot, the upper bounds
+    // of its norms, its entry-relative code, and the rows per block.
+    double* fold_rows;      // [fold_cap][fold_slots]
+    void* fold_lo;          // the same rows as staged in FP32 ([fold_cap][fold_slots]) or as
+                            // kBf16Planes bf16 planes ([plane][fold_cap][fold_slots])
+    float2* fold_norm;      // [fold_cap]
+    int32_t* fold_desc;     // [fold_cap] kFoldedRow - (entry << 5 | component)
+    int32_t* fold_n;        // [nblk]
+    int fold_per_block;
+    int fold_slots;
+    int fold_cap;           // rows allocated: blocks of the tables x fold_per_block
     // 1: also write kept, kept_list, kept_shell, kept_bits and nkshell (the
     // AO-P path and the masks entry point); 0: the AO->MO plan and nkept only.
     int lists;
@@ -181,6 +197,11 @@ constexpr int kRescueTile = 16; // points per rescue tile (the FP64 kernel's til
 // rescue and weighted modes; launch_ao_mo dispatches to it.
 bool ao_mo_wave_supported(const AoMoLaunch& a);
 void launch_ao_mo_wave(const AoMoLaunch& a, hipStream_t stream);
+// The folded rows the masks of nblk blocks placed in the fold workspace
+// (t.fold_per_block > 0), from the FP64 operand table mo: in FP64 and as
+// staged in `precision` (FP32 or bf16 planes; nothing more for FP64).
+void launch_fold(const MolView& m, const MoOperand& mo, const KeptTables& t, int nblk, int precision,
+                 hipStream_t stream);
 
 // AO-P contraction (density_derivatives_shared / densities_blocked,
 // src/density.cpp:149-229, 365-444): W = P_kept [Phi Phi_x Phi_y Phi_z] and
