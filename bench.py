@@ -367,7 +367,8 @@ def bench_config2(args, built, torch):
         main = prec == args.precision
         o = [torch.empty((nch, n), dtype=torch.float64, device="cuda") for n in counts]
         ptrs = [[t_[c].data_ptr() for c in range(nch)] for t_ in o]
-        b = Batch(ctx, mols, grids, PROPS_BATCH, ptrs, streams=args.streams, precision=int(Precision[prec]))
+        b = Batch(ctx, mols, grids, PROPS_BATCH, ptrs, streams=args.streams, precision=int(Precision[prec]),
+                  tile_points=args.tile_points if prec != "FP64" else 0)
         t = time.perf_counter()
         for _ in range(max(1, args.warmup if main else 1)):
             b.run(graph=True)
@@ -753,6 +754,8 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--chunk", type=int, default=131072)
     ap.add_argument("--precision", default="BF16X6", choices=list(PEAK))
+    ap.add_argument("--tile-points", type=int, default=0, choices=[0, 16, 32, 64],
+                    help="config 2: AO-MO workgroup tile of the reduced precision (0: the library's choice)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and oracle comparisons")
     ap.add_argument("--no-sub", action="store_true", help="config 2 without the configuration 1 and 4 sub-records")
     ap.add_argument("--cpu-blocks", type=int, default=0,
