@@ -606,7 +606,7 @@ def bench_config1(args, c1, torch, sub=True):
         errors["vs"] = f"FP64 oracle at {len(idx)} points ({len(blocks)} blocks); ELF where rho >= {ELF_FLOOR:g}"
     kernel_s = main_run["kernel_ms"] * 1e-3
     st = {"kernel_ms": main_run["kernel_ms"], "launches": main_run["launches"], "flops": main_run["flops"]}
-    traffic = load_traffic("traffic_aomo_bf16x6_tzvp640.json") if args.precision == "BF16X6" else None
+    traffic = load_traffic("traffic_aomo_wave_bf16x6_tzvp640.json") if args.precision == "BF16X6" else None
     res = {"workload": "40-atom def2-TZVP-shaped UHF (640 AOs, 60a/59b) on a 160^3 cube at 0.15 bohr: rho, grad, "
                        "lap, tau, ELF, LOL", "points": n, "precision": args.precision, "value": value,
            "unit": "grid points/s", "ms_per_step": 1e3 * main_run["seconds"] / steps, "steps": steps,
