@@ -1,8 +1,9 @@
 // AO-P contraction (src/density.cpp:149-229 density_derivatives_shared,
 // :365-444 densities_blocked): per screening block, the kept AO panels Phi
-// are staged in HBM, W = P_kept * [Phi Phi_x Phi_y Phi_z] is formed with an
-// LDS-tiled FP64 product that gathers P by the block's kept list, and the
-// per-point moments are the dot products of Phi and W over the kept rows.
+// are staged in HBM, W = P_kept * [Phi Phi_x Phi_y Phi_z] is formed on the
+// FP64 matrix cores (v_mfma_f64_16x16x4) from LDS tiles of P, gathered by
+// the block's kept list, and of Phi, and the per-point moments are the dot
+// products of Phi and W over the kept rows.
 #include "kernels.hpp"
 
 #include "common.hpp"
@@ -63,60 +64,89 @@ __global__ void __launch_bounds__(128) aop_phi_kernel(AoPLaunch a, int panels) {
     }
 }
 
-// W[blk][i][c][j] = sum_k Peff(kept_i, kept_k) Phi[k][c][j], c < wpanels.
-// Peff: mode 0 P0, 1 P1, 2 0.5 (P0 + P1), 3 0.5 (P0 - P1).
-__global__ void __launch_bounds__(256) aop_w_kernel(AoPLaunch a, int panels, int wpanels, int mode) {
-    constexpr int TI = 16, TK = 16;
-    __shared__ double ps_[TI][TK + 1];
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// W[blk][i][c][j] = sum_k P(kept_i, kept_k) Phi[k][c][j], c < wpanels, on
+// FP64 MFMA. A workgroup computes a 64 x 64 tile of W (rows i, columns
+// (c, j) = (col / bcount, col % bcount)); wave w its rows 16 w .. 16 w + 15
+// against the four 16-column subtiles. K runs over the kept rows in chunks
+// of 16 staged in LDS: P gathered by the kept list (zero past m), Phi rows
+// read along the points. Operand layouts of v_mfma_f64_16x16x4: lane l
+// holds A[l % 16][l / 16] and B[l / 16][l % 16], and accumulator element r
+// is D[l / 16 + 4 r][l % 16].
+constexpr int WT = 64, WK = 16;
+__global__ void __launch_bounds__(256) aop_w_kernel(AoPLaunch a, int panels, int wpanels) {
+    __shared__ double ps_[WT][WK + 1];
+    __shared__ double fs_[WK][WT];
     const int blk_local = blockIdx.z;
     const long long blk = a.first_block + blk_local;
     const long long bstart = blk * a.block_size;
     const int bcount = static_cast<int>(min(static_cast<long long>(a.block_size), a.npoints - bstart));
     const int m = a.tables.nkept[blk_local];
-    const int i0 = blockIdx.y * TI;
-    if (i0 >= m) return;
+    const int i0 = blockIdx.y * WT;
+    const int cols = wpanels * bcount;
+    const int col0 = blockIdx.x * WT;
+    if (i0 >= m || col0 >= cols) return; // uniform over the workgroup
     const int nao = a.mol.nao, B = a.block_size;
     const int32_t* kl = a.tables.kept_list + static_cast<long long>(blk_local) * nao;
     const double* phi = a.phi + static_cast<long long>(blk_local) * nao * panels * B;
     double* w = a.w + static_cast<long long>(blk_local) * nao * 4 * B;
-    const int cols = wpanels * bcount;
-    const int col = blockIdx.x * 256 + threadIdx.x; // (c, j) = (col / bcount, col % bcount)
-    const bool active = col < cols;
-    const int c = active ? col / bcount : 0, j = active ? col % bcount : 0;
-    double acc[TI];
+    const double* P = a.p[0];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    // the staging items of this thread: 4 of the P tile, 4 of the Phi tile
+    int prow[4], pcol[4], fk[4], fcol[4];
+    long long fofs[4];
+    bool fok[4];
 #pragma unroll
-    for (int r = 0; r < TI; ++r)
-        acc[r] = 0.0;
-    for (int k0 = 0; k0 < m; k0 += TK) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < TI * TK; e += 256) {
-            const int r = e / TK, kk = e % TK;
-            double v = 0.0;
-            if (i0 + r < m && k0 + kk < m) {
-                const long long idx = static_cast<long long>(kl[i0 + r]) * nao + kl[k0 + kk];
-                const double p0 = a.p[0][idx];
-                if (mode == 0) v = p0;
-                else if (mode == 1) v = a.p[1][idx];
-                else if (mode == 2) v = 0.5 * (p0 + a.p[1][idx]);
-                else v = 0.5 * (p0 - a.p[1][idx]);
-            }
-            ps_[r][kk] = v;
+    for (int u = 0; u < 4; ++u) {
+        const int e = tid + 256 * u;
+        prow[u] = e / WK;
+        pcol[u] = e % WK;
+        fk[u] = e / WT;
+        fcol[u] = e % WT;
+        const int col = col0 + fcol[u];
+        fok[u] = col < cols;
+        const int c = fok[u] ? col / bcount : 0, j = fok[u] ? col % bcount : 0;
+        fofs[u] = static_cast<long long>(c) * B + j;
+    }
+    const int pi[4] = {i0 + prow[0] < m ? kl[i0 + prow[0]] : -1, i0 + prow[1] < m ? kl[i0 + prow[1]] : -1,
+                       i0 + prow[2] < m ? kl[i0 + prow[2]] : -1, i0 + prow[3] < m ? kl[i0 + prow[3]] : -1};
+    f64x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < m; k0 += WK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + pcol[u];
+            ps_[prow[u]][pcol[u]] =
+                (pi[u] >= 0 && k < m) ? P[static_cast<long long>(pi[u]) * nao + kl[k]] : 0.0;
+            const int kf = k0 + fk[u];
+            fs_[fk[u]][fcol[u]] =
+                (fok[u] && kf < m) ? phi[static_cast<long long>(kf) * panels * B + fofs[u]] : 0.0;
         }
         __syncthreads();
-        if (active) {
-            const int kmax = min(TK, m - k0);
-            for (int kk = 0; kk < kmax; ++kk) {
-                const double x = phi[(static_cast<long long>(k0 + kk) * panels + c) * B + j];
 #pragma unroll
-                for (int r = 0; r < TI; ++r)
-                    acc[r] += ps_[r][kk] * x;
-            }
+        for (int ks = 0; ks < WK; ks += 4) {
+            const double av = ps_[16 * wave + r16][ks + g];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, fs_[ks + g][16 * t + r16], acc[t], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int col = col0 + 16 * t + r16;
+        if (col >= cols) continue;
+        const int c = col / bcount, j = col % bcount;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = i0 + 16 * wave + g + 4 * r;
+            if (i < m) w[(static_cast<long long>(i) * 4 + c) * B + j] = acc[t][r];
         }
     }
-    if (active)
-#pragma unroll
-        for (int r = 0; r < TI; ++r)
-            if (i0 + r < m) w[(static_cast<long long>(i0 + r) * 4 + c) * B + j] = acc[r];
 }
 
 // Moments of one matrix (contract_derivatives, src/density.cpp:26-44).
@@ -176,14 +206,14 @@ void launch_ao_p(const AoPLaunch& a, hipStream_t stream) {
     default: aop_phi_kernel<2><<<gphi, 128, 0, stream>>>(a, panels); break;
     }
     KUMO_HIP(hipGetLastError());
-    const dim3 gw(static_cast<unsigned>((wpanels * a.block_size + 255) / 256),
-                  static_cast<unsigned>((a.mol.nao + 15) / 16), static_cast<unsigned>(a.nblk));
+    const dim3 gw(static_cast<unsigned>((wpanels * a.block_size + WT - 1) / WT),
+                  static_cast<unsigned>((a.mol.nao + WT - 1) / WT), static_cast<unsigned>(a.nblk));
     for (int ch = 0; ch < 2; ++ch) {
         if (!a.p[ch]) continue;
-        // a.p[ch] encodes the matrix; modes are resolved by the caller through p[].
+        // a.p[ch] is the matrix of channel ch (the caller forms combinations).
         AoPLaunch b = a;
         b.p[0] = a.p[ch];
-        aop_w_kernel<<<gw, 256, 0, stream>>>(b, panels, wpanels, 0);
+        aop_w_kernel<<<gw, 256, 0, stream>>>(b, panels, wpanels);
         KUMO_HIP(hipGetLastError());
         aop_moments_kernel<<<gphi, 128, 0, stream>>>(a, panels, a.order, ch);
         KUMO_HIP(hipGetLastError());
