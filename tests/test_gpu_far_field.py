@@ -175,3 +175,53 @@ def test_invalid_method(ctx):
     g, _ = load(ctx, "pople20")
     with pytest.raises(Exception, match="electrostatic potential method"):
         g.evaluate(ESP, points=np.zeros((4, 3)), esp_far_field=7)
+
+
+def test_high_l_pairs(ctx):
+    # g and h shells: pair orders up to L = 10, expansions to order 20 (the
+    # largest recursion the target kernel holds) and near-pair kernels of
+    # every order
+    from kumo_hip.binding import Molecule
+    sm, grid = synthetic.high_l_molecule()
+    g = Molecule(ctx, data=sm)
+    o = oracle.Molecule(sm)
+    try:
+        print("[gpu] far field: high-L molecule, grid", tuple(grid["counts"]), flush=True)
+        far = esp(g, 1e-6, FAR, grid=grid)
+        exact = esp(g, 0.0, EXACT, grid=grid)
+        orc = o.evaluate(ESP, grid=grid, potential_tolerance=1e-6)[0][0][0]
+        err = np.abs(far - exact).max()
+        print(f"[gpu] far field: high-L max |far - exact| {err:.3g}, vs oracle far field "
+              f"{np.abs(far - orc).max():.3g}", flush=True)
+        assert 0.0 < err <= 1e-6
+        assert np.abs(far - orc).max() <= 1e-11
+    finally:
+        g.close()
+
+
+def test_special_points(ctx):
+    # points on the nuclei (their own charge left out), duplicates, points in
+    # cubes of their own and points 1000 bohr away (every source cluster
+    # translated), against the oracle's far field and the exact potential
+    g, o = load(ctx, "pople20")
+    xyz = o.data.xyz
+    far_away = xyz.mean(0) + np.array([[1000.0, 0.0, 0.0], [0.0, -1000.0, 0.0], [577.0, 577.0, 577.0]])
+    pts = np.concatenate([xyz, xyz[:3], xyz[:5] + 0.37, far_away, far_away + 0.1])
+    for tol in (1e-6, 1e-3):
+        far = esp(g, tol, FAR, points=pts)
+        orc = o.evaluate(ESP, points=pts, potential_tolerance=tol)[0][0][0]
+        exact = esp(g, 0.0, EXACT, points=pts)
+        assert np.array_equal(far[len(xyz):len(xyz) + 3], far[:3])
+        assert np.abs(far - orc).max() <= 1e-11 * max(1.0, np.abs(orc).max())
+        assert np.abs(far - exact).max() <= tol
+
+
+def test_loose_tolerance_grid(ctx):
+    # 1e-3: most pairs of a tile expanded, most clusters translated
+    g, o = load(ctx, "pople20")
+    grid = box_grid(synthetic.pople_molecule().xyz, 1.5, (17, 15, 13))
+    far = esp(g, 1e-3, FAR, grid=grid)
+    exact = esp(g, 0.0, EXACT, grid=grid)
+    orc = o.evaluate(ESP, grid=grid, potential_tolerance=1e-3)[0][0][0]
+    assert 0.0 < np.abs(far - exact).max() <= 1e-3
+    assert np.abs(far - orc).max() <= 1e-11
