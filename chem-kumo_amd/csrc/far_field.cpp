@@ -306,6 +306,7 @@ FarPlanHost far_field_plan(const std::vector<EspPairDev>& pairs, const std::vect
         const double dt = first_within(truncation, 1.25 * reach, 2.5 * reach + 1e-3, share);
         const double de = first_within(spread, 0.0, 1.0 / std::sqrt(pmin), share);
         cl.min_distance = std::max(dt, reach + de);
+        cl.max_L = top;
     }
     return plan;
 }

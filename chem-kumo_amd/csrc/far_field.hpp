@@ -38,12 +38,14 @@ constexpr int kFarTileSide = 8;    // grid points per edge of a target tile (src
 __host__ __device__ constexpr int far_hcount(int s) { return s < 0 ? 0 : (s + 1) * (s + 2) * (s + 3) / 6; }
 
 // Source cluster on the device: centre, the distance from a target centre
-// beyond which it is translated, and its pairs' range in the plan's pair list.
+// beyond which it is translated, its pairs' range in the plan's pair list
+// and their largest order L (its moments of shift order m vanish beyond
+// total order L + m).
 struct FarClusterDev {
     double c[3];
     double min_distance;
     int32_t first, count;
-    int32_t pad[2];
+    int32_t max_L, pad;
 };
 
 struct FarPlanHost {

@@ -279,17 +279,21 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
         const double x[3] = {cl.c[0] - b0, cl.c[1] - b1, cl.c[2] - b2};
         const double d2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
         // seeds (-1)^n (2n - 1)!! / |X|^(2n + 1) (coulomb_derivatives,
-        // src/electrostatic_potential.cpp:282-291)
+        // src/electrostatic_potential.cpp:282-291), to the order the
+        // cluster's moments reach: its largest pair order + N (the moments of
+        // higher total order are zero, and R^0 of a total order d needs only
+        // the seeds to order d, so every term is the one of the full order)
+        const int topc = cl.max_L + kFarMultipole;
         if (lane == 0) {
             const double inv = 1.0 / sqrt(d2), inv2 = inv * inv;
             double sd = inv;
-            for (int n = 0; n <= top; ++n) {
+            for (int n = 0; n <= topc; ++n) {
                 seeds[n] = sd;
                 sd *= -(2.0 * n + 1.0) * inv2;
             }
         }
         wave_sync();
-        const double* R = wave_hermite(seeds, x, top, buf_a, buf_b, recipe, lane);
+        const double* R = wave_hermite(seeds, x, topc, buf_a, buf_b, recipe, lane);
         const double* q0 = plan.moments + static_cast<long long>(ci) * (kFarMultipole + 1) * plan.moment_count;
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
@@ -298,7 +302,7 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
             const uchar4 hb = tri[be];
             const int sb = hb.x + hb.y + hb.z;
             const int m = kFarMultipole - sb;
-            acc[r] += shifted_dot(q0 + static_cast<long long>(m) * plan.moment_count, R, plan.max_L + m, hb.x, sb,
+            acc[r] += shifted_dot(q0 + static_cast<long long>(m) * plan.moment_count, R, cl.max_L + m, hb.x, sb,
                                   hb.z);
         }
         wave_sync(); // the buffers are rewritten by the next item
