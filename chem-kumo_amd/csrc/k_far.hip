@@ -481,12 +481,9 @@ void launch_far_targets(const FarPlanDev& plan, const EspPairDev* pairs, const d
     const size_t lds = ((H * 4 + 15) / 16) * 16 + NLOC * 4 + sizeof(double) * (FW * (2 * H + top + 2) + FW * NLOC) +
                        sizeof(int) * (kFarLevels + 4 + 4 + FW);
     if (lds > 160 * 1024) invalid("far-field expansion needs more LDS than a workgroup has");
-    static bool configured = false;
-    if (!configured) {
-        KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&far_targets_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        configured = true;
-    }
+    // (set on every launch: the attribute belongs to the current device)
+    KUMO_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&far_targets_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     far_targets_kernel<<<ntargets, FT, lds, stream>>>(plan, pairs, coef, boys_table, targets, local, near, nnear,
                                                       work);
     KUMO_HIP(hipGetLastError());
