@@ -134,3 +134,42 @@ def test_config1_whole_grid_invariants(ctx):
     assert np.all((got[5][0] >= 0) & (got[5][0] <= 1))
     assert np.all(t >= 0)
     g.close()
+
+
+def test_config1_folded_rows_beyond_the_workspace(ctx):
+    # Blocks of 512 points at threshold 1e-4 keep part of many pure shells:
+    # some blocks have more folded coefficient rows than the fold workspace
+    # holds per block (16), and the rest are formed at staging. FP64 (tile
+    # kernel) against the oracle at the same screening; BF16X6 on the wave
+    # kernel and on the 16-point tile kernel against the FP64 GPU path.
+    g, o, grid = load(ctx, lambda: synthetic.tzvp_cube(seed=1), "tzvp40_seed1")
+    props = [(K.Density, 0, 0, 0), (K.Gradient, 0, 0, 0), (K.Laplacian, 0, 0, 0), (K.Tau, 0, 0, 0),
+             (K.Elf, 0, 0, 0)]
+    allp = oracle.grid_points(grid)
+    blocks = np.random.default_rng(1).choice(len(allp) // 512, 40, replace=False)
+    pts = np.concatenate([allp[b * 512:(b + 1) * 512] for b in blocks])
+    opts = dict(points=pts, block_size=512, screening_threshold=1e-4)
+    m = o.masks(512, 2, points=pts, threshold=1e-4)
+    l, off, cnt, _ = o.basis.blocks()
+    ncart = (l + 1) * (l + 2) // 2
+    folded = [sum(int(ncart[s]) for s in range(len(l))
+                  if cnt[s] != ncart[s] and 0 < m[b, off[s]:off[s] + cnt[s]].sum() < cnt[s]) for b in range(len(m))]
+    print(f"[gpu] folded rows per block: max {max(folded)}, blocks above 16: {sum(f > 16 for f in folded)}",
+          flush=True)
+    assert max(folded) > 16
+    ref, _ = o.evaluate(props, points=pts, block_size=512, threshold=1e-4)
+    f64, _ = g.evaluate(props, precision=Precision.FP64, **opts)
+    rho_ref = ref[0][0]
+    for a, b in zip(f64[:4], ref[:4]):
+        assert np.max(np.abs(a - b)) <= 1e-10 * max(1.0, np.max(np.abs(b)))
+    # ELF where FP64 determines it: two FP64 summation orders agree within 1e-6
+    # above the floor (test_oracle_pins.py, test_elf_tail_is_undetermined_in_fp64)
+    sel = rho_ref >= ELF_FLOOR
+    assert np.max(np.abs(f64[4][0] - ref[4][0])[sel]) <= 1e-6
+    for tile in (64, 16):
+        got, _ = g.evaluate(props, precision=Precision.BF16X6, tile_points=tile, **opts)
+        rho = f64[0][0]
+        sig = rho >= 1e-6
+        assert np.max(np.abs(got[0][0] - rho)[sig] / rho[sig]) <= 1e-6, tile
+        assert np.max(np.abs(got[4][0] - f64[4][0])[rho >= ELF_FLOOR]) <= 1e-5, tile
+    g.close()
