@@ -200,6 +200,10 @@ struct EvaluationOptions {
     double fukui_delta = 0.1;
     ProgressCallback progress = {};
     Precision precision = Precision::FP64; // HIP only
+    // HIP only: the ESP at potential_tolerance > 0 through the reference's far
+    // field (true) or by the exact pair sums (false: within any tolerance, and
+    // the faster of the two on the GPU).
+    bool far_field = false;
 };
 
 struct PropertyField {
@@ -404,6 +408,7 @@ private:
         o.screening_threshold = options_.screening_threshold;
         o.potential_tolerance = options_.potential_tolerance;
         o.fukui_delta = options_.fukui_delta;
+        o.esp_far_field = options_.far_field ? KUMO_HIP_ESP_FAR : KUMO_HIP_ESP_AUTO;
         struct Hook {
             const ProgressCallback* cb;
             std::exception_ptr error;
@@ -478,7 +483,7 @@ struct SurfaceStatistics {
 struct SurfaceOptions { // include/chem/kumo/surface.hpp:108-116
     double isovalue = 1e-3;
     double spacing = 0.25;
-    double potential_tolerance = 1e-8; // the HIP path evaluates the potential exactly
+    double potential_tolerance = 1e-8; // the HIP path sums the potential exactly (within any tolerance)
 };
 
 struct PotentialSurface { // include/chem/kumo/surface.hpp:118-123

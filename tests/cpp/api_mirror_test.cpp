@@ -54,6 +54,20 @@ int main(int argc, char** argv) {
     const auto at = evaluator.evaluate(points, Request().add(density::total.value()));
     check(at.get(density::total.value()).component().size() == 2, "evaluation at explicit points");
 
+    // The ESP on the grid: exact sums and the far field at tolerance 1e-6.
+    EvaluationOptions far;
+    far.potential_tolerance = 1e-6;
+    far.far_field = true;
+    const auto exact_esp = evaluator.evaluate(grid, Request().add(density::total.electrostatic_potential()));
+    const auto far_esp = Evaluator(molecule, far).evaluate(grid, Request().add(density::total.electrostatic_potential()));
+    const auto ve = exact_esp.get(density::total.electrostatic_potential()).component();
+    const auto vf = far_esp.get(density::total.electrostatic_potential()).component();
+    double worst = 0.0;
+    for (std::size_t q = 0; q < ve.size(); ++q)
+        worst = std::max(worst, std::abs(ve[q] - vf[q]));
+    std::printf("far-field ESP: largest difference from the exact sums %.3g\n", worst);
+    check(vf.size() == ve.size() && worst <= 1e-6, "far-field ESP within its tolerance of the exact sums");
+
     // Cube output as tests/atom_charge_cube_test.cpp writes it: an iron atom
     // (Z 26) with an ECP charge of 16, two points per axis.
     Grid small;
