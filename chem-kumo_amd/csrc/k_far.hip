@@ -124,6 +124,46 @@ __device__ __attribute__((always_inline)) inline double shifted_dot(const double
     return s0 + s1;
 }
 
+constexpr int kSeedMax = 2 * kMaxL + kFarLocal; // highest seed order of a pair expansion (L + 6)
+
+// F_0..F_top(T) (top <= kSeedMax) in registers, in boys()'s arithmetic (the
+// nearest table point and the 7-term Taylor series below T = 40, the closed
+// form and upward recursion above): the loops run over the fixed bound with
+// the orders above top left alone, so that the array needs no scratch.
+__device__ __attribute__((always_inline)) inline void boys_regs(const double* __restrict__ table, double T, int top,
+                                                                double (&v)[kSeedMax + 1]) {
+    const double decay = exp(-T);
+    if (T >= 40.0) {
+        v[0] = 0.5 * sqrt(3.141592653589793 / T);
+#pragma unroll
+        for (int m = 0; m < kSeedMax; ++m)
+            if (m < top) v[m + 1] = ((2.0 * m + 1.0) * v[m] - decay) / (2.0 * T);
+        return;
+    }
+    const int point = static_cast<int>(llround(T / 0.05));
+    const double delta = T - point * 0.05;
+    const double* row = table + point * kBoysOrders + top;
+    double sum = 0.0, factor = 1.0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        sum += row[k] * factor;
+        factor *= -delta / (k + 1.0);
+    }
+    double cur = sum;
+#pragma unroll
+    for (int m = kSeedMax; m >= 0; --m) {
+        if (m <= top) v[m] = cur;
+        if (m <= top && m > 0) cur = (2.0 * T * cur + decay) / (2.0 * m - 1.0);
+    }
+}
+
+__device__ __attribute__((always_inline)) inline double lane_double(double v, int j) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(u), j);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(u >> 32), j);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const EspPairDev* __restrict__ pairs,
                                                          const double* __restrict__ coef,
                                                          const double* __restrict__ boys_table,
@@ -135,16 +175,18 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
     const int top = plan.max_L + kFarMultipole;
     const int H = far_hcount(top);
     // LDS: the recursion recipes recipe[H], the indices tri[NLOC] of the
-    // local coefficients, then per wave two recursion buffers and the seeds
-    // (top + 1 doubles), then the wave sums red[FW][NLOC], the level counts
-    // and the scan words.
+    // local coefficients, then per wave two recursion buffers, the seeds
+    // (top + 2 doubles) and a pair's Hermite coefficients, then the wave sums
+    // red[FW][NLOC], the level counts and the scan words.
     uint32_t* recipe = reinterpret_cast<uint32_t*>(smem);
     uchar4* tri = reinterpret_cast<uchar4*>(smem + ((H * 4 + 15) / 16) * 16);
     double* wbuf = reinterpret_cast<double*>(smem + ((H * 4 + 15) / 16) * 16 + NLOC * 4);
-    const int wstride = 2 * H + top + 2;
+    const int ncoef_max = far_hcount(plan.max_L) < 128 ? 128 : far_hcount(plan.max_L);
+    const int wstride = 2 * H + top + 2 + ncoef_max;
     double* buf_a = wbuf + wave * wstride;
     double* buf_b = buf_a + H;
     double* seeds = buf_b + H;
+    double* cbuf = seeds + top + 2; // the Hermite coefficients of the pair being expanded
     double* red = wbuf + FW * wstride;
     int* counts = reinterpret_cast<int*>(red + FW * NLOC);
     int* scan = counts + kFarLevels + 4; // cut, partial, -, -, then the wave totals [FW]
@@ -307,32 +349,65 @@ __global__ void __launch_bounds__(FT) far_targets_kernel(FarPlanDev plan, const 
         }
         wave_sync(); // the buffers are rewritten by the next item
     }
+    // Expanded pairs in batches of 64 per wave: each lane loads one pair of
+    // the batch and computes its seeds (-2p)^n F_n(p |PC|^2) (hermite_integrals,
+    // src/electrostatic_potential.cpp:269-280) in registers; the pairs are
+    // then expanded one after another, each pair's seeds and Hermite
+    // coefficients staged in LDS, the next pair's coefficients loaded while
+    // the current one is expanded.
     const int nexp = n_exp;
-    for (int e = wave; e < nexp; e += FW) {
-        const int i = nl[plan.npairs - 1 - e];
+    const int ncoef = far_hcount(plan.max_L);
+    for (int base = wave * 64; base < nexp; base += FW * 64) {
+        const int cnt = min(64, nexp - base);
+        const int i = nl[plan.npairs - 1 - (base + min(lane, cnt - 1))];
         const EspPairDev pr = pairs[i];
-        const double x[3] = {pr.qx - b0, pr.qy - b1, pr.qz - b2};
+        const double px = pr.qx - b0, py = pr.qy - b1, pz = pr.qz - b2;
         const int ord = pr.L + kFarLocal;
-        // seeds (-2p)^n F_n(p |PC|^2) (hermite_integrals, src/electrostatic_potential.cpp:269-280)
-        if (lane == 0) {
-            boys(boys_table, kBoysOrders, ord, pr.p * ((x[0] * x[0] + x[1] * x[1]) + x[2] * x[2]), ord, seeds);
+        double sd[kSeedMax + 1];
+        boys_regs(boys_table, pr.p * ((px * px + py * py) + pz * pz), ord, sd);
+        {
             double power = 1.0;
-            for (int n = 1; n <= ord; ++n) {
+#pragma unroll
+            for (int n = 1; n <= kSeedMax; ++n) {
                 power *= -2.0 * pr.p;
-                seeds[n] *= power;
+                if (n <= ord) sd[n] *= power;
             }
         }
-        wave_sync();
-        const double* R = wave_hermite(seeds, x, ord, buf_a, buf_b, recipe, lane);
-        const double* c = coef + pr.first;
+        // coefficients of the batch's first pair
+        const int first0 = __builtin_amdgcn_readlane(pr.first, 0);
+        const int nk0 = far_hcount(__builtin_amdgcn_readlane(pr.L, 0));
+        double cpre[2] = {lane < nk0 ? coef[first0 + lane] : 0.0, lane + 64 < nk0 ? coef[first0 + lane + 64] : 0.0};
+        for (int j = 0; j < cnt; ++j) {
+            const int Lj = __builtin_amdgcn_readlane(pr.L, j);
+            const int firstj = __builtin_amdgcn_readlane(pr.first, j);
+            const int nk = far_hcount(Lj);
+            if (lane == j) {
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) {
-            const int be = lane + 64 * r;
-            if (be >= NLOW) continue;
-            const uchar4 hb = tri[be];
-            acc[r] += shifted_dot(c, R, pr.L, hb.x, hb.x + hb.y + hb.z, hb.z);
+                for (int n = 0; n <= kSeedMax; ++n)
+                    if (n <= ord) seeds[n] = sd[n];
+            }
+            cbuf[lane] = cpre[0];
+            if (lane + 64 < ncoef) cbuf[lane + 64] = cpre[1];
+            for (int k = lane + 128; k < nk; k += 64) // beyond two chunks (L >= 7): straight from memory
+                cbuf[k] = coef[firstj + k];
+            if (j + 1 < cnt) {
+                const int fn = __builtin_amdgcn_readlane(pr.first, j + 1);
+                const int nn = far_hcount(__builtin_amdgcn_readlane(pr.L, j + 1));
+                cpre[0] = lane < nn ? coef[fn + lane] : 0.0;
+                cpre[1] = lane + 64 < nn ? coef[fn + lane + 64] : 0.0;
+            }
+            wave_sync();
+            const double xj[3] = {lane_double(px, j), lane_double(py, j), lane_double(pz, j)};
+            const double* R = wave_hermite(seeds, xj, Lj + kFarLocal, buf_a, buf_b, recipe, lane);
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int be = lane + 64 * r;
+                if (be >= NLOW) continue;
+                const uchar4 hb = tri[be];
+                acc[r] += shifted_dot(cbuf, R, Lj, hb.x, hb.x + hb.y + hb.z, hb.z);
+            }
+            wave_sync();
         }
-        wave_sync();
     }
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
@@ -478,7 +553,9 @@ void launch_far_targets(const FarPlanDev& plan, const EspPairDev* pairs, const d
     const int top = plan.max_L + kFarMultipole;
     const int H = far_hcount(top);
     if (H > 2048) invalid("far-field expansion order too high"); // recipe indices take 11 bits
-    const size_t lds = ((H * 4 + 15) / 16) * 16 + NLOC * 4 + sizeof(double) * (FW * (2 * H + top + 2) + FW * NLOC) +
+    const int ncoef_max = far_hcount(plan.max_L) < 128 ? 128 : far_hcount(plan.max_L);
+    const size_t lds = ((H * 4 + 15) / 16) * 16 + NLOC * 4 +
+                       sizeof(double) * (FW * (2 * H + top + 2 + ncoef_max) + FW * NLOC) +
                        sizeof(int) * (kFarLevels + 4 + 4 + FW);
     if (lds > 160 * 1024) invalid("far-field expansion needs more LDS than a workgroup has");
     // (set on every launch: the attribute belongs to the current device)
