@@ -489,9 +489,9 @@ __global__ void __launch_bounds__(FT) far_poly_kernel(const EspAtomDev* __restri
 }
 
 // The near pairs of order L (pair indices [lo, hi), a contiguous part of
-// each target's sorted near list), subtracted at the target's points. A
-// target of up to 64 (128) points splits the pairs into 4 (2) slices over
-// the threads, summed in slice order.
+// each target's sorted near list), subtracted at the target's points, 64
+// points at a time with the pairs split into 4 slices over the threads,
+// summed in slice order.
 template <int L>
 __global__ void __launch_bounds__(FT) far_near_kernel(const EspPairDev* __restrict__ pairs,
                                                       const double* __restrict__ coef,
@@ -517,8 +517,10 @@ __global__ void __launch_bounds__(FT) far_near_kernel(const EspPairDev* __restri
     };
     const int e0 = lower(lo), e1 = lower(hi);
     if (e0 >= e1) return; // uniform over the workgroup
-    const int span = tg.count <= 64 ? 64 : (tg.count <= 128 ? 128 : FT); // points per pass
-    const int slices = FT / span;
+    // 64 points per pass, the pairs in 4 slices: the same summation order for
+    // a point whatever else its target holds (a range of listed points holds
+    // part of a cube)
+    constexpr int span = 64, slices = FT / span;
     const int slice = threadIdx.x / span;
     for (int k0 = 0; k0 < tg.count; k0 += span) {
         const int k = k0 + static_cast<int>(threadIdx.x) % span;
@@ -531,15 +533,14 @@ __global__ void __launch_bounds__(FT) far_near_kernel(const EspPairDev* __restri
             for (int e = e0 + slice; e < e1; e += slices)
                 sum += pair_sum<L>(pairs[nl[e]], coef, boys_table, x, y, z);
         }
-        if (slices > 1) {
-            part[threadIdx.x] = sum;
-            __syncthreads();
-            if (slice == 0) {
-                for (int sl = 1; sl < slices; ++sl)
-                    sum += part[threadIdx.x + sl * span];
-            }
-            __syncthreads();
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        if (slice == 0) {
+#pragma unroll
+            for (int sl = 1; sl < slices; ++sl)
+                sum += part[threadIdx.x + sl * span];
         }
+        __syncthreads();
         if (slice == 0 && q >= 0) out[q] -= sum;
     }
 }

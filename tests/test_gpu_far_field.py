@@ -171,6 +171,22 @@ def test_ranges_cut_tiles(ctx):
     assert np.array_equal(res[0][0], whole)
 
 
+def test_ranges_of_listed_points(ctx):
+    # the serving loop's ranges of a point list: each range groups its own
+    # points into cubes, and a value depends only on its point and cube
+    from kumo_hip.binding import Server
+    g, _ = load(ctx, "pople20")
+    mol = synthetic.pople_molecule()
+    pts = synthetic.vdw_shell_points(mol.atomic_numbers, mol.xyz, 20000)
+    whole = esp(g, 1e-7, FAR, points=pts)
+    srv = Server(ctx, workers=3, chunk_points=1024)
+    try:
+        res, _ = srv.submit(g, ESP, points=pts, potential_tolerance=1e-7, esp_far_field=FAR).wait()
+    finally:
+        srv.close()
+    assert np.array_equal(res[0][0], whole)
+
+
 def test_invalid_method(ctx):
     g, _ = load(ctx, "pople20")
     with pytest.raises(Exception, match="electrostatic potential method"):
