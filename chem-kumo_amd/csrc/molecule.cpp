@@ -528,31 +528,51 @@ void kumo_hip_molecule::ensure_esp(hipStream_t stream) {
                     out.push_back({t, u, s - t - u});
         return out;
     };
+    // One Cartesian direction of a primitive pair, as McMurchie-Davidson coefficients E^{ij}_t:
+    // (x-A)^i (x-B)^j = sum_t E^{ij}_t Lambda_t, Lambda_t = (d/dPx)^t exp(-p u^2), u = x - Px.
+    // The product is first multiplied out as a polynomial in u, (u + pa)^i (u + pb)^j, and each
+    // power u^k is then rewritten over the Lambda_t with u Lambda_t = Lambda_{t+1} / 2p + t Lambda_{t-1}.
     struct Herm {
         int lb, width;
-        std::vector<double> v;
+        std::vector<double> e; // (i, j, t) at (i * (lb + 1) + j) * width + t
         Herm(int la, int lb_, double p, double pa, double pb) : lb(lb_), width(la + lb_ + 1) {
-            v.assign(static_cast<std::size_t>((la + 1) * (lb + 1) * width), 0.0);
+            const int L = la + lb;
             const double half = 0.5 / p;
-            at(0, 0, 0) = 1.0;
-            const auto step = [&](int si, int sj, int ti, int tj, double shift) {
-                for (int t = 0; t <= ti + tj; ++t) {
-                    double value = shift * get(si, sj, t);
-                    if (t > 0) value += half * get(si, sj, t - 1);
-                    if (t + 1 <= si + sj) value += (t + 1) * get(si, sj, t + 1);
-                    at(ti, tj, t) = value;
+            // u^k = sum_t power[k * width + t] Lambda_t
+            std::vector<double> power(static_cast<std::size_t>(width * width), 0.0);
+            power[0] = 1.0;
+            for (int k = 1; k <= L; ++k)
+                for (int t = 0; t <= k; ++t) {
+                    double d = t > 0 ? half * power[static_cast<std::size_t>((k - 1) * width + t - 1)] : 0.0;
+                    if (t + 1 <= k - 1) d += (t + 1) * power[static_cast<std::size_t>((k - 1) * width + t + 1)];
+                    power[static_cast<std::size_t>(k * width + t)] = d;
                 }
-            };
-            for (int i = 0; i < la; ++i)
-                step(i, 0, i + 1, 0, pa);
-            for (int i = 0; i <= la; ++i)
-                for (int j = 0; j < lb; ++j)
-                    step(i, j, i, j + 1, pb);
+            e.assign(static_cast<std::size_t>((la + 1) * (lb + 1) * width), 0.0);
+            std::vector<double> left(static_cast<std::size_t>(width), 0.0), poly;
+            left[0] = 1.0; // (u + pa)^i, coefficient of u^k at k
+            for (int i = 0; i <= la; ++i) {
+                if (i > 0)
+                    for (int k = i; k >= 0; --k)
+                        left[static_cast<std::size_t>(k)] = (k > 0 ? left[static_cast<std::size_t>(k - 1)] : 0.0) +
+                                                            pa * left[static_cast<std::size_t>(k)];
+                poly = left; // (u + pa)^i (u + pb)^j
+                for (int j = 0; j <= lb; ++j) {
+                    if (j > 0)
+                        for (int k = i + j; k >= 0; --k)
+                            poly[static_cast<std::size_t>(k)] = (k > 0 ? poly[static_cast<std::size_t>(k - 1)] : 0.0) +
+                                                                pb * poly[static_cast<std::size_t>(k)];
+                    for (int t = 0; t <= i + j; ++t) {
+                        double sum = 0.0;
+                        for (int k = t; k <= i + j; k += 2)
+                            sum += poly[static_cast<std::size_t>(k)] * power[static_cast<std::size_t>(k * width + t)];
+                        e[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)] = sum;
+                    }
+                }
+            }
         }
         double get(int i, int j, int t) const {
-            return (t < 0 || t > i + j) ? 0.0 : v[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)];
+            return (t < 0 || t > i + j) ? 0.0 : e[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)];
         }
-        double& at(int i, int j, int t) { return v[static_cast<std::size_t>((i * (lb + 1) + j) * width + t)]; }
     };
     std::vector<std::vector<EspPairDev>> groups(2 * kMaxL + 1);
     std::vector<std::vector<double>> gcoef(2 * kMaxL + 1);
